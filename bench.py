@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark of the hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dense256]
+
+A step is one pass of the hot path in the reference's order over the basin's live
+particles: particles to grid, classify, gravity, pressure projection (MG-PCG to a
+1e-6 relative FP64 residual, solved from zero each step), grid to particles,
+resample. Inputs are synthetic (the balanced Lamb-Oseen vortex of `initialize!`) and
+resident on the GPU before the timed region. Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "maelstrom.jl_amd"))
+
+METRIC = "ms/step at 5cm sparse, MP-MGPCG vs FP32 @1e-6 rel. residual; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (basin kwargs, dx, description)
+    "dense256": (dict(radius=18.0, depth=12.0, Γ=225.0, core=2.5), 0.15,
+                 "dense FlipBasin 256x256x128, dx 0.15 m, Lamb-Oseen vortex Γ 225 m^2/s core 2.5 m, 8 particles/cell "
+                 "in the 3-cell band, MG-PCG (6 levels, 2 RBGS sweeps) to 1e-6, CFL 4"),
+    "dense64": (dict(radius=18.0, depth=12.0, Γ=225.0, core=2.5), 0.6,
+                "dense FlipBasin 64x64x32, dx 0.6 m, the same vortex, MG-PCG (4 levels) to 1e-6"),
+}
+
+
+def algorithmic_bytes(kernel, s, n_particles):
+    """Compulsory bytes of one launch of `kernel` at the reference's element sizes."""
+    n1, n2, n3 = s.n_cells
+    C = n1 * n2 * n3
+    return {
+        # one colour: x of every cell (neighbours), types of every cell, b/diag/x-write of half
+        "rbgs": 11 * C,
+        # z, d, diag, q (4 B each) and types (1 B) per cell
+        "search": 17 * C,
+        # z, q read; d, x, r read and written (4 B each) per cell
+        "advance": 32 * C,
+        # x (12 B) read, v (12 B) and c (36 B) written per particle
+        "g2p": 60 * n_particles,
+        # x, v, c, lag (64 B) read per particle
+        "p2g": 64 * n_particles,
+        # 64 B read and 64 B written per particle, key and rank (8 B) read
+        "scatter": 136 * n_particles,
+        # r, x (4 B) read, types (1 B), r (4 B) written per cell
+        "residual": 13 * C,
+    }[kernel]
+
+
+def cpu_baseline(s, iters_per_step, dt):
+    """The oracle's FP32 CPU port on the same state (one core): resample, P2G and G2P
+    over all particles, and a few MG-PCG iterations, scaled to the GPU's iterations."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as orc
+    g = orc.Grid()
+    for a in range(3):
+        g.lo[a] = s.grid.lo[a]
+        g.n[a] = s.grid.n[a]
+    g.dx, g.radius, g.floor, g.levels = s.grid.dx, s.grid.radius, s.grid.floor, s.grid.levels
+    n = s.n
+    ps = {f: s.get(f) for f in orc.PFIELDS}
+    phi, u, v, w = (s.get(k) for k in ("PHI", "U", "V", "W"))
+    types = s.get("TYPES")
+    flags = [s.get(k) for k in ("FU", "FV", "FW")]
+    t = {}
+    t0 = time.perf_counter()
+    orc.dense_p2g(g, ps, n, dt, phi, u, v, w, fp32=True)
+    t["p2g"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    orc.dense_g2p(g, ps, n, u, v, w, *flags, dt)
+    t["g2p"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    orc.dense_resample(g, ps, n, s.capacity, phi, u, v, w, seed=s.steps + 1)
+    t["resample"] = time.perf_counter() - t0
+    b = orc.dense_rhs(g, u, v, w - np.float32(9.81 * dt), types, 0.0).astype(np.float32)
+    t0 = time.perf_counter()
+    orc.dense_pressure(g, types, phi, b, tol=1e-30, maxiter=1, fp32=True)
+    t1 = time.perf_counter()
+    orc.dense_pressure(g, types, phi, b, tol=1e-30, maxiter=4, fp32=True)
+    t2 = time.perf_counter()
+    per_iter = (t2 - t1 - (t1 - t0)) / 3
+    setup = (t1 - t0) - per_iter
+    ms = 1e3 * (t["p2g"] + t["g2p"] + t["resample"] + setup + per_iter * iters_per_step)
+    sample = ("oracle FP32 port, one core: P2G, G2P and resample over all %d particles timed once; MG-PCG set-up "
+              "plus 1 and 4 iterations timed, per-iteration cost scaled to the GPU's %.1f iterations/step "
+              "(%.1f s of CPU work)" % (n, iters_per_step, (t2 - t0) + t["p2g"] + t["g2p"] + t["resample"]))
+    return {"value": round(ms, 1), "unit": "ms/step", "cores": 1, "kind": "port", "sample": sample}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="dense256", choices=sorted(WORKLOADS))
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--kernel", default="rbgs", help="the kernel the roofline object reports")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build's hot path runs on a single MI355X (--gpus %d not supported)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import maelstrom_amd as m
+    from maelstrom_amd._lib import KERNEL_ID
+    kw, dx, desc = WORKLOADS[args.workload]
+    b = m.Basin(**kw)
+    s = m.FlipBasin(b, dx=dx, band=4, nmin=8, cfl=4.0, capacity=30_000_000)
+    m.initialize(s)
+    # run.jl:44: dt = left / ceil(left / (cfl dx / max(maxspeed, 1e-3))) within a 1/24 s frame
+    left = 1.0 / 24
+    dt = left / math.ceil(left / (s.cfl * dx / max(m.maxspeed(s), 1e-3)))
+    solve = dict(tol=args.tol, maxiter=400, precision=args.precision, warm=0)
+    for _ in range(args.warmup):
+        m.step(s, dt, **solve)
+    s.sync()
+    L = m.lib()
+    s.ctx.check(L.mael_dense_ktimer(s.h, KERNEL_ID[args.kernel], 64 * 1024))
+    iters, pressure_ms, nparts = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = m.step(s, dt, **solve)
+        iters.append(st.iterations)
+        pressure_ms.append(st.ms)
+        nparts.append(s.n)
+    s.sync()
+    t1 = time.perf_counter()
+    import ctypes as C
+    kms, kl = C.c_double(), C.c_int32()
+    s.ctx.check(L.mael_dense_ktime(s.h, C.byref(kms), C.byref(kl)))
+    ms_step = 1e3 * (t1 - t0) / args.steps
+    avg_launch_ms = kms.value / max(kl.value, 1)
+    n_avg = sum(nparts) / len(nparts)
+    abytes = algorithmic_bytes(args.kernel, s, n_avg)
+    achieved = abytes / (avg_launch_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(ms_step, 3),
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "cells": list(s.n_cells), "particles": int(n_avg), "dt_s": round(dt, 5),
+                   "tol": args.tol, "precision": ["fp32", "fp16", "bf16"][args.precision], "solve_start": "zero"},
+        "pressure_ms": round(sum(pressure_ms) / len(pressure_ms), 3),
+        "cg_iterations": round(sum(iters) / len(iters), 2),
+        "mparticles_per_s": round(n_avg / (ms_step * 1e-3) / 1e6, 2),
+        "roofline": {"bound": "hbm", "kernel": args.kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "launches": kl.value, "avg_launch_us": round(avg_launch_ms * 1e3, 3),
+                     "algorithmic_bytes_per_launch": int(abytes)},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(s, out["cg_iterations"], dt)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

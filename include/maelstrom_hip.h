@@ -1,0 +1,184 @@
+/*
+ * maelstrom_hip.h — the C ABI of the MI355X (gfx950) hot path of Maelstrom.jl.
+ *
+ * The reference runs its hot path as KernelAbstractions kernels launched from Julia
+ * (`k!(backend, wg)(args...; ndrange)`, e.g. flip.jl:596) with device arrays from
+ * `KA.zeros`/`Mantle.devicearray`. This library replaces that layer at phase
+ * granularity: one handle per basin owns the device state, and each entry point runs
+ * one phase of a step (resample, P2G, projection, G2P) on the basin's HIP stream, so
+ * that the phase's kernels can be fused and the Krylov loop can keep its scalars on the
+ * device. The Julia package calls these with `ccall`; see INTEGRATION.md.
+ *
+ * Conventions
+ *  - Every function returns a status (MAEL_OK = 0, negative on error) unless stated.
+ *    `mael_last_error(ctx)` then holds the message; where the reference raises an
+ *    error the text is the reference's (e.g. "the narrow band needs N particles, more
+ *    than the C allocated", flip.jl:1373).
+ *  - Indices are 0-based. Grid arrays are x-fastest (the reference's column-major
+ *    order). Particles are structure-of-arrays (x, y, z, vx, vy, vz, the nine entries
+ *    of the APIC rows cx, cy, cz, lag), unlike the reference's interleaved Point3f.
+ *  - Calls on one basin are issued in order on its stream; a call that returns a host
+ *    value (a count, statistics) synchronises. A basin is not re-entrant.
+ */
+#ifndef MAELSTROM_HIP_H
+#define MAELSTROM_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define MAEL_OK               0
+#define MAEL_E_ARG           -1  /* bad argument, shape or handle */
+#define MAEL_E_CAPACITY      -2  /* particle capacity exceeded, flip.jl:1373, sflip.jl:2237 */
+#define MAEL_E_BLOCKS        -3  /* fine-block pool exceeded, sflip.jl:530 */
+#define MAEL_E_NONFINITE     -4  /* "Nonfinite pressure ...", sflip.jl:1873-1898, fgmres.jl:31-65 */
+#define MAEL_E_CG_INDEFINITE -5  /* "Pressure CG requires a positive ...", sflip.jl:1882-1896 */
+#define MAEL_E_NOCONVERGE    -6  /* "Pressure did not converge ...", fgmres.jl:90 */
+#define MAEL_E_BREAKDOWN     -7  /* "Pressure Arnoldi breakdown", fgmres.jl:77 */
+#define MAEL_E_HIP           -8  /* a HIP runtime error */
+#define MAEL_E_GRID          -9  /* "the coarse grid ... does not halve ...", sflip.jl:326 */
+
+/* ── context ──────────────────────────────────────────────────────────────── */
+
+typedef struct mael_ctx mael_ctx;
+
+/* Open `device` (a HIP device ordinal). Replaces `Mantle.defaultbackend()`
+ * (test/runtests.jl:20): the backend every basin is built on. */
+int mael_create(int device, mael_ctx **out);
+void mael_destroy(mael_ctx *ctx);
+/* The message of the last failed call on `ctx` (or on a basin built on it). */
+const char *mael_last_error(const mael_ctx *ctx);
+/* The library's version string. */
+const char *mael_version(void);
+
+/* ── the dense grid (`FlipGrid`, flip.jl:30-62) ───────────────────────────── */
+
+typedef struct {
+    float lo[3];      /* the grid's low corner, m */
+    float dx;         /* cell width, m */
+    int32_t n[3];     /* cells per axis */
+    float radius;     /* the basin's wall radius, m */
+    float floor;      /* the basin's floor height, m (= -depth) */
+    int32_t levels;   /* multigrid levels: the coarsest has at most 1024 cells */
+} mael_dense_grid;
+
+/* `FlipGrid(b; dx, top = b.rimwall, pad)` (flip.jl:48-62): the sizing rule. */
+int mael_flipgrid(float radius, float depth, float top, float dx, int32_t pad, mael_dense_grid *out);
+
+/* ── a dense basin (`FlipBasin`, flip.jl:156-226) ─────────────────────────── */
+
+typedef struct mael_dense mael_dense;
+
+/* `FlipBasin(b, backend; dx, band, nmin, capacity)` (flip.jl:189-223): allocates the
+ * grid fields, the multigrid levels and two particle buffers of `capacity`. */
+int mael_dense_create(mael_ctx *ctx, const mael_dense_grid *grid, int64_t capacity, int32_t band, int32_t nmin,
+                      mael_dense **out);
+void mael_dense_destroy(mael_dense *d);
+
+/* Arrays of a basin, for `Array(s.u)`, `copyto!(s.φ, …)` and the like (the reference reads
+ * solver arrays this way in checkpoint.jl:10-13 and live.jl:36-37). Counts are elements. */
+enum {
+    MAEL_U = 0, MAEL_V, MAEL_W,           /* face velocities (float) */
+    MAEL_U0, MAEL_V0, MAEL_W0,            /* the grid's own velocity combine falls back to (float) */
+    MAEL_PHI, MAEL_PHI0,                  /* level set (float) */
+    MAEL_TYPES,                           /* SOLID 0 / LIQUID 1 / AIR 2 (uint8) */
+    MAEL_P,                               /* pressure P = p dt / rho (float) */
+    MAEL_FU, MAEL_FV, MAEL_FW,            /* valid-face flags (uint8) */
+    MAEL_RHS,                             /* the projection's right-hand side (float) */
+    MAEL_PX, MAEL_PY, MAEL_PZ,            /* particle positions (float, live buffer) */
+    MAEL_VX, MAEL_VY, MAEL_VZ,            /* particle velocities */
+    MAEL_C0, MAEL_C1, MAEL_C2, MAEL_C3, MAEL_C4, MAEL_C5, MAEL_C6, MAEL_C7, MAEL_C8,  /* APIC rows cx, cy, cz */
+    MAEL_LAG,                             /* particle time lag */
+    MAEL_KEY, MAEL_RANK,                  /* last resample: each particle's cell (-1 dropped) and rank */
+    MAEL_COUNTS, MAEL_TOTALS, MAEL_OFFSETS, /* last resample: per cell (offsets inclusive) */
+    MAEL_MOM_U, MAEL_MOM_V, MAEL_MOM_W,   /* last P2G: momentum accumulators (int64 fixed point, 2^-32) */
+    MAEL_WT_U, MAEL_WT_V, MAEL_WT_W,      /* last P2G: weight accumulators (int64 fixed point, 2^-32) */
+    MAEL_NARRAYS
+};
+/* Device pointer, element count and element size of array `which`. */
+int mael_dense_array(mael_dense *d, int32_t which, void **dptr, int64_t *count, int32_t *elsize);
+/* Copy `count` elements between host memory and array `which` (synchronous). */
+int mael_dense_upload(mael_dense *d, int32_t which, const void *host, int64_t count);
+int mael_dense_download(mael_dense *d, int32_t which, void *host, int64_t count);
+/* `s.particles.n`. */
+int mael_dense_count(mael_dense *d, int64_t *n);
+int mael_dense_set_count(mael_dense *d, int64_t n);
+
+/* `resample!(s; seeding)` (flip.jl:1364-1382). `seed` is `s.steps + 1`. On success
+ * `*total` is the new particle count. Ranks are stable (particle-index order within a
+ * cell), so the result is deterministic. MAEL_E_CAPACITY if the band needs more
+ * particles than the capacity (then the particles are unchanged). */
+int mael_dense_resample(mael_dense *d, int32_t seeding, uint32_t seed, int64_t *total);
+
+/* `particles2grid!(s, dt)` (flip.jl:587-606): APIC deposit (deterministic), splat,
+ * union with φ0 + dx, combine with u0 within (band - 1) dx of the surface. */
+int mael_dense_p2g(mael_dense *d, float dt);
+
+/* `grid2particles!(s, dt)` (flip.jl:1246-1252). */
+int mael_dense_g2p(mael_dense *d, float dt, float gravity);
+
+typedef struct {
+    double tol;          /* relative residual target (the reference's default 1e-4) */
+    int32_t maxiter;     /* CG iterations in all (60) */
+    int32_t sweeps;      /* RBGS sweeps per level (2) */
+    int32_t coarse;      /* coarsest-level sweeps (40) */
+    int32_t precision;   /* 0: FP32 V-cycle (the reference's) */
+    int32_t refine;      /* 1: FP64 true-residual restarts until tol is met */
+    int32_t warm;        /* 1: start from the last pressure (the reference's warm start); 0: from zero */
+} mael_solve_opts;
+
+typedef struct {
+    int32_t iterations;        /* CG iterations */
+    int32_t restarts;          /* FP64 residual restarts */
+    double rel_residual;       /* ||b - A P|| / ||b||, FP64 */
+    double rel_residual_rec;   /* the FP32 recursive residual at the last CG stop */
+    double ms;                 /* wall time of the solve */
+} mael_solve_stats;
+
+/* Defaults of `pressure!(s::FlipBasin)` (flip.jl:1082): tol 1e-4, maxiter 60, sweeps 2, coarse 40,
+ * warm start, FP64 refinement on. */
+void mael_solve_defaults(mael_solve_opts *o);
+
+/* `project!(s, dt)` (flip.jl:1144-1156) after the caller's γ (the volume control,
+ * flip.jl:1146-1147): rhs, clear air, `pressure!`, project the faces and flag them. */
+int mael_dense_project(mael_dense *d, float dt, float gamma, const mael_solve_opts *opts, mael_solve_stats *stats);
+
+/* Support phases around the hot path (not part of it): */
+/* `classify!(s)` (flip.jl:674-677). */
+int mael_dense_classify(mael_dense *d);
+/* `volume(s)` (flip.jl:293), m^3. */
+int mael_dense_volume(mael_dense *d, double *volume);
+/* The balanced vortex of `initialize!` (flip.jl:325-330): φ from the surface table
+ * `eta` (neta samples of `surface(b, r)` over r in [0, rmax]) and the Lamb-Oseen swirl. */
+int mael_dense_init_vortex(mael_dense *d, const float *eta, int32_t neta, float rmax, float gamma_circ, float core);
+/* `solidify!(s)` (flip.jl:800-807). */
+int mael_dense_solidify(mael_dense *d);
+/* The gravity term of `forces!` (flip.jl:765-775) followed by `solidify!`. */
+int mael_dense_gravity(mael_dense *d, float dt, float gravity);
+/* Copy array `src` into array `dst` of the same size and type on the device (e.g. the
+ * `copyto!(s.φ0, s.φ)` of flip.jl:342). */
+int mael_dense_copy(mael_dense *d, int32_t dst, int32_t src);
+/* Wait for the basin's stream. */
+int mael_dense_sync(mael_dense *d);
+
+/* ── launch timing (no counterpart in the reference, which has no profiler hooks) ── */
+/* Kernel ids for the launch timer. */
+enum {
+    MAEL_K_P2G = 0, MAEL_K_SPLAT, MAEL_K_COMBINE, MAEL_K_G2P, MAEL_K_BIN, MAEL_K_FIXRANK, MAEL_K_SCATTER,
+    MAEL_K_SPAWN, MAEL_K_SEARCH, MAEL_K_ADVANCE, MAEL_K_RBGS, MAEL_K_RESIDUAL, MAEL_K_RESTRICT, MAEL_K_PROLONG,
+    MAEL_K_COARSEST, MAEL_K_RESID64, MAEL_K_COUNT
+};
+/* Record HIP events around each of the next (up to `launches`) launches of `kernel`
+ * (fine-level launches only for the multigrid kernels), on the basin's stream. */
+int mael_dense_ktimer(mael_dense *d, int32_t kernel, int32_t launches);
+/* Total milliseconds and number of the recorded launches; stops recording. */
+int mael_dense_ktime(mael_dense *d, double *ms, int32_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAELSTROM_HIP_H */

@@ -1,0 +1,166 @@
+// Device state of one dense basin (the reference's `FlipBasin`, flip.jl:156-226) and
+// its phases. One HIP stream per basin; every phase is asynchronous on it except
+// where a host value is returned (the resample total, solver statistics).
+#pragma once
+#include "common.hpp"
+#include <vector>
+
+namespace mael {
+
+// A device allocation that frees itself.
+template <class T> struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) {
+            if (p) (void)hipFree(p);
+            p = o.p; n = o.n; o.p = nullptr; o.n = 0;
+        }
+        return *this;
+    }
+    ~DBuf() { if (p) (void)hipFree(p); }
+    void alloc(size_t count) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        n = count;
+        if (count) MAEL_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    }
+};
+
+struct SolveOpts {
+    double tol = 1e-4;
+    int maxiter = 60, sweeps = 2, coarse = 40;
+    int precision = 0;        // 0 FP32 V-cycle, 1 FP16 storage, 2 BF16 storage
+    int refine = 1;           // FP64 true-residual recomputation with restarts
+    int warm = 1;             // start from the last pressure (else from zero)
+};
+struct SolveStats {
+    int iterations = 0, restarts = 0;
+    double rel_residual = 0, rel_residual_recursive = 0;
+    double ms_total = 0;
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    void* pinned = nullptr;   // small pinned host scratch for scalars read back
+    Stream() {
+        MAEL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        MAEL_HIP(hipHostMalloc(&pinned, 4096, hipHostMallocDefault));
+    }
+    ~Stream() {
+        if (s) (void)hipStreamDestroy(s);
+        if (pinned) (void)hipHostFree(pinned);
+    }
+    void sync() { MAEL_HIP(hipStreamSynchronize(s)); }
+};
+
+// Kernel ids for the launch timer (mirrored as MAEL_K_* in include/maelstrom_hip.h).
+enum KernelId {
+    K_P2G = 0, K_SPLAT, K_COMBINE, K_G2P, K_BIN, K_FIXRANK, K_SCATTER, K_SPAWN, K_SEARCH, K_ADVANCE, K_RBGS,
+    K_RESIDUAL, K_RESTRICT, K_PROLONG, K_COARSEST, K_RESID64, K_NKERNELS
+};
+
+// HIP events around every launch of one chosen kernel, on the stream it runs on, so a
+// benchmark can read that kernel's launch durations over a timed region.
+struct KTimer {
+    int target = -1;
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    ~KTimer() { for (auto e : ev) (void)hipEventDestroy(e); }
+    void enable(int id, size_t launches) {
+        while (ev.size() < 2 * launches) {
+            hipEvent_t e;
+            MAEL_HIP(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        target = id;
+        used = 0;
+    }
+    bool on(int id) const { return id == target && used + 2 <= ev.size(); }
+    void begin(int id, hipStream_t s) { if (on(id)) MAEL_HIP(hipEventRecord(ev[used], s)); }
+    void end(int id, hipStream_t s) {
+        if (on(id)) { MAEL_HIP(hipEventRecord(ev[used + 1], s)); used += 2; }
+    }
+    double read(int* launches) {   // total ms of the recorded launches; disables the timer
+        double t = 0;
+        for (size_t i = 0; i < used; i += 2) {
+            MAEL_HIP(hipEventSynchronize(ev[i + 1]));
+            float ms = 0;
+            MAEL_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+            t += ms;
+        }
+        if (launches) *launches = (int)(used / 2);
+        target = -1;
+        used = 0;
+        return t;
+    }
+};
+
+// Levels of the dense multigrid (flip.jl:117-123, 195-202): types and x, b, r per level.
+struct DLevel {
+    int n[3];
+    DBuf<uint8_t> types;
+    DBuf<float> x, b, r;
+    size_t cells() const { return (size_t)n[0] * n[1] * n[2]; }
+};
+
+struct DenseBasin {
+    DGrid g;
+    int levels;
+    int32_t band, nmin;
+    Stream st;
+    KTimer kt;
+
+    // grid state (flip.jl:160-172)
+    DBuf<float> u[3], u0[3];
+    DBuf<float> phi, phi0;
+    DBuf<int32_t> phip;
+    DBuf<uint8_t> types, flag[3];
+    DBuf<float> P, b, r, z, d, q, diag;
+    DBuf<double> P64;                         // the pressure, accumulated in FP64
+    DBuf<float> e, rv;                        // CG's FP32 correction; level 0's V-cycle residual
+    DBuf<unsigned long long> mom[3], wt[3];   // fixed-point P2G accumulators
+    std::vector<DLevel> lv;                   // lv[0] mirrors the fine grid (only n is used)
+    DBuf<double> partials;                    // per-block partial dot products
+    DBuf<double> scal;                        // device scalars of the Krylov loop
+
+    // particles: two SoA buffers (flip.jl:131-151), `cur` is the live one
+    DBuf<float> pbuf[2][PFIELDS];
+    int cur = 0;
+    int64_t n = 0, capacity = 0;
+
+    // resampling (flip.jl:216-219)
+    DBuf<int32_t> key, rank, seg;
+    DBuf<int32_t> counts, totals, offsets, cstart;
+    DBuf<unsigned long long> bsums;
+
+    size_t cells() const { return (size_t)g.n[0] * g.n[1] * g.n[2]; }
+    size_t faces(int a) const { return (size_t)(g.n[0] + (a == 0)) * (g.n[1] + (a == 1)) * (g.n[2] + (a == 2)); }
+    PView view(int which) const {
+        PView v;
+        for (int f = 0; f < PFIELDS; ++f) v.f[f] = pbuf[which][f].p;
+        return v;
+    }
+
+    DenseBasin(const DGrid& grid, int levels, int64_t capacity, int32_t band, int32_t nmin);
+
+    // phases
+    int64_t resample(bool seeding, uint32_t seed);          // flip.jl:1364-1382
+    void p2g(float dt);                                      // flip.jl:587-606
+    void g2p(float dt, float gravity);                       // flip.jl:1246-1252
+    void project(float dt, float gamma, const SolveOpts& o, SolveStats& s);   // flip.jl:1144-1156
+    // support phases for initialisation and a whole step
+    void classify();                                         // flip.jl:674-677
+    void solidify();                                         // flip.jl:800-807
+    double volume();                                         // flip.jl:293
+    void init_vortex(const float* eta, int neta, float rmax, float Gamma, float core);   // flip.jl:298-314
+    void gravity(float dt, float g);                         // the gravity part of forces!, flip.jl:765-775
+};
+
+// scan helper shared by the dense and sparse resampling
+void launch_scan_bsums(unsigned long long* bsums, int nblocks, hipStream_t s);
+
+}  // namespace mael

@@ -1,0 +1,200 @@
+// Resampling of the narrow band on the dense grid (flip.jl:1254-1382): bin each
+// particle to its cell, count, scan per-cell (count, total) pairs in one pass, give
+// every particle a stable rank in its cell, scatter the survivors in cell order and
+// spawn new particles where a band cell has too few.
+//
+// Determinism: the reference ranks particles by atomic arrival (flip.jl:1282), so its
+// in-cell order and the survivors of the 2*nmin cap depend on scheduling. Here the
+// atomic only sizes the cell; the rank is the particle's position among the cell's
+// particles in index order (a counting sort whose segments are then put in index
+// order), so keys, ranks, counts, totals, offsets and the resampled arrays are the
+// same on every run.
+#include "dense.hpp"
+#include "scan.hpp"
+
+namespace mael {
+
+// ─── per-cell counts (flip.jl:1289-1297) ─────────────────────────────────────
+
+__device__ inline void cellcounts(int32_t count, float phic, bool solid, int32_t nmin, float deep, float dx, bool seeding,
+                                  int32_t& kept, int32_t& add) {
+    kept = min(count, 2 * nmin);
+    bool inband = !solid && -deep <= phic && phic <= (seeding ? 0.5f * dx : -dx);
+    add = (inband && kept < nmin) ? nmin - kept : 0;
+}
+
+struct DenseCells {   // what the scan needs to know about dense cell l
+    const int32_t* counts;
+    const float* phi;
+    DGrid g;
+    int32_t nmin;
+    float deep;
+    int seeding;
+    __device__ void get(size_t l, int32_t& count, int32_t& total) const {
+        int n1 = g.n[0], n2 = g.n[1];
+        int i = (int)(l % (size_t)n1), j = (int)((l / (size_t)n1) % (size_t)n2), k = (int)(l / ((size_t)n1 * n2));
+        count = counts[l];
+        int32_t kept, add;
+        cellcounts(count, phi[l], issolidcell(g, i, j, k), nmin, deep, g.dx, seeding != 0, kept, add);
+        total = kept + add;
+    }
+};
+
+// ─── binning (flip.jl:1258-1287) ─────────────────────────────────────────────
+
+__global__ void k_dense_bin(PView ps, int64_t n, const float* __restrict__ phi, DGrid g, float deep, float above,
+                            int32_t* __restrict__ key, int32_t* __restrict__ rank, int32_t* __restrict__ counts) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float xp = ps.f[0][p], yp = ps.f[1][p], zp = ps.f[2][p];
+    float c0 = __fdiv_rn(xp - g.lo[0], g.dx), c1 = __fdiv_rn(yp - g.lo[1], g.dx), c2 = __fdiv_rn(zp - g.lo[2], g.dx);
+    bool keep = isfinite(c0) && isfinite(c1) && isfinite(c2);
+    // floor(c) in [0, n) is c in [0, n) for finite c
+    keep = keep && c0 >= 0.0f && c0 < (float)g.n[0] && c1 >= 0.0f && c1 < (float)g.n[1] && c2 >= 0.0f &&
+           c2 < (float)g.n[2] && !issolid(g.radius, g.floor, xp, yp, zp);
+    if (keep) {
+        float f = trilinear(phi, g.n[0], g.n[1], g.n[2], c0 - 0.5f, c1 - 0.5f, c2 - 0.5f);
+        keep = -deep <= f && f < above;
+    }
+    if (keep) {
+        int32_t l = (int32_t)floorf(c0) + g.n[0] * ((int32_t)floorf(c1) + g.n[1] * (int32_t)floorf(c2));
+        key[p] = l;
+        rank[p] = atomicAdd(&counts[l], 1);   // arrival: only sizes the segment
+    } else {
+        key[p] = -1;
+        rank[p] = -1;
+    }
+}
+
+// Each kept particle into its cell's segment, at its arrival slot.
+__global__ void k_place(int64_t n, const int32_t* __restrict__ key, const int32_t* __restrict__ rank,
+                        const int32_t* __restrict__ cstart, int32_t* __restrict__ seg) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int32_t l = key[p];
+    if (l >= 0) seg[cstart[l] + rank[p]] = (int32_t)p;
+}
+
+// Each cell's segment into index order (insertion sort: segments are short and
+// nearly sorted, the previous resample wrote particles cell by cell); the rank is the
+// position there.
+__global__ void k_fixrank(size_t C, const int32_t* __restrict__ counts, const int32_t* __restrict__ cstart,
+                          int32_t* __restrict__ seg, int32_t* __restrict__ rank) {
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= C) return;
+    int32_t c = counts[l];
+    if (c < 2) return;
+    int32_t* s = seg + cstart[l];
+    for (int32_t a = 1; a < c; ++a) {
+        int32_t v = s[a];
+        int32_t b = a - 1;
+        while (b >= 0 && s[b] > v) { s[b + 1] = s[b]; --b; }
+        s[b + 1] = v;
+    }
+    for (int32_t a = 0; a < c; ++a) rank[s[a]] = a;
+}
+
+// flip.jl:1307-1319 scatter_kernel!: survivors to their place in cell order.
+__global__ void k_scatter(PView src, PView dst, int64_t n, const int32_t* __restrict__ key,
+                          const int32_t* __restrict__ rank, const int32_t* __restrict__ totals,
+                          const int32_t* __restrict__ offsets, int32_t nmax) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int32_t l = key[p];
+    int32_t r = rank[p];
+    if (l < 0 || r >= nmax) return;
+    int64_t d = (int64_t)offsets[l] - totals[l] + r;
+#pragma unroll
+    for (int f = 0; f < PFIELDS; ++f) dst.f[f][d] = src.f[f][p];
+}
+
+// flip.jl:1321-1354 spawn_kernel!: stratified, jittered new particles carrying the grid's
+// APIC velocity; when seeding, those the level set puts in the air get NaN positions.
+__global__ void k_dense_spawn(PView dst, const int32_t* __restrict__ counts, const int32_t* __restrict__ totals,
+                              const int32_t* __restrict__ offsets, const float* __restrict__ phi,
+                              const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                              DGrid g, int32_t nmin, float deep, int seeding, uint32_t seed) {
+    size_t C = (size_t)g.n[0] * g.n[1] * g.n[2];
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= C) return;
+    int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
+    int i = (int)(l % (size_t)n1), j = (int)((l / (size_t)n1) % (size_t)n2), k = (int)(l / ((size_t)n1 * n2));
+    int32_t kept, add;
+    cellcounts(counts[l], phi[l], issolidcell(g, i, j, k), nmin, deep, g.dx, seeding != 0, kept, add);
+    if (add <= 0) return;
+    int64_t base = (int64_t)offsets[l] - totals[l] + kept;
+    float cx = g.lo[0] + (float)i * g.dx, cy = g.lo[1] + (float)j * g.dx, cz = g.lo[2] + (float)k * g.dx;
+    uint32_t l1 = (uint32_t)(l + 1);   // the reference's 1-based linear index
+    int32_t s = (int32_t)seed;
+    float half = 0.5f * g.dx;
+    for (int32_t m = 0; m < add; ++m) {
+        int32_t h = (int32_t)(l1 * 31u + (uint32_t)m);
+        float jx = hash01(h, s + 1), jy = hash01(h, s + 2), jz = hash01(h, s + 3);
+        int32_t o = m % 8;
+        float xp = cx + ((float)(o & 1) + jx) * half;
+        float yp = cy + ((float)((o >> 1) & 1) + jy) * half;
+        float zp = cz + ((float)((o >> 2) & 1) + jz) * half;
+        int64_t d = base + m;
+        dst.f[15][d] = 0.0f;
+        bool air = false;
+        if (seeding)
+            air = trilinear(phi, n1, n2, n3, ispace(xp, g.lo[0], g.dx, 0.5f), ispace(yp, g.lo[1], g.dx, 0.5f),
+                            ispace(zp, g.lo[2], g.dx, 0.5f)) >= 0.0f;
+        if (air) {
+            float nan = __builtin_nanf("");
+            dst.f[0][d] = nan; dst.f[1][d] = nan; dst.f[2][d] = nan;
+#pragma unroll
+            for (int f = 3; f < 15; ++f) dst.f[f][d] = 0.0f;
+        } else {
+            dst.f[0][d] = xp; dst.f[1][d] = yp; dst.f[2][d] = zp;
+            float vel, a0, a1, a2;
+            gather(u, n1 + 1, n2, n3, g, 0, xp, yp, zp, vel, a0, a1, a2);
+            dst.f[3][d] = vel; dst.f[6][d] = a0; dst.f[7][d] = a1; dst.f[8][d] = a2;
+            gather(v, n1, n2 + 1, n3, g, 1, xp, yp, zp, vel, a0, a1, a2);
+            dst.f[4][d] = vel; dst.f[9][d] = a0; dst.f[10][d] = a1; dst.f[11][d] = a2;
+            gather(w, n1, n2, n3 + 1, g, 2, xp, yp, zp, vel, a0, a1, a2);
+            dst.f[5][d] = vel; dst.f[12][d] = a0; dst.f[13][d] = a1; dst.f[14][d] = a2;
+        }
+    }
+}
+
+int64_t DenseBasin::resample(bool seeding, uint32_t seed) {
+    hipStream_t s = st.s;
+    size_t C = cells();
+    float deep = (float)band * g.dx;
+    fill_async(counts.p, C, 0, s);
+    PView src = view(cur), dst = view(cur ^ 1);
+    if (n > 0) {
+        kt.begin(K_BIN, s);
+        hipLaunchKernelGGL(k_dense_bin, dim3(cdiv(n, 256)), dim3(256), 0, s, src, n, phi.p, g, deep, 3.0f * g.dx, key.p,
+                           rank.p, counts.p);
+        kt.end(K_BIN, s);
+        MAEL_LAUNCH_CHECK();
+    }
+    DenseCells cellsinfo{counts.p, phi.p, g, nmin, deep, seeding ? 1 : 0};
+    int64_t total = scan_cells(cellsinfo, C, totals.p, offsets.p, cstart.p, bsums.p, (unsigned long long*)st.pinned, s);
+    if (total > capacity)
+        throw Error(E_CAPACITY, "the narrow band needs " + std::to_string(total) + " particles, more than the " +
+                                    std::to_string(capacity) + " allocated");
+    if (n > 0) {
+        hipLaunchKernelGGL(k_place, dim3(cdiv(n, 256)), dim3(256), 0, s, n, key.p, rank.p, cstart.p, seg.p);
+        kt.begin(K_FIXRANK, s);
+        hipLaunchKernelGGL(k_fixrank, dim3(cdiv(C, 256)), dim3(256), 0, s, C, counts.p, cstart.p, seg.p, rank.p);
+        kt.end(K_FIXRANK, s);
+        kt.begin(K_SCATTER, s);
+        hipLaunchKernelGGL(k_scatter, dim3(cdiv(n, 256)), dim3(256), 0, s, src, dst, n, key.p, rank.p, totals.p,
+                           offsets.p, 2 * nmin);
+        kt.end(K_SCATTER, s);
+        MAEL_LAUNCH_CHECK();
+    }
+    kt.begin(K_SPAWN, s);
+    hipLaunchKernelGGL(k_dense_spawn, dim3(cdiv(C, 256)), dim3(256), 0, s, dst, counts.p, totals.p, offsets.p, phi.p,
+                       u[0].p, u[1].p, u[2].p, g, nmin, deep, seeding ? 1 : 0, seed);
+    kt.end(K_SPAWN, s);
+    MAEL_LAUNCH_CHECK();
+    cur ^= 1;
+    n = total;
+    return total;
+}
+
+}  // namespace mael

@@ -1,0 +1,135 @@
+"""ctypes binding of the HIP library's C ABI (include/maelstrom_hip.h).
+
+The library is loaded from `maelstrom.jl_amd/lib/libmaelstrom_hip.so`. There is no
+fallback: if it is missing or cannot open a GPU, the calls raise.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(os.path.dirname(HERE), "lib", "libmaelstrom_hip.so")
+
+# status codes (include/maelstrom_hip.h)
+OK, E_ARG, E_CAPACITY, E_BLOCKS, E_NONFINITE, E_CG_INDEFINITE, E_NOCONVERGE, E_BREAKDOWN, E_HIP, E_GRID = (
+    0, -1, -2, -3, -4, -5, -6, -7, -8, -9)
+
+# array ids (include/maelstrom_hip.h)
+ARRAYS = ["U", "V", "W", "U0", "V0", "W0", "PHI", "PHI0", "TYPES", "P", "FU", "FV", "FW", "RHS",
+          "PX", "PY", "PZ", "VX", "VY", "VZ", "C0", "C1", "C2", "C3", "C4", "C5", "C6", "C7", "C8", "LAG",
+          "KEY", "RANK", "COUNTS", "TOTALS", "OFFSETS",
+          "MOM_U", "MOM_V", "MOM_W", "WT_U", "WT_V", "WT_W"]
+ARRAY_ID = {n: i for i, n in enumerate(ARRAYS)}
+PARTICLE_FIELDS = ARRAYS[ARRAY_ID["PX"]:ARRAY_ID["LAG"] + 1]
+
+# kernel ids of the launch timer (include/maelstrom_hip.h MAEL_K_*)
+KERNELS = ["p2g", "splat", "combine", "g2p", "bin", "fixrank", "scatter", "spawn", "search", "advance", "rbgs",
+           "residual", "restrict", "prolong", "coarsest", "resid64"]
+KERNEL_ID = {n: i for i, n in enumerate(KERNELS)}
+
+
+class MaelstromError(RuntimeError):
+    """A failed call: `code` is the status, the message the library's (the reference's text where it has one)."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class DenseGrid(C.Structure):
+    _fields_ = [("lo", C.c_float * 3), ("dx", C.c_float), ("n", C.c_int32 * 3), ("radius", C.c_float),
+                ("floor", C.c_float), ("levels", C.c_int32)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("maxiter", C.c_int32), ("sweeps", C.c_int32), ("coarse", C.c_int32),
+                ("precision", C.c_int32), ("refine", C.c_int32), ("warm", C.c_int32)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("restarts", C.c_int32), ("rel_residual", C.c_double),
+                ("rel_residual_rec", C.c_double), ("ms", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library (loaded on first use; raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise MaelstromError(E_ARG, "the HIP library is not built: %s (run __graft_entry__.build())" % LIBPATH)
+    L = C.CDLL(LIBPATH)
+    p, i32, i64, f32, f64, u32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_uint32
+    sig = {
+        "mael_create": (C.c_int, [C.c_int, C.POINTER(p)]),
+        "mael_destroy": (None, [p]),
+        "mael_last_error": (C.c_char_p, [p]),
+        "mael_version": (C.c_char_p, []),
+        "mael_flipgrid": (C.c_int, [f32, f32, f32, f32, i32, C.POINTER(DenseGrid)]),
+        "mael_solve_defaults": (None, [C.POINTER(SolveOpts)]),
+        "mael_dense_create": (C.c_int, [p, C.POINTER(DenseGrid), i64, i32, i32, C.POINTER(p)]),
+        "mael_dense_destroy": (None, [p]),
+        "mael_dense_array": (C.c_int, [p, i32, C.POINTER(p), C.POINTER(i64), C.POINTER(i32)]),
+        "mael_dense_upload": (C.c_int, [p, i32, p, i64]),
+        "mael_dense_download": (C.c_int, [p, i32, p, i64]),
+        "mael_dense_count": (C.c_int, [p, C.POINTER(i64)]),
+        "mael_dense_set_count": (C.c_int, [p, i64]),
+        "mael_dense_resample": (C.c_int, [p, i32, u32, C.POINTER(i64)]),
+        "mael_dense_p2g": (C.c_int, [p, f32]),
+        "mael_dense_g2p": (C.c_int, [p, f32, f32]),
+        "mael_dense_project": (C.c_int, [p, f32, f32, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+        "mael_dense_classify": (C.c_int, [p]),
+        "mael_dense_volume": (C.c_int, [p, C.POINTER(f64)]),
+        "mael_dense_init_vortex": (C.c_int, [p, p, i32, f32, f32, f32]),
+        "mael_dense_solidify": (C.c_int, [p]),
+        "mael_dense_gravity": (C.c_int, [p, f32, f32]),
+        "mael_dense_copy": (C.c_int, [p, i32, i32]),
+        "mael_dense_ktimer": (C.c_int, [p, i32, i32]),
+        "mael_dense_ktime": (C.c_int, [p, C.POINTER(f64), C.POINTER(i32)]),
+        "mael_dense_sync": (C.c_int, [p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+class Context:
+    """A device context (`mael_create`): the backend basins are built on."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        st = lib().mael_create(device, C.byref(self.h))
+        if st != OK:
+            raise MaelstromError(st, "cannot open HIP device %d (status %d)" % (device, st))
+
+    def check(self, status):
+        if status != OK:
+            raise MaelstromError(status, lib().mael_last_error(self.h).decode())
+        return status
+
+    def close(self):
+        if self.h:
+            lib().mael_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default = None
+
+
+def default_context():
+    """The process's context on device 0 (`Mantle.defaultbackend()`)."""
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default

@@ -1,0 +1,70 @@
+"""The simulated basin (basin.jl:13-30) and the balanced vortex surface (sea.jl:38-58).
+
+Only what the hot path's inputs need: the basin's geometry and vortex, and the free
+surface in balance with it, which `initialize` turns into the level set.
+"""
+from dataclasses import dataclass, replace
+
+import numpy as np
+
+
+@dataclass(frozen=True)
+class Basin:
+    """basin.jl:13-30 `Basin`: field units as there (m, m^2/s, s)."""
+    radius: float = 14.0
+    depth: float = 6.0
+    spacing: float = 0.3
+    Γ: float = 150.0
+    core: float = 2.5
+    drain: float = 1.2
+    band: float = 3.0
+    relax: float = 0.3
+    shortest: float = 3.0
+    headroom: float = 6.0
+    g: float = 9.81
+    ρ0: float = 1000.0
+    sound: float = 100.0
+    layers: int = 4
+    rimwall: float = 3.0
+    arc: float = 1.5
+
+
+def narutobasin(**kw):
+    """basin.jl:47-49 `narutobasin`: a Naruto-sized basin (note rimwall 1.5)."""
+    d = dict(radius=10.0, depth=3.0, spacing=0.1, Γ=45.0, core=1.5, drain=0.0, band=2.5, shortest=1.0,
+             headroom=3.0, sound=60.0, rimwall=1.5)
+    d.update(kw)
+    return Basin(**d)
+
+
+def oseen(G, core, r):
+    """sea.jl:39: azimuthal speed of the Lamb-Oseen vortex, FP32."""
+    r = np.asarray(r, dtype=np.float32)
+    G, core = np.float32(G), np.float32(core)
+    q = r / core
+    with np.errstate(over="ignore"):
+        return G / (np.float32(2.0) * np.float32(np.pi) * np.maximum(r, np.float32(1e-4))) * (
+            np.float32(1.0) - np.exp(-(q * q)))
+
+
+def balanced(G, core, g, r, steps=2000):
+    """sea.jl:48-58: the free surface over the vortex in balance at radius r (FP32, trapezoid in log r)."""
+    r = np.asarray(r, dtype=np.float32)
+    f32 = np.float32
+    a = np.log(np.maximum(r, f32(1e-3)))
+    z = np.log(f32(50.0) * f32(core) + f32(50.0) * np.maximum(r, f32(1.0)))
+    h = (z - a) / f32(steps)
+    s = np.zeros_like(r)
+    for i in range(steps + 1):
+        rho = np.exp(a + f32(i) * h)
+        o = oseen(G, core, rho)
+        s += (f32(0.5) if i in (0, steps) else f32(1.0)) * (o * o) / f32(g)
+    return -s * h
+
+
+def surface(b, r):
+    """basin.jl:36 `surface`: the basin's balanced free surface at radius r."""
+    return balanced(b.Γ, b.core, b.g, r)
+
+
+__all__ = ["Basin", "narutobasin", "oseen", "balanced", "surface", "replace"]

@@ -1,0 +1,244 @@
+"""`FlipBasin`: the dense narrow-band FLIP basin on the HIP library (flip.jl:156-1410).
+
+The Python mirror of the reference's Julia API for the hot path: `FlipBasin`,
+`initialize`, `resample`, `particles2grid`, `project`, `grid2particles`, `step`
+(Julia's bang names without the `!`). Every phase runs in the gfx950 library; this
+module only holds host-side bookkeeping (time, step count, the volume control's
+reference volume) exactly where the reference keeps it on the host.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import ARRAY_ID, PARTICLE_FIELDS, DenseGrid, MaelstromError, SolveOpts, SolveStats
+from .basin import surface
+
+_DTYPES = {1: np.uint8, 4: np.float32, 8: np.int64}
+_INT_ARRAYS = {"KEY", "RANK", "COUNTS", "TOTALS", "OFFSETS"}
+
+
+def flipgrid(basin, dx, top=None, pad=2):
+    """flip.jl:48-62 `FlipGrid(b; dx, top = b.rimwall, pad = 2)`."""
+    g = DenseGrid()
+    top = basin.rimwall if top is None else top
+    st = _lib.lib().mael_flipgrid(basin.radius, basin.depth, top, dx, pad, C.byref(g))
+    if st != _lib.OK:
+        raise MaelstromError(st, "bad grid for dx=%r" % dx)
+    return g
+
+
+class FlipBasin:
+    """flip.jl:189 `FlipBasin(b, backend; dx, stir, band = 4, nmin = 8, cfl = 2, capacity = 12_000_000)`.
+
+    The stirring model is not on this path (only its gravity term is applied by `step`).
+    """
+
+    def __init__(self, basin, dx, band=4, nmin=8, cfl=2.0, capacity=12_000_000, ctx=None, top=None, pad=2):
+        self.basin = basin
+        self.ctx = ctx or _lib.default_context()
+        self.grid = flipgrid(basin, dx, top, pad)
+        self.band, self.nmin, self.cfl, self.capacity = int(band), int(nmin), float(cfl), int(capacity)
+        self.h = C.c_void_p()
+        self.ctx.check(_lib.lib().mael_dense_create(self.ctx.h, C.byref(self.grid), self.capacity, self.band,
+                                                    self.nmin, C.byref(self.h)))
+        self.volume0 = 0.0
+        self.t = 0.0
+        self.steps = 0
+        self.iterations = []
+        self.last_solve = None
+
+    # ── arrays ─────────────────────────────────────────────────────────────
+    @property
+    def n_cells(self):
+        return tuple(int(x) for x in self.grid.n)
+
+    def shape(self, name):
+        """numpy shape (z, y, x) of a grid array, or (capacity,) of a particle array."""
+        n1, n2, n3 = self.n_cells
+        if name in ("U", "U0", "FU", "MOM_U", "WT_U"):
+            return (n3, n2, n1 + 1)
+        if name in ("V", "V0", "FV", "MOM_V", "WT_V"):
+            return (n3, n2 + 1, n1)
+        if name in ("W", "W0", "FW", "MOM_W", "WT_W"):
+            return (n3 + 1, n2, n1)
+        if name in ("PHI", "PHI0", "TYPES", "P", "RHS", "COUNTS", "TOTALS", "OFFSETS"):
+            return (n3, n2, n1)
+        return (self.capacity,)
+
+    def _info(self, name):
+        ptr, cnt, es = C.c_void_p(), C.c_int64(), C.c_int32()
+        self.ctx.check(_lib.lib().mael_dense_array(self.h, ARRAY_ID[name], C.byref(ptr), C.byref(cnt), C.byref(es)))
+        dt = np.int32 if name in _INT_ARRAYS else _DTYPES[es.value]
+        return ptr.value, cnt.value, dt
+
+    def get(self, name, count=None):
+        """Download array `name` (`Array(s.x)`); particle arrays are cut to `count` (default n)."""
+        _, cnt, dt = self._info(name)
+        if name in PARTICLE_FIELDS or name in ("KEY", "RANK"):
+            cnt = self.n if count is None else count
+            out = np.empty(cnt, dtype=dt)
+        else:
+            out = np.empty(self.shape(name), dtype=dt)
+            assert out.size == cnt
+        if out.size:
+            self.ctx.check(_lib.lib().mael_dense_download(self.h, ARRAY_ID[name], out.ctypes.data, out.size))
+        return out
+
+    def set(self, name, arr):
+        """Upload into array `name` (`copyto!(s.x, arr)`)."""
+        _, cnt, dt = self._info(name)
+        a = np.ascontiguousarray(arr, dtype=dt).ravel()
+        if a.size > cnt:
+            raise ValueError("%s holds %d elements, got %d" % (name, cnt, a.size))
+        if a.size:
+            self.ctx.check(_lib.lib().mael_dense_upload(self.h, ARRAY_ID[name], a.ctypes.data, a.size))
+
+    def copy(self, dst, src):
+        self.ctx.check(_lib.lib().mael_dense_copy(self.h, ARRAY_ID[dst], ARRAY_ID[src]))
+
+    @property
+    def n(self):
+        v = C.c_int64()
+        self.ctx.check(_lib.lib().mael_dense_count(self.h, C.byref(v)))
+        return v.value
+
+    @n.setter
+    def n(self, value):
+        self.ctx.check(_lib.lib().mael_dense_set_count(self.h, int(value)))
+
+    def particles(self):
+        """The live particles as a dict of arrays (x, y, z, vx, vy, vz, c0..c8, lag)."""
+        return {f: self.get(f) for f in PARTICLE_FIELDS}
+
+    def set_particles(self, ps, n=None):
+        n = len(ps["PX"]) if n is None else n
+        for f in PARTICLE_FIELDS:
+            self.set(f, np.asarray(ps[f][:n], dtype=np.float32))
+        self.n = n
+
+    def sync(self):
+        self.ctx.check(_lib.lib().mael_dense_sync(self.h))
+
+    def close(self):
+        if self.h:
+            _lib.lib().mael_dense_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ── phases ───────────────────────────────────────────────────────────────────
+
+def resample(s, seeding=False):
+    """flip.jl:1364-1382 `resample!(s; seeding)`: returns the new particle count."""
+    total = C.c_int64()
+    s.ctx.check(_lib.lib().mael_dense_resample(s.h, 1 if seeding else 0, (s.steps + 1) & 0xffffffff, C.byref(total)))
+    return total.value
+
+
+def particles2grid(s, dt):
+    """flip.jl:587-606 `particles2grid!(s, dt)`."""
+    s.ctx.check(_lib.lib().mael_dense_p2g(s.h, float(dt)))
+    return s
+
+
+def grid2particles(s, dt):
+    """flip.jl:1246-1252 `grid2particles!(s, dt)`."""
+    s.ctx.check(_lib.lib().mael_dense_g2p(s.h, float(dt), float(s.basin.g)))
+    return s
+
+
+def solve_options(tol=1e-4, maxiter=60, sweeps=2, coarse=40, precision=0, refine=1, warm=1):
+    """`pressure!`'s options (flip.jl:1082 defaults) plus precision, FP64 refinement and warm start."""
+    return SolveOpts(float(tol), int(maxiter), int(sweeps), int(coarse), int(precision), int(refine), int(warm))
+
+
+def volume(s):
+    """flip.jl:293 `volume(s)`, m^3."""
+    v = C.c_double()
+    s.ctx.check(_lib.lib().mael_dense_volume(s.h, C.byref(v)))
+    return v.value
+
+
+def gamma_for(s, dt, recover=0.5):
+    """flip.jl:1146-1147: the volume control's divergence γ."""
+    loss = (s.volume0 - volume(s)) / s.volume0 if s.volume0 else 0.0
+    return float(np.float32(min(max(loss / recover, -0.01 / dt), 0.01 / dt)))
+
+
+def project(s, dt, recover=0.5, gamma=None, **solve):
+    """flip.jl:1144-1156 `project!(s, dt; recover)`; `solve` are `pressure!`'s options
+    (tol, maxiter, sweeps, coarse) plus precision and refine. Returns the solve's stats."""
+    g = gamma_for(s, dt, recover) if gamma is None else float(gamma)
+    opts = solve_options(**solve)
+    st = SolveStats()
+    s.ctx.check(_lib.lib().mael_dense_project(s.h, float(dt), g, C.byref(opts), C.byref(st)))
+    s.iterations.append(st.iterations)
+    s.last_solve = st
+    return st
+
+
+def classify(s):
+    s.ctx.check(_lib.lib().mael_dense_classify(s.h))
+    return s
+
+
+def initialize(s):
+    """flip.jl:325-348 `initialize!(s)`: the balanced vortex, the band seeded with
+    particles, their deposit. Redistancing (not on this path) is left out: the level
+    set is the height above the balanced surface, a distance where the surface is flat."""
+    g = s.grid
+    rmax = np.float32(math.hypot(g.lo[0], g.lo[1]))
+    r = np.linspace(np.float32(0), rmax, 2048, dtype=np.float32)
+    eta = np.ascontiguousarray(surface(s.basin, r), dtype=np.float32)
+    L = _lib.lib()
+    s.ctx.check(L.mael_dense_init_vortex(s.h, eta.ctypes.data, eta.size, float(rmax), float(s.basin.Γ),
+                                         float(s.basin.core)))
+    classify(s)
+    s.ctx.check(L.mael_dense_solidify(s.h))
+    for f in ("FU", "FV", "FW"):
+        s.set(f, np.ones(s.shape(f), dtype=np.uint8))
+    s.n = 0
+    resample(s, seeding=True)
+    resample(s)
+    s.copy("PHI0", "PHI")
+    for a, b in (("U0", "U"), ("V0", "V"), ("W0", "W")):
+        s.copy(a, b)
+    particles2grid(s, 1.0)
+    classify(s)
+    s.volume0 = volume(s)
+    s.t = 0.0
+    return s
+
+
+def maxspeed(s):
+    """flip.jl:1387 `maxspeed(s)`."""
+    return float(max(np.abs(s.get(f)).max() for f in ("U", "V", "W")))
+
+
+def step(s, dt, **solve):
+    """One step of the hot path in the reference's order (flip.jl:1396-1410): the grid
+    holds still in place of advection (φ0, u0 take φ, u), then particles to grid,
+    classify, gravity, projection, grid to particles, resample. Redistancing, trapped
+    air, the stir and extrapolation are not on this path and are left out."""
+    s.copy("PHI0", "PHI")
+    for a, b in (("U0", "U"), ("V0", "V"), ("W0", "W")):
+        s.copy(a, b)
+    particles2grid(s, dt)
+    classify(s)
+    s.ctx.check(_lib.lib().mael_dense_gravity(s.h, float(dt), float(s.basin.g)))
+    st = project(s, dt, **solve)
+    grid2particles(s, dt)
+    s.steps += 1
+    resample(s)
+    s.t += dt
+    return st
+
+
+advance = step

@@ -1,0 +1,345 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Only tests/, the smoke check and bench.py's CPU-baseline leg use this module, as the
+checker. It builds the oracle with `make -C oracle` on first use if needed.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liboracle.so")
+
+PFIELDS = ["PX", "PY", "PZ", "VX", "VY", "VZ", "C0", "C1", "C2", "C3", "C4", "C5", "C6", "C7", "C8", "LAG"]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIBPATH
+
+
+class Grid(C.Structure):
+    _fields_ = [("lo", C.c_float * 3), ("dx", C.c_float), ("n", C.c_int32 * 3), ("radius", C.c_float),
+                ("floor", C.c_float), ("levels", C.c_int32)]
+
+    @property
+    def dims(self):
+        return tuple(int(x) for x in self.n)
+
+
+class CParticles(C.Structure):
+    _fields_ = [("x", C.c_void_p * 3), ("v", C.c_void_p * 3), ("c", C.c_void_p * 9), ("lag", C.c_void_p),
+                ("n", C.c_int64), ("capacity", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            build()
+        L = C.CDLL(LIBPATH)
+        p, i32, i64, f32, f64, u32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_uint32
+        G, PP = C.POINTER(Grid), C.POINTER(CParticles)
+        sig = {
+            "orc_last_error": (C.c_char_p, []),
+            "orc_flipgrid": (C.c_int, [f32, f32, f32, f32, i32, G]),
+            "orc_hash01": (f32, [i32, i32]),
+            "orc_hash01_many": (None, [p, p, p, i64]),
+            "orc_bspline": (None, [f32, C.POINTER(i32), p]),
+            "orc_orderkey": (i32, [f32]),
+            "orc_orderfloat": (f32, [i32]),
+            "orc_temporal": (f32, [f32, f32]),
+            "orc_strength": (f32, [f32, f32, f32]),
+            "orc_ahead": (f32, [f32]),
+            "orc_oseen": (f32, [f32, f32, f32]),
+            "orc_balanced": (f32, [f32, f32, f32, f32]),
+            "orc_theta64": (f64, [f64, f64]),
+            "orc_trilinear": (f32, [p, i32, i32, i32, f32, f32, f32]),
+            "orc_gather": (None, [p, G, i32, f32, f32, f32, p, p]),
+            "orc_dense_resample": (C.c_int, [G, PP, PP, p, p, p, p, i32, i32, i32, u32, p, p, p, p, p, C.POINTER(i64)]),
+            "orc_dense_p2g": (C.c_int, [G, PP, f32, i32, i32] + [p] * 14),
+            "orc_dense_p2g_f32": (C.c_int, [G, PP, f32, i32, i32, p, p, p, p, p, p, p, p]),
+            "orc_dense_g2p": (C.c_int, [G, PP, p, p, p, p, p, p, f32, f32]),
+            "orc_dense_classify": (None, [G, p, p]),
+            "orc_dense_volume": (f64, [G, p]),
+            "orc_dense_rhs": (None, [G, p, p, p, p, f32, p]),
+            "orc_dense_apply": (None, [G, p, p, p, p]),
+            "orc_dense_vcycle": (None, [G, p, p, p, p, i32, i32]),
+            "orc_dense_matrix": (i64, [G, p, p, p, p, p]),
+            "orc_dense_pressure": (i32, [G, p, p, p, p, f64, i32, i32, i32, C.POINTER(f64)]),
+            "orc_dense_pressure_f32": (i32, [G, p, p, p, p, f64, i32, i32, i32, C.POINTER(f64)]),
+            "orc_dense_project_faces": (None, [G, p, p, p, p, p, p, p, p, p]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ── grid ─────────────────────────────────────────────────────────────────────
+
+def flipgrid(radius, depth, top, dx, pad=2):
+    g = Grid()
+    lib().orc_flipgrid(radius, depth, top, dx, pad, C.byref(g))
+    return g
+
+
+def cells_shape(g):
+    n1, n2, n3 = g.dims
+    return (n3, n2, n1)
+
+
+def face_shape(g, a):
+    n1, n2, n3 = g.dims
+    return (n3 + (a == 2), n2 + (a == 1), n1 + (a == 0))
+
+
+# ── particles ────────────────────────────────────────────────────────────────
+
+def empty_particles(capacity):
+    return {f: np.zeros(capacity, dtype=np.float32) for f in PFIELDS}
+
+
+def _cparts(ps, n, capacity):
+    arrs = {f: np.array(ps[f], dtype=np.float32, copy=True) for f in PFIELDS}   # never alias the caller's
+    for f in PFIELDS:
+        if arrs[f].size < capacity:
+            a = np.zeros(capacity, np.float32)
+            a[:arrs[f].size] = arrs[f]
+            arrs[f] = a
+    cp = CParticles()
+    for a in range(3):
+        cp.x[a] = arrs[PFIELDS[a]].ctypes.data
+        cp.v[a] = arrs[PFIELDS[3 + a]].ctypes.data
+    for a in range(9):
+        cp.c[a] = arrs[PFIELDS[6 + a]].ctypes.data
+    cp.lag = arrs["LAG"].ctypes.data
+    cp.n = n
+    cp.capacity = capacity
+    return cp, arrs
+
+
+# ── scalar helpers ───────────────────────────────────────────────────────────
+
+def hash01(i, j):
+    i = _c(np.atleast_1d(i), np.int32)
+    j = _c(np.atleast_1d(j), np.int32)
+    i, j = np.broadcast_arrays(i, j)
+    i, j = _c(i, np.int32), _c(j, np.int32)
+    out = np.empty(i.shape, np.float32)
+    lib().orc_hash01_many(i.ctypes.data, j.ctypes.data, out.ctypes.data, i.size)
+    return out
+
+
+def bspline(q):
+    b = C.c_int32()
+    w = np.zeros(3, np.float32)
+    lib().orc_bspline(q, C.byref(b), w.ctypes.data)
+    return b.value, w
+
+
+def trilinear(A, q):
+    A = _c(A, np.float32)
+    n3, n2, n1 = A.shape
+    return lib().orc_trilinear(A.ctypes.data, n1, n2, n3, q[0], q[1], q[2])
+
+
+def gather(u, g, a, p):
+    u = _c(u, np.float32)
+    vel = np.zeros(1, np.float32)
+    aff = np.zeros(3, np.float32)
+    lib().orc_gather(u.ctypes.data, C.byref(g), a, p[0], p[1], p[2], vel.ctypes.data, aff.ctypes.data)
+    return float(vel[0]), aff
+
+
+# ── the dense phases ─────────────────────────────────────────────────────────
+
+def dense_resample(g, ps, n, capacity, phi, u, v, w, band=4, nmin=8, seeding=False, seed=1):
+    """flip.jl:1364-1382; returns (particles, n_new, key, rank, counts, totals, offsets)."""
+    src, keep_src = _cparts(ps, n, max(n, 1))
+    dst_arrs = empty_particles(capacity)
+    dst, keep_dst = _cparts(dst_arrs, 0, capacity)
+    C_ = int(np.prod(g.dims))
+    key = np.zeros(max(n, 1), np.int32)
+    rank = np.zeros(max(n, 1), np.int32)
+    counts = np.zeros(C_, np.int32)
+    totals = np.zeros(C_, np.int32)
+    offsets = np.zeros(C_, np.int32)
+    total = C.c_int64()
+    phi, u, v, w = (_c(x, np.float32) for x in (phi, u, v, w))
+    st = lib().orc_dense_resample(C.byref(g), C.byref(src), C.byref(dst), phi.ctypes.data, u.ctypes.data,
+                                  v.ctypes.data, w.ctypes.data, band, nmin, 1 if seeding else 0, seed & 0xffffffff,
+                                  key.ctypes.data, rank.ctypes.data, counts.ctypes.data, totals.ctypes.data,
+                                  offsets.ctypes.data, C.byref(total))
+    if st != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    nn = total.value
+    out = {f: keep_dst[f][:nn].copy() for f in PFIELDS}
+    return out, nn, key[:n], rank[:n], counts, totals, offsets
+
+
+def dense_p2g(g, ps, n, dt, phi0, u0, v0, w0, band=4, nmin=8, fp32=False):
+    """flip.jl:587-606 in FP64 (or FP32 with fp32=True); returns (phi, u, v, w, mom, wt)."""
+    cp, keep = _cparts(ps, n, max(n, 1))
+    phi0, u0, v0, w0 = (_c(x, np.float32) for x in (phi0, u0, v0, w0))
+    phi = np.zeros(cells_shape(g), np.float32)
+    U = [np.zeros(face_shape(g, a), np.float32) for a in range(3)]
+    if fp32:
+        lib().orc_dense_p2g_f32(C.byref(g), C.byref(cp), dt, band, nmin, phi0.ctypes.data, u0.ctypes.data,
+                                v0.ctypes.data, w0.ctypes.data, phi.ctypes.data, *(x.ctypes.data for x in U))
+        return phi, U[0], U[1], U[2], None, None
+    M = [np.zeros(face_shape(g, a), np.float64) for a in range(3)]
+    W = [np.zeros(face_shape(g, a), np.float64) for a in range(3)]
+    lib().orc_dense_p2g(C.byref(g), C.byref(cp), dt, band, nmin, phi0.ctypes.data, u0.ctypes.data, v0.ctypes.data,
+                        w0.ctypes.data, phi.ctypes.data, *(x.ctypes.data for x in U), *(x.ctypes.data for x in M),
+                        *(x.ctypes.data for x in W))
+    return phi, U[0], U[1], U[2], M, W
+
+
+def dense_g2p(g, ps, n, u, v, w, fu, fv, fw, dt, gravity=9.81):
+    """flip.jl:1246-1252; returns the updated particle arrays."""
+    cp, keep = _cparts(ps, n, max(n, 1))
+    u, v, w = (_c(x, np.float32) for x in (u, v, w))
+    fu, fv, fw = (_c(x, np.uint8) for x in (fu, fv, fw))
+    lib().orc_dense_g2p(C.byref(g), C.byref(cp), u.ctypes.data, v.ctypes.data, w.ctypes.data, fu.ctypes.data,
+                        fv.ctypes.data, fw.ctypes.data, dt, gravity)
+    return {f: keep[f][:n].copy() for f in PFIELDS}
+
+
+def dense_classify(g, phi):
+    phi = _c(phi, np.float32)
+    t = np.zeros(cells_shape(g), np.uint8)
+    lib().orc_dense_classify(C.byref(g), phi.ctypes.data, t.ctypes.data)
+    return t
+
+
+def dense_volume(g, phi):
+    phi = _c(phi, np.float32)
+    return lib().orc_dense_volume(C.byref(g), phi.ctypes.data)
+
+
+def dense_rhs(g, u, v, w, types, gamma):
+    u, v, w = (_c(x, np.float32) for x in (u, v, w))
+    types = _c(types, np.uint8)
+    b = np.zeros(cells_shape(g), np.float64)
+    lib().orc_dense_rhs(C.byref(g), u.ctypes.data, v.ctypes.data, w.ctypes.data, types.ctypes.data, gamma,
+                        b.ctypes.data)
+    return b
+
+
+def dense_apply(g, types, phi, x):
+    types, phi, x = _c(types, np.uint8), _c(phi, np.float32), _c(x, np.float64)
+    q = np.zeros(cells_shape(g), np.float64)
+    lib().orc_dense_apply(C.byref(g), types.ctypes.data, phi.ctypes.data, x.ctypes.data, q.ctypes.data)
+    return q
+
+
+def dense_vcycle(g, types, phi, r, sweeps=2, coarse=40):
+    types, phi, r = _c(types, np.uint8), _c(phi, np.float32), _c(r, np.float64)
+    z = np.zeros(cells_shape(g), np.float64)
+    lib().orc_dense_vcycle(C.byref(g), types.ctypes.data, phi.ctypes.data, r.ctypes.data, z.ctypes.data, sweeps,
+                           coarse)
+    return z
+
+
+def dense_matrix(g, types, phi):
+    """The fine level's operator as a scipy CSR matrix over all cells."""
+    import scipy.sparse as sp
+    types, phi = _c(types, np.uint8), _c(phi, np.float32)
+    nnz = lib().orc_dense_matrix(C.byref(g), types.ctypes.data, phi.ctypes.data, None, None, None)
+    rows = np.zeros(nnz, np.int64)
+    cols = np.zeros(nnz, np.int64)
+    vals = np.zeros(nnz, np.float64)
+    lib().orc_dense_matrix(C.byref(g), types.ctypes.data, phi.ctypes.data, rows.ctypes.data, cols.ctypes.data,
+                           vals.ctypes.data)
+    N = int(np.prod(g.dims))
+    return sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
+
+
+def dense_pressure(g, types, phi, b, P0=None, tol=1e-12, maxiter=500, sweeps=2, coarse=40, fp32=False):
+    """flip.jl:1075-1111 in FP64 (or FP32); returns (P, iterations, relres)."""
+    types, phi = _c(types, np.uint8), _c(phi, np.float32)
+    dt = np.float32 if fp32 else np.float64
+    b = _c(b, dt)
+    P = np.zeros(cells_shape(g), dt) if P0 is None else _c(P0, dt).copy()
+    rr = C.c_double()
+    fn = lib().orc_dense_pressure_f32 if fp32 else lib().orc_dense_pressure
+    it = fn(C.byref(g), types.ctypes.data, phi.ctypes.data, b.ctypes.data, P.ctypes.data, tol, maxiter, sweeps, coarse,
+            C.byref(rr))
+    return P, it, rr.value
+
+
+def dense_project_faces(g, P, types, phi, u, v, w):
+    P, types, phi = _c(P, np.float64), _c(types, np.uint8), _c(phi, np.float32)
+    U = [_c(x, np.float32).copy() for x in (u, v, w)]
+    F = [np.zeros(face_shape(g, a), np.uint8) for a in range(3)]
+    lib().orc_dense_project_faces(C.byref(g), P.ctypes.data, types.ctypes.data, phi.ctypes.data,
+                                  *(x.ctypes.data for x in U), *(x.ctypes.data for x in F))
+    return U, F
+
+
+# ── a scene for CPU tests (the balanced vortex of initialize!, restated in numpy) ──
+
+def _oseen(G, core, r):
+    r = np.asarray(r, np.float32)
+    q = r / np.float32(core)
+    return np.float32(G) / (np.float32(2) * np.float32(np.pi) * np.maximum(r, np.float32(1e-4))) * (
+        np.float32(1) - np.exp(-(q * q)))
+
+
+def surface_table(G, core, gravity, rmax, n=2048, steps=2000):
+    """sea.jl:48-58 `balanced` at n radii in [0, rmax] (FP32)."""
+    r = np.linspace(np.float32(0), np.float32(rmax), n, dtype=np.float32)
+    a = np.log(np.maximum(r, np.float32(1e-3)))
+    z = np.log(np.float32(50) * np.float32(core) + np.float32(50) * np.maximum(r, np.float32(1)))
+    h = (z - a) / np.float32(steps)
+    s = np.zeros_like(r)
+    for i in range(steps + 1):
+        o = _oseen(G, core, np.exp(a + np.float32(i) * h))
+        s += (np.float32(0.5) if i in (0, steps) else np.float32(1)) * (o * o) / np.float32(gravity)
+    return -s * h
+
+
+def dense_scene(radius, depth, top, dx, G, core, gravity=9.81):
+    """The balanced vortex on a FlipGrid (flip.jl:298-330): phi, u, v, w, types."""
+    g = flipgrid(radius, depth, top, dx)
+    n1, n2, n3 = g.dims
+    lo = np.array(g.lo, np.float32)
+    dxf = np.float32(g.dx)
+    rmax = np.float32(np.hypot(lo[0], lo[1]))
+    eta = surface_table(G, core, gravity, rmax)
+    k, j, i = np.meshgrid(np.arange(n3), np.arange(n2), np.arange(n1), indexing="ij")
+    px = lo[0] + (i.astype(np.float32) + np.float32(0.5)) * dxf
+    py = lo[1] + (j.astype(np.float32) + np.float32(0.5)) * dxf
+    pz = lo[2] + (k.astype(np.float32) + np.float32(0.5)) * dxf
+    r = np.sqrt(px * px + py * py)
+    t = np.clip(r / rmax, 0, 1).astype(np.float32) * np.float32(eta.size - 1)
+    ia = np.minimum(t.astype(np.int64), eta.size - 2)
+    f = t - ia.astype(np.float32)
+    phi = (pz - (eta[ia] * (np.float32(1) - f) + eta[ia + 1] * f)).astype(np.float32)
+    vel = []
+    for a in range(3):
+        sh = face_shape(g, a)
+        kk, jj, ii = np.meshgrid(np.arange(sh[0]), np.arange(sh[1]), np.arange(sh[2]), indexing="ij")
+        fx = lo[0] + (ii.astype(np.float32) + np.float32(0 if a == 0 else 0.5)) * dxf
+        fy = lo[1] + (jj.astype(np.float32) + np.float32(0 if a == 1 else 0.5)) * dxf
+        rr = np.sqrt(fx * fx + fy * fy)
+        turn = np.where(rr > 1e-4, _oseen(G, core, rr) / np.maximum(rr, np.float32(1e-4)), 0).astype(np.float32)
+        vel.append((-turn * fy if a == 0 else (turn * fx if a == 1 else np.zeros_like(fx))).astype(np.float32))
+    types = dense_classify(g, phi)
+    return g, phi, vel[0], vel[1], vel[2], types

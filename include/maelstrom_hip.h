@@ -150,6 +150,8 @@ int mael_dense_project(mael_dense *d, float dt, float gamma, const mael_solve_op
 /* Support phases around the hot path (not part of it): */
 /* `classify!(s)` (flip.jl:674-677). */
 int mael_dense_classify(mael_dense *d);
+/* `redistance!(s; iterations)` (flip.jl:672-679): Jacobi redistancing sweeps of φ (φ0 is scratch). */
+int mael_dense_redistance(mael_dense *d, int32_t iterations);
 /* `volume(s)` (flip.jl:293), m^3. */
 int mael_dense_volume(mael_dense *d, double *volume);
 /* The balanced vortex of `initialize!` (flip.jl:325-330): φ from the surface table

@@ -21,6 +21,10 @@ FLAGS = [
     "-ffp-contract=off",
     # no packed FP32 instructions in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no sinking of branch tails into shared stores: a sunk store whose target is chosen
+    # by a divergent branch was emitted with a scalar base pointer (wrong for the other
+    # branch's lanes); kernels also store branch results unconditionally
+    "-mllvm", "-simplifycfg-sink-common=false",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 

@@ -162,6 +162,79 @@ void DenseBasin::gravity(float dt, float grav) {
     solidify();
 }
 
+// flip.jl:611-622 eikonal: the distance at which |grad phi| = 1 from the smallest
+// neighbour distances along each axis (Godunov upwind).
+__device__ inline float eikonal(float a, float b, float c, float h) {
+    float t;
+    if (b < a) { t = a; a = b; b = t; }
+    if (c < b) { t = b; b = c; c = t; }
+    if (b < a) { t = a; a = b; b = t; }
+    float d = a + h;
+    if (d > b) {
+        float ab = a - b;
+        d = 0.5f * (a + b + sqrtf(fmaxf(2.0f * h * h - ab * ab, 0.0f)));
+        if (d > c) {
+            float s = a + b + c;
+            d = (s + sqrtf(fmaxf(s * s - 3.0f * (a * a + b * b + c * c - h * h), 0.0f))) / 3.0f;
+        }
+    }
+    return d;
+}
+
+// flip.jl:624-664 reinit_kernel!: one Jacobi sweep of redistancing.
+__global__ void k_reinit(float* __restrict__ out, const float* __restrict__ phi, DGrid g, float phimax) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
+    if (i >= n1 || j >= n2 || k >= n3) return;
+    const int nb[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    size_t c0 = (size_t)i + (size_t)n1 * ((size_t)j + (size_t)n2 * k);
+    float c = phi[c0];
+    float res;
+    if (issolidcell(g, i, j, k)) {
+        float lowest = c;
+        bool found = false;
+        for (int m = 0; m < 6; ++m) {
+            int a = i + nb[m][0], b = j + nb[m][1], cc = k + nb[m][2];
+            if (a < 0 || b < 0 || cc < 0 || a >= n1 || b >= n2 || cc >= n3 || issolidcell(g, a, b, cc)) continue;
+            float f = phi[(size_t)a + (size_t)n1 * ((size_t)b + (size_t)n2 * cc)];
+            lowest = found ? fminf(lowest, f) : f;
+            found = true;
+        }
+        res = found ? lowest : jclamp(c, -phimax, phimax);
+    } else {
+        bool inside = c < 0.0f, frozen = false;
+        float ax = phimax, ay = phimax, az = phimax;
+        for (int m = 0; m < 6; ++m) {
+            int a = i + nb[m][0], b = j + nb[m][1], cc = k + nb[m][2];
+            if (a < 0 || b < 0 || cc < 0 || a >= n1 || b >= n2 || cc >= n3 || issolidcell(g, a, b, cc)) continue;
+            float f = phi[(size_t)a + (size_t)n1 * ((size_t)b + (size_t)n2 * cc)];
+            if ((f < 0.0f) != inside) frozen = true;
+            float d = fabsf(f);
+            if (m < 2) ax = fminf(ax, d);
+            else if (m < 4) ay = fminf(ay, d);
+            else az = fminf(az, d);
+        }
+        if (frozen) res = c;
+        else {
+            float d = fminf(eikonal(ax, ay, az, g.dx), phimax);
+            res = inside ? -d : d;
+        }
+    }
+    out[c0] = res;
+}
+
+// flip.jl:672-679 redistance!: `iterations` Jacobi sweeps, phi and phi0 ping-ponged (phi0
+// is the step's scratch here, as in the reference).
+void DenseBasin::redistance(int iterations) {
+    float phimax = (float)(iterations + 1) * g.dx;
+    for (int it = 0; it < iterations; ++it) {
+        hipLaunchKernelGGL(k_reinit, dim3(cdiv(g.n[0], 64), cdiv(g.n[1], 4), g.n[2]), dim3(64, 4), 0, st.s, phi0.p, phi.p,
+                           g, phimax);
+        std::swap(phi.p, phi0.p);
+    }
+    MAEL_LAUNCH_CHECK();
+}
+
 // flip.jl:262-279 volume_kernel!, summed in FP64 in a fixed order.
 __global__ __launch_bounds__(256) void k_volume(double* __restrict__ partials, const float* __restrict__ phi, DGrid g) {
     __shared__ double lds[4];

@@ -223,6 +223,10 @@ int mael_dense_classify(mael_dense* d) {
     if (!d) return MAEL_E_ARG;
     return guard(d->ctx, [&] { d->b->classify(); });
 }
+int mael_dense_redistance(mael_dense* d, int32_t iterations) {
+    if (!d || iterations < 0) return MAEL_E_ARG;
+    return guard(d->ctx, [&] { d->b->redistance(iterations); });
+}
 int mael_dense_volume(mael_dense* d, double* v) {
     if (!d || !v) return MAEL_E_ARG;
     return guard(d->ctx, [&] { *v = d->b->volume(); });

@@ -154,6 +154,7 @@ struct DenseBasin {
     void project(float dt, float gamma, const SolveOpts& o, SolveStats& s);   // flip.jl:1144-1156
     // support phases for initialisation and a whole step
     void classify();                                         // flip.jl:674-677
+    void redistance(int iterations);                         // flip.jl:672-679
     void solidify();                                         // flip.jl:800-807
     double volume();                                         // flip.jl:293
     void init_vortex(const float* eta, int neta, float rmax, float Gamma, float core);   // flip.jl:298-314

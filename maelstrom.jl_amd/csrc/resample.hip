@@ -135,26 +135,29 @@ __global__ void k_dense_spawn(PView dst, const int32_t* __restrict__ counts, con
         float yp = cy + ((float)((o >> 1) & 1) + jy) * half;
         float zp = cz + ((float)((o >> 2) & 1) + jz) * half;
         int64_t d = base + m;
-        dst.f[15][d] = 0.0f;
         bool air = false;
         if (seeding)
             air = trilinear(phi, n1, n2, n3, ispace(xp, g.lo[0], g.dx, 0.5f), ispace(yp, g.lo[1], g.dx, 0.5f),
                             ispace(zp, g.lo[2], g.dx, 0.5f)) >= 0.0f;
+        // The record is built in registers and every field stored unconditionally: a
+        // branch that picks which field a store goes to lets the compiler merge the two
+        // branches' tail stores through one scalar base pointer chosen by a divergent
+        // condition, which is wrong for the lanes of the other branch.
+        float rec[PFIELDS];
         if (air) {
             float nan = __builtin_nanf("");
-            dst.f[0][d] = nan; dst.f[1][d] = nan; dst.f[2][d] = nan;
+            rec[0] = nan; rec[1] = nan; rec[2] = nan;
 #pragma unroll
-            for (int f = 3; f < 15; ++f) dst.f[f][d] = 0.0f;
+            for (int f = 3; f < PFIELDS; ++f) rec[f] = 0.0f;
         } else {
-            dst.f[0][d] = xp; dst.f[1][d] = yp; dst.f[2][d] = zp;
-            float vel, a0, a1, a2;
-            gather(u, n1 + 1, n2, n3, g, 0, xp, yp, zp, vel, a0, a1, a2);
-            dst.f[3][d] = vel; dst.f[6][d] = a0; dst.f[7][d] = a1; dst.f[8][d] = a2;
-            gather(v, n1, n2 + 1, n3, g, 1, xp, yp, zp, vel, a0, a1, a2);
-            dst.f[4][d] = vel; dst.f[9][d] = a0; dst.f[10][d] = a1; dst.f[11][d] = a2;
-            gather(w, n1, n2, n3 + 1, g, 2, xp, yp, zp, vel, a0, a1, a2);
-            dst.f[5][d] = vel; dst.f[12][d] = a0; dst.f[13][d] = a1; dst.f[14][d] = a2;
+            rec[0] = xp; rec[1] = yp; rec[2] = zp;
+            gather(u, n1 + 1, n2, n3, g, 0, xp, yp, zp, rec[3], rec[6], rec[7], rec[8]);
+            gather(v, n1, n2 + 1, n3, g, 1, xp, yp, zp, rec[4], rec[9], rec[10], rec[11]);
+            gather(w, n1, n2, n3 + 1, g, 2, xp, yp, zp, rec[5], rec[12], rec[13], rec[14]);
+            rec[15] = 0.0f;
         }
+#pragma unroll
+        for (int f = 0; f < PFIELDS; ++f) dst.f[f][d] = rec[f];
     }
 }
 

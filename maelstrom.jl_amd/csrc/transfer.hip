@@ -189,19 +189,22 @@ __global__ __launch_bounds__(256) void k_dense_g2p(PView ps, int64_t n, DGrid g,
     bool k1 = gather_value(fu, n1 + 1, n2, n3, g, 0, xp, yp, zp) > 0.999f;
     bool k2 = gather_value(fv, n1, n2 + 1, n3, g, 1, xp, yp, zp) > 0.999f;
     bool k3 = gather_value(fw, n1, n2, n3 + 1, g, 2, xp, yp, zp) > 0.999f;
+    // v and c (fields 3..14) built in registers and stored unconditionally, so no store's
+    // target depends on the divergent branch (see k_dense_spawn).
+    float rec[12];
     if (k1 && k2 && k3) {
-        float vel, a0, a1, a2;
-        gather(u, n1 + 1, n2, n3, g, 0, xp, yp, zp, vel, a0, a1, a2);
-        ps.f[3][p] = vel; ps.f[6][p] = a0; ps.f[7][p] = a1; ps.f[8][p] = a2;
-        gather(v, n1, n2 + 1, n3, g, 1, xp, yp, zp, vel, a0, a1, a2);
-        ps.f[4][p] = vel; ps.f[9][p] = a0; ps.f[10][p] = a1; ps.f[11][p] = a2;
-        gather(w, n1, n2, n3 + 1, g, 2, xp, yp, zp, vel, a0, a1, a2);
-        ps.f[5][p] = vel; ps.f[12][p] = a0; ps.f[13][p] = a1; ps.f[14][p] = a2;
+        gather(u, n1 + 1, n2, n3, g, 0, xp, yp, zp, rec[0], rec[3], rec[4], rec[5]);
+        gather(v, n1, n2 + 1, n3, g, 1, xp, yp, zp, rec[1], rec[6], rec[7], rec[8]);
+        gather(w, n1, n2, n3 + 1, g, 2, xp, yp, zp, rec[2], rec[9], rec[10], rec[11]);
     } else {
-        ps.f[5][p] = ps.f[5][p] - gravity * dt;
+        rec[0] = ps.f[3][p];
+        rec[1] = ps.f[4][p];
+        rec[2] = ps.f[5][p] - gravity * dt;
 #pragma unroll
-        for (int f = 6; f < 15; ++f) ps.f[f][p] = 0.0f;
+        for (int f = 3; f < 12; ++f) rec[f] = 0.0f;
     }
+#pragma unroll
+    for (int f = 0; f < 12; ++f) ps.f[3 + f][p] = rec[f];
 }
 
 void DenseBasin::g2p(float dt, float grav) {

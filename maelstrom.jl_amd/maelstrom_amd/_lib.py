@@ -82,6 +82,7 @@ def lib():
         "mael_dense_project": (C.c_int, [p, f32, f32, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
         "mael_dense_classify": (C.c_int, [p]),
         "mael_dense_volume": (C.c_int, [p, C.POINTER(f64)]),
+        "mael_dense_redistance": (C.c_int, [p, i32]),
         "mael_dense_init_vortex": (C.c_int, [p, p, i32, f32, f32, f32]),
         "mael_dense_solidify": (C.c_int, [p]),
         "mael_dense_gravity": (C.c_int, [p, f32, f32]),

@@ -185,14 +185,21 @@ def project(s, dt, recover=0.5, gamma=None, **solve):
 
 
 def classify(s):
+    """flip.jl:674-677 `classify!(s)`."""
     s.ctx.check(_lib.lib().mael_dense_classify(s.h))
+    return s
+
+
+def redistance(s, iterations=None):
+    """flip.jl:672-679 `redistance!(s; iterations = band + 3)`."""
+    it = s.band + 3 if iterations is None else int(iterations)
+    s.ctx.check(_lib.lib().mael_dense_redistance(s.h, it))
     return s
 
 
 def initialize(s):
     """flip.jl:325-348 `initialize!(s)`: the balanced vortex, the band seeded with
-    particles, their deposit. Redistancing (not on this path) is left out: the level
-    set is the height above the balanced surface, a distance where the surface is flat."""
+    particles, their deposit."""
     g = s.grid
     rmax = np.float32(math.hypot(g.lo[0], g.lo[1]))
     r = np.linspace(np.float32(0), rmax, 2048, dtype=np.float32)
@@ -200,6 +207,7 @@ def initialize(s):
     L = _lib.lib()
     s.ctx.check(L.mael_dense_init_vortex(s.h, eta.ctypes.data, eta.size, float(rmax), float(s.basin.Γ),
                                          float(s.basin.core)))
+    redistance(s, 2 * s.band + 4)
     classify(s)
     s.ctx.check(L.mael_dense_solidify(s.h))
     for f in ("FU", "FV", "FW"):
@@ -211,6 +219,7 @@ def initialize(s):
     for a, b in (("U0", "U"), ("V0", "V"), ("W0", "W")):
         s.copy(a, b)
     particles2grid(s, 1.0)
+    redistance(s)
     classify(s)
     s.volume0 = volume(s)
     s.t = 0.0
@@ -225,12 +234,13 @@ def maxspeed(s):
 def step(s, dt, **solve):
     """One step of the hot path in the reference's order (flip.jl:1396-1410): the grid
     holds still in place of advection (φ0, u0 take φ, u), then particles to grid,
-    classify, gravity, projection, grid to particles, resample. Redistancing, trapped
-    air, the stir and extrapolation are not on this path and are left out."""
+    redistance, classify, gravity, projection, grid to particles, resample. Trapped air,
+    the stir and extrapolation are not on this path and are left out."""
     s.copy("PHI0", "PHI")
     for a, b in (("U0", "U"), ("V0", "V"), ("W0", "W")):
         s.copy(a, b)
     particles2grid(s, dt)
+    redistance(s)
     classify(s)
     s.ctx.check(_lib.lib().mael_dense_gravity(s.h, float(dt), float(s.basin.g)))
     st = project(s, dt, **solve)

@@ -177,22 +177,21 @@ def test_projection_parity(mael, orc, ctx, tol):
     s, sc = make_basin(mael, orc)
     g = sc["g"]
     dt = 0.05
-    w = sc["w"].copy()
-    s.set("W", w)
+    # gravity, then the walls (flip.jl:775-777): the projection's input velocities
     s.ctx.check(mael.lib().mael_dense_gravity(s.h, dt, 9.81))
-    w = s.get("W")
+    u, v, w = s.get("U"), s.get("V"), s.get("W")
     gamma = 0.001
     s.set("P", np.zeros(orc.cells_shape(g), np.float32))
     st = mael.project(s, dt, gamma=gamma, tol=tol, maxiter=200)
     assert st.rel_residual <= tol and st.iterations > 0
-    b = orc.dense_rhs(g, sc["u"], sc["v"], w, sc["types"], gamma)
+    b = orc.dense_rhs(g, u, v, w, sc["types"], gamma)
     assert rel_l2(s.get("RHS"), b) < 1e-6
     P, it, rr = orc.dense_pressure(g, sc["types"], sc["phi"], b, tol=1e-13, maxiter=500)
     # the solution error is bounded by the residual times the conditioning
     Pg = s.get("P")
     err = rel_l2(Pg, P)
     assert err < (1e-5 if tol <= 1e-6 else 5e-3), err
-    U, F = orc.dense_project_faces(g, P, sc["types"], sc["phi"], sc["u"], sc["v"], w)
+    U, F = orc.dense_project_faces(g, P, sc["types"], sc["phi"], u, v, w)
     for name, ref in (("U", U[0]), ("V", U[1]), ("W", U[2])):
         assert rel_l2(s.get(name), ref) < (1e-5 if tol <= 1e-6 else 1e-3), name
     for name, ref in (("FU", F[0]), ("FV", F[1]), ("FW", F[2])):

@@ -13,6 +13,7 @@ struct orc_particles { float* x[3]; float* v[3]; float* c[9]; float* lag; int64_
 
 static thread_local std::string g_err;
 const char* orc_last_error() { return g_err.c_str(); }
+void orc_set_error(const char* msg) { g_err = msg; }
 
 static DenseGrid to_grid(const orc_grid* g) {
     DenseGrid d;

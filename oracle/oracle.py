@@ -343,3 +343,268 @@ def dense_scene(radius, depth, top, dx, G, core, gravity=9.81):
         vel.append((-turn * fy if a == 0 else (turn * fx if a == 1 else np.zeros_like(fx))).astype(np.float32))
     types = dense_classify(g, phi)
     return g, phi, vel[0], vel[1], vel[2], types
+
+
+# ── the sparse basin (sflip.jl) ──────────────────────────────────────────────
+
+class SGrid(C.Structure):
+    _fields_ = [("lo", C.c_float * 3), ("dx", C.c_float), ("nb", C.c_int32 * 3), ("radius", C.c_float),
+                ("floor", C.c_float)]
+
+    @property
+    def nc(self):
+        return tuple(int(x) * 4 for x in self.nb)
+
+
+_SPARSE_SIG = None
+
+
+def _slib():
+    global _SPARSE_SIG
+    L = lib()
+    if _SPARSE_SIG is None:
+        p, i32, i64, f32, f64, u32, u8 = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_uint32, C.c_uint8
+        P32, P64, PF64 = C.POINTER(i32), C.POINTER(i64), C.POINTER(f64)
+        _SPARSE_SIG = {
+            "orc_sparsegrid": (C.c_int, [f32, f32, f32, f32, i32, C.POINTER(SGrid)]),
+            "orc_sparse_create": (p, [f32, f32, f32, f32, i32, i32, i64, i32, i32]),
+            "orc_sparse_destroy": (None, [p]),
+            "orc_sparse_info": (None, [p, C.POINTER(SGrid), P32, P32, P64, P32]),
+            "orc_sparse_free_list": (None, [p, p]),
+            "orc_sparse_set_n": (None, [p, i64]),
+            "orc_sparse_array": (C.c_int, [p, C.c_char_p, C.POINTER(p), P64]),
+            "orc_sparse_marks": (None, [p, f32, f32, i32, i32, p]),
+            "orc_sparse_refine": (C.c_int, [p, p, p, P32, p, P32]),
+            "orc_sparse_init_coarse": (None, [p, p, i32, f32, f32, f32]),
+            "orc_sparse_init_fine": (None, [p, p, i32, f32, f32, f32]),
+            "orc_sparse_classify": (None, [p]),
+            "orc_sparse_solidify": (None, [p]),
+            "orc_sparse_restrict_velocity": (None, [p]),
+            "orc_sparse_redistance": (None, [p, i32]),
+            "orc_sparse_volume": (f64, [p]),
+            "orc_sparse_fill_flags": (None, [p, u8]),
+            "orc_sparse_keep": (None, [p]),
+            "orc_sparse_resample": (C.c_int, [p, i32, u32, P64, p, p, p, p, p]),
+            "orc_sparse_p2g": (None, [p, f32, i32, p]),
+            "orc_sparse_g2p": (None, [p, f32, f32]),
+            "orc_sparse_rhs": (None, [p, f32, p, p]),
+            "orc_sparse_matrix": (i64, [p, i32, p, p, p]),
+            "orc_sparse_apply": (None, [p, p, p, p, p]),
+            "orc_sparse_vcycle": (None, [p, p, p, p, p, i32, i32, f64]),
+            "orc_sparse_levels": (i32, [p, p]),
+            "orc_sparse_project": (C.c_int, [p, f32, f64, i32, i32, i32, f64, i32, i32, i32, P32, PF64, P32]),
+        }
+        for name, (res, args) in _SPARSE_SIG.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+    return L
+
+
+def sparsegrid(radius, depth, top, dx, pad=2):
+    g = SGrid()
+    _slib().orc_sparsegrid(radius, depth, top, dx, pad, C.byref(g))
+    return g
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+_U8 = ("types_f", "types_c", "fu_f", "fu_c", "fv_f", "fv_c", "fw_f", "fw_c")
+_I32 = ("slot", "where")
+
+
+class SparseOracle:
+    """One sparse basin's state in the CPU oracle; arrays are numpy views into it."""
+
+    def __init__(self, radius=18.0, depth=12.0, top=3.0, dx=0.6, pad=2, blocks=64, capacity=400000, band=4, nmin=8):
+        L = _slib()
+        self.h = L.orc_sparse_create(radius, depth, top, dx, pad, blocks, capacity, band, nmin)
+        self.band, self.nmin, self.blocks, self.capacity = band, nmin, blocks, capacity
+        self.params = dict(radius=radius, depth=depth, top=top, dx=dx, pad=pad)
+
+    def __del__(self):
+        try:
+            if self.h:
+                _slib().orc_sparse_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def info(self):
+        g, ns, nf, n, b = SGrid(), C.c_int32(), C.c_int32(), C.c_int64(), C.c_int32()
+        _slib().orc_sparse_info(self.h, C.byref(g), C.byref(ns), C.byref(nf), C.byref(n), C.byref(b))
+        return g, ns.value, nf.value, n.value
+
+    @property
+    def grid(self):
+        return self.info()[0]
+
+    @property
+    def nslots(self):
+        return self.info()[1]
+
+    @property
+    def n(self):
+        return self.info()[3]
+
+    @n.setter
+    def n(self, v):
+        _slib().orc_sparse_set_n(self.h, int(v))
+
+    def free_list(self):
+        k = self.info()[2]
+        out = np.zeros(k, np.int32)
+        _slib().orc_sparse_free_list(self.h, _ptr(out))
+        return out
+
+    def array(self, name):
+        """A numpy view (no copy) of a state array; particle fields are the current buffer."""
+        ptr, n = C.c_void_p(), C.c_int64()
+        if _slib().orc_sparse_array(self.h, name.encode(), C.byref(ptr), C.byref(n)) != 0:
+            raise KeyError(name)
+        dt = np.uint8 if name in _U8 else (np.int32 if name in _I32 else np.float32)
+        buf = (C.c_char * (n.value * np.dtype(dt).itemsize)).from_address(ptr.value)
+        return np.frombuffer(buf, dtype=dt)
+
+    def marks(self, vary=0.5, reach=None, rings=0, particles=True):
+        g = self.grid
+        if reach is None:
+            reach = np.float32((np.ceil(8.0) + 3) * g.dx)
+        out = np.zeros(int(np.prod(list(g.nb))), np.uint8)
+        _slib().orc_sparse_marks(self.h, vary, reach, rings, 1 if particles else 0, _ptr(out))
+        return out
+
+    def refine(self, want):
+        want = _c(want, np.uint8)
+        added = np.zeros(self.blocks, np.int32)
+        removed = np.zeros(self.blocks, np.int32)
+        na, nr = C.c_int32(), C.c_int32()
+        st = _slib().orc_sparse_refine(self.h, _ptr(want), _ptr(added), C.byref(na), _ptr(removed), C.byref(nr))
+        if st != 0:
+            raise OracleError(st, lib().orc_last_error().decode())
+        return added[:na.value].copy(), removed[:nr.value].copy()
+
+    def init_coarse(self, eta, rmax, G, core):
+        eta = _c(eta, np.float32)
+        _slib().orc_sparse_init_coarse(self.h, _ptr(eta), eta.size, rmax, G, core)
+
+    def init_fine(self, eta, rmax, G, core):
+        eta = _c(eta, np.float32)
+        _slib().orc_sparse_init_fine(self.h, _ptr(eta), eta.size, rmax, G, core)
+
+    def classify(self):
+        _slib().orc_sparse_classify(self.h)
+
+    def solidify(self):
+        _slib().orc_sparse_solidify(self.h)
+
+    def restrict_velocity(self):
+        _slib().orc_sparse_restrict_velocity(self.h)
+
+    def redistance(self, iterations=None):
+        _slib().orc_sparse_redistance(self.h, self.band + 3 if iterations is None else iterations)
+
+    def volume(self):
+        return _slib().orc_sparse_volume(self.h)
+
+    def fill_flags(self, v=1):
+        _slib().orc_sparse_fill_flags(self.h, v)
+
+    def keep(self):
+        _slib().orc_sparse_keep(self.h)
+
+    def resample(self, seeding=False, seed=1):
+        n, ns = self.n, self.nslots
+        key, rank = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
+        cells = ns * 4096
+        counts, totals, offsets = (np.zeros(max(cells, 1), np.int32) for _ in range(3))
+        total = C.c_int64()
+        st = _slib().orc_sparse_resample(self.h, 1 if seeding else 0, seed, C.byref(total), _ptr(key), _ptr(rank),
+                                         _ptr(counts), _ptr(totals), _ptr(offsets))
+        if st != 0:
+            raise OracleError(st, lib().orc_last_error().decode())
+        return dict(total=total.value, key=key[:n], rank=rank[:n], counts=counts[:cells], totals=totals[:cells],
+                    offsets=offsets[:cells])
+
+    def p2g(self, dt, fp32=False, momwt=False):
+        out = np.zeros(6 * self.nslots * 4352, np.float64) if momwt else None
+        _slib().orc_sparse_p2g(self.h, dt, 1 if fp32 else 0, _ptr(out))
+        return None if out is None else out.reshape(6, -1)
+
+    def g2p(self, dt, gravity=9.81):
+        _slib().orc_sparse_g2p(self.h, dt, gravity)
+
+    def rhs(self, gamma=0.0):
+        nc = self.grid.nc
+        bf = np.zeros(self.nslots * 4096, np.float64)
+        bc = np.zeros(nc[0] * nc[1] * nc[2], np.float64)
+        _slib().orc_sparse_rhs(self.h, gamma, _ptr(bf), _ptr(bc))
+        return bf, bc
+
+    def matrix(self, N=16):
+        """The level's composite operator as scipy CSR over (fine pool, then coarse) unknowns."""
+        import scipy.sparse as sp
+        nnz = _slib().orc_sparse_matrix(self.h, N, None, None, None)
+        r, c, v = np.zeros(nnz, np.int64), np.zeros(nnz, np.int64), np.zeros(nnz, np.float64)
+        _slib().orc_sparse_matrix(self.h, N, _ptr(r), _ptr(c), _ptr(v))
+        nc = self.grid.nc
+        size = self.nslots * N ** 3 + nc[0] * nc[1] * nc[2]
+        return sp.csr_matrix((v, (r, c)), shape=(size, size))
+
+    def apply(self, xf, xc):
+        xf, xc = _c(xf, np.float64), _c(xc, np.float64)
+        qf, qc = np.zeros_like(xf), np.zeros_like(xc)
+        _slib().orc_sparse_apply(self.h, _ptr(xf), _ptr(xc), _ptr(qf), _ptr(qc))
+        return qf, qc
+
+    def vcycle(self, rf, rc, sweeps=1, coarse=40, boost=1.0):
+        rf, rc = _c(rf, np.float64), _c(rc, np.float64)
+        zf, zc = np.zeros_like(rf), np.zeros_like(rc)
+        _slib().orc_sparse_vcycle(self.h, _ptr(rf), _ptr(rc), _ptr(zf), _ptr(zc), sweeps, coarse, boost)
+        return zf, zc
+
+    def levels(self):
+        n = _slib().orc_sparse_levels(self.h, None)
+        dims = np.zeros(3 * n, np.int32)
+        _slib().orc_sparse_levels(self.h, _ptr(dims))
+        return [tuple(int(x) for x in dims[3 * l:3 * l + 3]) for l in range(n)]
+
+    def project(self, gamma=0.0, tol=1e-6, maxiter=200, sweeps=1, coarse=40, boost=1.0, restart=8, policy=0, fp32=False):
+        it, rr, fg = C.c_int32(), C.c_double(), C.c_int32()
+        st = _slib().orc_sparse_project(self.h, gamma, tol, maxiter, sweeps, coarse, boost, restart, policy,
+                                        1 if fp32 else 0, C.byref(it), C.byref(rr), C.byref(fg))
+        if st != 0:
+            raise OracleError(st, lib().orc_last_error().decode())
+        return dict(iterations=it.value, rel_residual=rr.value, fgmres=bool(fg.value))
+
+
+def sparse_scene(radius=18.0, depth=12.0, top=3.0, dx=0.6, G=225.0, core=2.5, gravity=9.81, blocks=64,
+                 capacity=400000, band=4, nmin=8, vary=0.5, particles=True):
+    """The sparse basin's start (sflip.jl:681-713) in the oracle: the balanced surface and
+    the swirl on the coarse grid, fine blocks round the surface, the band seeded (two
+    resamples with seed 1), then a first particles-to-grid and redistance."""
+    o = SparseOracle(radius, depth, top, dx, 2, blocks, capacity, band, nmin)
+    g = o.grid
+    rmax = np.float32(np.hypot(np.float32(g.lo[0]), np.float32(g.lo[1])))
+    eta = surface_table(G, core, gravity, rmax)
+    o.init_coarse(eta, rmax, G, core)
+    o.refine(o.marks(vary=vary, particles=False))
+    o.init_fine(eta, rmax, G, core)
+    o.redistance(2 * band + 4)
+    o.classify()
+    o.solidify()
+    o.fill_flags(1)
+    o.n = 0
+    if particles:
+        o.resample(seeding=True, seed=1)
+        o.resample(seed=1)
+        o.keep()
+        o.p2g(1.0, fp32=True)
+        o.redistance()
+        o.classify()
+    o.eta, o.rmax, o.G, o.core = eta, rmax, G, core
+    return o

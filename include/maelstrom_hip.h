@@ -167,6 +167,121 @@ int mael_dense_copy(mael_dense *d, int32_t dst, int32_t src);
 /* Wait for the basin's stream. */
 int mael_dense_sync(mael_dense *d);
 
+/* ── the sparse grid (`SparseGrid`, sflip.jl:31-53) ───────────────────────── */
+
+typedef struct {
+    float lo[3];      /* the grid's low corner, m */
+    float dx;         /* fine cell width, m (coarse cells are 4 dx, blocks 16 dx) */
+    int32_t nb[3];    /* blocks per axis (multiples of 4) */
+    float radius;     /* the basin's wall radius, m */
+    float floor;      /* the basin's floor height, m (= -depth) */
+} mael_sparse_grid;
+
+/* `SparseGrid(b; dx, top = b.rimwall, pad)` (sflip.jl:46-53): the sizing rule. */
+int mael_sparsegrid(float radius, float depth, float top, float dx, int32_t pad, mael_sparse_grid *out);
+
+/* ── a sparse basin (`SparseBasin`, sflip.jl:242-347) ─────────────────────── */
+
+typedef struct mael_sparse mael_sparse;
+
+/* `SparseBasin(b, backend; dx, band, nmin, capacity, blocks)` (sflip.jl:301-347): the fine
+ * pools for `blocks` 16^3 blocks, the coarse grid and its multigrid halvings, two
+ * particle buffers of `capacity`. MAEL_E_GRID if the coarse grid does not halve to at
+ * most 1024 cells (sflip.jl:326). */
+int mael_sparse_create(mael_ctx *ctx, const mael_sparse_grid *grid, int32_t blocks, int64_t capacity, int32_t band,
+                       int32_t nmin, mael_sparse **out);
+void mael_sparse_destroy(mael_sparse *s);
+
+/* Arrays of a sparse basin. A field has a fine part (the pool: 4096 cells or 4352 faces
+ * per slot, slot-major; see the layout notes in the library) and a coarse part (dense,
+ * x fastest). Counts are elements; fine parts count the whole pool (`blocks` slots). */
+enum {
+    MAEL_S_PHI_F = 0, MAEL_S_PHI_C, MAEL_S_PHI0_F, MAEL_S_PHI0_C, MAEL_S_P_F, MAEL_S_P_C,
+    MAEL_S_TYPES_F, MAEL_S_TYPES_C,                                   /* uint8 */
+    MAEL_S_U_F, MAEL_S_U_C, MAEL_S_V_F, MAEL_S_V_C, MAEL_S_W_F, MAEL_S_W_C,
+    MAEL_S_U0_F, MAEL_S_U0_C, MAEL_S_V0_F, MAEL_S_V0_C, MAEL_S_W0_F, MAEL_S_W0_C,
+    MAEL_S_FU_F, MAEL_S_FU_C, MAEL_S_FV_F, MAEL_S_FV_C, MAEL_S_FW_F, MAEL_S_FW_C,   /* uint8 */
+    MAEL_S_RHS_F, MAEL_S_RHS_C,
+    MAEL_S_PX, MAEL_S_PY, MAEL_S_PZ, MAEL_S_VX, MAEL_S_VY, MAEL_S_VZ,
+    MAEL_S_C0, MAEL_S_C1, MAEL_S_C2, MAEL_S_C3, MAEL_S_C4, MAEL_S_C5, MAEL_S_C6, MAEL_S_C7, MAEL_S_C8, MAEL_S_LAG,
+    MAEL_S_KEY, MAEL_S_RANK,                                          /* last resample, per particle (int32) */
+    MAEL_S_COUNTS, MAEL_S_TOTALS, MAEL_S_OFFSETS,                     /* last resample, per fine cell (int32) */
+    MAEL_S_MOM_U, MAEL_S_MOM_V, MAEL_S_MOM_W, MAEL_S_WT_U, MAEL_S_WT_V, MAEL_S_WT_W,   /* int64 fixed point 2^-32 */
+    MAEL_S_SLOT,                                                      /* per block: 0 or the 1-based slot (int32) */
+    MAEL_S_WHERE,                                                     /* per slot: block x, y, z, 0 (int32 x4; -1 free) */
+    MAEL_S_NARRAYS
+};
+int mael_sparse_array(mael_sparse *s, int32_t which, void **dptr, int64_t *count, int32_t *elsize);
+int mael_sparse_upload(mael_sparse *s, int32_t which, const void *host, int64_t count);
+int mael_sparse_download(mael_sparse *s, int32_t which, void *host, int64_t count);
+int mael_sparse_count(mael_sparse *s, int64_t *n);
+int mael_sparse_set_count(mael_sparse *s, int64_t n);
+
+/* The host side of the topology (sflip.jl:252-290 `slots`, `wheres`, `free`, `nslots`):
+ * `slot` has nb0*nb1*nb2 entries, `where` 3 per pool slot, `freelist` up to `blocks`. */
+int mael_sparse_get_topology(mael_sparse *s, int32_t *slot, int32_t *where, int32_t *nslots, int32_t *freelist,
+                             int32_t *nfree);
+int mael_sparse_set_topology(mael_sparse *s, const int32_t *slot, const int32_t *where, int32_t nslots,
+                             const int32_t *freelist, int32_t nfree);
+
+/* The marks of `refine!` (sflip.jl:518-526): the surface's blocks, those where the coarse
+ * velocity varies by more than `vary`, those within `reach` of a particle (if
+ * `particles`), grown by `rings`; written to `marks` (one byte a block). */
+int mael_sparse_marks(mael_sparse *s, float vary, float reach, int32_t rings, int32_t particles, uint8_t *marks);
+/* `refine!`'s allocation from given marks (sflip.jl:527-575): release unmarked blocks, then
+ * give marked ones a slot (last freed first), zero the released slots and start new
+ * blocks from the coarse grid. Bit-exact: the same marks give the same slots.
+ * MAEL_E_BLOCKS with the reference's text if more blocks are wanted than the pool holds. */
+int mael_sparse_refine_marks(mael_sparse *s, const uint8_t *marks, int32_t *added, int32_t *removed);
+/* `refine!(s)` (sflip.jl:516-576): the marks, then the allocation. */
+int mael_sparse_refine(mael_sparse *s, float vary, float reach, int32_t rings, int32_t *added, int32_t *removed);
+
+/* `resample!(s; seeding)` (sflip.jl:2227-2246), new particles from (u, v, w). */
+int mael_sparse_resample(mael_sparse *s, int32_t seeding, uint32_t seed, int64_t *total);
+/* `particles2grid!(s, dt)` (sflip.jl:808-831). */
+int mael_sparse_p2g(mael_sparse *s, float dt);
+/* `grid2particles!(s, dt)` (sflip.jl:2141-2144). */
+int mael_sparse_g2p(mael_sparse *s, float dt, float gravity);
+
+typedef struct {
+    double tol;          /* relative residual target */
+    int32_t maxiter;     /* iterations in all */
+    int32_t sweeps;      /* damped Jacobi sweep pairs per level */
+    int32_t coarse;      /* coarsest-level RBGS sweeps */
+    int32_t restart;     /* FGMRES restart length (at most 40) */
+    int32_t policy;      /* 0 flexible CG (sflip.jl:1867), 1 FGMRES (fgmres.jl:26), 2 CG then FGMRES (fgmres.jl:96) */
+    int32_t precision;   /* 0: FP32 hierarchy */
+    float boost;         /* coarse-correction scale */
+} mael_sparse_opts;
+typedef struct {
+    int32_t iterations, fcg_iterations, fgmres_iterations, used_fgmres;
+    double rel_residual;   /* ||b - A P|| / ||b|| of the returned pressure (FP64 sums) */
+    double ms;
+} mael_sparse_stats;
+/* The defaults of `pressure!(::SparseBasin{B,<:SurfaceStir})` (fgmres.jl:96): tol 5e-4,
+ * maxiter 240, sweeps 1, coarse 40, restart 8, policy 2, boost 1.5. */
+void mael_sparse_defaults(mael_sparse_opts *o);
+/* `project!(s, dt)` (sflip.jl:1998-2015) after the caller's γ: couplings, rhs, clear,
+ * the solve by `opts->policy`, the faces and flags. Errors carry the reference's texts. */
+int mael_sparse_project(mael_sparse *s, float gamma, const mael_sparse_opts *opts, mael_sparse_stats *stats);
+
+/* Support phases around the hot path (not part of it): */
+/* The start of `initialize!` (sflip.jl:681-694): φ from the surface table and the swirl,
+ * on the coarse grid, then on the fine blocks. */
+int mael_sparse_init_coarse(mael_sparse *s, const float *eta, int32_t neta, float rmax, float gamma_circ, float core);
+int mael_sparse_init_fine(mael_sparse *s, const float *eta, int32_t neta, float rmax, float gamma_circ, float core);
+int mael_sparse_classify(mael_sparse *s);                        /* sflip.jl:942-946 */
+int mael_sparse_solidify(mael_sparse *s);                        /* sflip.jl:1034-1041 */
+int mael_sparse_redistance(mael_sparse *s, int32_t iterations);  /* sflip.jl:910-919 */
+int mael_sparse_restrict_velocity(mael_sparse *s);               /* sflip.jl:618 */
+int mael_sparse_keep(mael_sparse *s);                            /* φ0 = φ, u0 = u (sflip.jl:700-703) */
+int mael_sparse_fill_flags(mael_sparse *s, uint8_t value);       /* sflip.jl:696-698 */
+int mael_sparse_volume(mael_sparse *s, double *volume);          /* sflip.jl:1990-1996 */
+int mael_sparse_gravity(mael_sparse *s, float dt, float gravity); /* forces! without stirring, sflip.jl:1043-1056 */
+int mael_sparse_sync(mael_sparse *s);
+int mael_sparse_ktimer(mael_sparse *s, int32_t kernel, int32_t launches);
+int mael_sparse_ktime(mael_sparse *s, double *ms, int32_t *launches);
+
 /* ── launch timing (no counterpart in the reference, which has no profiler hooks) ── */
 /* Kernel ids for the launch timer. */
 enum {

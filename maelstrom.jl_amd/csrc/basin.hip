@@ -162,25 +162,6 @@ void DenseBasin::gravity(float dt, float grav) {
     solidify();
 }
 
-// flip.jl:611-622 eikonal: the distance at which |grad phi| = 1 from the smallest
-// neighbour distances along each axis (Godunov upwind).
-__device__ inline float eikonal(float a, float b, float c, float h) {
-    float t;
-    if (b < a) { t = a; a = b; b = t; }
-    if (c < b) { t = b; b = c; c = t; }
-    if (b < a) { t = a; a = b; b = t; }
-    float d = a + h;
-    if (d > b) {
-        float ab = a - b;
-        d = 0.5f * (a + b + sqrtf(fmaxf(2.0f * h * h - ab * ab, 0.0f)));
-        if (d > c) {
-            float s = a + b + c;
-            d = (s + sqrtf(fmaxf(s * s - 3.0f * (a * a + b * b + c * c - h * h), 0.0f))) / 3.0f;
-        }
-    }
-    return d;
-}
-
 // flip.jl:624-664 reinit_kernel!: one Jacobi sweep of redistancing.
 __global__ void k_reinit(float* __restrict__ out, const float* __restrict__ phi, DGrid g, float phimax) {
     int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;

@@ -2,16 +2,13 @@
 // Every entry point catches the library's errors and turns them into a status code and
 // the context's last-error text.
 #include "../../include/maelstrom_hip.h"
+#include "capi_common.hpp"
 #include "dense.hpp"
 #include "scan.hpp"
 #include <cmath>
 #include <memory>
 #include <string>
 
-struct mael_ctx {
-    int device = 0;
-    std::string err;
-};
 
 struct mael_dense {
     mael_ctx* ctx;
@@ -20,21 +17,6 @@ struct mael_dense {
 
 namespace {
 
-template <class F> int guard(mael_ctx* ctx, F&& f) {
-    try {
-        if (ctx) MAEL_HIP(hipSetDevice(ctx->device));
-        f();
-        return MAEL_OK;
-    } catch (const mael::Error& e) {
-        if (ctx) ctx->err = e.what();
-        return e.code;
-    } catch (const std::exception& e) {
-        if (ctx) ctx->err = e.what();
-        return MAEL_E_ARG;
-    }
-}
-
-struct ArrRef { void* p; int64_t n; int32_t es; };
 
 ArrRef dense_array(mael::DenseBasin& b, int which) {
     using namespace mael;

@@ -179,6 +179,31 @@ __device__ inline void gather(const V* __restrict__ u, int n1, int n2, int n3, c
     a0 *= c; a1 *= c; a2 *= c;
 }
 
+// flip.jl:611-622 eikonal: the distance at which |grad phi| = 1 from the smallest
+// neighbour distances along each axis (Godunov upwind).
+__device__ inline float eikonal(float a, float b, float c, float h) {
+    float t;
+    if (b < a) { t = a; a = b; b = t; }
+    if (c < b) { t = b; b = c; c = t; }
+    if (b < a) { t = a; a = b; b = t; }
+    float d = a + h;
+    if (d > b) {
+        float ab = a - b;
+        d = 0.5f * (a + b + sqrtf(fmaxf(2.0f * h * h - ab * ab, 0.0f)));
+        if (d > c) {
+            float s = a + b + c;
+            d = (s + sqrtf(fmaxf(s * s - 3.0f * (a * a + b * b + c * c - h * h), 0.0f))) / 3.0f;
+        }
+    }
+    return d;
+}
+
+// sea.jl:39 oseen: the Lamb-Oseen vortex's azimuthal speed at radius r.
+__device__ inline float oseen_dev(float G, float core, float r) {
+    float q = __fdiv_rn(r, core);
+    return __fdiv_rn(G, 2.0f * 3.14159265f * fmaxf(r, 1e-4f)) * (1.0f - expf(-(q * q)));
+}
+
 // ─── particles ───────────────────────────────────────────────────────────────
 
 constexpr int PFIELDS = 16;   // x y z vx vy vz c0..c8 lag

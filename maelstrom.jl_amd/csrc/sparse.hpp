@@ -1,0 +1,347 @@
+// Device state of one sparse two-resolution basin (the reference's `SparseBasin`,
+// sflip.jl:242-347) and its phases.
+//
+// Layout in HBM (0-based throughout):
+//  * the block table `slot` (nb0*nb1*nb2 int32: 0, or the 1-based pool slot) and per
+//    slot its block coordinates `where` (int4, x = -1 for a free slot);
+//  * fine pools, slot-major: a slot's 4096 cells x fastest (i + 16 (j + 16 k)), its
+//    faces along axis a in (16+ea) x (16+eb) x (16+ec) order, 4352 per slot and axis;
+//    a face on the plane between two fine blocks is held by both, the upper block's
+//    copy canonical (sflip.jl:110-126);
+//  * the coarse grid (cells 4 fine cells wide) as dense arrays, x fastest;
+//  * level 1 of the multigrid at 8^3 a block (512 cells, 576 faces per axis), and the
+//    dense halvings of the coarse grid.
+#pragma once
+#include "dense.hpp"
+
+namespace mael {
+
+constexpr int SBLOCK = 16, SRATIO = 4, SBCELLS = 4096, SBFACES = 4352, SBCELLS1 = 512, SBFACES1 = 576;
+constexpr float FINE_FINE = 1.0f, FINE_COARSE = 0.4f, COARSE_COARSE = 4.0f;   // sflip.jl:1071-1073
+constexpr float OMEGA = 6.0f / 7.0f;                                            // sflip.jl:1074
+
+struct SGrid {   // sflip.jl:38-44
+    float lo[3];
+    float dx;
+    int nb[3];
+    float radius, floor;
+    __host__ __device__ int nc(int a) const { return nb[a] * 4; }
+    __host__ __device__ float cdx() const { return 4.0f * dx; }
+};
+
+// The topology as kernels see it (sflip.jl:78-84).
+struct STopo {
+    SGrid g;
+    const int32_t* slot;
+    const int4* where;
+};
+
+__host__ __device__ inline int ffn(int N, int a, int i, int j, int k) {   // a level-N block's face (sflip.jl:1081-1085)
+    return a == 0 ? i + (N + 1) * (j + N * k) : (a == 1 ? i + N * (j + (N + 1) * k) : i + N * (j + N * k));
+}
+__host__ __device__ inline size_t cl(int i, int j, int k, int n0, int n1) {
+    return (size_t)i + (size_t)n0 * ((size_t)j + (size_t)n1 * k);
+}
+
+__device__ inline bool inblocks(const SGrid& g, int bi, int bj, int bk) {
+    return bi >= 0 && bi < g.nb[0] && bj >= 0 && bj < g.nb[1] && bk >= 0 && bk < g.nb[2];
+}
+__device__ inline int blockid(const SGrid& g, int bi, int bj, int bk) { return bi + g.nb[0] * (bj + g.nb[1] * bk); }
+
+// Level-N cell (N = 16: the fine cells; 8: level 1) at global (I, J, K), or -1 (sflip.jl:1087-1094).
+template <int N> __device__ inline int32_t cellat(const STopo& t, int I, int J, int K) {
+    constexpr int sh = N == 16 ? 4 : 3, m = N - 1;
+    int bi = I >> sh, bj = J >> sh, bk = K >> sh;
+    if (!inblocks(t.g, bi, bj, bk)) return -1;
+    int32_t s = t.slot[blockid(t.g, bi, bj, bk)];
+    return s == 0 ? -1 : (s - 1) * (N * N * N) + (I & m) + N * ((J & m) + N * (K & m));
+}
+__device__ inline int32_t fineat(const STopo& t, int I, int J, int K) { return cellat<16>(t, I, J, K); }
+
+// Level-N face along a on the low side of cell (I, J, K), in either block that holds it;
+// the upper block's copy first (sflip.jl:1096-1109, 110-126).
+template <int N> __device__ inline int32_t faceat(const STopo& t, int a, int I, int J, int K) {
+    constexpr int sh = N == 16 ? 4 : 3, m = N - 1, per = (N + 1) * N * N;
+    int bi = I >> sh, bj = J >> sh, bk = K >> sh;
+    if (inblocks(t.g, bi, bj, bk)) {
+        int32_t s = t.slot[blockid(t.g, bi, bj, bk)];
+        if (s != 0) return (s - 1) * per + ffn(N, a, I & m, J & m, K & m);
+    }
+    int Im = I - (a == 0), Jm = J - (a == 1), Km = K - (a == 2);
+    if ((Im >> sh) == bi && (Jm >> sh) == bj && (Km >> sh) == bk) return -1;
+    if (!inblocks(t.g, Im >> sh, Jm >> sh, Km >> sh)) return -1;
+    int32_t s = t.slot[blockid(t.g, Im >> sh, Jm >> sh, Km >> sh)];
+    if (s == 0) return -1;
+    return (s - 1) * per + ffn(N, a, a == 0 ? N : (Im & m), a == 1 ? N : (Jm & m), a == 2 ? N : (Km & m));
+}
+__device__ inline int32_t finefaceat(const STopo& t, int a, int I, int J, int K) { return faceat<16>(t, a, I, J, K); }
+// Whether coarse cell (I, J, K) lies in a fine block (sflip.jl:127-128).
+__device__ inline bool covered(const STopo& t, int I, int J, int K) {
+    int bi = I >> 2, bj = J >> 2, bk = K >> 2;
+    return inblocks(t.g, bi, bj, bk) && t.slot[blockid(t.g, bi, bj, bk)] != 0;
+}
+
+// A thread's fine cell n of the pool: slot (1-based), local i, j, k and the block coordinates.
+struct SCell { int s, i, j, k; int4 w; };
+__device__ inline SCell finecell(const STopo& t, int32_t n) {
+    SCell c;
+    c.s = (n >> 12) + 1;
+    int l = n & 4095;
+    c.i = l & 15; c.j = (l >> 4) & 15; c.k = l >> 8;
+    c.w = t.where[c.s - 1];
+    return c;
+}
+// A thread's fine face n along a (sflip.jl:364-377).
+__device__ inline SCell fineface(const STopo& t, int a, int32_t n) {
+    SCell c;
+    c.s = n / SBFACES + 1;
+    int f = n - (c.s - 1) * SBFACES;
+    if (a == 0) { c.i = f % 17; int r = f / 17; c.j = r & 15; c.k = r >> 4; }
+    else if (a == 1) { c.i = f & 15; int r = f >> 4; c.j = r % 17; c.k = r / 17; }
+    else { c.i = f & 15; c.j = (f >> 4) & 15; c.k = f >> 8; }
+    c.w = t.where[c.s - 1];
+    return c;
+}
+
+__device__ inline bool sissolid(const SGrid& g, float px, float py, float pz) { return issolid(g.radius, g.floor, px, py, pz); }
+__device__ inline void finecentre(const SGrid& g, int I, int J, int K, float p[3]) {
+    p[0] = g.lo[0] + ((float)I + 0.5f) * g.dx; p[1] = g.lo[1] + ((float)J + 0.5f) * g.dx; p[2] = g.lo[2] + ((float)K + 0.5f) * g.dx;
+}
+__device__ inline void coarsecentre(const SGrid& g, int I, int J, int K, float p[3]) {
+    float cd = g.cdx();
+    p[0] = g.lo[0] + ((float)I + 0.5f) * cd; p[1] = g.lo[1] + ((float)J + 0.5f) * cd; p[2] = g.lo[2] + ((float)K + 0.5f) * cd;
+}
+__device__ inline void facepos(const SGrid& g, int a, int I, int J, int K, float h, float p[3]) {
+    p[0] = g.lo[0] + ((float)I + (a == 0 ? 0.0f : 0.5f)) * h;
+    p[1] = g.lo[1] + ((float)J + (a == 1 ? 0.0f : 0.5f)) * h;
+    p[2] = g.lo[2] + ((float)K + (a == 2 ? 0.0f : 0.5f)) * h;
+}
+
+// sflip.jl:132-134 lerp8.
+__device__ inline float lerp8(const float c[8], float fx, float fy, float fz) {
+    float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+    return ((c[0] * gx + c[1] * fx) * gy + (c[2] * gx + c[3] * fx) * fy) * gz +
+           ((c[4] * gx + c[5] * fx) * gy + (c[6] * gx + c[7] * fx) * fy) * fz;
+}
+
+// sflip.jl:136-155 cellsample: fine trilinear where all 8 taps are fine, else the coarse grid's.
+template <class V>
+__device__ inline float cellsample(const STopo& t, const V* __restrict__ fine, const V* __restrict__ coarse, float px, float py,
+                                   float pz) {
+    const SGrid& g = t.g;
+    float q0 = __fdiv_rn(px - g.lo[0], g.dx) - 0.5f, q1 = __fdiv_rn(py - g.lo[1], g.dx) - 0.5f,
+          q2 = __fdiv_rn(pz - g.lo[2], g.dx) - 0.5f;
+    float x = floorf(q0), y = floorf(q1), z = floorf(q2);
+    int I = (int)x, J = (int)y, K = (int)z;
+    int32_t a[8];
+    bool all = true;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        a[m] = fineat(t, I + (m & 1), J + ((m >> 1) & 1), K + ((m >> 2) & 1));
+        all = all && a[m] >= 0;
+    }
+    if (all) {
+        float c[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) c[m] = (float)fine[a[m]];
+        return lerp8(c, q0 - x, q1 - y, q2 - z);
+    }
+    float cd = g.cdx();
+    return trilinear(coarse, g.nc(0), g.nc(1), g.nc(2), __fdiv_rn(px - g.lo[0], cd) - 0.5f, __fdiv_rn(py - g.lo[1], cd) - 0.5f,
+                     __fdiv_rn(pz - g.lo[2], cd) - 0.5f);
+}
+
+// sflip.jl:157-175 facesample.
+template <class V>
+__device__ inline float facesample(const STopo& t, const V* __restrict__ fine, const V* __restrict__ coarse, int a, float px,
+                                   float py, float pz) {
+    const SGrid& g = t.g;
+    float o0 = a == 0 ? 0.0f : 0.5f, o1 = a == 1 ? 0.0f : 0.5f, o2 = a == 2 ? 0.0f : 0.5f;
+    float q0 = __fdiv_rn(px - g.lo[0], g.dx) - o0, q1 = __fdiv_rn(py - g.lo[1], g.dx) - o1, q2 = __fdiv_rn(pz - g.lo[2], g.dx) - o2;
+    float x = floorf(q0), y = floorf(q1), z = floorf(q2);
+    int I = (int)x, J = (int)y, K = (int)z;
+    int32_t ix[8];
+    bool all = true;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        ix[m] = finefaceat(t, a, I + (m & 1), J + ((m >> 1) & 1), K + ((m >> 2) & 1));
+        all = all && ix[m] >= 0;
+    }
+    if (all) {
+        float c[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) c[m] = (float)fine[ix[m]];
+        return lerp8(c, q0 - x, q1 - y, q2 - z);
+    }
+    float cd = g.cdx();
+    return trilinear(coarse, g.nc(0) + (a == 0), g.nc(1) + (a == 1), g.nc(2) + (a == 2), __fdiv_rn(px - g.lo[0], cd) - o0,
+                     __fdiv_rn(py - g.lo[1], cd) - o1, __fdiv_rn(pz - g.lo[2], cd) - o2);
+}
+
+// sflip.jl:193-215 finegather: APIC gather over the fine faces only, with the weight found.
+template <class V>
+__device__ inline void finegather(const STopo& t, const V* __restrict__ u, int a, float px, float py, float pz, float& vel,
+                                  float& a0, float& a1, float& a2) {
+    const SGrid& g = t.g;
+    float q0 = ispace(px, g.lo[0], g.dx, a == 0 ? 0.0f : 0.5f);
+    float q1 = ispace(py, g.lo[1], g.dx, a == 1 ? 0.0f : 0.5f);
+    float q2 = ispace(pz, g.lo[2], g.dx, a == 2 ? 0.0f : 0.5f);
+    BSpline bx = bspline(q0), by = bspline(q1), bz = bspline(q2);
+    vel = 0.0f; a0 = 0.0f; a1 = 0.0f; a2 = 0.0f;
+#pragma unroll
+    for (int mz = 0; mz < 3; ++mz) {
+        float wz = pick(bz, mz), dz = (float)(bz.b + mz) - q2;
+#pragma unroll
+        for (int my = 0; my < 3; ++my) {
+            float wy = pick(by, my), dy = (float)(by.b + my) - q1;
+#pragma unroll
+            for (int mx = 0; mx < 3; ++mx) {
+                int32_t idx = finefaceat(t, a, bx.b + mx, by.b + my, bz.b + mz);
+                if (idx >= 0) {
+                    float wgt = pick(bx, mx) * wy * wz;
+                    float tt = wgt * (float)u[idx];
+                    vel += tt;
+                    a0 += (tt * ((float)(bx.b + mx) - q0)) * g.dx;
+                    a1 += (tt * dy) * g.dx;
+                    a2 += (tt * dz) * g.dx;
+                }
+            }
+        }
+    }
+    float c = __fdiv_rn(4.0f, g.dx * g.dx);
+    a0 *= c; a1 *= c; a2 *= c;
+}
+template <class V>
+__device__ inline float finegather_value(const STopo& t, const V* __restrict__ u, int a, float px, float py, float pz) {
+    const SGrid& g = t.g;
+    float q0 = ispace(px, g.lo[0], g.dx, a == 0 ? 0.0f : 0.5f);
+    float q1 = ispace(py, g.lo[1], g.dx, a == 1 ? 0.0f : 0.5f);
+    float q2 = ispace(pz, g.lo[2], g.dx, a == 2 ? 0.0f : 0.5f);
+    BSpline bx = bspline(q0), by = bspline(q1), bz = bspline(q2);
+    float vel = 0.0f;
+#pragma unroll
+    for (int mz = 0; mz < 3; ++mz)
+#pragma unroll
+        for (int my = 0; my < 3; ++my)
+#pragma unroll
+            for (int mx = 0; mx < 3; ++mx) {
+                int32_t idx = finefaceat(t, a, bx.b + mx, by.b + my, bz.b + mz);
+                float wgt = pick(bx, mx) * pick(by, my) * pick(bz, mz);
+                if (idx >= 0) vel += wgt * (float)u[idx];
+            }
+    return vel;
+}
+
+// A field: its fine pool and the dense coarse array (sflip.jl:86-90).
+template <class T> struct SField {
+    DBuf<T> fine, coarse;
+    void alloc(size_t nf, size_t nc) { fine.alloc(nf); coarse.alloc(nc); }
+};
+
+struct SparseOpts {          // the sparse solve's options (sflip.jl:1867, fgmres.jl:26,96)
+    double tol = 5e-4;
+    int maxiter = 240, sweeps = 1, coarse = 40, restart = 8;
+    float boost = 1.5f;
+    int policy = 2;          // 0 flexible CG, 1 FGMRES, 2 CG (at most 80) then FGMRES
+    int precision = 0;       // 0 FP32 hierarchy
+};
+struct SparseStats {
+    int iterations = 0, fcg_iterations = 0, fgmres_iterations = 0, used_fgmres = 0;
+    double rel_residual = 0;
+    double ms = 0;
+};
+
+// Dense levels of the sparse hierarchy (sflip.jl:234-240).
+struct SLevel {
+    int n[3];
+    DBuf<float> x, b, r, t, c[3], dD;
+    DBuf<uint8_t> act;
+    size_t cells() const { return (size_t)n[0] * n[1] * n[2]; }
+    size_t faces(int a) const { return (size_t)(n[0] + (a == 0)) * (n[1] + (a == 1)) * (n[2] + (a == 2)); }
+};
+
+struct SparseBasin {
+    SGrid g;
+    int blocks;               // the pool's capacity in blocks
+    int32_t band, nmin;
+    Stream st;
+    KTimer kt;
+
+    // host side of the topology (sflip.jl:252-290: slots, wheres, free, nslots)
+    std::vector<int32_t> hslot;
+    std::vector<int4> hwhere;
+    std::vector<int32_t> freel;
+    int nslots = 0;
+    DBuf<int32_t> slot;
+    DBuf<int4> where;
+    DBuf<int32_t> list;       // added / removed slots of a refinement
+    DBuf<uint8_t> marks, marks2;
+
+    SField<float> phi, phi0, P;
+    SField<uint8_t> types;
+    SField<float> u[3], u0[3];
+    SField<uint8_t> fl[3];
+    DBuf<int32_t> phip;
+    DBuf<unsigned long long> mom[3], wt[3];
+
+    // pressure (sflip.jl:1059-1904)
+    SField<float> b, r, z, d, q, rold, tmp, dD;
+    DBuf<float> cf[3], cc[3];                 // level-0 face couplings, fine pools and coarse faces
+    DBuf<float> c1[3], dD1;                   // level 1 (8^3 a block)
+    DBuf<uint8_t> act1;
+    SField<float> X1x, X1b, X1r, X1t;         // level-1 vectors (fine 512 a slot, coarse part)
+    std::vector<SLevel> D;                    // D[0] = the coarse grid as level 2, then halvings
+    std::vector<SField<float>> V, Z;          // FGMRES bases, allocated on first use
+    DBuf<double> partials, scal;
+
+    // particles
+    DBuf<float> pbuf[2][PFIELDS];
+    int cur = 0;
+    int64_t n = 0, capacity = 0;
+    DBuf<int32_t> key, rank, seg, counts, totals, offsets, cstart;
+    DBuf<unsigned long long> bsums;
+
+    SparseBasin(const SGrid& grid, int blocks, int64_t capacity, int32_t band, int32_t nmin);
+
+    size_t ncells() const { return (size_t)g.nc(0) * g.nc(1) * g.nc(2); }
+    size_t nfaces(int a) const { return (size_t)(g.nc(0) + (a == 0)) * (g.nc(1) + (a == 1)) * (g.nc(2) + (a == 2)); }
+    size_t finecells() const { return (size_t)nslots * SBCELLS; }
+    size_t finefaces() const { return (size_t)nslots * SBFACES; }
+    size_t nblocks() const { return (size_t)g.nb[0] * g.nb[1] * g.nb[2]; }
+    STopo topo() const { return STopo{g, slot.p, where.p}; }
+    PView view(int which) const {
+        PView v;
+        for (int f = 0; f < PFIELDS; ++f) v.f[f] = pbuf[which][f].p;
+        return v;
+    }
+
+    // topology (sflip.jl:516-576)
+    void marks_compute(float vary, float reach, int rings, bool particles);
+    void refine_host(const uint8_t* want, std::vector<int32_t>& added, std::vector<int32_t>& removed);
+    void refine(float vary, float reach, int rings, int* nadd, int* nrem);
+    void refine_from(const uint8_t* want, int* nadd, int* nrem);
+    void upload_topology();
+    // phases
+    int64_t resample(bool seeding, uint32_t seed);                    // sflip.jl:2227-2246
+    void p2g(float dt);                                               // sflip.jl:808-831
+    void g2p(float dt, float gravity);                                // sflip.jl:2141-2144
+    void project(float gamma, const SparseOpts& o, SparseStats& s);   // sflip.jl:1998-2015
+    // support phases
+    void init_coarse(const float* eta, int neta, float rmax, float G, float core);
+    void init_fine(const float* eta, int neta, float rmax, float G, float core);
+    void classify();
+    void solidify();
+    void redistance(int iterations);
+    void restrict_velocity();
+    void keep();
+    void fill_flags(uint8_t v);
+    double volume();
+    void gravity(float dt, float gacc);
+};
+
+// restriction kernels shared by several phases (sflip.jl:583-612)
+void launch_restrictcells(const SparseBasin& s, float* coarse, const float* fine);
+void launch_restrictfaces_f(const SparseBasin& s, float* coarse, const float* fine, int a);
+void launch_restrictfaces_u8(const SparseBasin& s, uint8_t* coarse, const uint8_t* fine, int a);
+
+}  // namespace mael

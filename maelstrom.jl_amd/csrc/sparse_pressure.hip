@@ -1,0 +1,1083 @@
+// The sparse basin's pressure projection (sflip.jl:1059-2015, fgmres.jl): level-0
+// couplings and right-hand side, the Galerkin hierarchy by aggregation, the composite
+// damped-Jacobi V-cycle (fine blocks at 16^3 and 8^3 over the coarse grid, then the
+// coarse grid and its halvings, the coarsest in one workgroup), flexible
+// Polak-Ribiere CG, restarted FGMRES, the CG-then-FGMRES policy, and the velocity update.
+//
+// Reductions are FP64 partial sums per workgroup summed in a fixed order, so a solve is
+// reproducible run to run. Krylov scalars stay on the device; the host reads one value
+// per CG iteration (the residual norm, with the error flags) and one per Arnoldi step.
+#include "sparse.hpp"
+#include "scan.hpp"
+#include <chrono>
+#include <cmath>
+
+namespace mael {
+
+// ─── level-0 couplings (sflip.jl:1321-1406) ──────────────────────────────────
+
+// sflip.jl:1334-1342 celltype: the fine cell's type and whether it is fine.
+__device__ inline uint8_t celltype(const STopo& t, const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc, int I, int J,
+                                   int K, bool& isfine) {
+    int32_t idx = fineat(t, I, J, K);
+    if (idx >= 0) { isfine = true; return tf[idx]; }
+    isfine = false;
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    int ci = I >> 2, cj = J >> 2, ck = K >> 2;
+    if (ci < 0 || cj < 0 || ck < 0 || ci >= n0 || cj >= n1 || ck >= n2) return SOLID;
+    return tc[cl(ci, cj, ck, n0, n1)];
+}
+// The value at fine cell (I, J, K): a fine block's, else the coarse cell's (clamped).
+template <class V>
+__device__ inline V fineorcoarse(const STopo& t, const V* __restrict__ fine, const V* __restrict__ coarse, int I, int J, int K) {
+    int32_t idx = fineat(t, I, J, K);
+    if (idx >= 0) return fine[idx];
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    return coarse[cl(min(max(I >> 2, 0), n0 - 1), min(max(J >> 2, 0), n1 - 1), min(max(K >> 2, 0), n2 - 1), n0, n1)];
+}
+
+__global__ void k_finecoef(float* __restrict__ c, const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc, STopo t, int a,
+                           int32_t nf) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nf) return;
+    SCell s = fineface(t, a, n);
+    if (s.w.x < 0) return;
+    int I = s.w.x * 16 + s.i, J = s.w.y * 16 + s.j, K = s.w.z * 16 + s.k;
+    bool fp, fm;
+    uint8_t tp = celltype(t, tf, tc, I, J, K, fp), tm = celltype(t, tf, tc, I - (a == 0), J - (a == 1), K - (a == 2), fm);
+    c[n] = (tp == LIQUID && tm == LIQUID) ? (fp && fm ? FINE_FINE : FINE_COARSE) : 0.0f;
+}
+__global__ void k_coarsecoef(float* __restrict__ c, const uint8_t* __restrict__ tc, STopo t, int a) {
+    int nc0 = t.g.nc(0), nc1 = t.g.nc(1), nc2 = t.g.nc(2);
+    int n0 = nc0 + (a == 0), n1 = nc1 + (a == 1), n2 = nc2 + (a == 2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int Im = I - (a == 0), Jm = J - (a == 1), Km = K - (a == 2);
+    bool ok = Im >= 0 && Jm >= 0 && Km >= 0 && I < nc0 && J < nc1 && K < nc2;
+    c[l] = (ok && !covered(t, I, J, K) && !covered(t, Im, Jm, Km) && tc[cl(I, J, K, nc0, nc1)] == LIQUID &&
+            tc[cl(Im, Jm, Km, nc0, nc1)] == LIQUID)
+               ? COARSE_COARSE
+               : 0.0f;
+}
+__global__ void k_finedirichlet(float* __restrict__ dD, const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc,
+                                const float* __restrict__ phi, const float* __restrict__ phic, STopo t, int32_t nf) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nf) return;
+    SCell s = finecell(t, n);
+    if (s.w.x < 0) return;
+    float d = 0.0f;
+    if (tf[n] == LIQUID) {
+        int I = s.w.x * 16 + s.i, J = s.w.y * 16 + s.j, K = s.w.z * 16 + s.k;
+        float pc = phi[n];
+        // neighbour order of flip.jl:84-92
+        const int nb[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+        for (int m = 0; m < 6; ++m) {
+            int a = I + nb[m][0], b = J + nb[m][1], cc = K + nb[m][2];
+            bool isf;
+            if (celltype(t, tf, tc, a, b, cc, isf) == AIR)
+                d += __fdiv_rn(isf ? FINE_FINE : FINE_COARSE, theta(pc, fineorcoarse(t, phi, phic, a, b, cc)));
+        }
+    }
+    dD[n] = d;
+}
+__global__ void k_coarsedirichlet(float* __restrict__ dD, const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc,
+                                  const float* __restrict__ phi, const float* __restrict__ phic, STopo t) {
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    float d = 0.0f;
+    if (tc[l] == LIQUID && !covered(t, I, J, K)) {
+        float pc = phic[l];
+        for (int a = 0; a < 3; ++a)
+            for (int sg = -1; sg <= 1; sg += 2) {
+                int In = I + sg * (a == 0), Jn = J + sg * (a == 1), Kn = K + sg * (a == 2);
+                if (In < 0 || Jn < 0 || Kn < 0 || In >= n0 || Jn >= n1 || Kn >= n2) continue;
+                if (covered(t, In, Jn, Kn)) {
+                    for (int q = 0; q < 4; ++q)
+                        for (int p = 0; p < 4; ++p) {
+                            int FI = a == 0 ? (sg > 0 ? 4 * In : 4 * I - 1) : 4 * I + p;
+                            int FJ = a == 1 ? (sg > 0 ? 4 * Jn : 4 * J - 1) : 4 * J + (a == 0 ? p : q);
+                            int FK = a == 2 ? (sg > 0 ? 4 * Kn : 4 * K - 1) : 4 * K + q;
+                            int32_t idx = fineat(t, FI, FJ, FK);
+                            if (idx >= 0 && tf[idx] == AIR) d += __fdiv_rn(FINE_COARSE, theta(pc, phi[idx]));
+                        }
+                } else if (tc[cl(In, Jn, Kn, n0, n1)] == AIR) {
+                    d += __fdiv_rn(COARSE_COARSE, theta(pc, phic[cl(In, Jn, Kn, n0, n1)]));
+                }
+            }
+    }
+    dD[l] = d;
+}
+__global__ void k_activecoarse(uint8_t* __restrict__ act, const uint8_t* __restrict__ tc, STopo t) {
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    act[l] = (tc[l] == LIQUID && !covered(t, I, J, K)) ? 1 : 0;
+}
+
+// sflip.jl:1408-1435 the right-hand side; sflip.jl:1964-1973 P cleared off the unknowns.
+__global__ void k_finerhs(float* __restrict__ b, float* __restrict__ P, const uint8_t* __restrict__ tf,
+                          const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w, STopo t,
+                          float gamma, int32_t nf) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nf) return;
+    SCell s = finecell(t, n);
+    if (s.w.x < 0) return;
+    float h = t.g.dx;
+    if (tf[n] == LIQUID) {
+        size_t base = (size_t)(s.s - 1) * SBFACES;
+        float div = u[base + ffn(16, 0, s.i + 1, s.j, s.k)] - u[base + ffn(16, 0, s.i, s.j, s.k)] +
+                    v[base + ffn(16, 1, s.i, s.j + 1, s.k)] - v[base + ffn(16, 1, s.i, s.j, s.k)] +
+                    w[base + ffn(16, 2, s.i, s.j, s.k + 1)] - w[base + ffn(16, 2, s.i, s.j, s.k)];
+        b[n] = -h * div + gamma * h * h;
+    } else {
+        b[n] = 0.0f;
+        P[n] = 0.0f;
+    }
+}
+__global__ void k_coarserhs(float* __restrict__ b, float* __restrict__ P, const uint8_t* __restrict__ tc,
+                            const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w, STopo t,
+                            float gamma) {
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    if (tc[l] == LIQUID && !covered(t, I, J, K)) {
+        float h = t.g.dx;
+        float div = u[cl(I + 1, J, K, n0 + 1, n1)] - u[cl(I, J, K, n0 + 1, n1)] + v[cl(I, J + 1, K, n0, n1 + 1)] -
+                    v[cl(I, J, K, n0, n1 + 1)] + w[cl(I, J, K + 1, n0, n1)] - w[cl(I, J, K, n0, n1)];
+        b[l] = -16.0f * h * div + 64.0f * gamma * h * h;
+    } else {
+        b[l] = 0.0f;
+        P[l] = 0.0f;
+    }
+}
+
+// ─── the hierarchy (sflip.jl:1450-1570) ──────────────────────────────────────
+
+__global__ void k_coef1(float* __restrict__ c1, const float* __restrict__ c0, STopo t, int a, int nslots) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nslots * SBFACES1) return;
+    int s = n / SBFACES1 + 1, f = n - (s - 1) * SBFACES1, i, j, k;
+    if (a == 0) { i = f % 9; int r = f / 9; j = r & 7; k = r >> 3; }
+    else if (a == 1) { i = f & 7; int r = f >> 3; j = r % 9; k = r / 9; }
+    else { i = f & 7; j = (f >> 3) & 7; k = f >> 6; }
+    if (t.where[s - 1].x < 0) return;
+    float acc = 0.0f;
+    for (int q = 0; q < 2; ++q)
+        for (int p = 0; p < 2; ++p) {
+            int ii = a == 0 ? 2 * i : 2 * i + p;
+            int jj = a == 1 ? 2 * j : 2 * j + (a == 0 ? p : q);
+            int kk = a == 2 ? 2 * k : 2 * k + q;
+            acc += c0[(size_t)(s - 1) * SBFACES + ffn(16, a, ii, jj, kk)];
+        }
+    c1[n] = acc;
+}
+__global__ void k_cells1(float* __restrict__ dD1, uint8_t* __restrict__ act1, const float* __restrict__ dD0,
+                         const uint8_t* __restrict__ tf, STopo t, int nslots) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nslots * SBCELLS1) return;
+    int s = (n >> 9) + 1, c = n & 511, i = c & 7, j = (c >> 3) & 7, k = c >> 6;
+    if (t.where[s - 1].x < 0) return;
+    float d = 0.0f;
+    uint8_t act = 0;
+    for (int cz = 0; cz < 2; ++cz)
+        for (int by = 0; by < 2; ++by)
+            for (int ax = 0; ax < 2; ++ax) {
+                int32_t m = (s - 1) * SBCELLS + (2 * i + ax) + 16 * ((2 * j + by) + 16 * (2 * k + cz));
+                d += dD0[m];
+                if (tf[m] == LIQUID) act = 1;
+            }
+    dD1[n] = d;
+    act1[n] = act;
+}
+__global__ void k_coef2(float* __restrict__ c2, const float* __restrict__ c0c, const float* __restrict__ c1, STopo t, int a) {
+    int n0 = t.g.nc(0) + (a == 0), n1 = t.g.nc(1) + (a == 1), n2 = t.g.nc(2) + (a == 2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    if (covered(t, I, J, K) || covered(t, I - (a == 0), J - (a == 1), K - (a == 2))) {
+        float acc = 0.0f;
+        for (int q = 0; q < 2; ++q)
+            for (int p = 0; p < 2; ++p) {
+                int FI = a == 0 ? 2 * I : 2 * I + p;
+                int FJ = a == 1 ? 2 * J : 2 * J + (a == 0 ? p : q);
+                int FK = a == 2 ? 2 * K : 2 * K + q;
+                int32_t idx = faceat<8>(t, a, FI, FJ, FK);
+                if (idx >= 0) acc += c1[idx];
+            }
+        c2[l] = acc;
+    } else {
+        c2[l] = c0c[l];
+    }
+}
+__global__ void k_cells2(float* __restrict__ dD2, uint8_t* __restrict__ act, const float* __restrict__ dD0c,
+                         const float* __restrict__ dD1, const uint8_t* __restrict__ act1, STopo t) {
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    if (covered(t, I, J, K)) {
+        float d = 0.0f;
+        uint8_t ac = 0;
+        for (int c = 0; c < 2; ++c)
+            for (int b = 0; b < 2; ++b)
+                for (int a = 0; a < 2; ++a) {
+                    int32_t m = cellat<8>(t, 2 * I + a, 2 * J + b, 2 * K + c);
+                    d += dD1[m];
+                    if (act1[m]) ac = 1;
+                }
+        dD2[l] = d;
+        act[l] = ac;
+    } else {
+        dD2[l] = dD0c[l];
+    }
+}
+__global__ void k_coefdense(float* __restrict__ cc, const float* __restrict__ cf, int3 nc, int3 nf, int a) {
+    int n0 = nc.x + (a == 0), n1 = nc.y + (a == 1), n2 = nc.z + (a == 2);
+    int m0 = nf.x + (a == 0), m1 = nf.y + (a == 1);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    float acc = 0.0f;
+    for (int q = 0; q < 2; ++q)
+        for (int p = 0; p < 2; ++p) {
+            int FI = a == 0 ? 2 * I : 2 * I + p;
+            int FJ = a == 1 ? 2 * J : 2 * J + (a == 0 ? p : q);
+            int FK = a == 2 ? 2 * K : 2 * K + q;
+            acc += cf[cl(FI, FJ, FK, m0, m1)];
+        }
+    cc[l] = acc;
+}
+__global__ void k_cellsdense(float* __restrict__ dDc, uint8_t* __restrict__ actc, const float* __restrict__ dDf,
+                             const uint8_t* __restrict__ actf, int3 nc, int3 nf) {
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)nc.x * nc.y * nc.z) return;
+    int I = (int)(l % nc.x), J = (int)((l / nc.x) % nc.y), K = (int)(l / ((size_t)nc.x * nc.y));
+    float d = 0.0f;
+    uint8_t ac = 0;
+    for (int c = 0; c < 2; ++c)
+        for (int b = 0; b < 2; ++b)
+            for (int a = 0; a < 2; ++a) {
+                size_t m = cl(2 * I + a, 2 * J + b, 2 * K + c, nf.x, nf.y);
+                d += dDf[m];
+                if (actf[m]) ac = 1;
+            }
+    dDc[l] = d;
+    actc[l] = ac;
+}
+
+// ─── the composite rows (sflip.jl:1139-1216) ─────────────────────────────────
+
+// A level's fine cell (N = 16 or 8): diag and the neighbours' sum, A x = diag x - off.
+template <int N>
+__device__ inline void finerow(const STopo& t, const float* __restrict__ x, const float* __restrict__ xc,
+                               const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
+                               const float* __restrict__ dD, int s, int i, int j, int k, int4 w, float& diag, float& off) {
+    constexpr int sh = N == 16 ? 2 : 1, per = (N + 1) * N * N;
+    int I = w.x * N + i, J = w.y * N + j, K = w.z * N + k;
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    diag = dD[(s - 1) * (N * N * N) + i + N * (j + N * k)];
+    off = 0.0f;
+    auto nv = [&](int In, int Jn, int Kn) -> float {
+        int32_t idx = cellat<N>(t, In, Jn, Kn);
+        if (idx >= 0) return x[idx];
+        return xc[cl(min(max(In >> sh, 0), n0 - 1), min(max(Jn >> sh, 0), n1 - 1), min(max(Kn >> sh, 0), n2 - 1), n0, n1)];
+    };
+    size_t base = (size_t)(s - 1) * per;
+    float c;
+    c = cx[base + ffn(N, 0, i, j, k)];     if (c > 0.0f) { diag += c; off += c * nv(I - 1, J, K); }
+    c = cx[base + ffn(N, 0, i + 1, j, k)]; if (c > 0.0f) { diag += c; off += c * nv(I + 1, J, K); }
+    c = cy[base + ffn(N, 1, i, j, k)];     if (c > 0.0f) { diag += c; off += c * nv(I, J - 1, K); }
+    c = cy[base + ffn(N, 1, i, j + 1, k)]; if (c > 0.0f) { diag += c; off += c * nv(I, J + 1, K); }
+    c = cz[base + ffn(N, 2, i, j, k)];     if (c > 0.0f) { diag += c; off += c * nv(I, J, K - 1); }
+    c = cz[base + ffn(N, 2, i, j, k + 1)]; if (c > 0.0f) { diag += c; off += c * nv(I, J, K + 1); }
+}
+
+struct CoarseCoef { const float* c[3]; };
+struct FineCoef { const float* c[3]; };
+
+// An uncovered coarse cell: coarse faces, or a covered neighbour's (N/4)^2 sub-faces.
+template <int N>
+__device__ inline void coarserow(const STopo& t, const float* __restrict__ xf, const FineCoef& cf, const float* __restrict__ xc,
+                                 const CoarseCoef& ccf, const float* __restrict__ dDc, int I, int J, int K, float& diag,
+                                 float& off) {
+    constexpr int R = N / 4;
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    diag = dDc[cl(I, J, K, n0, n1)];
+    off = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        int f0 = n0 + (a == 0), f1 = n1 + (a == 1);
+        for (int sg = -1; sg <= 1; sg += 2) {
+            int In = I + sg * (a == 0), Jn = J + sg * (a == 1), Kn = K + sg * (a == 2);
+            if (In < 0 || Jn < 0 || Kn < 0 || In >= n0 || Jn >= n1 || Kn >= n2) continue;
+            if (covered(t, In, Jn, Kn)) {
+                for (int q = 0; q < R; ++q)
+                    for (int p = 0; p < R; ++p) {
+                        int FI = a == 0 ? (sg > 0 ? R * In : R * I - 1) : R * I + p;
+                        int FJ = a == 1 ? (sg > 0 ? R * Jn : R * J - 1) : R * J + (a == 0 ? p : q);
+                        int FK = a == 2 ? (sg > 0 ? R * Kn : R * K - 1) : R * K + q;
+                        int PI = (a == 0 && sg < 0) ? FI + 1 : FI;
+                        int PJ = (a == 1 && sg < 0) ? FJ + 1 : FJ;
+                        int PK = (a == 2 && sg < 0) ? FK + 1 : FK;
+                        int32_t fidx = faceat<N>(t, a, PI, PJ, PK), cidx = cellat<N>(t, FI, FJ, FK);
+                        if (fidx >= 0 && cidx >= 0) {
+                            float c = cf.c[a][fidx];
+                            diag += c;
+                            off += c * xf[cidx];
+                        }
+                    }
+            } else {
+                float c = sg < 0 ? ccf.c[a][cl(I, J, K, f0, f1)] : ccf.c[a][cl(I + (a == 0), J + (a == 1), K + (a == 2), f0, f1)];
+                if (c > 0.0f) { diag += c; off += c * xc[cl(In, Jn, Kn, n0, n1)]; }
+            }
+        }
+    }
+}
+
+// Level views: fine couplings and diagonal at N, the coarse grid's level-0 couplings.
+struct Comp {
+    FineCoef cf;
+    const float* dDf;
+    const uint8_t* actf;
+    CoarseCoef cc;
+    const float* dDc;
+    const uint8_t* actc;
+};
+
+// sflip.jl:1218-1269: mode 0 Jacobi (out = x + w (b - A x)/diag), 1 residual (out = b - A x),
+// 2 apply (out = A x), on the fine cells of a level; inactive cells get 0.
+template <int N, int MODE>
+__global__ __launch_bounds__(256) void k_rowfine(float* __restrict__ out, const float* __restrict__ x, const float* __restrict__ b,
+                                                 const float* __restrict__ xc, Comp L, STopo t, int32_t nf) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nf) return;
+    constexpr int lg = N == 16 ? 12 : 9, msk = N * N * N - 1, l2 = N == 16 ? 4 : 3;
+    int s = (n >> lg) + 1, c = n & msk;
+    int4 w = t.where[s - 1];
+    if (w.x < 0) return;
+    bool act = MODE == 2 ? L.actf[n] == LIQUID : L.actf[n] != 0;
+    float res = 0.0f;
+    if (act) {
+        float diag, off;
+        finerow<N>(t, x, xc, L.cf.c[0], L.cf.c[1], L.cf.c[2], L.dDf, s, c & (N - 1), (c >> l2) & (N - 1), c >> (2 * l2), w, diag,
+                   off);
+        float ax = diag * x[n] - off;
+        if (MODE == 0) res = diag > 0.0f ? x[n] + __fdiv_rn(OMEGA * (b[n] - ax), diag) : 0.0f;
+        else if (MODE == 1) res = b[n] - ax;
+        else res = ax;
+    }
+    out[n] = res;
+}
+template <int N, int MODE>
+__global__ __launch_bounds__(256) void k_rowcoarse(float* __restrict__ out, const float* __restrict__ x,
+                                                   const float* __restrict__ b, const float* __restrict__ xf, Comp L, STopo t) {
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    float res = 0.0f;
+    if (L.actc[l] != 0 && !covered(t, I, J, K)) {
+        float diag, off;
+        coarserow<N>(t, xf, L.cf, x, L.cc, L.dDc, I, J, K, diag, off);
+        float ax = diag * x[l] - off;
+        if (MODE == 0) res = diag > 0.0f ? x[l] + __fdiv_rn(OMEGA * (b[l] - ax), diag) : 0.0f;
+        else if (MODE == 1) res = b[l] - ax;
+        else res = ax;
+    }
+    out[l] = res;
+}
+
+// ─── dense levels (sflip.jl:1574-1657) ───────────────────────────────────────
+
+struct DView {
+    int3 n;
+    const float* c[3];
+    const float* dD;
+    const uint8_t* act;
+};
+__device__ inline void denserow(const DView& L, const float* __restrict__ x, int i, int j, int k, float& diag, float& off) {
+    int n0 = L.n.x, n1 = L.n.y, n2 = L.n.z;
+    diag = L.dD[cl(i, j, k, n0, n1)];
+    off = 0.0f;
+    float c;
+    c = L.c[0][cl(i, j, k, n0 + 1, n1)];     if (c > 0.0f && i > 0) { diag += c; off += c * x[cl(i - 1, j, k, n0, n1)]; }
+    c = L.c[0][cl(i + 1, j, k, n0 + 1, n1)]; if (c > 0.0f && i < n0 - 1) { diag += c; off += c * x[cl(i + 1, j, k, n0, n1)]; }
+    c = L.c[1][cl(i, j, k, n0, n1 + 1)];     if (c > 0.0f && j > 0) { diag += c; off += c * x[cl(i, j - 1, k, n0, n1)]; }
+    c = L.c[1][cl(i, j + 1, k, n0, n1 + 1)]; if (c > 0.0f && j < n1 - 1) { diag += c; off += c * x[cl(i, j + 1, k, n0, n1)]; }
+    c = L.c[2][cl(i, j, k, n0, n1)];         if (c > 0.0f && k > 0) { diag += c; off += c * x[cl(i, j, k - 1, n0, n1)]; }
+    c = L.c[2][cl(i, j, k + 1, n0, n1)];     if (c > 0.0f && k < n2 - 1) { diag += c; off += c * x[cl(i, j, k + 1, n0, n1)]; }
+}
+template <int MODE>
+__global__ __launch_bounds__(256) void k_rowdense(float* __restrict__ out, const float* __restrict__ x, const float* __restrict__ b,
+                                                  DView L) {
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)L.n.x * L.n.y * L.n.z) return;
+    int i = (int)(l % L.n.x), j = (int)((l / L.n.x) % L.n.y), k = (int)(l / ((size_t)L.n.x * L.n.y));
+    float res = 0.0f;
+    if (L.act[l] != 0) {
+        float diag, off;
+        denserow(L, x, i, j, k, diag, off);
+        float ax = diag * x[l] - off;
+        res = MODE == 0 ? (diag > 0.0f ? x[l] + __fdiv_rn(OMEGA * (b[l] - ax), diag) : 0.0f) : b[l] - ax;
+    }
+    out[l] = res;
+}
+__global__ void k_restrictdense(float* __restrict__ bc, const float* __restrict__ rf, int3 nc, int3 nf) {
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)nc.x * nc.y * nc.z) return;
+    int I = (int)(l % nc.x), J = (int)((l / nc.x) % nc.y), K = (int)(l / ((size_t)nc.x * nc.y));
+    float acc = 0.0f;
+    for (int c = 0; c < 2; ++c)
+        for (int b = 0; b < 2; ++b)
+            for (int a = 0; a < 2; ++a) acc += rf[cl(2 * I + a, 2 * J + b, 2 * K + c, nf.x, nf.y)];
+    bc[l] = acc;
+}
+__global__ void k_prolongdense(float* __restrict__ xf, const float* __restrict__ xc, const uint8_t* __restrict__ actf, int3 nf,
+                               int3 nc, float boost) {
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)nf.x * nf.y * nf.z) return;
+    int i = (int)(l % nf.x), j = (int)((l / nf.x) % nf.y), k = (int)(l / ((size_t)nf.x * nf.y));
+    if (actf[l] != 0) xf[l] += boost * xc[cl(i >> 1, j >> 1, k >> 1, nc.x, nc.y)];
+}
+// sflip.jl:1623-1657 coarsest_kernel!: red-black Gauss-Seidel from zero in one workgroup.
+__global__ __launch_bounds__(1024) void k_scoarsest(float* __restrict__ x, const float* __restrict__ b, DView L, int sweeps) {
+    __shared__ float xs[1024];
+    int t = threadIdx.x;
+    int n0 = L.n.x, n1 = L.n.y, n2 = L.n.z, N = n0 * n1 * n2;
+    bool inside = t < N;
+    int i = t % n0, j = (t / n0) % n1, k = t / (n0 * n1);
+    bool act = inside && L.act[t] != 0;
+    float bi = inside ? b[t] : 0.0f;
+    float dg = 0.0f, cw[6] = {0, 0, 0, 0, 0, 0};
+    int nb[6] = {0, 0, 0, 0, 0, 0};
+    if (act) {
+        dg = L.dD[t];
+        float c;
+        c = L.c[0][cl(i, j, k, n0 + 1, n1)];     if (c > 0.0f && i > 0) { cw[0] = c; nb[0] = t - 1; }
+        c = L.c[0][cl(i + 1, j, k, n0 + 1, n1)]; if (c > 0.0f && i < n0 - 1) { cw[1] = c; nb[1] = t + 1; }
+        c = L.c[1][cl(i, j, k, n0, n1 + 1)];     if (c > 0.0f && j > 0) { cw[2] = c; nb[2] = t - n0; }
+        c = L.c[1][cl(i, j + 1, k, n0, n1 + 1)]; if (c > 0.0f && j < n1 - 1) { cw[3] = c; nb[3] = t + n0; }
+        c = L.c[2][cl(i, j, k, n0, n1)];         if (c > 0.0f && k > 0) { cw[4] = c; nb[4] = t - n0 * n1; }
+        c = L.c[2][cl(i, j, k + 1, n0, n1)];     if (c > 0.0f && k < n2 - 1) { cw[5] = c; nb[5] = t + n0 * n1; }
+    }
+    xs[t] = 0.0f;
+    __syncthreads();
+    int par = (i + j + k + 1) & 1;
+    for (int s = 1; s <= 2 * sweeps; ++s)
+        for (int half = 0; half < 2; ++half) {
+            int colour = s <= sweeps ? half : 1 - half;
+            if (act && par == colour) {
+                float diag = dg, acc = bi;
+#pragma unroll
+                for (int m = 0; m < 6; ++m)
+                    if (cw[m] > 0.0f) { diag += cw[m]; acc += cw[m] * xs[nb[m]]; }
+                xs[t] = diag > 0.0f ? __fdiv_rn(acc, diag) : 0.0f;
+            }
+            __syncthreads();
+        }
+    if (inside) x[t] = xs[t];
+}
+
+// ─── between the composite levels (sflip.jl:1661-1705) ───────────────────────
+
+__global__ void k_restrict01(float* __restrict__ b1, const float* __restrict__ r0, STopo t, int nslots) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nslots * SBCELLS1) return;
+    int s = (n >> 9) + 1, c = n & 511, i = c & 7, j = (c >> 3) & 7, k = c >> 6;
+    if (t.where[s - 1].x < 0) return;
+    float acc = 0.0f;
+    for (int cz = 0; cz < 2; ++cz)
+        for (int by = 0; by < 2; ++by)
+            for (int ax = 0; ax < 2; ++ax)
+                acc += r0[(s - 1) * SBCELLS + (2 * i + ax) + 16 * ((2 * j + by) + 16 * (2 * k + cz))];
+    b1[n] = acc;
+}
+__global__ void k_restrict12(float* __restrict__ b2, const float* __restrict__ r1f, const float* __restrict__ r1c, STopo t) {
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    if (covered(t, I, J, K)) {
+        float acc = 0.0f;
+        for (int c = 0; c < 2; ++c)
+            for (int b = 0; b < 2; ++b)
+                for (int a = 0; a < 2; ++a) acc += r1f[cellat<8>(t, 2 * I + a, 2 * J + b, 2 * K + c)];
+        b2[l] = acc;
+    } else {
+        b2[l] = r1c[l];
+    }
+}
+__global__ void k_prolong21(float* __restrict__ x1f, const float* __restrict__ x2, const uint8_t* __restrict__ act1, STopo t,
+                            float boost, int nslots) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nslots * SBCELLS1) return;
+    int s = (n >> 9) + 1, c = n & 511, i = c & 7, j = (c >> 3) & 7, k = c >> 6;
+    int4 w = t.where[s - 1];
+    if (w.x < 0 || act1[n] == 0) return;
+    x1f[n] += boost * x2[cl((w.x * 8 + i) >> 1, (w.y * 8 + j) >> 1, (w.z * 8 + k) >> 1, t.g.nc(0), t.g.nc(1))];
+}
+__global__ void k_prolongcoarse(float* __restrict__ xc, const float* __restrict__ from, const uint8_t* __restrict__ act, STopo t,
+                                float boost) {
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    if (act[l] != 0 && !covered(t, I, J, K)) xc[l] += boost * from[l];
+}
+__global__ void k_prolong10(float* __restrict__ x0f, const float* __restrict__ x1f, const uint8_t* __restrict__ tf, STopo t,
+                            float boost, int32_t nf) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nf) return;
+    SCell c = finecell(t, n);
+    if (c.w.x < 0 || tf[n] != LIQUID) return;
+    x0f[n] += boost * x1f[(c.s - 1) * SBCELLS1 + (c.i >> 1) + 8 * ((c.j >> 1) + 8 * (c.k >> 1))];
+}
+
+// ─── vectors: dot products over the unknowns, axpy and friends (sflip.jl:1766-1822) ──
+
+enum { SS_BB = 0, SS_RR, SS_RHO, SS_DEN, SS_ALPHA, SS_RHONEW, SS_ZROLD, SS_BETA, SS_RNORM, SS_FLAG, SS_TMP, SS_H0 = 16 };
+// error flags (the reference's messages, sflip.jl:1880-1900)
+enum { EF_RHO_NONFINITE = 1, EF_RHO_NONPOS = 2, EF_DEN_NONPOS = 4, EF_ALPHA_NONFINITE = 8, EF_RHONEW_NONPOS = 16,
+       EF_BETA_NONFINITE = 32 };
+
+constexpr int DOT_BLOCKS = 1024;
+
+__device__ inline double block_sum_red(double v, double* lds) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += lds[i];
+    return s;
+}
+
+// sflip.jl:1772-1805 inner: LIQUID fine cells of live slots, LIQUID uncovered coarse cells.
+__global__ __launch_bounds__(256) void k_sdot(double* __restrict__ partials, const float* __restrict__ af,
+                                              const float* __restrict__ bf, const float* __restrict__ ac,
+                                              const float* __restrict__ bc, const uint8_t* __restrict__ tf,
+                                              const uint8_t* __restrict__ tc, STopo t, int32_t nf) {
+    __shared__ double lds[4];
+    int n0 = t.g.nc(0), n1 = t.g.nc(1);
+    size_t Cc = (size_t)n0 * n1 * t.g.nc(2);
+    double v = 0;
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < (size_t)nf + Cc; l += (size_t)gridDim.x * blockDim.x) {
+        if (l < (size_t)nf) {
+            if (tf[l] == LIQUID && t.where[l >> 12].x >= 0) v += (double)af[l] * (double)bf[l];
+        } else {
+            size_t m = l - nf;
+            int I = (int)(m % n0), J = (int)((m / n0) % n1), K = (int)(m / ((size_t)n0 * n1));
+            if (tc[m] == LIQUID && !covered(t, I, J, K)) v += (double)ac[m] * (double)bc[m];
+        }
+    }
+    double s = block_sum_red(v, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+enum { SF_SET = 0, SF_NORM, SF_RHO0, SF_ALPHA, SF_BETA };
+__global__ __launch_bounds__(256) void k_sfinish(const double* __restrict__ partials, int np, double* __restrict__ scal, int op,
+                                                 int slot) {
+    __shared__ double lds[4];
+    double v = 0;
+    for (int l = threadIdx.x; l < np; l += blockDim.x) v += partials[l];
+    double s = block_sum_red(v, lds);
+    if (threadIdx.x != 0) return;
+    long long flags = (long long)scal[SS_FLAG];
+    switch (op) {
+        case SF_SET: scal[slot] = s; break;
+        case SF_NORM: scal[slot] = sqrt(s); break;
+        case SF_RHO0:   // rho = z.r, with the reference's checks (sflip.jl:1880-1882)
+            scal[SS_RHO] = s;
+            if (!isfinite(s)) flags |= EF_RHO_NONFINITE;
+            else if (!(s > 0)) flags |= EF_RHO_NONPOS;
+            break;
+        case SF_ALPHA: {   // den = d.q, alpha = rho / den (sflip.jl:1885-1888)
+            scal[SS_DEN] = s;
+            if (!(s > 0)) flags |= EF_DEN_NONPOS;
+            double al = (double)((float)scal[SS_RHO] / (float)s);
+            if (!isfinite(al)) flags |= EF_ALPHA_NONFINITE;
+            scal[SS_ALPHA] = al;
+            break;
+        }
+        case SF_BETA: {   // z.rold; beta = max(0, (rho_new - z.rold) / rho) (sflip.jl:1895-1900)
+            double rn = scal[SS_RHONEW];
+            if (!(rn > 0)) flags |= EF_RHONEW_NONPOS;
+            double be = (double)fmaxf(0.0f, ((float)rn - (float)s) / (float)scal[SS_RHO]);
+            if (!isfinite(be)) flags |= EF_BETA_NONFINITE;
+            scal[SS_BETA] = be;
+            scal[SS_RHO] = rn;
+            break;
+        }
+    }
+    scal[SS_FLAG] = (double)flags;
+}
+
+// y = x + coef*y (XPBY) or y += coef*x (AXPY) over n values; coef = mult*scal[slot] (slot >= 0) or mult.
+template <int XPBY>
+__global__ void k_saxpy(float* __restrict__ y, const float* __restrict__ x, size_t n, const double* __restrict__ scal, int slot,
+                        float mult) {
+    float c = slot >= 0 ? mult * (float)scal[slot] : mult;
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (size_t)gridDim.x * blockDim.x)
+        y[l] = XPBY ? x[l] + c * y[l] : y[l] + c * x[l];
+}
+
+// ─── projection (sflip.jl:1906-1962) ─────────────────────────────────────────
+
+__global__ void k_projectfine(float* __restrict__ u, uint8_t* __restrict__ valid, const float* __restrict__ P,
+                              const float* __restrict__ Pc, const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc,
+                              const float* __restrict__ phi, const float* __restrict__ phic, STopo t, int a, int32_t nfa) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nfa) return;
+    SCell s = fineface(t, a, n);
+    if (s.w.x < 0) return;
+    int I = s.w.x * 16 + s.i, J = s.w.y * 16 + s.j, K = s.w.z * 16 + s.k;
+    int Im = I - (a == 0), Jm = J - (a == 1), Km = K - (a == 2);
+    bool fp, fm;
+    uint8_t tp = celltype(t, tf, tc, I, J, K, fp), tm = celltype(t, tf, tc, Im, Jm, Km, fm);
+    bool ok = tp != SOLID && tm != SOLID && (tp == LIQUID || tm == LIQUID);
+    if (ok) {
+        float d = (fp && fm ? 1.0f : 2.5f) * t.g.dx;
+        float Pp = fineorcoarse(t, P, Pc, I, J, K), Pm = fineorcoarse(t, P, Pc, Im, Jm, Km);
+        float Dl;
+        if (tp == LIQUID && tm == LIQUID) Dl = Pp - Pm;
+        else if (tm == LIQUID) Dl = __fdiv_rn(-Pm, theta(fineorcoarse(t, phi, phic, Im, Jm, Km), fineorcoarse(t, phi, phic, I, J, K)));
+        else Dl = __fdiv_rn(Pp, theta(fineorcoarse(t, phi, phic, I, J, K), fineorcoarse(t, phi, phic, Im, Jm, Km)));
+        u[n] -= __fdiv_rn(Dl, d);
+    }
+    valid[n] = ok ? 1 : 0;
+}
+__global__ void k_projectcoarse(float* __restrict__ u, uint8_t* __restrict__ valid, const float* __restrict__ Pc,
+                                const uint8_t* __restrict__ tc, const float* __restrict__ phic, STopo t, int a) {
+    int nc0 = t.g.nc(0), nc1 = t.g.nc(1), nc2 = t.g.nc(2);
+    int n0 = nc0 + (a == 0), n1 = nc1 + (a == 1), n2 = nc2 + (a == 2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int Im = I - (a == 0), Jm = J - (a == 1), Km = K - (a == 2);
+    bool inside = Im >= 0 && Jm >= 0 && Km >= 0 && I < nc0 && J < nc1 && K < nc2;
+    if (inside && !covered(t, I, J, K) && !covered(t, Im, Jm, Km)) {
+        size_t cp = cl(I, J, K, nc0, nc1), cm = cl(Im, Jm, Km, nc0, nc1);
+        uint8_t tp = tc[cp], tm = tc[cm];
+        bool ok = tp != SOLID && tm != SOLID && (tp == LIQUID || tm == LIQUID);
+        if (ok) {
+            float Pp = Pc[cp], Pm = Pc[cm], fp = phic[cp], fm = phic[cm];
+            float Dl = (tp == LIQUID && tm == LIQUID) ? Pp - Pm
+                                                      : (tm == LIQUID ? __fdiv_rn(-Pm, theta(fm, fp)) : __fdiv_rn(Pp, theta(fp, fm)));
+            u[l] -= __fdiv_rn(Dl, t.g.cdx());
+        }
+        valid[l] = ok ? 1 : 0;
+    } else if (!inside) {
+        valid[l] = 0;
+    }
+}
+
+// ─── the solver ──────────────────────────────────────────────────────────────
+
+struct SVec {   // a pressure-sized vector: fine pool part and coarse part
+    float* f;
+    float* c;
+};
+
+struct SparseSolver {
+    SparseBasin& s;
+    hipStream_t st;
+    STopo t;
+    int32_t nf;        // fine cells in use (nslots * 4096)
+    size_t Cc;
+
+    explicit SparseSolver(SparseBasin& b) : s(b), st(b.st.s), t(b.topo()), nf((int32_t)b.finecells()), Cc(b.ncells()) {}
+
+    SVec vec(SField<float>& f) { return SVec{f.fine.p, f.coarse.p}; }
+    Comp level0() {
+        return Comp{FineCoef{{s.cf[0].p, s.cf[1].p, s.cf[2].p}}, s.dD.fine.p, s.types.fine.p,
+                    CoarseCoef{{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p};
+    }
+    Comp level1() {
+        return Comp{FineCoef{{s.c1[0].p, s.c1[1].p, s.c1[2].p}}, s.dD1.p, s.act1.p,
+                    CoarseCoef{{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p};
+    }
+    DView dview(int l) {
+        SLevel& L = s.D[l];
+        return DView{make_int3(L.n[0], L.n[1], L.n[2]), {L.c[0].p, L.c[1].p, L.c[2].p}, L.dD.p, L.act.p};
+    }
+    static int3 i3(const int n[3]) { return make_int3(n[0], n[1], n[2]); }
+
+    void coefficients() {
+        for (int a = 0; a < 3; ++a) {
+            if (s.nslots)
+                hipLaunchKernelGGL(k_finecoef, dim3(cdiv(s.finefaces(), 256)), dim3(256), 0, st, s.cf[a].p, s.types.fine.p,
+                                   s.types.coarse.p, t, a, (int32_t)s.finefaces());
+            hipLaunchKernelGGL(k_coarsecoef, dim3(cdiv(s.nfaces(a), 256)), dim3(256), 0, st, s.cc[a].p, s.types.coarse.p, t, a);
+        }
+        if (nf)
+            hipLaunchKernelGGL(k_finedirichlet, dim3(cdiv(nf, 256)), dim3(256), 0, st, s.dD.fine.p, s.types.fine.p,
+                               s.types.coarse.p, s.phi.fine.p, s.phi.coarse.p, t, nf);
+        hipLaunchKernelGGL(k_coarsedirichlet, dim3(cdiv(Cc, 256)), dim3(256), 0, st, s.dD.coarse.p, s.types.fine.p,
+                           s.types.coarse.p, s.phi.fine.p, s.phi.coarse.p, t);
+        hipLaunchKernelGGL(k_activecoarse, dim3(cdiv(Cc, 256)), dim3(256), 0, st, s.D[0].act.p, s.types.coarse.p, t);
+        MAEL_LAUNCH_CHECK();
+    }
+    void hierarchy() {
+        int ns = s.nslots;
+        for (int a = 0; a < 3; ++a)
+            if (ns)
+                hipLaunchKernelGGL(k_coef1, dim3(cdiv(ns * SBFACES1, 256)), dim3(256), 0, st, s.c1[a].p, s.cf[a].p, t, a, ns);
+        if (ns)
+            hipLaunchKernelGGL(k_cells1, dim3(cdiv(ns * SBCELLS1, 256)), dim3(256), 0, st, s.dD1.p, s.act1.p, s.dD.fine.p,
+                               s.types.fine.p, t, ns);
+        SLevel& D2 = s.D[0];
+        for (int a = 0; a < 3; ++a)
+            hipLaunchKernelGGL(k_coef2, dim3(cdiv(D2.faces(a), 256)), dim3(256), 0, st, D2.c[a].p, s.cc[a].p, s.c1[a].p, t, a);
+        hipLaunchKernelGGL(k_cells2, dim3(cdiv(Cc, 256)), dim3(256), 0, st, D2.dD.p, D2.act.p, s.dD.coarse.p, s.dD1.p, s.act1.p,
+                           t);
+        for (size_t l = 1; l < s.D.size(); ++l) {
+            SLevel &F = s.D[l - 1], &C = s.D[l];
+            for (int a = 0; a < 3; ++a)
+                hipLaunchKernelGGL(k_coefdense, dim3(cdiv(C.faces(a), 256)), dim3(256), 0, st, C.c[a].p, F.c[a].p, i3(C.n),
+                                   i3(F.n), a);
+            hipLaunchKernelGGL(k_cellsdense, dim3(cdiv(C.cells(), 256)), dim3(256), 0, st, C.dD.p, C.act.p, F.dD.p, F.act.p,
+                               i3(C.n), i3(F.n));
+        }
+        MAEL_LAUNCH_CHECK();
+    }
+
+    // One composite-level pass (MODE 0 Jacobi, 1 residual, 2 apply) over the fine part
+    // (N^3 a slot) and the coarse part.
+    template <int N, int MODE> void rows(SVec out, SVec x, SVec b, const Comp& L) {
+        int32_t nfl = s.nslots * N * N * N;
+        if (nfl) {
+            if (N == 16 && MODE == 0) s.kt.begin(K_RBGS, st);
+            hipLaunchKernelGGL((k_rowfine<N, MODE>), dim3(cdiv(nfl, 256)), dim3(256), 0, st, out.f, x.f, b.f, x.c, L, t, nfl);
+            if (N == 16 && MODE == 0) s.kt.end(K_RBGS, st);
+        }
+        hipLaunchKernelGGL((k_rowcoarse<N, MODE>), dim3(cdiv(Cc, 256)), dim3(256), 0, st, out.c, x.c, b.c, x.f, L, t);
+    }
+    template <int N> void smooth(SVec x, SVec b, SVec tmp, const Comp& L, int sweeps) {
+        for (int k = 0; k < sweeps; ++k) {
+            rows<N, 0>(tmp, x, b, L);
+            rows<N, 0>(x, tmp, b, L);
+        }
+    }
+    void zero(SVec v, size_t nfine) {
+        fill_async(v.f, nfine, 0.0f, st);
+        fill_async(v.c, Cc, 0.0f, st);
+    }
+
+    // sflip.jl:1722-1762 vcycle!: z = M^-1 r.
+    void vcycle(SVec z, SVec r, int sweeps, int coarse, float boost) {
+        Comp L0 = level0(), L1 = level1();
+        int ns = s.nslots;
+        SVec q = vec(s.q), tmp = vec(s.tmp);
+        SVec X1x = vec(s.X1x), X1b = vec(s.X1b), X1r = vec(s.X1r), X1t = vec(s.X1t);
+        zero(z, (size_t)nf);
+        smooth<16>(z, r, tmp, L0, sweeps);
+        rows<16, 1>(q, z, r, L0);
+        if (ns) hipLaunchKernelGGL(k_restrict01, dim3(cdiv(ns * SBCELLS1, 256)), dim3(256), 0, st, X1b.f, q.f, t, ns);
+        MAEL_HIP(hipMemcpyAsync(X1b.c, q.c, Cc * sizeof(float), hipMemcpyDeviceToDevice, st));
+        zero(X1x, (size_t)ns * SBCELLS1);
+        smooth<8>(X1x, X1b, X1t, L1, sweeps);
+        rows<8, 1>(X1r, X1x, X1b, L1);
+        SLevel& D2 = s.D[0];
+        hipLaunchKernelGGL(k_restrict12, dim3(cdiv(Cc, 256)), dim3(256), 0, st, D2.b.p, X1r.f, X1r.c, t);
+        int nl = (int)s.D.size();
+        for (int l = 0; l + 1 < nl; ++l) {
+            SLevel& Dl = s.D[l];
+            DView V = dview(l);
+            size_t c = Dl.cells();
+            fill_async(Dl.x.p, c, 0.0f, st);
+            for (int k = 0; k < sweeps; ++k) {
+                hipLaunchKernelGGL(k_rowdense<0>, dim3(cdiv(c, 256)), dim3(256), 0, st, Dl.t.p, Dl.x.p, Dl.b.p, V);
+                hipLaunchKernelGGL(k_rowdense<0>, dim3(cdiv(c, 256)), dim3(256), 0, st, Dl.x.p, Dl.t.p, Dl.b.p, V);
+            }
+            hipLaunchKernelGGL(k_rowdense<1>, dim3(cdiv(c, 256)), dim3(256), 0, st, Dl.r.p, Dl.x.p, Dl.b.p, V);
+            SLevel& Cn = s.D[l + 1];
+            hipLaunchKernelGGL(k_restrictdense, dim3(cdiv(Cn.cells(), 256)), dim3(256), 0, st, Cn.b.p, Dl.r.p, i3(Cn.n), i3(Dl.n));
+        }
+        SLevel& Dm = s.D[nl - 1];
+        hipLaunchKernelGGL(k_scoarsest, dim3(1), dim3(1024), 0, st, Dm.x.p, Dm.b.p, dview(nl - 1), coarse);
+        for (int l = nl - 2; l >= 0; --l) {
+            SLevel &Dl = s.D[l], &Cn = s.D[l + 1];
+            DView V = dview(l);
+            size_t c = Dl.cells();
+            hipLaunchKernelGGL(k_prolongdense, dim3(cdiv(c, 256)), dim3(256), 0, st, Dl.x.p, Cn.x.p, Dl.act.p, i3(Dl.n), i3(Cn.n),
+                               boost);
+            for (int k = 0; k < sweeps; ++k) {
+                hipLaunchKernelGGL(k_rowdense<0>, dim3(cdiv(c, 256)), dim3(256), 0, st, Dl.t.p, Dl.x.p, Dl.b.p, V);
+                hipLaunchKernelGGL(k_rowdense<0>, dim3(cdiv(c, 256)), dim3(256), 0, st, Dl.x.p, Dl.t.p, Dl.b.p, V);
+            }
+        }
+        if (ns)
+            hipLaunchKernelGGL(k_prolong21, dim3(cdiv(ns * SBCELLS1, 256)), dim3(256), 0, st, X1x.f, D2.x.p, s.act1.p, t, boost,
+                               ns);
+        hipLaunchKernelGGL(k_prolongcoarse, dim3(cdiv(Cc, 256)), dim3(256), 0, st, X1x.c, D2.x.p, D2.act.p, t, boost);
+        smooth<8>(X1x, X1b, X1t, L1, sweeps);
+        if (nf) hipLaunchKernelGGL(k_prolong10, dim3(cdiv(nf, 256)), dim3(256), 0, st, z.f, X1x.f, s.types.fine.p, t, boost, nf);
+        hipLaunchKernelGGL(k_prolongcoarse, dim3(cdiv(Cc, 256)), dim3(256), 0, st, z.c, X1x.c, D2.act.p, t, boost);
+        smooth<16>(z, r, tmp, L0, sweeps);
+        MAEL_LAUNCH_CHECK();
+    }
+
+    void apply(SVec q, SVec x) { rows<16, 2>(q, x, x, level0()); }
+    void dot(SVec a, SVec b, int op, int slot) {
+        hipLaunchKernelGGL(k_sdot, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, a.f, b.f, a.c, b.c, s.types.fine.p,
+                           s.types.coarse.p, t, nf);
+        hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(256), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p, op, slot);
+        MAEL_LAUNCH_CHECK();
+    }
+    // y += c x (c = mult * scal[slot], or mult when slot < 0), over the used fine cells and the coarse part.
+    void axpy(SVec y, SVec x, int slot, float mult) {
+        if (nf) hipLaunchKernelGGL(k_saxpy<0>, dim3(std::min<unsigned>(cdiv(nf, 256), 8192)), dim3(256), 0, st, y.f, x.f, (size_t)nf, s.scal.p, slot, mult);
+        hipLaunchKernelGGL(k_saxpy<0>, dim3(std::min<unsigned>(cdiv(Cc, 256), 8192)), dim3(256), 0, st, y.c, x.c, Cc, s.scal.p, slot, mult);
+    }
+    // y = x + c y
+    void xpby(SVec y, SVec x, int slot, float mult) {
+        if (nf) hipLaunchKernelGGL(k_saxpy<1>, dim3(std::min<unsigned>(cdiv(nf, 256), 8192)), dim3(256), 0, st, y.f, x.f, (size_t)nf, s.scal.p, slot, mult);
+        hipLaunchKernelGGL(k_saxpy<1>, dim3(std::min<unsigned>(cdiv(Cc, 256), 8192)), dim3(256), 0, st, y.c, x.c, Cc, s.scal.p, slot, mult);
+    }
+    void copy(SVec y, SVec x) {
+        if (nf) MAEL_HIP(hipMemcpyAsync(y.f, x.f, (size_t)nf * sizeof(float), hipMemcpyDeviceToDevice, st));
+        MAEL_HIP(hipMemcpyAsync(y.c, x.c, Cc * sizeof(float), hipMemcpyDeviceToDevice, st));
+    }
+    double read(int slot) {
+        MAEL_HIP(hipMemcpyAsync(s.st.pinned, s.scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, st));
+        MAEL_HIP(hipStreamSynchronize(st));
+        return *(double*)s.st.pinned;
+    }
+    void read2(int slot, double& a, double& b) {   // scal[slot] and the error flags, one copy
+        MAEL_HIP(hipMemcpyAsync(s.st.pinned, s.scal.p, 16 * sizeof(double), hipMemcpyDeviceToHost, st));
+        MAEL_HIP(hipStreamSynchronize(st));
+        a = ((double*)s.st.pinned)[slot];
+        b = ((double*)s.st.pinned)[SS_FLAG];
+    }
+    void set(int slot, double v) { fill_async(s.scal.p + slot, (size_t)1, v, st); }
+
+    static void raise_flags(long long f) {
+        if (f & EF_RHO_NONFINITE) throw Error(E_NONFINITE, "Nonfinite pressure preconditioner");
+        if (f & EF_RHO_NONPOS) throw Error(E_CG_INDEFINITE, "Pressure CG requires a positive preconditioner");
+        if (f & EF_DEN_NONPOS) throw Error(E_CG_INDEFINITE, "Pressure CG requires a positive operator");
+        if (f & EF_ALPHA_NONFINITE) throw Error(E_NONFINITE, "Nonfinite pressure update");
+        if (f & EF_RHONEW_NONPOS) throw Error(E_CG_INDEFINITE, "Pressure CG requires a positive preconditioner");
+        if (f & EF_BETA_NONFINITE) throw Error(E_NONFINITE, "Nonfinite pressure direction");
+    }
+
+    // sflip.jl:1867-1904 pressure!: flexible CG from the pressure in s.P. Returns the
+    // iterations; throws the reference's errors; `converged` false at maxiter.
+    int fcg(double tol, int maxiter, int sweeps, int coarse, float boost, bool& converged, double& relres) {
+        SVec P = vec(s.P), b = vec(s.b), r = vec(s.r), z = vec(s.z), d = vec(s.d), q = vec(s.q), rold = vec(s.rold);
+        converged = false;
+        set(SS_FLAG, 0.0);
+        apply(r, P);
+        xpby(r, b, -1, -1.0f);
+        dot(b, b, SF_NORM, SS_BB);
+        double bnorm = read(SS_BB);
+        relres = 0;
+        if (!std::isfinite(bnorm)) throw Error(E_NONFINITE, "Nonfinite pressure right-hand side");
+        if (bnorm == 0) { converged = true; return 0; }
+        double target = (double)((float)tol * (float)bnorm);
+        dot(r, r, SF_NORM, SS_RNORM);
+        double rnorm = read(SS_RNORM);
+        relres = rnorm / bnorm;
+        if (rnorm <= target) { converged = true; return 0; }
+        vcycle(z, r, sweeps, coarse, boost);
+        copy(d, z);
+        dot(z, r, SF_RHO0, 0);
+        for (int it = 1; it <= maxiter; ++it) {
+            apply(q, d);
+            dot(d, q, SF_ALPHA, 0);
+            axpy(P, d, SS_ALPHA, 1.0f);
+            copy(rold, r);
+            axpy(r, q, SS_ALPHA, -1.0f);
+            dot(r, r, SF_NORM, SS_RNORM);
+            double flags;
+            read2(SS_RNORM, rnorm, flags);
+            raise_flags((long long)flags);
+            relres = rnorm / bnorm;
+            if (rnorm <= target) { converged = true; return it; }
+            vcycle(z, r, sweeps, coarse, boost);
+            dot(z, r, SF_SET, SS_RHONEW);
+            dot(z, rold, SF_BETA, 0);
+            xpby(d, z, SS_BETA, 1.0f);
+        }
+        double flags;
+        read2(SS_RNORM, rnorm, flags);
+        raise_flags((long long)flags);
+        return maxiter;
+    }
+
+    SVec vfield(std::vector<SField<float>>& pool, int k) { return SVec{pool[k].fine.p, pool[k].coarse.p}; }
+    void workspace(int restart) {
+        if ((int)s.Z.size() == restart) return;
+        s.V.clear(); s.Z.clear();
+        s.V.resize(restart + 1); s.Z.resize(restart);
+        for (auto* pool : {&s.V, &s.Z})
+            for (auto& f : *pool) {
+                f.alloc((size_t)s.blocks * SBCELLS, Cc);
+                fill_async(f.fine.p, (size_t)s.blocks * SBCELLS, 0.0f, st);
+                fill_async(f.coarse.p, Cc, 0.0f, st);
+            }
+    }
+
+    // fgmres.jl:26-92 pressure_fgmres!: right-preconditioned restarted FGMRES (Saad 1993),
+    // Z_j = M V_j + w D^-1 V_j, two passes of Gram-Schmidt, the small system in FP64.
+    int fgmres(double tol, int maxiter, int restart, int sweeps, int coarse, float boost, double& relres) {
+        workspace(restart);
+        SVec P = vec(s.P), b = vec(s.b), r = vec(s.r), q = vec(s.q), z = vec(s.z), tmp = vec(s.tmp);
+        dot(b, b, SF_NORM, SS_BB);
+        double bnorm = read(SS_BB);
+        if (!std::isfinite(bnorm)) throw Error(E_NONFINITE, "Nonfinite pressure RHS");
+        if (bnorm == 0) { zero(P, (size_t)nf); relres = 0; return 0; }
+        double target = (double)((float)tol * (float)bnorm);
+        auto actual = [&]() {
+            apply(q, P);
+            copy(r, b);
+            axpy(r, q, -1, -1.0f);
+            dot(r, r, SF_NORM, SS_RNORM);
+            return read(SS_RNORM);
+        };
+        double residual = actual();
+        int iterations = 0;
+        while (residual > target && iterations < maxiter) {
+            if (!std::isfinite(residual)) throw Error(E_NONFINITE, "Nonfinite pressure residual");
+            std::vector<double> H((size_t)(restart + 1) * restart, 0.0), g(restart + 1, 0.0), cs(restart, 0.0), sn(restart, 0.0);
+            auto Hs = [&](int i, int j) -> double& { return H[(size_t)i + (size_t)(restart + 1) * j]; };
+            g[0] = residual;
+            SVec V0 = vfield(s.V, 0);
+            zero(V0, (size_t)nf);
+            axpy(V0, r, -1, (float)(1.0f / (float)residual));
+            int used = 0;
+            for (int j = 0; j < std::min(restart, maxiter - iterations); ++j) {
+                SVec Vj = vfield(s.V, j), Zj = vfield(s.Z, j);
+                vcycle(z, Vj, sweeps, coarse, boost);
+                // the diagonal part: one damped Jacobi sweep from zero (fgmres.jl:15-23)
+                zero(tmp, (size_t)nf);
+                rows<16, 0>(Zj, tmp, Vj, level0());
+                axpy(Zj, z, -1, 1.0f);
+                apply(q, Zj);
+                int k = 0;
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int i = 0; i <= j; ++i, ++k) {
+                        dot(q, vfield(s.V, i), SF_SET, SS_H0 + k);
+                        axpy(q, vfield(s.V, i), SS_H0 + k, -1.0f);
+                    }
+                dot(q, q, SF_NORM, SS_TMP);
+                std::vector<double> hv(SS_H0 + k);
+                MAEL_HIP(hipMemcpyAsync(s.st.pinned, s.scal.p, (SS_H0 + k) * sizeof(double), hipMemcpyDeviceToHost, st));
+                MAEL_HIP(hipStreamSynchronize(st));
+                std::memcpy(hv.data(), s.st.pinned, (SS_H0 + k) * sizeof(double));
+                k = 0;
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int i = 0; i <= j; ++i, ++k) Hs(i, j) += (double)(float)hv[SS_H0 + k];
+                double hnext = (double)(float)hv[SS_TMP];
+                if (!std::isfinite(hnext)) throw Error(E_NONFINITE, "Nonfinite pressure Arnoldi vector");
+                Hs(j + 1, j) = hnext;
+                if (hnext > 0) {
+                    SVec Vn = vfield(s.V, j + 1);
+                    zero(Vn, (size_t)nf);
+                    axpy(Vn, q, -1, (float)(1.0f / (float)hnext));
+                }
+                for (int i = 0; i < j; ++i) {
+                    double a = Hs(i, j), bb = Hs(i + 1, j);
+                    Hs(i, j) = cs[i] * a + sn[i] * bb;
+                    Hs(i + 1, j) = -sn[i] * a + cs[i] * bb;
+                }
+                double a = Hs(j, j), bb = Hs(j + 1, j), hyp = std::hypot(a, bb);
+                if (!(hyp > 0)) throw Error(E_BREAKDOWN, "Pressure Arnoldi breakdown");
+                cs[j] = a / hyp; sn[j] = bb / hyp;
+                Hs(j, j) = hyp; Hs(j + 1, j) = 0;
+                g[j + 1] = -sn[j] * g[j]; g[j] = cs[j] * g[j];
+                used = j + 1;
+                ++iterations;
+                if (std::fabs(g[j + 1]) <= target || hnext == 0) break;
+            }
+            std::vector<double> y(used);
+            for (int i = used - 1; i >= 0; --i) {
+                double acc = g[i];
+                for (int kk = i + 1; kk < used; ++kk) acc -= Hs(i, kk) * y[kk];
+                y[i] = acc / Hs(i, i);
+            }
+            for (int j = 0; j < used; ++j) axpy(P, vfield(s.Z, j), -1, (float)y[j]);
+            residual = actual();
+        }
+        relres = residual / bnorm;
+        if (!(residual <= target))
+            throw Error(E_NOCONVERGE, "Pressure did not converge: " + std::to_string(residual / bnorm) + " after " +
+                                          std::to_string(iterations) + " iterations");
+        return iterations;
+    }
+};
+
+void SparseBasin::project(float gamma, const SparseOpts& o, SparseStats& stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = st.s;
+    SparseSolver S(*this);
+    STopo t = topo();
+    stats = SparseStats{};
+    S.coefficients();
+    if (S.nf)
+        hipLaunchKernelGGL(k_finerhs, dim3(cdiv(S.nf, 256)), dim3(256), 0, s, b.fine.p, P.fine.p, types.fine.p, u[0].fine.p,
+                           u[1].fine.p, u[2].fine.p, t, gamma, S.nf);
+    hipLaunchKernelGGL(k_coarserhs, dim3(cdiv(S.Cc, 256)), dim3(256), 0, s, b.coarse.p, P.coarse.p, types.coarse.p, u[0].coarse.p,
+                       u[1].coarse.p, u[2].coarse.p, t, gamma);
+    MAEL_LAUNCH_CHECK();
+    S.hierarchy();
+    double rr = 0;
+    bool conv = false;
+    if (o.policy == 0) {
+        stats.fcg_iterations = S.fcg(o.tol, o.maxiter, o.sweeps, o.coarse, o.boost, conv, rr);
+    } else if (o.policy == 1) {
+        stats.fgmres_iterations = S.fgmres(o.tol, o.maxiter, o.restart, o.sweeps, o.coarse, o.boost, rr);
+        stats.used_fgmres = 1;
+    } else {
+        // fgmres.jl:96-110: CG first; FGMRES when it fails or misses the tolerance
+        try {
+            stats.fcg_iterations = S.fcg(o.tol, std::min(80, o.maxiter), o.sweeps, o.coarse, o.boost, conv, rr);
+        } catch (const Error& e) {
+            std::string m = e.what();
+            if (m.rfind("Nonfinite pressure", 0) != 0 && m.rfind("Pressure CG ", 0) != 0) throw;
+        }
+        SVec Pv = S.vec(P), bv = S.vec(b), rv = S.vec(r), qv = S.vec(q);
+        S.apply(qv, Pv);
+        S.copy(rv, bv);
+        S.axpy(rv, qv, -1, -1.0f);
+        S.dot(bv, bv, SF_NORM, SS_BB);
+        S.dot(rv, rv, SF_NORM, SS_RNORM);
+        double bn = S.read(SS_BB), rn = S.read(SS_RNORM);
+        if (!(rn <= (double)((float)o.tol * (float)bn))) {
+            if (!std::isfinite(rn) || rn > 2.0 * bn) S.zero(Pv, (size_t)S.nf);
+            stats.used_fgmres = 1;
+            stats.fgmres_iterations = S.fgmres(o.tol, o.maxiter, o.restart, o.sweeps, o.coarse, o.boost, rr);
+        }
+    }
+    stats.iterations = stats.fcg_iterations + stats.fgmres_iterations;
+    {   // the true relative residual of the returned pressure, in FP64 sums
+        SVec Pv = S.vec(P), bv = S.vec(b), rv = S.vec(r), qv = S.vec(q);
+        S.apply(qv, Pv);
+        S.copy(rv, bv);
+        S.axpy(rv, qv, -1, -1.0f);
+        S.dot(bv, bv, SF_NORM, SS_BB);
+        S.dot(rv, rv, SF_NORM, SS_RNORM);
+        double bn = S.read(SS_BB), rn = S.read(SS_RNORM);
+        stats.rel_residual = bn > 0 ? rn / bn : 0.0;
+    }
+    // sflip.jl:2008-2013 the faces, then the coarse faces under fine blocks from theirs
+    for (int a = 0; a < 3; ++a) {
+        if (S.nf)
+            hipLaunchKernelGGL(k_projectfine, dim3(cdiv(finefaces(), 256)), dim3(256), 0, s, u[a].fine.p, fl[a].fine.p, P.fine.p,
+                               P.coarse.p, types.fine.p, types.coarse.p, phi.fine.p, phi.coarse.p, t, a, (int32_t)finefaces());
+        hipLaunchKernelGGL(k_projectcoarse, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, s, u[a].coarse.p, fl[a].coarse.p, P.coarse.p,
+                           types.coarse.p, phi.coarse.p, t, a);
+        launch_restrictfaces_f(*this, u[a].coarse.p, u[a].fine.p, a);
+        launch_restrictfaces_u8(*this, fl[a].coarse.p, fl[a].fine.p, a);
+    }
+    MAEL_LAUNCH_CHECK();
+    st.sync();
+    stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace mael

@@ -26,6 +26,14 @@ KERNELS = ["p2g", "splat", "combine", "g2p", "bin", "fixrank", "scatter", "spawn
            "residual", "restrict", "prolong", "coarsest", "resid64"]
 KERNEL_ID = {n: i for i, n in enumerate(KERNELS)}
 
+# sparse array ids (include/maelstrom_hip.h MAEL_S_*)
+SPARSE_ARRAYS = ["PHI_F", "PHI_C", "PHI0_F", "PHI0_C", "P_F", "P_C", "TYPES_F", "TYPES_C",
+                 "U_F", "U_C", "V_F", "V_C", "W_F", "W_C", "U0_F", "U0_C", "V0_F", "V0_C", "W0_F", "W0_C",
+                 "FU_F", "FU_C", "FV_F", "FV_C", "FW_F", "FW_C", "RHS_F", "RHS_C"] + PARTICLE_FIELDS + [
+                 "KEY", "RANK", "COUNTS", "TOTALS", "OFFSETS",
+                 "MOM_U", "MOM_V", "MOM_W", "WT_U", "WT_V", "WT_W", "SLOT", "WHERE"]
+SPARSE_ID = {n: i for i, n in enumerate(SPARSE_ARRAYS)}
+
 
 class MaelstromError(RuntimeError):
     """A failed call: `code` is the status, the message the library's (the reference's text where it has one)."""
@@ -38,6 +46,25 @@ class MaelstromError(RuntimeError):
 class DenseGrid(C.Structure):
     _fields_ = [("lo", C.c_float * 3), ("dx", C.c_float), ("n", C.c_int32 * 3), ("radius", C.c_float),
                 ("floor", C.c_float), ("levels", C.c_int32)]
+
+
+class SparseGrid(C.Structure):
+    _fields_ = [("lo", C.c_float * 3), ("dx", C.c_float), ("nb", C.c_int32 * 3), ("radius", C.c_float),
+                ("floor", C.c_float)]
+
+    @property
+    def nc(self):
+        return tuple(4 * int(x) for x in self.nb)
+
+
+class SparseOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("maxiter", C.c_int32), ("sweeps", C.c_int32), ("coarse", C.c_int32),
+                ("restart", C.c_int32), ("policy", C.c_int32), ("precision", C.c_int32), ("boost", C.c_float)]
+
+
+class SparseStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("fcg_iterations", C.c_int32), ("fgmres_iterations", C.c_int32),
+                ("used_fgmres", C.c_int32), ("rel_residual", C.c_double), ("ms", C.c_double)]
 
 
 class SolveOpts(C.Structure):
@@ -90,6 +117,37 @@ def lib():
         "mael_dense_ktimer": (C.c_int, [p, i32, i32]),
         "mael_dense_ktime": (C.c_int, [p, C.POINTER(f64), C.POINTER(i32)]),
         "mael_dense_sync": (C.c_int, [p]),
+        "mael_sparsegrid": (C.c_int, [f32, f32, f32, f32, i32, C.POINTER(SparseGrid)]),
+        "mael_sparse_create": (C.c_int, [p, C.POINTER(SparseGrid), i32, i64, i32, i32, C.POINTER(p)]),
+        "mael_sparse_destroy": (None, [p]),
+        "mael_sparse_array": (C.c_int, [p, i32, C.POINTER(p), C.POINTER(i64), C.POINTER(i32)]),
+        "mael_sparse_upload": (C.c_int, [p, i32, p, i64]),
+        "mael_sparse_download": (C.c_int, [p, i32, p, i64]),
+        "mael_sparse_count": (C.c_int, [p, C.POINTER(i64)]),
+        "mael_sparse_set_count": (C.c_int, [p, i64]),
+        "mael_sparse_get_topology": (C.c_int, [p, p, p, C.POINTER(i32), p, C.POINTER(i32)]),
+        "mael_sparse_set_topology": (C.c_int, [p, p, p, i32, p, i32]),
+        "mael_sparse_marks": (C.c_int, [p, f32, f32, i32, i32, p]),
+        "mael_sparse_refine_marks": (C.c_int, [p, p, C.POINTER(i32), C.POINTER(i32)]),
+        "mael_sparse_refine": (C.c_int, [p, f32, f32, i32, C.POINTER(i32), C.POINTER(i32)]),
+        "mael_sparse_resample": (C.c_int, [p, i32, u32, C.POINTER(i64)]),
+        "mael_sparse_p2g": (C.c_int, [p, f32]),
+        "mael_sparse_g2p": (C.c_int, [p, f32, f32]),
+        "mael_sparse_defaults": (None, [C.POINTER(SparseOpts)]),
+        "mael_sparse_project": (C.c_int, [p, f32, C.POINTER(SparseOpts), C.POINTER(SparseStats)]),
+        "mael_sparse_init_coarse": (C.c_int, [p, p, i32, f32, f32, f32]),
+        "mael_sparse_init_fine": (C.c_int, [p, p, i32, f32, f32, f32]),
+        "mael_sparse_classify": (C.c_int, [p]),
+        "mael_sparse_solidify": (C.c_int, [p]),
+        "mael_sparse_redistance": (C.c_int, [p, i32]),
+        "mael_sparse_restrict_velocity": (C.c_int, [p]),
+        "mael_sparse_keep": (C.c_int, [p]),
+        "mael_sparse_fill_flags": (C.c_int, [p, C.c_uint8]),
+        "mael_sparse_volume": (C.c_int, [p, C.POINTER(f64)]),
+        "mael_sparse_gravity": (C.c_int, [p, f32, f32]),
+        "mael_sparse_sync": (C.c_int, [p]),
+        "mael_sparse_ktimer": (C.c_int, [p, i32, i32]),
+        "mael_sparse_ktime": (C.c_int, [p, C.POINTER(f64), C.POINTER(i32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -1,0 +1,311 @@
+"""`SparseBasin`: the two-resolution narrow-band FLIP basin on the HIP library
+(sflip.jl:242-2318, fgmres.jl).
+
+The Python mirror of the reference's Julia API for the hot path: `SparseBasin`,
+`refine`, `resample`, `particles2grid`, `project` (with `pressure!`'s policies),
+`grid2particles`, `initialize` and `step` (Julia's bang names without the `!`). Every
+phase runs in the gfx950 library; this module keeps only what the reference keeps on
+the host: time, step count, the reference volume, and the block table's bookkeeping
+mirrored from the library.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import SPARSE_ID, PARTICLE_FIELDS, MaelstromError, SparseGrid, SparseOpts, SparseStats
+from .basin import surface
+
+BCELLS, BFACES = 4096, 4352
+_INT32 = {"KEY", "RANK", "COUNTS", "TOTALS", "OFFSETS", "SLOT", "WHERE"}
+_UINT8 = {"TYPES_F", "TYPES_C", "FU_F", "FU_C", "FV_F", "FV_C", "FW_F", "FW_C"}
+_INT64 = {"MOM_U", "MOM_V", "MOM_W", "WT_U", "WT_V", "WT_W"}
+
+
+def sparsegrid(basin, dx, top=None, pad=2):
+    """sflip.jl:46-53 `SparseGrid(b; dx, top = b.rimwall, pad = 2)`."""
+    g = SparseGrid()
+    top = basin.rimwall if top is None else top
+    st = _lib.lib().mael_sparsegrid(basin.radius, basin.depth, top, dx, pad, C.byref(g))
+    if st != _lib.OK:
+        raise MaelstromError(st, "bad sparse grid for dx=%r" % dx)
+    return g
+
+
+class SparseBasin:
+    """sflip.jl:301 `SparseBasin(b, backend; dx, stir, band = 4, nmin = 8, cfl = 8, capacity = 30_000_000,
+    blocks = 8000, rings = 0, vary = 0.5, reflect = true)`."""
+
+    def __init__(self, basin, dx, band=4, nmin=8, cfl=8.0, capacity=30_000_000, blocks=8000, rings=0, vary=0.5,
+                 reflect=True, ctx=None, top=None, pad=2):
+        self.basin = basin
+        self.ctx = ctx or _lib.default_context()
+        self.grid = sparsegrid(basin, dx, top, pad)
+        self.band, self.nmin, self.cfl, self.capacity = int(band), int(nmin), float(cfl), int(capacity)
+        self.blocks, self.rings, self.vary, self.reflect = int(blocks), int(rings), float(vary), bool(reflect)
+        self.h = C.c_void_p()
+        self.ctx.check(_lib.lib().mael_sparse_create(self.ctx.h, C.byref(self.grid), self.blocks, self.capacity,
+                                                     self.band, self.nmin, C.byref(self.h)))
+        self.volume0 = 0.0
+        self.t = 0.0
+        self.steps = 0
+        self.iterations = []
+        self.last_solve = None
+
+    # ── arrays ─────────────────────────────────────────────────────────────
+    @property
+    def nc(self):
+        return self.grid.nc
+
+    @property
+    def nblocks(self):
+        return int(np.prod([int(x) for x in self.grid.nb]))
+
+    def shape(self, name):
+        """numpy shape of an array: coarse parts (z, y, x); fine parts and particles flat."""
+        n1, n2, n3 = self.nc
+        coarse = {"PHI_C": (n3, n2, n1), "PHI0_C": (n3, n2, n1), "P_C": (n3, n2, n1), "TYPES_C": (n3, n2, n1),
+                  "RHS_C": (n3, n2, n1), "U_C": (n3, n2, n1 + 1), "U0_C": (n3, n2, n1 + 1), "FU_C": (n3, n2, n1 + 1),
+                  "V_C": (n3, n2 + 1, n1), "V0_C": (n3, n2 + 1, n1), "FV_C": (n3, n2 + 1, n1),
+                  "W_C": (n3 + 1, n2, n1), "W0_C": (n3 + 1, n2, n1), "FW_C": (n3 + 1, n2, n1)}
+        if name in coarse:
+            return coarse[name]
+        return (self._info(name)[1],)
+
+    def _info(self, name):
+        ptr, cnt, es = C.c_void_p(), C.c_int64(), C.c_int32()
+        self.ctx.check(_lib.lib().mael_sparse_array(self.h, SPARSE_ID[name], C.byref(ptr), C.byref(cnt), C.byref(es)))
+        dt = np.int32 if name in _INT32 else (np.uint8 if name in _UINT8 else (np.int64 if name in _INT64 else np.float32))
+        return ptr.value, cnt.value, dt
+
+    def get(self, name, count=None):
+        """Download array `name`; particle arrays are cut to `count` (default n), fine
+        parts to the slots in use unless `count` is given."""
+        _, cnt, dt = self._info(name)
+        if name in PARTICLE_FIELDS or name in ("KEY", "RANK"):
+            cnt = self.n if count is None else count
+        elif count is not None:
+            cnt = count
+        elif name.endswith("_F") or name in ("COUNTS", "TOTALS", "OFFSETS"):
+            cnt = self.nslots * BCELLS if name in ("PHI_F", "PHI0_F", "P_F", "TYPES_F", "RHS_F", "COUNTS", "TOTALS",
+                                                    "OFFSETS") else self.nslots * BFACES
+        elif name in _INT64:
+            cnt = self.nslots * BFACES
+        out = np.empty(cnt, dtype=dt)
+        if out.size:
+            self.ctx.check(_lib.lib().mael_sparse_download(self.h, SPARSE_ID[name], out.ctypes.data, out.size))
+        if name.endswith("_C"):
+            out = out.reshape(self.shape(name))
+        return out
+
+    def set(self, name, arr):
+        _, cnt, dt = self._info(name)
+        a = np.ascontiguousarray(arr, dtype=dt).ravel()
+        if a.size > cnt:
+            raise ValueError("%s holds %d elements, got %d" % (name, cnt, a.size))
+        if a.size:
+            self.ctx.check(_lib.lib().mael_sparse_upload(self.h, SPARSE_ID[name], a.ctypes.data, a.size))
+
+    @property
+    def n(self):
+        v = C.c_int64()
+        self.ctx.check(_lib.lib().mael_sparse_count(self.h, C.byref(v)))
+        return v.value
+
+    @n.setter
+    def n(self, value):
+        self.ctx.check(_lib.lib().mael_sparse_set_count(self.h, int(value)))
+
+    def particles(self):
+        return {f: self.get(f) for f in PARTICLE_FIELDS}
+
+    def set_particles(self, ps, n=None):
+        n = len(ps["PX"]) if n is None else n
+        for f in PARTICLE_FIELDS:
+            self.set(f, np.asarray(ps[f][:n], dtype=np.float32))
+        self.n = n
+
+    # ── topology ───────────────────────────────────────────────────────────
+    def topology(self):
+        """(slot per block, where per slot (blocks x 3), nslots, free list)."""
+        slot = np.zeros(self.nblocks, np.int32)
+        where = np.zeros(3 * self.blocks, np.int32)
+        free = np.zeros(self.blocks, np.int32)
+        ns, nf = C.c_int32(), C.c_int32()
+        self.ctx.check(_lib.lib().mael_sparse_get_topology(self.h, slot.ctypes.data, where.ctypes.data, C.byref(ns),
+                                                           free.ctypes.data, C.byref(nf)))
+        return slot, where.reshape(-1, 3), ns.value, free[:nf.value].copy()
+
+    @property
+    def nslots(self):
+        ns = C.c_int32()
+        self.ctx.check(_lib.lib().mael_sparse_get_topology(self.h, None, None, C.byref(ns), None, None))
+        return ns.value
+
+    def set_topology(self, slot, where, nslots, free=()):
+        slot = np.ascontiguousarray(slot, np.int32).ravel()
+        where = np.ascontiguousarray(where, np.int32).ravel()
+        free = np.ascontiguousarray(free, np.int32).ravel()
+        self.ctx.check(_lib.lib().mael_sparse_set_topology(self.h, slot.ctypes.data, where.ctypes.data, int(nslots),
+                                                           free.ctypes.data if free.size else None, free.size))
+
+    def sync(self):
+        self.ctx.check(_lib.lib().mael_sparse_sync(self.h))
+
+    def close(self):
+        if self.h:
+            _lib.lib().mael_sparse_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ── phases ───────────────────────────────────────────────────────────────────
+
+def reach(s):
+    """sflip.jl:522: the particles' reach, (ceil(cfl) + 3) dx."""
+    return float(np.float32((math.ceil(s.cfl) + 3) * s.grid.dx))
+
+
+def marks(s, particles=True):
+    """The marks of `refine!` (sflip.jl:518-526), one byte a block."""
+    out = np.zeros(s.nblocks, np.uint8)
+    s.ctx.check(_lib.lib().mael_sparse_marks(s.h, s.vary, reach(s), s.rings, 1 if particles else 0, out.ctypes.data))
+    return out
+
+
+def refine(s, want=None):
+    """sflip.jl:516-576 `refine!(s)` -> (added, removed); with `want`, from those marks."""
+    na, nr = C.c_int32(), C.c_int32()
+    L = _lib.lib()
+    if want is None:
+        s.ctx.check(L.mael_sparse_refine(s.h, s.vary, reach(s), s.rings, C.byref(na), C.byref(nr)))
+    else:
+        w = np.ascontiguousarray(want, np.uint8)
+        s.ctx.check(L.mael_sparse_refine_marks(s.h, w.ctypes.data, C.byref(na), C.byref(nr)))
+    return na.value, nr.value
+
+
+def resample(s, seeding=False):
+    """sflip.jl:2227-2246 `resample!(s; seeding)`: returns the new particle count."""
+    total = C.c_int64()
+    s.ctx.check(_lib.lib().mael_sparse_resample(s.h, 1 if seeding else 0, (s.steps + 1) & 0xffffffff, C.byref(total)))
+    return total.value
+
+
+def particles2grid(s, dt):
+    """sflip.jl:808-831 `particles2grid!(s, dt)`."""
+    s.ctx.check(_lib.lib().mael_sparse_p2g(s.h, float(dt)))
+    return s
+
+
+def grid2particles(s, dt):
+    """sflip.jl:2141-2144 `grid2particles!(s, dt)`."""
+    s.ctx.check(_lib.lib().mael_sparse_g2p(s.h, float(dt), float(s.basin.g)))
+    return s
+
+
+def solve_options(tol=5e-4, maxiter=240, sweeps=1, coarse=40, restart=8, policy=2, precision=0, boost=1.5):
+    """`pressure!(::SparseBasin{B,<:SurfaceStir})`'s defaults (fgmres.jl:96); policy 0 is the
+    plain flexible CG of sflip.jl:1867 (its defaults: tol 1e-4, maxiter 200, boost 1),
+    1 FGMRES alone (fgmres.jl:26)."""
+    return SparseOpts(float(tol), int(maxiter), int(sweeps), int(coarse), int(restart), int(policy), int(precision),
+                      float(boost))
+
+
+def volume(s):
+    """sflip.jl:1990-1996 `volume(s)`, m^3."""
+    v = C.c_double()
+    s.ctx.check(_lib.lib().mael_sparse_volume(s.h, C.byref(v)))
+    return v.value
+
+
+def gamma_for(s, dt, recover=0.5):
+    """sflip.jl:2000-2001: the volume control's divergence γ."""
+    loss = (s.volume0 - volume(s)) / s.volume0 if s.volume0 else 0.0
+    return float(np.float32(min(max(loss / recover, -0.01 / dt), 0.01 / dt)))
+
+
+def project(s, dt, recover=0.5, gamma=None, **solve):
+    """sflip.jl:1998-2015 `project!(s, dt; recover)`; `solve` are the solver's options."""
+    g = gamma_for(s, dt, recover) if gamma is None else float(gamma)
+    opts = solve_options(**solve)
+    st = SparseStats()
+    s.ctx.check(_lib.lib().mael_sparse_project(s.h, g, C.byref(opts), C.byref(st)))
+    s.iterations.append(st.iterations)
+    s.last_solve = st
+    return st
+
+
+def classify(s):
+    s.ctx.check(_lib.lib().mael_sparse_classify(s.h))
+    return s
+
+
+def redistance(s, iterations=None):
+    """sflip.jl:910-919 `redistance!(s; iterations = band + 3)`."""
+    it = s.band + 3 if iterations is None else int(iterations)
+    s.ctx.check(_lib.lib().mael_sparse_redistance(s.h, it))
+    return s
+
+
+def keep(s):
+    """φ0 = φ, u0 = u (sflip.jl:700-703)."""
+    s.ctx.check(_lib.lib().mael_sparse_keep(s.h))
+
+
+def surface_table(s):
+    """The balanced surface at 2048 radii over [0, hypot(lo)] (sflip.jl:684-685)."""
+    g = s.grid
+    rmax = np.float32(math.hypot(g.lo[0], g.lo[1]))
+    r = np.linspace(np.float32(0), rmax, 2048, dtype=np.float32)
+    return np.ascontiguousarray(surface(s.basin, r), dtype=np.float32), rmax
+
+
+def initialize(s):
+    """sflip.jl:681-713 `initialize!(s)`: the balanced surface and swirl on the coarse grid,
+    fine blocks round the surface with both on them too, the band seeded, their deposit."""
+    L = _lib.lib()
+    eta, rmax = surface_table(s)
+    args = (eta.ctypes.data, eta.size, float(rmax), float(s.basin.Γ), float(s.basin.core))
+    s.ctx.check(L.mael_sparse_init_coarse(s.h, *args))
+    refine(s)
+    s.ctx.check(L.mael_sparse_init_fine(s.h, *args))
+    redistance(s, 2 * s.band + 4)
+    classify(s)
+    s.ctx.check(L.mael_sparse_solidify(s.h))
+    s.ctx.check(L.mael_sparse_fill_flags(s.h, 1))
+    s.n = 0
+    resample(s, seeding=True)
+    resample(s)
+    keep(s)
+    particles2grid(s, 1.0)
+    redistance(s)
+    classify(s)
+    s.volume0 = volume(s)
+    s.t = 0.0
+    return s
+
+
+def step(s, dt, **solve):
+    """One step of the hot path in the reference's order (sflip.jl:2295-2318): refine, the
+    grid holds still in place of advection (φ0, u0 take φ, u), particles to grid,
+    redistance, classify, gravity, projection, grid to particles, resample."""
+    L = _lib.lib()
+    refine(s)
+    keep(s)
+    particles2grid(s, dt)
+    redistance(s)
+    classify(s)
+    s.ctx.check(L.mael_sparse_gravity(s.h, float(dt), float(s.basin.g)))
+    st = project(s, dt, **solve)
+    grid2particles(s, dt)
+    s.steps += 1
+    resample(s)
+    s.t += dt
+    return st

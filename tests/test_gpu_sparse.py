@@ -1,0 +1,273 @@
+"""Parity of the sparse basin's HIP path (SparseBasin) with the CPU oracle, through the
+C ABI: the block allocator and marks bit-exact, resampling and G2P bit-exact, P2G to
+FP32 rounding of the deposit, the composite-multigrid projection against the FP64
+oracle solve, and the reference's error texts."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PNAMES = ["px", "py", "pz", "vx", "vy", "vz"] + ["c%d" % k for k in range(9)] + ["lag"]
+BCELLS, BFACES = 4096, 4352
+DX = 0.6
+
+
+@pytest.fixture(scope="module")
+def M():
+    import maelstrom_amd
+    return maelstrom_amd
+
+
+def _basin(M):
+    return M.Basin(radius=18.0, depth=12.0, Γ=225.0, core=2.5)
+
+
+def _scene(**kw):
+    kw.setdefault("blocks", 64)
+    kw.setdefault("capacity", 300000)
+    return O.sparse_scene(dx=DX, **kw)
+
+
+def _upload(M, o, cfl=8.0):
+    """A GPU SparseBasin holding the oracle's state (topology, fields, particles)."""
+    s = M.SparseBasin(_basin(M), dx=DX, blocks=o.blocks, capacity=o.capacity, band=o.band, nmin=o.nmin, cfl=cfl)
+    g = o.grid
+    assert list(s.grid.nb) == list(g.nb) and np.float32(s.grid.dx) == np.float32(g.dx)
+    s.set_topology(o.array("slot"), o.array("where"), o.nslots, o.free_list())
+    for gname, oname in [("PHI", "phi"), ("PHI0", "phi0"), ("P", "P"), ("TYPES", "types"), ("U", "u"), ("V", "v"),
+                         ("W", "w"), ("U0", "u0"), ("V0", "v0"), ("W0", "w0"), ("FU", "fu"), ("FV", "fv"),
+                         ("FW", "fw")]:
+        s.set(gname + "_F", o.array(oname + "_f"))
+        s.set(gname + "_C", o.array(oname + "_c"))
+    n = o.n
+    for nm in PNAMES:
+        s.set(nm.upper(), o.array(nm)[:n])
+    s.n = n
+    return s
+
+
+def _fine(o, name, per):
+    return o.array(name)[:o.nslots * per]
+
+
+# ── topology ─────────────────────────────────────────────────────────────────
+
+def test_refine_allocator_bitexact(M):
+    """Random mark sequences: slots, wheres, free list and the prolonged fields match."""
+    print("case: refine allocator, 15 random mark sets", flush=True)
+    o = _scene(particles=False)
+    s = _upload(M, o)
+    rng = np.random.default_rng(5)
+    nblk = s.nblocks
+    for it in range(15):
+        want = (rng.random(nblk) < rng.uniform(0.1, 0.6)).astype(np.uint8)
+        if want.sum() > 64:
+            want[np.flatnonzero(want)[64:]] = 0
+        oa, orem = o.refine(want)
+        na, nr = M.sparse.refine(s, want)
+        assert (na, nr) == (len(oa), len(orem))
+        slot, where, ns, free = s.topology()
+        assert ns == o.nslots
+        np.testing.assert_array_equal(slot, o.array("slot"))
+        np.testing.assert_array_equal(where, o.array("where").reshape(-1, 3))
+        np.testing.assert_array_equal(free, o.free_list())
+    # the fields of the last arrivals came from the coarse grid exactly as the oracle's
+    np.testing.assert_array_equal(s.get("PHI_F"), _fine(o, "phi_f", BCELLS))
+    np.testing.assert_array_equal(s.get("U_F"), _fine(o, "u_f", BFACES))
+    np.testing.assert_array_equal(s.get("FW_F"), _fine(o, "fw_f", BFACES))
+
+
+def test_refine_pool_error(M):
+    o = _scene(particles=False)
+    s = _upload(M, o)
+    want = np.ones(s.nblocks, np.uint8)
+    s2 = M.SparseBasin(_basin(M), dx=DX, blocks=8, capacity=10)
+    with pytest.raises(M.MaelstromError, match="the sparse grid needs 64 fine blocks, more than its 8 allocated"):
+        M.sparse.refine(s2, want)
+    del s
+
+
+def test_marks_bitexact(M):
+    o = _scene()
+    s = _upload(M, o, cfl=8.0)
+    for particles in (False, True):
+        for rings in (0, 1):
+            s.rings = rings
+            got = M.sparse.marks(s, particles=particles)
+            exp = o.marks(vary=0.5, reach=np.float32((8 + 3) * np.float32(DX)), rings=rings, particles=particles)
+            np.testing.assert_array_equal(got, exp)
+
+
+def test_initialize_matches_oracle(M):
+    """initialize! on the GPU from the same surface table gives the oracle's blocks,
+    particles and level set."""
+    print("case: sparse initialize", flush=True)
+    o = _scene()
+    s = M.SparseBasin(_basin(M), dx=DX, blocks=64, capacity=300000)
+    M.sparse.initialize(s)
+    slot, where, ns, free = s.topology()
+    assert ns == o.nslots
+    np.testing.assert_array_equal(slot, o.array("slot"))
+    assert s.n == o.n
+    # the start uses the library's own Lamb-Oseen (expf) and surface table, so compare
+    # the level set and types, which the particles' positions decide
+    np.testing.assert_array_equal(s.get("TYPES_F"), _fine(o, "types_f", BCELLS))
+    for nm in ("px", "py", "pz"):
+        np.testing.assert_array_equal(s.get(nm.upper()), o.array(nm)[:o.n])
+    np.testing.assert_allclose(s.get("PHI_F"), _fine(o, "phi_f", BCELLS), rtol=0, atol=1e-6)
+    assert abs(M.sparse.volume(s) / o.volume() - 1) < 1e-6
+
+
+# ── particles ────────────────────────────────────────────────────────────────
+
+def test_resample_bitexact(M):
+    print("case: sparse resample", flush=True)
+    o = _scene()
+    s = _upload(M, o)
+    n = o.n
+    s.steps = 4
+    total = M.sparse.resample(s)
+    r = o.resample(seed=5)
+    assert total == r["total"] == s.n
+    C = o.nslots * BCELLS
+    np.testing.assert_array_equal(s.get("KEY", count=n), r["key"])
+    np.testing.assert_array_equal(s.get("RANK", count=n), r["rank"])
+    np.testing.assert_array_equal(s.get("COUNTS"), r["counts"][:C])
+    np.testing.assert_array_equal(s.get("TOTALS"), r["totals"][:C])
+    np.testing.assert_array_equal(s.get("OFFSETS"), r["offsets"][:C])
+    for nm in PNAMES:
+        np.testing.assert_array_equal(s.get(nm.upper()), o.array(nm)[:total], err_msg=nm)
+
+
+def test_resample_seeding_and_capacity(M):
+    o = _scene(particles=False)
+    s = _upload(M, o)
+    s.n = 0
+    total = M.sparse.resample(s, seeding=True)
+    r = o.resample(seeding=True, seed=1)
+    assert total == r["total"]
+    for nm in PNAMES:
+        np.testing.assert_array_equal(s.get(nm.upper()), o.array(nm)[:total], err_msg=nm)
+    s2 = M.SparseBasin(_basin(M), dx=DX, blocks=64, capacity=1000)
+    s2.set_topology(o.array("slot"), o.array("where"), o.nslots, o.free_list())
+    s2.set("PHI_F", o.array("phi_f"))
+    s2.set("PHI_C", o.array("phi_c"))
+    with pytest.raises(M.MaelstromError, match="the narrow band needs [0-9]+ particles, more than the 1000 allocated"):
+        M.sparse.resample(s2, seeding=True)
+
+
+def test_p2g_parity(M):
+    print("case: sparse p2g", flush=True)
+    o = _scene()
+    s = _upload(M, o)
+    dt = np.float32(0.1)
+    M.sparse.particles2grid(s, dt)
+    momwt = o.p2g(dt, momwt=True)
+    F = o.nslots * BFACES
+    for a, nm in enumerate("UVW"):
+        got_m = s.get("MOM_" + nm).astype(np.float64) / 2.0 ** 32
+        got_w = s.get("WT_" + nm).astype(np.float64) / 2.0 ** 32
+        scale = max(1.0, np.abs(momwt[a]).max())
+        np.testing.assert_allclose(got_m, momwt[a][:F], rtol=0, atol=1e-6 * scale)
+        np.testing.assert_allclose(got_w, momwt[3 + a][:F], rtol=0, atol=1e-6 * max(1.0, momwt[3 + a].max()))
+    np.testing.assert_array_equal(s.get("PHI_F"), _fine(o, "phi_f", BCELLS))
+    np.testing.assert_array_equal(s.get("PHI_C"), o.array("phi_c").reshape(s.shape("PHI_C")))
+    for nm in "uvw":
+        exp = _fine(o, nm + "_f", BFACES)
+        got = s.get(nm.upper() + "_F")
+        err = np.linalg.norm(got - exp) / max(np.linalg.norm(exp), 1e-30)
+        assert err < 1e-6, (nm, err)
+        np.testing.assert_allclose(s.get(nm.upper() + "_C").ravel(), o.array(nm + "_c"), rtol=0,
+                                   atol=1e-5 * max(1.0, np.abs(exp).max()))
+
+
+def test_p2g_deterministic(M):
+    o = _scene()
+    s = _upload(M, o)
+    M.sparse.particles2grid(s, 0.1)
+    a = [s.get(x) for x in ("U_F", "V_F", "W_F", "PHI_F")]
+    s2 = _upload(M, o)
+    M.sparse.particles2grid(s2, 0.1)
+    b = [s2.get(x) for x in ("U_F", "V_F", "W_F", "PHI_F")]
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_g2p_bitexact(M):
+    print("case: sparse g2p", flush=True)
+    o = _scene()
+    rng = np.random.default_rng(2)
+    for nm in ("u_f", "v_f", "w_f"):
+        arr = o.array(nm)
+        arr[:] += rng.normal(0, 0.2, arr.size).astype(np.float32)
+    fw = o.array("fw_f")
+    fw[rng.random(fw.size) < 0.01] = 0
+    s = _upload(M, o)
+    dt = np.float32(0.08)
+    M.sparse.grid2particles(s, dt)
+    o.g2p(dt, 9.81)
+    for nm in PNAMES:
+        np.testing.assert_array_equal(s.get(nm.upper()), o.array(nm)[:o.n], err_msg=nm)
+
+
+# ── projection ───────────────────────────────────────────────────────────────
+
+def _projection_inputs(o, seed=11):
+    rng = np.random.default_rng(seed)
+    for nm in ("u_f", "v_f", "w_f", "u_c", "v_c", "w_c"):
+        arr = o.array(nm)
+        arr[:] += rng.normal(0, 0.3, arr.size).astype(np.float32)
+    o.solidify()
+    o.restrict_velocity()
+    o.array("P_f")[:] = 0
+    o.array("P_c")[:] = 0
+
+
+@pytest.mark.parametrize("policy", [0, 2, 1])
+def test_project_parity(M, policy):
+    """The pressure to 1e-6 against the FP64 oracle's solve of the same system, then the
+    faces and their flags."""
+    print("case: sparse project policy %d" % policy, flush=True)
+    o = _scene()
+    _projection_inputs(o)
+    s = _upload(M, o)
+    gamma = 0.01
+    st = M.sparse.project(s, 0.1, gamma=gamma, tol=1e-6, maxiter=400, policy=policy, boost=1.0 if policy == 0 else 1.5)
+    assert st.rel_residual <= 1e-6 * 1.01, st.rel_residual
+    o.project(gamma=gamma, tol=1e-12, maxiter=600, policy=0)
+    ns = o.nslots
+    Pe = np.concatenate([o.array("P_f")[:ns * BCELLS], o.array("P_c")])
+    Pg = np.concatenate([s.get("P_F"), s.get("P_C").ravel()])
+    scale = np.abs(Pe).max()
+    np.testing.assert_allclose(Pg, Pe, rtol=0, atol=2e-5 * scale)
+    for nm in "uvw":
+        np.testing.assert_allclose(s.get(nm.upper() + "_F"), _fine(o, nm + "_f", BFACES), rtol=0, atol=2e-4)
+        np.testing.assert_array_equal(s.get("F" + nm.upper() + "_F"), _fine(o, "f" + nm + "_f", BFACES))
+        np.testing.assert_array_equal(s.get("F" + nm.upper() + "_C").ravel(), o.array("f" + nm + "_c"))
+
+
+def test_project_errors(M):
+    o = _scene()
+    s = _upload(M, o)
+    nan = np.full(o.nslots * BFACES, np.nan, np.float32)
+    s.set("U_F", nan)
+    with pytest.raises(M.MaelstromError, match="Nonfinite pressure right-hand side"):
+        M.sparse.project(s, 0.1, gamma=0.0, policy=0)
+    with pytest.raises(M.MaelstromError, match="Nonfinite pressure RHS"):
+        M.sparse.project(s, 0.1, gamma=0.0, policy=1)
+
+
+def test_steps_keep_volume(M):
+    """initialize! and three steps of the hot path: finite, volume kept to a few percent."""
+    print("case: sparse steps", flush=True)
+    s = M.SparseBasin(_basin(M), dx=DX, blocks=64, capacity=300000, cfl=4)
+    M.sparse.initialize(s)
+    v0 = s.volume0
+    for _ in range(3):
+        st = M.sparse.step(s, 0.1, tol=1e-6)
+        assert st.rel_residual <= 1.01e-6
+    v = M.sparse.volume(s)
+    assert 0.95 < v / v0 < 1.05
+    assert np.isfinite(s.get("U_F")).all() and s.n > 0

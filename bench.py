@@ -1,13 +1,15 @@
 #!/usr/bin/env python3
 """Benchmark of the hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dense256]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sparse5cm]
 
 A step is one pass of the hot path in the reference's order over the basin's live
-particles: particles to grid, classify, gravity, pressure projection (MG-PCG to a
-1e-6 relative FP64 residual, solved from zero each step), grid to particles,
-resample. Inputs are synthetic (the balanced Lamb-Oseen vortex of `initialize!`) and
-resident on the GPU before the timed region. Prints one JSON line.
+particles. On the sparse basin (the default, 5 cm fine cells): refine the blocks,
+particles to grid, redistance, classify, gravity, pressure projection (the composite
+multigrid-preconditioned flexible CG, FGMRES where CG fails, to a 1e-6 relative
+residual), grid to particles, resample. On the dense basin the same without
+refinement (MG-PCG). Inputs are synthetic (the balanced Lamb-Oseen vortex of
+`initialize!`) and resident on the GPU before the timed region. Prints one JSON line.
 """
 import argparse
 import json
@@ -24,12 +26,31 @@ HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
     # name: (basin kwargs, dx, description)
+    "sparse5cm": (dict(radius=18.0, depth=12.0, Γ=225.0, core=2.5), 0.05,
+                  "sparse SparseBasin at 5 cm: 48x48x20 blocks of 16^3 fine cells over a 192x192x80 coarse grid "
+                  "(20 cm), Lamb-Oseen vortex Γ 225 m^2/s core 2.5 m, 8 particles/cell in the 4-cell band, "
+                  "composite MG flexible CG then FGMRES(20) to 1e-6, CFL 4"),
+    "sparse60cm": (dict(radius=18.0, depth=12.0, Γ=225.0, core=2.5), 0.6,
+                   "sparse SparseBasin at 60 cm: 4x4x4 blocks over a 16x16x16 coarse grid, the same vortex and solve"),
     "dense256": (dict(radius=18.0, depth=12.0, Γ=225.0, core=2.5), 0.15,
                  "dense FlipBasin 256x256x128, dx 0.15 m, Lamb-Oseen vortex Γ 225 m^2/s core 2.5 m, 8 particles/cell "
                  "in the 3-cell band, MG-PCG (6 levels, 2 RBGS sweeps) to 1e-6, CFL 4"),
     "dense64": (dict(radius=18.0, depth=12.0, Γ=225.0, core=2.5), 0.6,
                 "dense FlipBasin 64x64x32, dx 0.6 m, the same vortex, MG-PCG (4 levels) to 1e-6"),
 }
+
+
+def sparse_bytes(kernel, live_cells, n_particles):
+    """Compulsory bytes of one launch of `kernel` on the sparse basin, at the reference's
+    element sizes, over the live fine cells (the fine part of the level-0 kernels)."""
+    return {
+        # x, b, diag (4 B each), three face couplings (each face shared by two cells), the
+        # type (1 B), and the result (4 B) per fine cell
+        "jacobi": 29 * live_cells,
+        "g2p": 60 * n_particles,
+        "p2g": 64 * n_particles,
+        "scatter": 136 * n_particles,
+    }[kernel]
 
 
 def algorithmic_bytes(kernel, s, n_particles):
@@ -95,12 +116,134 @@ def cpu_baseline(s, iters_per_step, dt):
     return {"value": round(ms, 1), "unit": "ms/step", "cores": 1, "kind": "port", "sample": sample}
 
 
+def sparse_cpu_baseline(s, iters_per_step, dt, sample_particles=1_000_000):
+    """The oracle's FP32 CPU port on the same sparse state (one core): P2G, G2P and
+    resample on a sample of the particles (scaled to all), the pressure set-up and flexible
+    CG iterations on the whole grid (scaled to the GPU's iterations per step)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as orc
+    from maelstrom_amd.sparse import BCELLS, BFACES
+    slot, where, ns, free = s.topology()
+    n = s.n
+    m = min(n, sample_particles)
+    o = orc.SparseOracle(s.basin.radius, s.basin.depth, s.basin.rimwall, s.grid.dx, 2, max(ns, 1), int(1.5 * n) + 4096,
+                         s.band, s.nmin)
+    # the topology and the fields from the GPU
+    o.array("slot")[:] = slot
+    o.array("where")[:3 * ns] = where[:ns].ravel()
+    _set_nslots(o, ns, free)
+    for gname, oname in [("PHI", "phi"), ("PHI0", "phi0"), ("TYPES", "types"), ("U", "u"), ("V", "v"), ("W", "w"),
+                         ("U0", "u0"), ("V0", "v0"), ("W0", "w0"), ("FU", "fu"), ("FV", "fv"), ("FW", "fw")]:
+        o.array(oname + "_f")[:] = s.get(gname + "_F")
+        o.array(oname + "_c")[:] = s.get(gname + "_C").ravel()
+    names = ["px", "py", "pz", "vx", "vy", "vz"] + ["c%d" % k for k in range(9)] + ["lag"]
+    for nm in names:
+        o.array(nm)[:m] = s.get(nm.upper(), count=m)
+    o.n = m
+    t = {}
+    t0 = time.perf_counter()
+    o.p2g(dt, fp32=True)
+    t["p2g"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.g2p(dt)
+    t["g2p"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.resample(seed=s.steps + 1)
+    t["resample"] = time.perf_counter() - t0
+    scale = n / max(m, 1)
+    t0 = time.perf_counter()
+    o.project(gamma=0.0, tol=1e-30, maxiter=1, policy=0, fp32=True)
+    t1 = time.perf_counter()
+    o.project(gamma=0.0, tol=1e-30, maxiter=3, policy=0, fp32=True)
+    t2 = time.perf_counter()
+    per_iter = (t2 - t1 - (t1 - t0)) / 2
+    setup = max((t1 - t0) - per_iter, 0.0)
+    ms = 1e3 * ((t["p2g"] + t["g2p"] + t["resample"]) * scale + setup + per_iter * iters_per_step)
+    sample = ("oracle FP32 port, one core: P2G, G2P and resample on %d of the %d particles (scaled by %.1f); the "
+              "pressure set-up and 1 and 3 flexible-CG iterations on the whole grid, per-iteration cost scaled to the "
+              "GPU's %.1f iterations/step (%.1f s of CPU work)" % (m, n, scale, iters_per_step,
+                                                                    (t2 - t0) + t["p2g"] + t["g2p"] + t["resample"]))
+    return {"value": round(ms, 1), "unit": "ms/step", "cores": 1, "kind": "port", "sample": sample}
+
+
+def _set_nslots(o, ns, free):
+    """Give the oracle handle the GPU's slot count and free list (its where/slot arrays are
+    already filled)."""
+    import oracle as orc
+    import numpy as np
+    free = np.ascontiguousarray(free, np.int32)
+    orc._slib().orc_sparse_set_topology_counts(o.h, int(ns), free.ctypes.data if free.size else None, free.size)
+
+
+def sparse_main(args, desc, kw, dx):
+    import ctypes as C
+    import numpy as np
+    import maelstrom_amd as m
+    from maelstrom_amd import sparse as sp
+    from maelstrom_amd._lib import KERNEL_ID
+    b = m.Basin(**kw)
+    s = m.SparseBasin(b, dx=dx, band=4, nmin=8, cfl=4.0, capacity=30_000_000, blocks=8000)
+    sp.initialize(s)
+    umax = max(float(np.abs(s.get(f)).max()) for f in ("U_F", "V_F", "W_F", "U_C", "V_C", "W_C"))
+    left = 1.0 / 24
+    dt = left / math.ceil(left / (s.cfl * dx / max(umax, 1e-3)))
+    solve = dict(tol=args.tol, maxiter=240, restart=20, policy=2, boost=1.5, precision=args.precision)
+    for _ in range(args.warmup):
+        sp.step(s, dt, **solve)
+    s.sync()
+    L = m.lib()
+    kernel = "jacobi" if args.kernel in ("rbgs", "jacobi") else args.kernel
+    kid = KERNEL_ID["rbgs" if kernel == "jacobi" else kernel]
+    s.ctx.check(L.mael_sparse_ktimer(s.h, kid, 64 * 1024))
+    iters, fg, pressure_ms, nparts, live = [], [], [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = sp.step(s, dt, **solve)
+        iters.append(st.iterations)
+        fg.append(st.used_fgmres)
+        pressure_ms.append(st.ms)
+        nparts.append(s.n)
+    s.sync()
+    t1 = time.perf_counter()
+    slot, where, ns, free = s.topology()
+    live_slots = int((where[:ns, 0] >= 0).sum())
+    kms, kl = C.c_double(), C.c_int32()
+    s.ctx.check(L.mael_sparse_ktime(s.h, C.byref(kms), C.byref(kl)))
+    ms_step = 1e3 * (t1 - t0) / args.steps
+    avg_launch_ms = kms.value / max(kl.value, 1)
+    n_avg = sum(nparts) / len(nparts)
+    abytes = sparse_bytes(kernel, live_slots * 4096, n_avg)
+    achieved = abytes / (avg_launch_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": round(ms_step, 3), "unit": "ms/step", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": desc, "blocks": list(int(x) for x in s.grid.nb), "coarse_cells": list(s.nc),
+                   "live_blocks": live_slots, "fine_cells": live_slots * 4096, "particles": int(n_avg),
+                   "dt_s": round(dt, 5), "tol": args.tol, "policy": "fcg-then-fgmres(20)",
+                   "precision": ["fp32", "fp16", "bf16"][args.precision],
+                   "phases": "refine, p2g, redistance, classify, gravity, project, g2p, resample"},
+        "pressure_ms": round(sum(pressure_ms) / len(pressure_ms), 3),
+        "iterations": round(sum(iters) / len(iters), 2),
+        "fgmres_steps": int(sum(fg)),
+        "mparticles_per_s": round(n_avg / (ms_step * 1e-3) / 1e6, 2),
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "launches": kl.value, "avg_launch_us": round(avg_launch_ms * 1e3, 3),
+                     "algorithmic_bytes_per_launch": int(abytes)},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = sparse_cpu_baseline(s, out["iterations"], dt)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="dense256", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="sparse5cm", choices=sorted(WORKLOADS))
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--precision", type=int, default=0)
     ap.add_argument("--kernel", default="rbgs", help="the kernel the roofline object reports")
@@ -111,9 +254,11 @@ def main():
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    kw, dx, desc = WORKLOADS[args.workload]
+    if args.workload.startswith("sparse"):
+        return sparse_main(args, desc, kw, dx)
     import maelstrom_amd as m
     from maelstrom_amd._lib import KERNEL_ID
-    kw, dx, desc = WORKLOADS[args.workload]
     b = m.Basin(**kw)
     s = m.FlipBasin(b, dx=dx, band=4, nmin=8, cfl=4.0, capacity=30_000_000)
     m.initialize(s)

@@ -50,6 +50,12 @@ void orc_sparse_free_list(void* p, int32_t* out) {
     std::copy(s.t.freel.begin(), s.t.freel.end(), out);
 }
 void orc_sparse_set_n(void* p, int64_t n) { auto& s = ((OrcSparse*)p)->s; s.ps[s.cur].n = n; }
+// The slot count and free list after `slot` and `where` were written through their arrays.
+void orc_sparse_set_topology_counts(void* p, int32_t nslots, const int32_t* freelist, int32_t nfree) {
+    auto& s = ((OrcSparse*)p)->s;
+    s.t.nslots = nslots;
+    s.t.freel.assign(freelist, freelist + nfree);
+}
 
 // A named array of the state: its address and element count (element type by name).
 int orc_sparse_array(void* p, const char* name, void** ptr, int64_t* len) {

@@ -372,6 +372,7 @@ def _slib():
             "orc_sparse_info": (None, [p, C.POINTER(SGrid), P32, P32, P64, P32]),
             "orc_sparse_free_list": (None, [p, p]),
             "orc_sparse_set_n": (None, [p, i64]),
+            "orc_sparse_set_topology_counts": (None, [p, i32, p, i32]),
             "orc_sparse_array": (C.c_int, [p, C.c_char_p, C.POINTER(p), P64]),
             "orc_sparse_marks": (None, [p, f32, f32, i32, i32, p]),
             "orc_sparse_refine": (C.c_int, [p, p, p, P32, p, P32]),

@@ -282,6 +282,41 @@ int mael_sparse_sync(mael_sparse *s);
 int mael_sparse_ktimer(mael_sparse *s, int32_t kernel, int32_t launches);
 int mael_sparse_ktime(mael_sparse *s, double *ms, int32_t *launches);
 
+/* The rest of a sparse step (sflip.jl:2295-2318), support phases around the hot path: */
+/* `advect!(s, dt; carried)` (sflip.jl:773-788): the particles along (u, v, w) with ST-FLIP's
+ * lag (seed `s.steps`), φ into φ0 and the carried velocity (u, or the reflected ur when
+ * `carry_reflected`) into u0. */
+int mael_sparse_advect(mael_sparse *s, float dt, int32_t carry_reflected, float gravity, int32_t seed);
+/* `filltrapped!(s; maxrounds)` (sflip.jl:972-990). */
+int mael_sparse_filltrapped(mael_sparse *s, int32_t maxrounds);
+/* The stirring model of `forces!` (basin.jl: Basin, Race, Stir / SurfaceStir). */
+typedef struct {
+    float radius, band, relax, core, gamma_circ;   /* the basin's */
+    float current[2];                              /* the race's */
+    float eddy, eddyscale, eddytime, guide, free, turbulence, boil, reach;
+    float origin[2], offset;                       /* the sea's coordinates and film time offset */
+    int32_t surface;                               /* 0 Stir, 1 SurfaceStir */
+    float hradius, ramp;                           /* SurfaceStir: the heights table's radius, the onset ramp */
+} mael_stir;
+/* `waves`: 7 floats a wave (direction x, y, wave number, angular frequency, amplitude
+ * scaled by the race, steepness, phase), those the reference keeps (basin.jl:101);
+ * `heights`: SurfaceStir's surface table (basin.jl:198). */
+int mael_sparse_set_stir(mael_sparse *s, const mael_stir *stir, const float *waves, int32_t nwaves, const float *heights,
+                         int32_t nheights);
+/* `forces!(s, dt)` (sflip.jl:1043-1056) at time `time`. */
+int mael_sparse_forces(mael_sparse *s, float dt, float time, float gravity);
+/* Advection-reflection (sflip.jl:2118-2139): ur = u before the projection; the kick
+ * ur = valid ? u - ur : 0 after it; ur += u after the extrapolation. */
+int mael_sparse_save_reflect(mael_sparse *s);
+int mael_sparse_kick(mael_sparse *s);
+int mael_sparse_reflected(mael_sparse *s);
+/* `extrapolate!(s; layers, coarselayers, kick)` (sflip.jl:2097-2116). */
+int mael_sparse_extrapolate(mael_sparse *s, int32_t layers, int32_t coarselayers, int32_t kick);
+/* Which flow G2P and resampling read: 0 (u, v, w), 1 the reflected (ur, vr, wr). */
+int mael_sparse_set_flow(mael_sparse *s, int32_t reflected);
+/* `maxspeed(s)` (sflip.jl:2264-2272). */
+int mael_sparse_maxspeed(mael_sparse *s, float *speed);
+
 /* ── launch timing (no counterpart in the reference, which has no profiler hooks) ── */
 /* Kernel ids for the launch timer. */
 enum {

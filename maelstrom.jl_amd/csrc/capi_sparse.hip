@@ -312,4 +312,57 @@ int mael_sparse_ktime(mael_sparse* s, double* ms, int32_t* launches) {
     });
 }
 
+int mael_sparse_advect(mael_sparse* s, float dt, int32_t carry_reflected, float gravity, int32_t seed) {
+    if (!s) return MAEL_E_ARG;
+    return guard(s->ctx, [&] { s->b->advect(dt, carry_reflected != 0, gravity, seed); });
+}
+int mael_sparse_filltrapped(mael_sparse* s, int32_t maxrounds) {
+    if (!s || maxrounds < 0) return MAEL_E_ARG;
+    return guard(s->ctx, [&] { s->b->filltrapped(maxrounds); });
+}
+int mael_sparse_set_stir(mael_sparse* s, const mael_stir* p, const float* waves, int32_t nwaves, const float* heights,
+                         int32_t nheights) {
+    if (!s || !p || nwaves < 0 || nheights < 0 || (nwaves && !waves) || (nheights && !heights)) return MAEL_E_ARG;
+    if (p->surface && nheights < 2) return MAEL_E_ARG;
+    return guard(s->ctx, [&] {
+        mael::StirParams q;
+        q.radius = p->radius; q.band = p->band; q.relax = p->relax; q.core = p->core; q.Gamma = p->gamma_circ;
+        q.current[0] = p->current[0]; q.current[1] = p->current[1];
+        q.eddy = p->eddy; q.eddyscale = p->eddyscale; q.eddytime = p->eddytime; q.guide = p->guide; q.free = p->free;
+        q.turbulence = p->turbulence; q.boil = p->boil; q.reach = p->reach;
+        q.origin[0] = p->origin[0]; q.origin[1] = p->origin[1]; q.offset = p->offset;
+        q.surface = p->surface; q.hradius = p->hradius; q.ramp = p->ramp;
+        s->b->set_stir(q, waves, nwaves, heights, nheights);
+    });
+}
+int mael_sparse_forces(mael_sparse* s, float dt, float time, float gravity) {
+    if (!s) return MAEL_E_ARG;
+    return guard(s->ctx, [&] { s->b->forces(dt, time, gravity); });
+}
+int mael_sparse_save_reflect(mael_sparse* s) {
+    if (!s) return MAEL_E_ARG;
+    return guard(s->ctx, [&] { s->b->save_reflect(); });
+}
+int mael_sparse_kick(mael_sparse* s) {
+    if (!s) return MAEL_E_ARG;
+    return guard(s->ctx, [&] { s->b->kick(); });
+}
+int mael_sparse_reflected(mael_sparse* s) {
+    if (!s) return MAEL_E_ARG;
+    return guard(s->ctx, [&] { s->b->reflected(); });
+}
+int mael_sparse_extrapolate(mael_sparse* s, int32_t layers, int32_t coarselayers, int32_t kick) {
+    if (!s || layers < 0 || coarselayers < 0) return MAEL_E_ARG;
+    return guard(s->ctx, [&] { s->b->extrapolate(layers, coarselayers, kick != 0); });
+}
+int mael_sparse_set_flow(mael_sparse* s, int32_t reflected) {
+    if (!s) return MAEL_E_ARG;
+    s->b->reflect_flow = reflected != 0;
+    return MAEL_OK;
+}
+int mael_sparse_maxspeed(mael_sparse* s, float* speed) {
+    if (!s || !speed) return MAEL_E_ARG;
+    return guard(s->ctx, [&] { *speed = s->b->maxspeed(); });
+}
+
 }  // extern "C"

@@ -232,6 +232,35 @@ __device__ inline float finegather_value(const STopo& t, const V* __restrict__ u
     return vel;
 }
 
+// sflip.jl:1030-1031 walledface: a wall on either side of the face at p (cells h wide).
+__device__ inline bool walledface_dev(const SGrid& g, const float p[3], int a, float h) {
+    float e = 0.5f * h;
+    float lo[3] = {p[0], p[1], p[2]}, hi[3] = {p[0], p[1], p[2]};
+    lo[a] = p[a] - e;
+    hi[a] = p[a] + e;
+    return sissolid(g, lo[0], lo[1], lo[2]) || sissolid(g, hi[0], hi[1], hi[2]);
+}
+
+// The stirring model's parameters (basin.jl:13-30 Basin, :69-80 Race, :92-103 Stir,
+// :191-205 SurfaceStir); the waves (7 floats each: direction x, y, wave number, angular
+// frequency, amplitude already scaled by the race, steepness, phase) and the surface
+// table live in device arrays.
+struct StirParams {
+    float radius = 18, band = 3, relax = 0.3f, core = 2.5f, Gamma = 225;
+    float current[2] = {0, 0};
+    float eddy = 0.6f, eddyscale = 3.5f, eddytime = 4, guide = 4, free = 0.6f, turbulence = 0, boil = 0.5f, reach = 2.5f;
+    float origin[2] = {0, 0};
+    float offset = 0;
+    int surface = 0;                 // 0 Stir, 1 SurfaceStir
+    float hradius = 0, ramp = 2;     // SurfaceStir's table radius and onset ramp
+    const float* waves = nullptr;
+    int nwaves = 0;
+    const float* heights = nullptr;
+    int nheights = 0;
+};
+
+struct SFlow;
+
 // A field: its fine pool and the dense coarse array (sflip.jl:86-90).
 template <class T> struct SField {
     DBuf<T> fine, coarse;
@@ -281,6 +310,12 @@ struct SparseBasin {
     SField<uint8_t> types;
     SField<float> u[3], u0[3];
     SField<uint8_t> fl[3];
+    SField<float> ur[3], ur0[3];              // the reflected velocity and its scratch (sflip.jl:279-283)
+    SField<uint8_t> fl2[3];                   // the extrapolation's scratch flags
+    DBuf<uint8_t> reach;                      // trapped air: the coarse cells the atmosphere reaches
+    DBuf<int> changed;
+    StirParams stirp;
+    DBuf<float> stir_waves, stir_heights;
     DBuf<int32_t> phip;
     DBuf<unsigned long long> mom[3], wt[3];
 
@@ -337,6 +372,18 @@ struct SparseBasin {
     void fill_flags(uint8_t v);
     double volume();
     void gravity(float dt, float gacc);
+    // the rest of a step (sflip.jl:2295-2318)
+    SFlow flow(bool reflected) const;
+    void advect(float dt, bool carry_reflected, float gacc, int32_t seed);
+    void filltrapped(int maxrounds);
+    void set_stir(const StirParams& p, const float* waves, int nwaves, const float* heights, int nheights);
+    void forces(float dt, float time, float gacc);
+    void save_reflect();
+    void kick();
+    void extrapolate(int layers, int coarselayers, bool withkick);
+    void reflected();
+    float maxspeed();
+    bool reflect_flow = false;                // G2P and resampling read ur (the reflected velocity)
 };
 
 // restriction kernels shared by several phases (sflip.jl:583-612)

@@ -5,6 +5,7 @@
 // Ranks are stable and deposits fixed point, as on the dense grid.
 #include "sparse.hpp"
 #include "scan.hpp"
+#include "p2g_tiled.hpp"
 
 namespace mael {
 
@@ -147,9 +148,10 @@ int64_t SparseBasin::resample(bool seeding, uint32_t seed) {
         MAEL_LAUNCH_CHECK();
     }
     if (C) {
+        const SField<float>* fv = reflect_flow ? ur : u;   // the flow new particles take (sflip.jl:2312-2316)
         kt.begin(K_SPAWN, s);
         hipLaunchKernelGGL(k_sparse_spawn, dim3(cdiv(C, 256)), dim3(256), 0, s, dst, counts.p, totals.p, offsets.p, phi.fine.p,
-                           phi.coarse.p, u[0].fine.p, u[1].fine.p, u[2].fine.p, t, nmin, deep, seeding ? 1 : 0, seed,
+                           phi.coarse.p, fv[0].fine.p, fv[1].fine.p, fv[2].fine.p, t, nmin, deep, seeding ? 1 : 0, seed,
                            (int32_t)C);
         kt.end(K_SPAWN, s);
         MAEL_LAUNCH_CHECK();
@@ -162,46 +164,11 @@ int64_t SparseBasin::resample(bool seeding, uint32_t seed) {
 // ─── particles to grid ───────────────────────────────────────────────────────
 
 // flip.jl:513-537 over the topology: nodes are fine faces (the upper block's copy on a
-// plane between blocks), fixed-point integer atomics.
-__global__ __launch_bounds__(256) void k_sparse_p2g(PView ps, int64_t n, STopo t, float dt, unsigned long long* __restrict__ mu,
-                                                    unsigned long long* __restrict__ mv, unsigned long long* __restrict__ mw,
-                                                    unsigned long long* __restrict__ wu, unsigned long long* __restrict__ wv,
-                                                    unsigned long long* __restrict__ ww) {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const SGrid& g = t.g;
-    float xp = ps.f[0][p], yp = ps.f[1][p], zp = ps.f[2][p];
-    float wT = temporal(ps.f[15][p], dt);
-    unsigned long long* mom[3] = {mu, mv, mw};
-    unsigned long long* wt[3] = {wu, wv, ww};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float q0 = ispace(xp, g.lo[0], g.dx, a == 0 ? 0.0f : 0.5f);
-        float q1 = ispace(yp, g.lo[1], g.dx, a == 1 ? 0.0f : 0.5f);
-        float q2 = ispace(zp, g.lo[2], g.dx, a == 2 ? 0.0f : 0.5f);
-        BSpline bx = bspline(q0), by = bspline(q1), bz = bspline(q2);
-        float vp = ps.f[3 + a][p];
-        float c0 = ps.f[6 + 3 * a][p], c1 = ps.f[7 + 3 * a][p], c2 = ps.f[8 + 3 * a][p];
-        for (int mz = 0; mz < 3; ++mz) {
-            int K = bz.b + mz;
-            float wz = pick(bz, mz), d2 = ((float)K - q2) * g.dx;
-            for (int my = 0; my < 3; ++my) {
-                int J = by.b + my;
-                float wy = pick(by, my), d1 = ((float)J - q1) * g.dx;
-#pragma unroll
-                for (int mx = 0; mx < 3; ++mx) {
-                    int I = bx.b + mx;
-                    int32_t idx = finefaceat(t, a, I, J, K);
-                    if (idx < 0) continue;
-                    float wgt = wT * pick(bx, mx) * wy * wz;
-                    float d0 = ((float)I - q0) * g.dx;
-                    atomicAdd(&mom[a][idx], tofix(wgt * (vp + (c0 * d0 + c1 * d1 + c2 * d2))));
-                    atomicAdd(&wt[a][idx], tofix(wgt));
-                }
-            }
-        }
-    }
-}
+// plane between blocks), deposited by the tiled fixed-point reduction.
+struct SparseNodes {
+    STopo t;
+    __device__ int32_t index(int a, int I, int J, int K) const { return finefaceat(t, a, I, J, K); }
+};
 
 // flip.jl:539-560 over the topology.
 __global__ __launch_bounds__(256) void k_sparse_splat(PView ps, int64_t n, STopo t, float dt, int32_t* __restrict__ phip) {
@@ -285,8 +252,8 @@ void SparseBasin::p2g(float dt) {
     if (n > 0) {
         PView ps = view(cur);
         kt.begin(K_P2G, s);
-        hipLaunchKernelGGL(k_sparse_p2g, dim3(cdiv(n, 256)), dim3(256), 0, s, ps, n, t, dt, mom[0].p, mom[1].p, mom[2].p,
-                           wt[0].p, wt[1].p, wt[2].p);
+        P2GArgs A{{g.lo[0], g.lo[1], g.lo[2]}, g.dx, dt, {mom[0].p, mom[1].p, mom[2].p}, {wt[0].p, wt[1].p, wt[2].p}};
+        hipLaunchKernelGGL(k_p2g_tiled<SparseNodes>, dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s, ps, n, SparseNodes{t}, A);
         kt.end(K_P2G, s);
         for (int a = 0; a < 3; ++a) {
             hipLaunchKernelGGL(k_syncplanes, dim3(cdiv(nslots * 256, 256)), dim3(256), 0, s, mom[a].p, t, a, nslots);
@@ -345,8 +312,9 @@ __global__ __launch_bounds__(256) void k_sparse_g2p(PView ps, int64_t n, STopo t
 void SparseBasin::g2p(float dt, float grav) {
     if (n == 0) return;
     kt.begin(K_G2P, st.s);
-    hipLaunchKernelGGL(k_sparse_g2p, dim3(cdiv(n, 256)), dim3(256), 0, st.s, view(cur), n, topo(), u[0].fine.p, u[1].fine.p,
-                       u[2].fine.p, fl[0].fine.p, fl[1].fine.p, fl[2].fine.p, dt, grav);
+    const SField<float>* fv = reflect_flow ? ur : u;   // sflip.jl:2312: the reflected velocity in a first half-step
+    hipLaunchKernelGGL(k_sparse_g2p, dim3(cdiv(n, 256)), dim3(256), 0, st.s, view(cur), n, topo(), fv[0].fine.p, fv[1].fine.p,
+                       fv[2].fine.p, fl[0].fine.p, fl[1].fine.p, fl[2].fine.p, dt, grav);
     kt.end(K_G2P, st.s);
     MAEL_LAUNCH_CHECK();
 }

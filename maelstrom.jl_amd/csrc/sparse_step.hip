@@ -1,0 +1,630 @@
+// The sparse basin's support phases around the hot path, so a step runs the reference's
+// `substep!` (sflip.jl:2295-2318): advection of the particles and the grid
+// (sflip.jl:715-788, flip.jl:426-499), trapped air (sflip.jl:950-990, flip.jl:690-760),
+// the stirring forces (sflip.jl:992-1056, basin.jl:105-235), and the projection's kick,
+// extrapolation and reflection (sflip.jl:2017-2139).
+#include "sparse.hpp"
+#include "scan.hpp"
+
+namespace mael {
+
+__constant__ int NBX[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+
+// A velocity field of the topology: face arrays (fine pool, coarse) and their flags.
+struct SFlow {
+    STopo t;
+    const float* uf[3];
+    const float* uc[3];
+    const uint8_t* ff[3];
+    const uint8_t* fc[3];
+};
+
+// sflip.jl:177-183 velocity and reached through the topology.
+__device__ inline void svelocity(const SFlow& f, float px, float py, float pz, float v[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[a] = facesample(f.t, f.uf[a], f.uc[a], a, px, py, pz);
+}
+__device__ inline bool sreached(const SFlow& f, float px, float py, float pz) {
+    float m = facesample(f.t, f.ff[0], f.fc[0], 0, px, py, pz);
+    m = jmin(m, facesample(f.t, f.ff[1], f.fc[1], 1, px, py, pz));
+    m = jmin(m, facesample(f.t, f.ff[2], f.fc[2], 2, px, py, pz));
+    return m > 0.999f;
+}
+// flip.jl:367 substeps.
+__device__ inline int substeps(float cells) {
+    float c = ceilf(cells);
+    int m = c != c ? 1 : (c >= 32.0f ? 32 : (c <= 1.0f ? 1 : (int)c));
+    return m;
+}
+// sflip.jl:185-191 inwalls.
+__device__ inline void sinwalls(const SGrid& g, float& x, float& y, float& z) {
+    float m = 0.25f * g.dx;
+    float r = sqrtf(x * x + y * y);
+    if (r > g.radius - m) {
+        x = __fdiv_rn(x * (g.radius - m), r);
+        y = __fdiv_rn(y * (g.radius - m), r);
+    }
+    if (z < g.floor + m) z = g.floor + m;
+}
+
+// flip.jl:438-464 advectparticles_kernel! over the topology.
+__global__ __launch_bounds__(256) void k_sadvect_particles(PView ps, int64_t n, SFlow flow, float dt, float gravity,
+                                                           int32_t seed) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float y0 = ps.f[0][p], y1 = ps.f[1][p], y2 = ps.f[2][p];
+    float b0 = ps.f[3][p], b1 = ps.f[4][p], b2 = ps.f[5][p];
+    float h0 = flow.t.g.dx;
+    float speed = sqrtf(b0 * b0 + b1 * b1 + b2 * b2);
+    float gm = strength(speed, dt, h0);
+    float xi = hash01((int32_t)(p + 1), seed) - 0.5f;
+    float tau = ps.f[15][p];
+    float act = jclamp(dt + tau + gm * xi * dt, 0.0f, 2.0f * dt);
+    ps.f[15][p] = dt + tau - act;
+    float vf[3];
+    svelocity(flow, y0, y1, y2, vf);
+    int m = substeps(__fdiv_rn(jmax(speed, sqrtf(vf[0] * vf[0] + vf[1] * vf[1] + vf[2] * vf[2])) * act, h0));
+    float h = __fdiv_rn(act, (float)m);
+    for (int s = 0; s < m; ++s) {
+        if (sreached(flow, y0, y1, y2)) {
+            float v1[3];
+            svelocity(flow, y0, y1, y2, v1);
+            float hh = 0.5f * h;
+            svelocity(flow, y0 + hh * v1[0], y1 + hh * v1[1], y2 + hh * v1[2], vf);
+            b0 = vf[0]; b1 = vf[1]; b2 = vf[2];
+            y0 = y0 + h * b0; y1 = y1 + h * b1; y2 = y2 + h * b2;
+        } else {
+            float dz = 0.5f * gravity * h;
+            y0 = y0 + h * b0; y1 = y1 + h * b1; y2 = y2 + h * (b2 - dz);
+            b2 = b2 - gravity * h;
+        }
+        sinwalls(flow.t.g, y0, y1, y2);
+    }
+    ps.f[0][p] = y0; ps.f[1][p] = y1; ps.f[2][p] = y2;
+}
+
+// sflip.jl:715-731 backtrace: RK2 sub-steps of at most `cells` cells `h` wide.
+__device__ inline void backtrace(const SFlow& f, float& px, float& py, float& pz, float dt, float h) {
+    float v[3];
+    svelocity(f, px, py, pz, v);
+    int m = substeps(__fdiv_rn(sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * dt, 4.0f * h));
+    float k = __fdiv_rn(dt, (float)m);
+    for (int s = 0; s < m; ++s) {
+        float v1[3], v2[3];
+        svelocity(f, px, py, pz, v1);
+        float hk = 0.5f * k;
+        svelocity(f, px - hk * v1[0], py - hk * v1[1], pz - hk * v1[2], v2);
+        px = px - k * v2[0]; py = py - k * v2[1]; pz = pz - k * v2[2];
+    }
+}
+
+// sflip.jl:733-763: cells (phi) and faces (the carried velocity) along the flow, into
+// phi0 and u0; fine parts over the pool, coarse cells outside blocks, all coarse faces.
+__global__ void k_sadvect_finecells(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ srcc,
+                                    SFlow f, float dt, int32_t nf) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nf) return;
+    SCell c = finecell(f.t, n);
+    if (c.w.x < 0) return;
+    float p[3];
+    finecentre(f.t.g, c.w.x * 16 + c.i, c.w.y * 16 + c.j, c.w.z * 16 + c.k, p);
+    backtrace(f, p[0], p[1], p[2], dt, f.t.g.dx);
+    out[n] = cellsample(f.t, src, srcc, p[0], p[1], p[2]);
+}
+__global__ void k_sadvect_finefaces(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ srcc,
+                                    SFlow f, int a, float dt, int32_t nfa) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nfa) return;
+    SCell c = fineface(f.t, a, n);
+    if (c.w.x < 0) return;
+    float p[3];
+    facepos(f.t.g, a, c.w.x * 16 + c.i, c.w.y * 16 + c.j, c.w.z * 16 + c.k, f.t.g.dx, p);
+    backtrace(f, p[0], p[1], p[2], dt, f.t.g.dx);
+    out[n] = facesample(f.t, src, srcc, a, p[0], p[1], p[2]);
+}
+__global__ void k_sadvect_coarsecells(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ srcc,
+                                      SFlow f, float dt) {
+    int n0 = f.t.g.nc(0), n1 = f.t.g.nc(1), n2 = f.t.g.nc(2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    if (covered(f.t, I, J, K)) return;
+    float p[3];
+    coarsecentre(f.t.g, I, J, K, p);
+    backtrace(f, p[0], p[1], p[2], dt, f.t.g.cdx());
+    out[l] = cellsample(f.t, src, srcc, p[0], p[1], p[2]);
+}
+__global__ void k_sadvect_coarsefaces(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ srcc,
+                                      SFlow f, int a, float dt) {
+    int n0 = f.t.g.nc(0) + (a == 0), n1 = f.t.g.nc(1) + (a == 1), n2 = f.t.g.nc(2) + (a == 2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    float p[3];
+    facepos(f.t.g, a, I, J, K, f.t.g.cdx(), p);
+    backtrace(f, p[0], p[1], p[2], dt, f.t.g.cdx());
+    out[l] = facesample(f.t, src, srcc, a, p[0], p[1], p[2]);
+}
+
+SFlow SparseBasin::flow(bool reflected) const {
+    SFlow f;
+    f.t = topo();
+    for (int a = 0; a < 3; ++a) {
+        f.uf[a] = reflected ? ur[a].fine.p : u[a].fine.p;
+        f.uc[a] = reflected ? ur[a].coarse.p : u[a].coarse.p;
+        f.ff[a] = fl[a].fine.p;
+        f.fc[a] = fl[a].coarse.p;
+    }
+    return f;
+}
+
+void SparseBasin::advect(float dt, bool carry_reflected, float gacc, int32_t seed) {
+    hipStream_t s = st.s;
+    SFlow f = flow(false);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_sadvect_particles, dim3(cdiv(n, 256)), dim3(256), 0, s, view(cur), n, f, dt, gacc, seed);
+    }
+    if (nslots)
+        hipLaunchKernelGGL(k_sadvect_finecells, dim3(cdiv(finecells(), 256)), dim3(256), 0, s, phi0.fine.p, phi.fine.p,
+                           phi.coarse.p, f, dt, (int32_t)finecells());
+    hipLaunchKernelGGL(k_sadvect_coarsecells, dim3(cdiv(ncells(), 256)), dim3(256), 0, s, phi0.coarse.p, phi.fine.p,
+                       phi.coarse.p, f, dt);
+    for (int a = 0; a < 3; ++a) {
+        const SField<float>& c = carry_reflected ? ur[a] : u[a];
+        if (nslots)
+            hipLaunchKernelGGL(k_sadvect_finefaces, dim3(cdiv(finefaces(), 256)), dim3(256), 0, s, u0[a].fine.p, c.fine.p,
+                               c.coarse.p, f, a, dt, (int32_t)finefaces());
+        hipLaunchKernelGGL(k_sadvect_coarsefaces, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, s, u0[a].coarse.p, c.fine.p,
+                           c.coarse.p, f, a, dt);
+    }
+    MAEL_LAUNCH_CHECK();
+}
+
+// ─── trapped air (flip.jl:698-745, sflip.jl:950-990) ─────────────────────────
+
+__global__ void k_seedair(uint8_t* __restrict__ reach, const uint8_t* __restrict__ types, int3 n) {
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n.x * n.y * n.z) return;
+    int k = (int)(l / ((size_t)n.x * n.y));
+    reach[l] = (k == n.z - 1 && types[l] == AIR) ? 1 : 0;
+}
+__global__ void k_sweepair(uint8_t* __restrict__ reach, int* __restrict__ changed, const uint8_t* __restrict__ types, int3 n,
+                           int axis) {
+    int na = axis == 0 ? n.y : n.x, nb = axis == 2 ? n.y : n.z;
+    int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y * blockDim.y + threadIdx.y;
+    if (a >= na || b >= nb) return;
+    int len = axis == 0 ? n.x : (axis == 1 ? n.y : n.z);
+    int count = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        bool carry = false;
+        for (int m = 0; m < len; ++m) {
+            int l = pass == 0 ? m : len - 1 - m;
+            int i, j, k;
+            if (axis == 0) { i = l; j = a; k = b; }
+            else if (axis == 1) { i = a; j = l; k = b; }
+            else { i = a; j = b; k = l; }
+            size_t c = cl(i, j, k, n.x, n.y);
+            if (types[c] == AIR) {
+                if (reach[c]) carry = true;
+                else if (carry) { reach[c] = 1; ++count; }
+            } else {
+                carry = false;
+            }
+        }
+    }
+    if (count > 0) atomicAdd(changed, count);
+}
+__global__ void k_fillfine(uint8_t* __restrict__ tf, float* __restrict__ phi, const uint8_t* __restrict__ reach,
+                           const uint8_t* __restrict__ tc, STopo t, float depth, int32_t nf) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nf) return;
+    SCell c = finecell(t, n);
+    if (c.w.x < 0 || tf[n] != AIR) return;
+    int I = c.w.x * 16 + c.i, J = c.w.y * 16 + c.j, K = c.w.z * 16 + c.k;
+    size_t cc = cl(I >> 2, J >> 2, K >> 2, t.g.nc(0), t.g.nc(1));
+    if (reach[cc] == 0 && tc[cc] == AIR) {
+        tf[n] = LIQUID;
+        phi[n] = jmin(phi[n], -depth);
+    }
+}
+__global__ void k_fillcoarse(uint8_t* __restrict__ tc, float* __restrict__ phic, const uint8_t* __restrict__ reach, STopo t,
+                             float depth) {
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    if (!covered(t, I, J, K) && tc[l] == AIR && reach[l] == 0) {
+        tc[l] = LIQUID;
+        phic[l] = jmin(phic[l], -depth);
+    }
+}
+
+void SparseBasin::filltrapped(int maxrounds) {
+    hipStream_t s = st.s;
+    int3 nn = make_int3(g.nc(0), g.nc(1), g.nc(2));
+    hipLaunchKernelGGL(k_seedair, dim3(cdiv(ncells(), 256)), dim3(256), 0, s, reach.p, types.coarse.p, nn);
+    for (int r = 0; r < maxrounds; ++r) {
+        fill_async(changed.p, (size_t)1, 0, s);
+        hipLaunchKernelGGL(k_sweepair, dim3(cdiv(nn.x, 16), cdiv(nn.y, 16)), dim3(16, 16), 0, s, reach.p, changed.p,
+                           types.coarse.p, nn, 2);
+        hipLaunchKernelGGL(k_sweepair, dim3(cdiv(nn.y, 16), cdiv(nn.z, 16)), dim3(16, 16), 0, s, reach.p, changed.p,
+                           types.coarse.p, nn, 0);
+        hipLaunchKernelGGL(k_sweepair, dim3(cdiv(nn.x, 16), cdiv(nn.z, 16)), dim3(16, 16), 0, s, reach.p, changed.p,
+                           types.coarse.p, nn, 1);
+        MAEL_LAUNCH_CHECK();
+        MAEL_HIP(hipMemcpyAsync(st.pinned, changed.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        st.sync();
+        if (*(int*)st.pinned == 0) break;
+    }
+    if (nslots)
+        hipLaunchKernelGGL(k_fillfine, dim3(cdiv(finecells(), 256)), dim3(256), 0, s, types.fine.p, phi.fine.p, reach.p,
+                           types.coarse.p, topo(), 2.0f * g.dx, (int32_t)finecells());
+    hipLaunchKernelGGL(k_fillcoarse, dim3(cdiv(ncells(), 256)), dim3(256), 0, s, types.coarse.p, phi.coarse.p, reach.p, topo(),
+                       2.0f * g.cdx());
+    MAEL_LAUNCH_CHECK();
+}
+
+// ─── the stirring forces (basin.jl:105-235) ──────────────────────────────────
+
+// sea.jl:68-76 value noise.
+__device__ inline float vnoise(float x, float y) {
+    float xi = floorf(x), yi = floorf(y);
+    float fx = x - xi, fy = y - yi;
+    float sx = fx * fx * (3.0f - 2.0f * fx), sy = fy * fy * (3.0f - 2.0f * fy);
+    int32_t i = (int32_t)xi, j = (int32_t)yi;
+    float a = hash01(i, j), b = hash01(i + 1, j), c = hash01(i, j + 1), d = hash01(i + 1, j + 1);
+    return (a + (b - a) * sx) + ((c + (d - c) * sx) - (a + (b - a) * sx)) * sy;
+}
+// basin.jl:110-121 eddies.
+__device__ inline void eddies(const StirParams& P, float x, float y, float t, float e[2]) {
+    float s = P.eddyscale;
+    float a = __fdiv_rn(x - P.current[0] * t, s), b = __fdiv_rn(y - P.current[1] * t, s);
+    float tau = __fdiv_rn(t, P.eddytime);
+    auto psi = [&](float p, float q) {
+        return vnoise(p + 17.0f, q + tau) + 0.5f * vnoise(2.1f * p - 5.0f, 2.1f * q - 1.7f * tau);
+    };
+    float dl = 0.25f;
+    float dpdy = __fdiv_rn(psi(a, b + dl) - psi(a, b - dl), 2.0f * dl);
+    float dpdx = __fdiv_rn(psi(a + dl, b) - psi(a - dl, b), 2.0f * dl);
+    e[0] = P.eddy * dpdy;
+    e[1] = -P.eddy * dpdx;
+}
+// basin.jl:124-127 boils.
+__device__ inline float boils(const StirParams& P, float x, float y, float t) {
+    float nn = vnoise(__fdiv_rn(0.6f * (x - P.current[0] * t), P.eddyscale) + 41.0f,
+                      __fdiv_rn(0.6f * (y - P.current[1] * t), P.eddyscale) + __fdiv_rn(0.2f * t, P.eddytime));
+    return P.boil * (__fdiv_rn(jmax(nn - 0.6f, 0.0f), 0.4f) - 0.15f);
+}
+// basin.jl:139-151 orbital: the waves' velocity at sea coordinates (x, y), depth z, time t.
+__device__ inline void orbital(const StirParams& P, float x, float y, float z, float t, float u[2], float& w) {
+    u[0] = 0.0f; u[1] = 0.0f; w = 0.0f;
+    for (int k = 0; k < P.nwaves; ++k) {
+        const float* W = P.waves + 7 * k;   // d0 d1 k omega a q phase
+        float th = W[2] * (W[0] * x + W[1] * y) - W[3] * t + W[6];
+        float s = sinf(th), c = cosf(th);
+        float e = W[4] * W[3] * expf(W[2] * jmin(z, 0.0f));
+        u[0] += (e * s) * W[0];
+        u[1] += (e * s) * W[1];
+        w -= e * c;
+    }
+}
+__device__ inline float swirl_dev(const StirParams& P, float r) { return oseen_dev(P.Gamma, P.core, r); }
+
+// The stirring acceleration at (x, y, z) for the water's velocity vel (basin.jl:153-181
+// `Stir`, or basin.jl:208-235 `SurfaceStir` when P.surface).
+__device__ inline void stir(const StirParams& P, float x, float y, float z, const float vel[3], float t, float acc[3]) {
+    float r = sqrtf(x * x + y * y);
+    float k = jclamp(__fdiv_rn(r - (P.radius - P.band), P.band), 0.0f, 1.0f);
+    k = k * k * (3.0f - 2.0f * k);
+    float v = swirl_dev(P, r);
+    float turn = r > 1e-4f ? __fdiv_rn(v, r) : 0.0f;
+    float eddy[2];
+    eddies(P, x, y, t, eddy);
+    if (!P.surface) {
+        float core = jclamp(2.0f - __fdiv_rn(r, P.core), 0.0f, 1.0f);
+        float hold = (1.0f - k) * jmax(jclamp(__fdiv_rn(-z - P.free, P.free), 0.0f, 1.0f), core);
+        float churn = jclamp(__fdiv_rn(r - 2.0f, 2.0f), 0.0f, 1.0f);
+        float aim0 = -turn * y + P.current[0] + churn * eddy[0], aim1 = turn * x + P.current[1] + churn * eddy[1];
+        float gh = __fdiv_rn(hold, P.guide);
+        float guide0 = gh * (aim0 - vel[0]), guide1 = gh * (aim1 - vel[1]);
+        float top = (1.0f - k) * churn * jclamp(1.0f + __fdiv_rn(z, P.free), 0.0f, 1.0f);
+        float pf = P.eddy > 0.0f ? __fdiv_rn(top * P.turbulence, P.eddy) : 0.0f;
+        float push0 = P.eddy > 0.0f ? pf * eddy[0] : 0.0f, push1 = P.eddy > 0.0f ? pf * eddy[1] : 0.0f;
+        float rising = (1.0f - k) * expf(__fdiv_rn(jmin(z, 0.0f), P.reach));
+        acc[0] = guide0 + push0;
+        acc[1] = guide1 + push1;
+        acc[2] = rising * churn * boils(P, x, y, t);
+        if (k == 0.0f) return;
+        float sea[2], up;
+        orbital(P, P.origin[0] + x, P.origin[1] + y, z, t - P.offset, sea, up);
+        float t0 = __fdiv_rn(-v * y, r) + P.current[0] + sea[0], t1 = __fdiv_rn(v * x, r) + P.current[1] + sea[1], t2 = up;
+        float kr = __fdiv_rn(k, P.relax);
+        acc[0] = acc[0] + kr * (t0 - vel[0]);
+        acc[1] = acc[1] + kr * (t1 - vel[1]);
+        acc[2] = acc[2] + kr * (t2 - vel[2]);
+        return;
+    }
+    int nh = P.nheights;
+    float hh = jclamp(__fdiv_rn(r, P.hradius) * (float)(nh - 1), 0.0f, (float)(nh - 1));
+    int i = min((int)floorf(hh), nh - 2);
+    float eta = P.heights[i] + (hh - (float)i) * (P.heights[i + 1] - P.heights[i]);
+    float relative = z - eta;
+    float onset = jclamp(__fdiv_rn(t, P.ramp), 0.0f, 1.0f);
+    onset = onset * onset * (3.0f - 2.0f * onset);
+    float churn = jclamp(__fdiv_rn(r, 1.3f * P.core), 0.0f, 1.0f);
+    churn = churn * churn * (3.0f - 2.0f * churn);
+    float aim0 = -turn * y + onset * churn * eddy[0], aim1 = turn * x + onset * churn * eddy[1];
+    float hold = (1.0f - k) * jclamp(__fdiv_rn(-relative - P.free, P.free), 0.0f, 1.0f);
+    float gh = __fdiv_rn(hold, P.guide);
+    float guide0 = gh * (aim0 - vel[0]), guide1 = gh * (aim1 - vel[1]);
+    float top = (1.0f - k) * churn * expf(__fdiv_rn(jmin(relative, 0.0f), P.free)) * onset;
+    float pf = P.eddy > 0.0f ? __fdiv_rn(top * P.turbulence, P.eddy) : 0.0f;
+    float push0 = P.eddy > 0.0f ? pf * eddy[0] : 0.0f, push1 = P.eddy > 0.0f ? pf * eddy[1] : 0.0f;
+    float rising = (1.0f - k) * churn * expf(__fdiv_rn(jmin(relative, 0.0f), P.reach)) * onset;
+    acc[0] = guide0 + push0;
+    acc[1] = guide1 + push1;
+    acc[2] = rising * boils(P, x, y, t);
+    if (k == 0.0f) return;
+    float sea[2], up;
+    orbital(P, P.origin[0] + x, P.origin[1] + y, relative, t - P.offset, sea, up);
+    float t0 = -turn * y + sea[0], t1 = turn * x + sea[1], t2 = up;
+    float kr = __fdiv_rn(k, P.relax);
+    acc[0] = acc[0] + kr * (t0 - vel[0]);
+    acc[1] = acc[1] + kr * (t1 - vel[1]);
+    acc[2] = acc[2] + kr * (t2 - vel[2]);
+}
+
+// sflip.jl:992-1016: u += dt (stir - g z) on the faces out of the walls, the flow read from `old`.
+__global__ void k_sforcesfine(float* __restrict__ u, SFlow old, StirParams P, int a, float dt, float time, float gravity,
+                              int32_t nfa) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nfa) return;
+    SCell c = fineface(old.t, a, n);
+    if (c.w.x < 0) return;
+    float p[3];
+    facepos(old.t.g, a, c.w.x * 16 + c.i, c.w.y * 16 + c.j, c.w.z * 16 + c.k, old.t.g.dx, p);
+    if (sissolid(old.t.g, p[0], p[1], p[2])) return;
+    float vel[3], acc[3];
+    svelocity(old, p[0], p[1], p[2], vel);
+    stir(P, p[0], p[1], p[2], vel, time, acc);
+    u[n] += dt * (a == 0 ? acc[0] : (a == 1 ? acc[1] : acc[2] - gravity));
+}
+__global__ void k_sforcescoarse(float* __restrict__ u, SFlow old, StirParams P, int a, float dt, float time, float gravity) {
+    const SGrid& g = old.t.g;
+    int n0 = g.nc(0) + (a == 0), n1 = g.nc(1) + (a == 1), n2 = g.nc(2) + (a == 2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    float p[3];
+    facepos(g, a, I, J, K, g.cdx(), p);
+    if (sissolid(g, p[0], p[1], p[2])) return;
+    float vel[3], acc[3];
+    svelocity(old, p[0], p[1], p[2], vel);
+    stir(P, p[0], p[1], p[2], vel, time, acc);
+    u[l] += dt * (a == 0 ? acc[0] : (a == 1 ? acc[1] : acc[2] - gravity));
+}
+
+void SparseBasin::forces(float dt, float time, float gacc) {
+    hipStream_t s = st.s;
+    for (int a = 0; a < 3; ++a) {   // u0 = u: the old flow the stir reads
+        MAEL_HIP(hipMemcpyAsync(u0[a].fine.p, u[a].fine.p, finefaces() * sizeof(float), hipMemcpyDeviceToDevice, s));
+        MAEL_HIP(hipMemcpyAsync(u0[a].coarse.p, u[a].coarse.p, nfaces(a) * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
+    SFlow old;
+    old.t = topo();
+    for (int a = 0; a < 3; ++a) {
+        old.uf[a] = u0[a].fine.p; old.uc[a] = u0[a].coarse.p;
+        old.ff[a] = fl[a].fine.p; old.fc[a] = fl[a].coarse.p;
+    }
+    StirParams P = stirp;
+    P.waves = stir_waves.p;
+    P.heights = stir_heights.p;
+    for (int a = 0; a < 3; ++a) {
+        if (nslots)
+            hipLaunchKernelGGL(k_sforcesfine, dim3(cdiv(finefaces(), 256)), dim3(256), 0, s, u[a].fine.p, old, P, a, dt, time,
+                               gacc, (int32_t)finefaces());
+        hipLaunchKernelGGL(k_sforcescoarse, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, s, u[a].coarse.p, old, P, a, dt, time,
+                           gacc);
+    }
+    MAEL_LAUNCH_CHECK();
+    solidify();
+    restrict_velocity();
+}
+
+void SparseBasin::set_stir(const StirParams& p, const float* waves, int nwaves, const float* heights, int nheights) {
+    stirp = p;
+    stirp.nwaves = nwaves;
+    stirp.nheights = nheights;
+    stir_waves.alloc(std::max(7 * nwaves, 1));
+    stir_heights.alloc(std::max(nheights, 2));
+    if (nwaves)
+        MAEL_HIP(hipMemcpyAsync(stir_waves.p, waves, 7 * nwaves * sizeof(float), hipMemcpyHostToDevice, st.s));
+    if (nheights)
+        MAEL_HIP(hipMemcpyAsync(stir_heights.p, heights, nheights * sizeof(float), hipMemcpyHostToDevice, st.s));
+    st.sync();
+}
+
+// ─── kick, extrapolation, reflection (sflip.jl:2017-2139) ────────────────────
+
+// One layer: an unflagged face takes the mean of its flagged neighbours (fine faces;
+// neighbours through finefaceat).
+__global__ void k_sextrapfine(float* __restrict__ out, uint8_t* __restrict__ fout, const float* __restrict__ src,
+                              const uint8_t* __restrict__ fin, float* __restrict__ out2, const float* __restrict__ src2, STopo t,
+                              int a, int32_t nfa) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nfa) return;
+    SCell c = fineface(t, a, n);
+    if (c.w.x < 0) return;
+    if (fin[n]) {
+        out[n] = src[n];
+        if (out2) out2[n] = src2[n];
+        fout[n] = 1;
+        return;
+    }
+    int I = c.w.x * 16 + c.i, J = c.w.y * 16 + c.j, K = c.w.z * 16 + c.k;
+    float acc = 0.0f, acc2 = 0.0f;
+    int m = 0;
+    float p[3];
+    facepos(t.g, a, I, J, K, t.g.dx, p);
+    if (!walledface_dev(t.g, p, a, t.g.dx)) {
+        for (int q = 0; q < 6; ++q) {
+            int32_t nidx = finefaceat(t, a, I + NBX[q][0], J + NBX[q][1], K + NBX[q][2]);
+            if (nidx >= 0 && fin[nidx]) {
+                acc += src[nidx];
+                if (out2) acc2 += src2[nidx];
+                ++m;
+            }
+        }
+    }
+    out[n] = m > 0 ? __fdiv_rn(acc, (float)m) : src[n];
+    if (out2) out2[n] = m > 0 ? __fdiv_rn(acc2, (float)m) : src2[n];
+    fout[n] = m > 0 ? 1 : 0;
+}
+__global__ void k_sextrapcoarse(float* __restrict__ out, uint8_t* __restrict__ fout, const float* __restrict__ src,
+                                const uint8_t* __restrict__ fin, float* __restrict__ out2, const float* __restrict__ src2,
+                                SGrid g, int a) {
+    int n0 = g.nc(0) + (a == 0), n1 = g.nc(1) + (a == 1), n2 = g.nc(2) + (a == 2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    if (fin[l]) {
+        out[l] = src[l];
+        if (out2) out2[l] = src2[l];
+        fout[l] = 1;
+        return;
+    }
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    float h = g.cdx(), p[3];
+    facepos(g, a, I, J, K, h, p);
+    float acc = 0.0f, acc2 = 0.0f;
+    int m = 0;
+    if (!walledface_dev(g, p, a, h)) {
+        for (int q = 0; q < 6; ++q) {
+            int x = I + NBX[q][0], y = J + NBX[q][1], z = K + NBX[q][2];
+            if (x < 0 || y < 0 || z < 0 || x >= n0 || y >= n1 || z >= n2) continue;
+            size_t e = cl(x, y, z, n0, n1);
+            if (fin[e]) {
+                acc += src[e];
+                if (out2) acc2 += src2[e];
+                ++m;
+            }
+        }
+    }
+    out[l] = m > 0 ? __fdiv_rn(acc, (float)m) : src[l];
+    if (out2) out2[l] = m > 0 ? __fdiv_rn(acc2, (float)m) : src2[l];
+    fout[l] = m > 0 ? 1 : 0;
+}
+__global__ void k_unreached(float* __restrict__ u, const uint8_t* __restrict__ reached, size_t n) {
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n && reached[l] == 0) u[l] = 0.0f;
+}
+// sflip.jl:2118-2122: k = valid ? u - k : 0.
+__global__ void k_kick(float* __restrict__ k, const float* __restrict__ u, const uint8_t* __restrict__ valid, size_t n) {
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) k[l] = valid[l] ? u[l] - k[l] : 0.0f;
+}
+__global__ void k_addto(float* __restrict__ k, const float* __restrict__ u, size_t n) {
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) k[l] += u[l];
+}
+
+void SparseBasin::save_reflect() {   // ur = u before the projection (sflip.jl:2306-2309)
+    for (int a = 0; a < 3; ++a) {
+        MAEL_HIP(hipMemcpyAsync(ur[a].fine.p, u[a].fine.p, finefaces() * sizeof(float), hipMemcpyDeviceToDevice, st.s));
+        MAEL_HIP(hipMemcpyAsync(ur[a].coarse.p, u[a].coarse.p, nfaces(a) * sizeof(float), hipMemcpyDeviceToDevice, st.s));
+    }
+}
+void SparseBasin::kick() {
+    for (int a = 0; a < 3; ++a) {
+        hipLaunchKernelGGL(k_kick, dim3(cdiv(finefaces(), 256)), dim3(256), 0, st.s, ur[a].fine.p, u[a].fine.p, fl[a].fine.p,
+                           finefaces());
+        hipLaunchKernelGGL(k_kick, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, st.s, ur[a].coarse.p, u[a].coarse.p,
+                           fl[a].coarse.p, nfaces(a));
+    }
+    MAEL_LAUNCH_CHECK();
+}
+void SparseBasin::reflected() {
+    for (int a = 0; a < 3; ++a) {
+        hipLaunchKernelGGL(k_addto, dim3(cdiv(finefaces(), 256)), dim3(256), 0, st.s, ur[a].fine.p, u[a].fine.p, finefaces());
+        hipLaunchKernelGGL(k_addto, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, st.s, ur[a].coarse.p, u[a].coarse.p, nfaces(a));
+    }
+    MAEL_LAUNCH_CHECK();
+}
+
+// sflip.jl:2097-2116 extrapolate!: `layers` fine layers and `coarselayers` coarse ones
+// (even counts leave the results in place), the kick along with the velocity.
+void SparseBasin::extrapolate(int layers, int coarselayers, bool withkick) {
+    hipStream_t s = st.s;
+    STopo t = topo();
+    size_t F = finefaces();
+    for (int a = 0; a < 3; ++a) {
+        float *f = u[a].fine.p, *tmp = u0[a].fine.p, *k = withkick ? ur[a].fine.p : nullptr, *ktmp = withkick ? ur0[a].fine.p : nullptr;
+        uint8_t *fin = fl[a].fine.p, *fout = fl2[a].fine.p;
+        for (int l = 0; l < layers; ++l) {
+            if (F)
+                hipLaunchKernelGGL(k_sextrapfine, dim3(cdiv(F, 256)), dim3(256), 0, s, tmp, fout, f, fin, ktmp, k, t, a, (int32_t)F);
+            std::swap(f, tmp); std::swap(k, ktmp); std::swap(fin, fout);
+        }
+        if (f != u[a].fine.p) {   // an odd count: copy back
+            MAEL_HIP(hipMemcpyAsync(u[a].fine.p, f, F * sizeof(float), hipMemcpyDeviceToDevice, s));
+            if (withkick) MAEL_HIP(hipMemcpyAsync(ur[a].fine.p, k, F * sizeof(float), hipMemcpyDeviceToDevice, s));
+            MAEL_HIP(hipMemcpyAsync(fl[a].fine.p, fin, F, hipMemcpyDeviceToDevice, s));
+        }
+        launch_restrictfaces_f(*this, u[a].coarse.p, u[a].fine.p, a);
+        if (withkick) launch_restrictfaces_f(*this, ur[a].coarse.p, ur[a].fine.p, a);
+        launch_restrictfaces_u8(*this, fl[a].coarse.p, fl[a].fine.p, a);
+        size_t Fc = nfaces(a);
+        float *fc = u[a].coarse.p, *tc = u0[a].coarse.p, *kc = withkick ? ur[a].coarse.p : nullptr,
+              *ktc = withkick ? ur0[a].coarse.p : nullptr;
+        uint8_t *finc = fl[a].coarse.p, *foutc = fl2[a].coarse.p;
+        for (int l = 0; l < coarselayers; ++l) {
+            hipLaunchKernelGGL(k_sextrapcoarse, dim3(cdiv(Fc, 256)), dim3(256), 0, s, tc, foutc, fc, finc, ktc, kc, g, a);
+            std::swap(fc, tc); std::swap(kc, ktc); std::swap(finc, foutc);
+        }
+        if (fc != u[a].coarse.p) {
+            MAEL_HIP(hipMemcpyAsync(u[a].coarse.p, fc, Fc * sizeof(float), hipMemcpyDeviceToDevice, s));
+            if (withkick) MAEL_HIP(hipMemcpyAsync(ur[a].coarse.p, kc, Fc * sizeof(float), hipMemcpyDeviceToDevice, s));
+            MAEL_HIP(hipMemcpyAsync(fl[a].coarse.p, finc, Fc, hipMemcpyDeviceToDevice, s));
+        }
+        for (int w = 0; w < (withkick ? 2 : 1); ++w) {
+            SField<float>& x = w == 0 ? u[a] : ur[a];
+            if (F) hipLaunchKernelGGL(k_unreached, dim3(cdiv(F, 256)), dim3(256), 0, s, x.fine.p, fl[a].fine.p, F);
+            hipLaunchKernelGGL(k_unreached, dim3(cdiv(Fc, 256)), dim3(256), 0, s, x.coarse.p, fl[a].coarse.p, Fc);
+        }
+        MAEL_LAUNCH_CHECK();
+    }
+    solidify();
+    restrict_velocity();
+}
+
+// ─── the fastest water (sflip.jl:2250-2272) ──────────────────────────────────
+
+__global__ __launch_bounds__(256) void k_maxabs(float* __restrict__ partials, const float* __restrict__ a, size_t n) {
+    __shared__ float lds[4];
+    float v = 0.0f;
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (size_t)gridDim.x * blockDim.x)
+        v = jmax(v, fabsf(a[l]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = jmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = jmax(jmax(lds[0], lds[1]), jmax(lds[2], lds[3]));
+}
+float SparseBasin::maxspeed() {
+    const int NB = 512;
+    std::vector<float> h(NB);
+    float m = 0.0f;
+    for (int a = 0; a < 3; ++a)
+        for (int part = 0; part < 2; ++part) {
+            size_t cnt = part == 0 ? finefaces() : nfaces(a);
+            const float* A = part == 0 ? u[a].fine.p : u[a].coarse.p;
+            if (!cnt) continue;
+            hipLaunchKernelGGL(k_maxabs, dim3(NB), dim3(256), 0, st.s, (float*)partials.p, A, cnt);
+            MAEL_LAUNCH_CHECK();
+            MAEL_HIP(hipMemcpyAsync(h.data(), partials.p, NB * sizeof(float), hipMemcpyDeviceToHost, st.s));
+            st.sync();
+            for (float x : h) m = std::max(m, x);
+        }
+    return m;
+}
+
+}  // namespace mael

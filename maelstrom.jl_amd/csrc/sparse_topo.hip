@@ -26,6 +26,7 @@ SparseBasin::SparseBasin(const SGrid& grid, int nblk, int64_t cap, int32_t bnd, 
     for (int a = 0; a < 3; ++a) {
         u[a].alloc(nfa, nfaces(a)); u0[a].alloc(nfa, nfaces(a)); fl[a].alloc(nfa, nfaces(a));
         mom[a].alloc(nfa); wt[a].alloc(nfa);
+        ur[a].alloc(nfa, nfaces(a)); ur0[a].alloc(nfa, nfaces(a)); fl2[a].alloc(nfa, nfaces(a));
         cf[a].alloc(nfa); cc[a].alloc(nfaces(a));
         c1[a].alloc((size_t)blocks * SBFACES1);
     }
@@ -46,6 +47,8 @@ SparseBasin::SparseBasin(const SGrid& grid, int nblk, int64_t cap, int32_t bnd, 
     if ((long)m[0] * m[1] * m[2] > 1024)
         throw Error(E_GRID, "the coarse grid (" + std::to_string(g.nc(0)) + ", " + std::to_string(g.nc(1)) + ", " +
                                 std::to_string(g.nc(2)) + ") does not halve to at most 1024 cells");
+    reach.alloc(Cc); changed.alloc(1);
+    stir_waves.alloc(1); stir_heights.alloc(2);
     partials.alloc(4096);
     scal.alloc(128);
     for (int w = 0; w < 2; ++w)
@@ -68,6 +71,9 @@ SparseBasin::SparseBasin(const SGrid& grid, int nblk, int64_t cap, int32_t bnd, 
         fill_async(fl[a].coarse.p, nfaces(a), (uint8_t)0, s);
         fill_async(cf[a].p, nfa, 0.0f, s);
         fill_async(cc[a].p, nfaces(a), 0.0f, s);
+        for (auto* f : {&ur[a], &ur0[a]}) { fill_async(f->fine.p, nfa, 0.0f, s); fill_async(f->coarse.p, nfaces(a), 0.0f, s); }
+        fill_async(fl2[a].fine.p, nfa, (uint8_t)0, s);
+        fill_async(fl2[a].coarse.p, nfaces(a), (uint8_t)0, s);
     }
     for (auto* f : {&X1x, &X1b, &X1r, &X1t}) {
         fill_async(f->fine.p, (size_t)blocks * SBCELLS1, 0.0f, s);
@@ -319,6 +325,8 @@ void SparseBasin::refine_from(const uint8_t* want, int* nadd, int* nrem) {
             hipLaunchKernelGGL(k_prolongcells<float>, dim3(gc), dim3(256), 0, s, f->fine.p, f->coarse.p, t, list.p, cnt);
         for (int a = 0; a < 3; ++a) {
             hipLaunchKernelGGL(k_prolongfaces<float>, dim3(gf), dim3(256), 0, s, u[a].fine.p, u[a].coarse.p, t, list.p, cnt, a);
+            // the reflected velocity too: a step's second half carries it (sflip.jl:569-572)
+            hipLaunchKernelGGL(k_prolongfaces<float>, dim3(gf), dim3(256), 0, s, ur[a].fine.p, ur[a].coarse.p, t, list.p, cnt, a);
             hipLaunchKernelGGL(k_prolongfaces<uint8_t>, dim3(gf), dim3(256), 0, s, fl[a].fine.p, fl[a].coarse.p, t, list.p, cnt,
                                a);
         }
@@ -497,13 +505,6 @@ void SparseBasin::classify() {
 
 // ─── walls (sflip.jl:1019-1041) ──────────────────────────────────────────────
 
-__device__ inline bool walledface(const SGrid& g, const float p[3], int a, float h) {
-    float e = 0.5f * h;
-    float lo[3] = {p[0], p[1], p[2]}, hi[3] = {p[0], p[1], p[2]};
-    lo[a] = p[a] - e;
-    hi[a] = p[a] + e;
-    return sissolid(g, lo[0], lo[1], lo[2]) || sissolid(g, hi[0], hi[1], hi[2]);
-}
 __global__ void k_solidfine(float* __restrict__ u, STopo t, int a, int32_t nf) {
     int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nf) return;
@@ -511,7 +512,7 @@ __global__ void k_solidfine(float* __restrict__ u, STopo t, int a, int32_t nf) {
     if (c.w.x < 0) return;
     float p[3];
     facepos(t.g, a, c.w.x * 16 + c.i, c.w.y * 16 + c.j, c.w.z * 16 + c.k, t.g.dx, p);
-    if (walledface(t.g, p, a, t.g.dx)) u[n] = 0.0f;
+    if (walledface_dev(t.g, p, a, t.g.dx)) u[n] = 0.0f;
 }
 __global__ void k_solidcoarse(float* __restrict__ u, SGrid g, int a) {
     int nc0 = g.nc(0), nc1 = g.nc(1), nc2 = g.nc(2);
@@ -523,7 +524,7 @@ __global__ void k_solidcoarse(float* __restrict__ u, SGrid g, int a) {
     float p[3];
     float h = g.cdx();
     facepos(g, a, I, J, K, h, p);
-    if (outside || walledface(g, p, a, h)) u[l] = 0.0f;
+    if (outside || walledface_dev(g, p, a, h)) u[l] = 0.0f;
 }
 void SparseBasin::solidify() {
     for (int a = 0; a < 3; ++a) {

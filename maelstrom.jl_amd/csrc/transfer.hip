@@ -8,51 +8,19 @@
 // reference's integer atomic minimum over order keys, which is order-independent too.
 #include "dense.hpp"
 #include "scan.hpp"
+#include "p2g_tiled.hpp"
 
 namespace mael {
 
-// flip.jl:513-537 p2g_kernel!, all three components per thread (x and lag read once).
-__global__ __launch_bounds__(256) void k_dense_p2g(PView ps, int64_t n, DGrid g, float dt, unsigned long long* __restrict__ mu,
-                                                   unsigned long long* __restrict__ mv, unsigned long long* __restrict__ mw,
-                                                   unsigned long long* __restrict__ wu, unsigned long long* __restrict__ wv,
-                                                   unsigned long long* __restrict__ ww) {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    float xp = ps.f[0][p], yp = ps.f[1][p], zp = ps.f[2][p];
-    float wT = temporal(ps.f[15][p], dt);
-    unsigned long long* mom[3] = {mu, mv, mw};
-    unsigned long long* wt[3] = {wu, wv, ww};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        int n1 = g.n[0] + (a == 0), n2 = g.n[1] + (a == 1), n3 = g.n[2] + (a == 2);
-        float q0 = ispace(xp, g.lo[0], g.dx, a == 0 ? 0.0f : 0.5f);
-        float q1 = ispace(yp, g.lo[1], g.dx, a == 1 ? 0.0f : 0.5f);
-        float q2 = ispace(zp, g.lo[2], g.dx, a == 2 ? 0.0f : 0.5f);
-        BSpline bx = bspline(q0), by = bspline(q1), bz = bspline(q2);
-        float vp = ps.f[3 + a][p];
-        float c0 = ps.f[6 + 3 * a][p], c1 = ps.f[7 + 3 * a][p], c2 = ps.f[8 + 3 * a][p];
-        for (int mz = 0; mz < 3; ++mz) {
-            int K = bz.b + mz;
-            if (K < 0 || K >= n3) continue;
-            float wz = pick(bz, mz), d2 = ((float)K - q2) * g.dx;
-            for (int my = 0; my < 3; ++my) {
-                int J = by.b + my;
-                if (J < 0 || J >= n2) continue;
-                float wy = pick(by, my), d1 = ((float)J - q1) * g.dx;
-#pragma unroll
-                for (int mx = 0; mx < 3; ++mx) {
-                    int I = bx.b + mx;
-                    if (I < 0 || I >= n1) continue;
-                    float wgt = wT * pick(bx, mx) * wy * wz;
-                    float d0 = ((float)I - q0) * g.dx;
-                    size_t idx = (size_t)I + (size_t)n1 * ((size_t)J + (size_t)n2 * K);
-                    atomicAdd(&mom[a][idx], tofix(wgt * (vp + (c0 * d0 + c1 * d1 + c2 * d2))));
-                    atomicAdd(&wt[a][idx], tofix(wgt));
-                }
-            }
-        }
+// flip.jl:513-537 p2g_kernel!: the tiled deposit over the dense grid's face arrays.
+struct DenseNodes {
+    int n[3];
+    __device__ int64_t index(int a, int I, int J, int K) const {
+        int n1 = n[0] + (a == 0), n2 = n[1] + (a == 1), n3 = n[2] + (a == 2);
+        if (I < 0 || I >= n1 || J < 0 || J >= n2 || K < 0 || K >= n3) return -1;
+        return (int64_t)I + (int64_t)n1 * ((int64_t)J + (int64_t)n2 * K);
     }
-}
+};
 
 // flip.jl:539-560 splat_kernel!: the particles' level set by atomic min over order keys.
 __global__ __launch_bounds__(256) void k_dense_splat(PView ps, int64_t n, DGrid g, float dt, int32_t* __restrict__ phip) {
@@ -126,8 +94,9 @@ void DenseBasin::p2g(float dt) {
     if (n > 0) {
         PView ps = view(cur);
         kt.begin(K_P2G, s);
-        hipLaunchKernelGGL(k_dense_p2g, dim3(cdiv(n, 256)), dim3(256), 0, s, ps, n, g, dt, mom[0].p, mom[1].p, mom[2].p,
-                           wt[0].p, wt[1].p, wt[2].p);
+        DenseNodes map{{g.n[0], g.n[1], g.n[2]}};
+        P2GArgs A{{g.lo[0], g.lo[1], g.lo[2]}, g.dx, dt, {mom[0].p, mom[1].p, mom[2].p}, {wt[0].p, wt[1].p, wt[2].p}};
+        hipLaunchKernelGGL(k_p2g_tiled<DenseNodes>, dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s, ps, n, map, A);
         kt.end(K_P2G, s);
         kt.begin(K_SPLAT, s);
         hipLaunchKernelGGL(k_dense_splat, dim3(cdiv(n, 256)), dim3(256), 0, s, ps, n, g, dt, phip.p);

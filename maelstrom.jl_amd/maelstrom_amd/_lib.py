@@ -67,6 +67,14 @@ class SparseStats(C.Structure):
                 ("used_fgmres", C.c_int32), ("rel_residual", C.c_double), ("ms", C.c_double)]
 
 
+class StirC(C.Structure):
+    _fields_ = [("radius", C.c_float), ("band", C.c_float), ("relax", C.c_float), ("core", C.c_float),
+                ("gamma_circ", C.c_float), ("current", C.c_float * 2), ("eddy", C.c_float), ("eddyscale", C.c_float),
+                ("eddytime", C.c_float), ("guide", C.c_float), ("free", C.c_float), ("turbulence", C.c_float),
+                ("boil", C.c_float), ("reach", C.c_float), ("origin", C.c_float * 2), ("offset", C.c_float),
+                ("surface", C.c_int32), ("hradius", C.c_float), ("ramp", C.c_float)]
+
+
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int32), ("sweeps", C.c_int32), ("coarse", C.c_int32),
                 ("precision", C.c_int32), ("refine", C.c_int32), ("warm", C.c_int32)]
@@ -146,6 +154,16 @@ def lib():
         "mael_sparse_volume": (C.c_int, [p, C.POINTER(f64)]),
         "mael_sparse_gravity": (C.c_int, [p, f32, f32]),
         "mael_sparse_sync": (C.c_int, [p]),
+        "mael_sparse_advect": (C.c_int, [p, f32, i32, f32, i32]),
+        "mael_sparse_filltrapped": (C.c_int, [p, i32]),
+        "mael_sparse_set_stir": (C.c_int, [p, C.POINTER(StirC), p, i32, p, i32]),
+        "mael_sparse_forces": (C.c_int, [p, f32, f32, f32]),
+        "mael_sparse_save_reflect": (C.c_int, [p]),
+        "mael_sparse_kick": (C.c_int, [p]),
+        "mael_sparse_reflected": (C.c_int, [p]),
+        "mael_sparse_extrapolate": (C.c_int, [p, i32, i32, i32]),
+        "mael_sparse_set_flow": (C.c_int, [p, i32]),
+        "mael_sparse_maxspeed": (C.c_int, [p, C.POINTER(f32)]),
         "mael_sparse_ktimer": (C.c_int, [p, i32, i32]),
         "mael_sparse_ktime": (C.c_int, [p, C.POINTER(f64), C.POINTER(i32)]),
     }

@@ -3,7 +3,7 @@
 Only what the hot path's inputs need: the basin's geometry and vortex, and the free
 surface in balance with it, which `initialize` turns into the level set.
 """
-from dataclasses import dataclass, replace
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -68,3 +68,43 @@ def surface(b, r):
 
 
 __all__ = ["Basin", "narutobasin", "oseen", "balanced", "surface", "replace"]
+
+
+@dataclass
+class Race:
+    """basin.jl:69-80 `Race`: the tidal race the stirring reads."""
+    current: tuple = (0.0, 0.0)
+    eddy: float = 0.6
+    eddyscale: float = 3.5
+    eddytime: float = 4.0
+    guide: float = 4.0
+    free: float = 0.6
+    turbulence: float = 0.0
+    boil: float = 0.5
+    reach: float = 2.5
+    waves: float = 2.5
+
+
+@dataclass
+class Wave:
+    """sea.jl:6-13 `Wave`: direction, wave number, angular frequency, amplitude, steepness, phase."""
+    d: tuple
+    k: float
+    omega: float
+    a: float
+    q: float
+    phase: float
+
+
+@dataclass
+class Stir:
+    """basin.jl:92-103 `Stir(b, waves; origin, offset, race)`; with `surface`, basin.jl:191-205
+    `SurfaceStir(b, waves; race, ramp)`. The waves are a table given by the caller (the
+    reference draws them from Julia's random stream)."""
+    basin: Basin
+    waves: list = field(default_factory=list)
+    race: Race = field(default_factory=Race)
+    origin: tuple = (0.0, 0.0)
+    offset: float = 0.0
+    surface: bool = False
+    ramp: float = 2.0

@@ -14,8 +14,8 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import SPARSE_ID, PARTICLE_FIELDS, MaelstromError, SparseGrid, SparseOpts, SparseStats
-from .basin import surface
+from ._lib import SPARSE_ID, PARTICLE_FIELDS, MaelstromError, SparseGrid, SparseOpts, SparseStats, StirC
+from .basin import Stir, surface
 
 BCELLS, BFACES = 4096, 4352
 _INT32 = {"KEY", "RANK", "COUNTS", "TOTALS", "OFFSETS", "SLOT", "WHERE"}
@@ -37,8 +37,8 @@ class SparseBasin:
     """sflip.jl:301 `SparseBasin(b, backend; dx, stir, band = 4, nmin = 8, cfl = 8, capacity = 30_000_000,
     blocks = 8000, rings = 0, vary = 0.5, reflect = true)`."""
 
-    def __init__(self, basin, dx, band=4, nmin=8, cfl=8.0, capacity=30_000_000, blocks=8000, rings=0, vary=0.5,
-                 reflect=True, ctx=None, top=None, pad=2):
+    def __init__(self, basin, dx, stir=None, band=4, nmin=8, cfl=8.0, capacity=30_000_000, blocks=8000, rings=0,
+                 vary=0.5, reflect=True, ctx=None, top=None, pad=2):
         self.basin = basin
         self.ctx = ctx or _lib.default_context()
         self.grid = sparsegrid(basin, dx, top, pad)
@@ -52,6 +52,7 @@ class SparseBasin:
         self.steps = 0
         self.iterations = []
         self.last_solve = None
+        set_stir(self, Stir(basin) if stir is None else stir)
 
     # ── arrays ─────────────────────────────────────────────────────────────
     @property
@@ -292,20 +293,76 @@ def initialize(s):
     return s
 
 
-def step(s, dt, **solve):
-    """One step of the hot path in the reference's order (sflip.jl:2295-2318): refine, the
-    grid holds still in place of advection (φ0, u0 take φ, u), particles to grid,
-    redistance, classify, gravity, projection, grid to particles, resample."""
+def set_stir(s, stir):
+    """The stirring model `forces!` applies (basin.jl:92-103 `Stir`, 191-205 `SurfaceStir`)."""
+    b, race = stir.basin, stir.race
+    c = StirC()
+    c.radius, c.band, c.relax, c.core, c.gamma_circ = b.radius, b.band, b.relax, b.core, b.Γ
+    c.current[0], c.current[1] = race.current
+    c.eddy, c.eddyscale, c.eddytime, c.guide = race.eddy, race.eddyscale, race.eddytime, race.guide
+    c.free, c.turbulence, c.boil, c.reach = race.free, race.turbulence, race.boil, race.reach
+    c.origin[0], c.origin[1] = stir.origin
+    c.offset = stir.offset
+    c.surface = 1 if stir.surface else 0
+    c.ramp = stir.ramp
+    # basin.jl:101: the waves at least `shortest` long, scaled by the race
+    w = [(wv.d[0], wv.d[1], wv.k, wv.omega, race.waves * wv.a, wv.q, wv.phase) for wv in stir.waves
+         if 2 * math.pi / wv.k >= b.shortest]
+    waves = np.ascontiguousarray(np.array(w, np.float32).reshape(-1), np.float32)
+    heights = np.zeros(0, np.float32)
+    if stir.surface:   # basin.jl:197-198: the balanced surface over [0, sqrt(2) (radius + 1)]
+        c.hradius = float(np.float32(math.sqrt(2.0) * (b.radius + 1.0)))
+        r = np.linspace(np.float32(0), np.float32(c.hradius), 2048, dtype=np.float32)
+        heights = np.ascontiguousarray(surface(b, r), dtype=np.float32)
+    s.ctx.check(_lib.lib().mael_sparse_set_stir(s.h, C.byref(c), waves.ctypes.data if waves.size else None, len(w),
+                                                heights.ctypes.data if heights.size else None, heights.size))
+    s.stir = stir
+
+
+def maxspeed(s):
+    """sflip.jl:2264-2272 `maxspeed(s)`."""
+    v = C.c_float()
+    s.ctx.check(_lib.lib().mael_sparse_maxspeed(s.h, C.byref(v)))
+    return v.value
+
+
+def substep(s, dt, reflect=False, carried_reflected=False, **solve):
+    """sflip.jl:2295-2318 `substep!(s, dt; reflect, carried)`: one step `dt` carrying the
+    velocity (or the reflected one); with `reflect` the reflected velocity is left in
+    ur, vr, wr for the particles and the next half-step. Returns the solve's stats."""
     L = _lib.lib()
     refine(s)
-    keep(s)
+    s.ctx.check(L.mael_sparse_advect(s.h, float(dt), 1 if carried_reflected else 0, float(s.basin.g), int(s.steps)))
     particles2grid(s, dt)
     redistance(s)
     classify(s)
-    s.ctx.check(L.mael_sparse_gravity(s.h, float(dt), float(s.basin.g)))
+    s.ctx.check(L.mael_sparse_filltrapped(s.h, 32))
+    s.ctx.check(L.mael_sparse_forces(s.h, float(dt), float(s.t), float(s.basin.g)))
+    if reflect:
+        s.ctx.check(L.mael_sparse_save_reflect(s.h))
     st = project(s, dt, **solve)
+    if reflect:
+        s.ctx.check(L.mael_sparse_kick(s.h))
+    s.ctx.check(L.mael_sparse_extrapolate(s.h, 6, 2, 1 if reflect else 0))
+    if reflect:
+        s.ctx.check(L.mael_sparse_reflected(s.h))
+    s.ctx.check(L.mael_sparse_set_flow(s.h, 1 if reflect else 0))
     grid2particles(s, dt)
     s.steps += 1
     resample(s)
+    s.ctx.check(L.mael_sparse_set_flow(s.h, 0))
     s.t += dt
     return st
+
+
+def step(s, dt, **solve):
+    """sflip.jl:2288-2292 `step!(s, dt)`: with `s.reflect`, advection-reflection's two
+    halves (the first ending in the reflected velocity, the second carrying it), else one
+    substep. Returns the last half's solve stats; `s.iterations` holds both."""
+    if not s.reflect:
+        return substep(s, dt, **solve)
+    substep(s, dt / 2, reflect=True, **solve)
+    return substep(s, dt / 2, carried_reflected=True, **solve)
+
+
+advance = step

@@ -111,12 +111,13 @@ def test_initialize_matches_oracle(M):
     assert ns == o.nslots
     np.testing.assert_array_equal(slot, o.array("slot"))
     assert s.n == o.n
-    # the start uses the library's own Lamb-Oseen (expf) and surface table, so compare
-    # the level set and types, which the particles' positions decide
+    # the positions are bit-exact; the start's velocities come from the library's own
+    # Lamb-Oseen (expf), so the level set the first deposit splats (positions moved along
+    # those velocities) agrees to a few ulps
     np.testing.assert_array_equal(s.get("TYPES_F"), _fine(o, "types_f", BCELLS))
     for nm in ("px", "py", "pz"):
         np.testing.assert_array_equal(s.get(nm.upper()), o.array(nm)[:o.n])
-    np.testing.assert_allclose(s.get("PHI_F"), _fine(o, "phi_f", BCELLS), rtol=0, atol=1e-6)
+    np.testing.assert_allclose(s.get("PHI_F"), _fine(o, "phi_f", BCELLS), rtol=0, atol=2e-5)
     assert abs(M.sparse.volume(s) / o.volume() - 1) < 1e-6
 
 
@@ -259,10 +260,14 @@ def test_project_errors(M):
         M.sparse.project(s, 0.1, gamma=0.0, policy=1)
 
 
-def test_steps_keep_volume(M):
-    """initialize! and three steps of the hot path: finite, volume kept to a few percent."""
-    print("case: sparse steps", flush=True)
-    s = M.SparseBasin(_basin(M), dx=DX, blocks=64, capacity=300000, cfl=4)
+@pytest.mark.parametrize("surface", [False, True])
+def test_steps_keep_volume(M, surface):
+    """initialize! and three whole steps (advection-reflection halves, the stir, trapped
+    air, extrapolation): finite, the volume kept to a few percent."""
+    print("case: sparse steps, surface stir %s" % surface, flush=True)
+    b = _basin(M)
+    stir = M.Stir(b, race=M.Race(eddy=3.0), surface=surface)
+    s = M.SparseBasin(b, dx=DX, blocks=64, capacity=300000, cfl=4, stir=stir)
     M.sparse.initialize(s)
     v0 = s.volume0
     for _ in range(3):

@@ -183,7 +183,9 @@ def sparse_main(args, desc, kw, dx):
     from maelstrom_amd import sparse as sp
     from maelstrom_amd._lib import KERNEL_ID
     b = m.Basin(**kw)
-    s = m.SparseBasin(b, dx=dx, band=4, nmin=8, cfl=4.0, capacity=30_000_000, blocks=8000)
+    # README.md:86-90: SurfaceStir(b, narutowaves(); race = Race(; eddy = 3)), cfl 4
+    stir = m.Stir(b, waves=m.narutowaves(), race=m.Race(eddy=3.0), surface=True)
+    s = m.SparseBasin(b, dx=dx, stir=stir, band=4, nmin=8, cfl=4.0, capacity=30_000_000, blocks=8000)
     sp.initialize(s)
     umax = max(float(np.abs(s.get(f)).max()) for f in ("U_F", "V_F", "W_F", "U_C", "V_C", "W_C"))
     left = 1.0 / 24
@@ -223,7 +225,9 @@ def sparse_main(args, desc, kw, dx):
                    "live_blocks": live_slots, "fine_cells": live_slots * 4096, "particles": int(n_avg),
                    "dt_s": round(dt, 5), "tol": args.tol, "policy": "fcg-then-fgmres(20)",
                    "precision": ["fp32", "fp16", "bf16"][args.precision],
-                   "phases": "refine, p2g, redistance, classify, gravity, project, g2p, resample"},
+                   "stir": "surface stir, 32 seeded narutowaves-shaped waves, race eddy 3",
+                   "phases": "two advection-reflection halves of refine, advect, p2g, redistance, classify, "
+                             "trapped air, stir forces, project, kick, extrapolate, reflect, g2p, resample"},
         "pressure_ms": round(sum(pressure_ms) / len(pressure_ms), 3),
         "iterations": round(sum(iters) / len(iters), 2),
         "fgmres_steps": int(sum(fg)),

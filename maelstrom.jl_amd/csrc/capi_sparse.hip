@@ -242,7 +242,7 @@ int mael_sparse_project(mael_sparse* s, float gamma, const mael_sparse_opts* opt
             o.restart = opts->restart; o.policy = opts->policy; o.precision = opts->precision; o.boost = opts->boost;
         }
         if (o.sweeps < 1 || o.coarse < 1 || o.maxiter < 0 || !(o.tol >= 0.0) || o.restart < 1 || o.restart > 40 ||
-            o.policy < 0 || o.policy > 2 || o.precision != 0)
+            o.policy < 0 || o.policy > 2 || o.precision < 0 || o.precision > 2)
             throw mael::Error(mael::E_ARG, "bad solver options");
         mael::SparseStats st;
         s->b->project(gamma, o, st);

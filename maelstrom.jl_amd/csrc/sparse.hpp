@@ -13,6 +13,7 @@
 //    dense halvings of the coarse grid.
 #pragma once
 #include "dense.hpp"
+#include <array>
 
 namespace mael {
 
@@ -261,6 +262,18 @@ struct StirParams {
 
 struct SFlow;
 
+// The mixed-precision hierarchy's storage (16-bit: FP16 or BF16 bit patterns), allocated
+// on the first solve that asks for it: the V-cycle's level-0 vectors, level 1's, each
+// dense level's, and rounded copies of every coupling.
+struct MPStore {
+    bool ready = false;
+    DBuf<uint16_t> z[2], r[2], q[2], tmp[2];        // fine pool, coarse
+    DBuf<uint16_t> X1[4][2];                        // level-1 x, b, r, t
+    std::vector<std::array<DBuf<uint16_t>, 4>> Dv;  // dense levels: x, b, r, t
+    DBuf<uint16_t> cf[3], cc[3], c1[3], dDf, dDc, dD1;
+    std::vector<std::array<DBuf<uint16_t>, 4>> Dc;  // dense levels: couplings x, y, z, diagonal
+};
+
 // A field: its fine pool and the dense coarse array (sflip.jl:86-90).
 template <class T> struct SField {
     DBuf<T> fine, coarse;
@@ -327,6 +340,8 @@ struct SparseBasin {
     SField<float> X1x, X1b, X1r, X1t;         // level-1 vectors (fine 512 a slot, coarse part)
     std::vector<SLevel> D;                    // D[0] = the coarse grid as level 2, then halvings
     std::vector<SField<float>> V, Z;          // FGMRES bases, allocated on first use
+    MPStore mp;
+    void mp_alloc();
     DBuf<double> partials, scal;
 
     // particles

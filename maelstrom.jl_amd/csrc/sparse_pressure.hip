@@ -9,6 +9,7 @@
 // per CG iteration (the residual norm, with the error flags) and one per Arnoldi step.
 #include "sparse.hpp"
 #include "scan.hpp"
+#include "precision.hpp"
 #include <chrono>
 #include <cmath>
 
@@ -271,43 +272,45 @@ __global__ void k_cellsdense(float* __restrict__ dDc, uint8_t* __restrict__ actc
 }
 
 // ─── the composite rows (sflip.jl:1139-1216) ─────────────────────────────────
+// The V-cycle's kernels are templates on the storage of its vectors (T) and of its
+// couplings (C): FP32 for the reference's solve, FP16 or BF16 for the mixed-precision
+// one. Arithmetic is FP32 either way (the coarsest level's in A).
+
+template <class C> struct FineCoefT { const C* c[3]; };
+template <class C> struct CoarseCoefT { const C* c[3]; };
 
 // A level's fine cell (N = 16 or 8): diag and the neighbours' sum, A x = diag x - off.
-template <int N>
-__device__ inline void finerow(const STopo& t, const float* __restrict__ x, const float* __restrict__ xc,
-                               const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
-                               const float* __restrict__ dD, int s, int i, int j, int k, int4 w, float& diag, float& off) {
+template <int N, class T, class C>
+__device__ inline void finerow(const STopo& t, const T* __restrict__ x, const T* __restrict__ xc, const FineCoefT<C>& cf,
+                               const C* __restrict__ dD, int s, int i, int j, int k, int4 w, float& diag, float& off) {
     constexpr int sh = N == 16 ? 2 : 1, per = (N + 1) * N * N;
     int I = w.x * N + i, J = w.y * N + j, K = w.z * N + k;
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
-    diag = dD[(s - 1) * (N * N * N) + i + N * (j + N * k)];
+    diag = lf(dD[(s - 1) * (N * N * N) + i + N * (j + N * k)]);
     off = 0.0f;
     auto nv = [&](int In, int Jn, int Kn) -> float {
         int32_t idx = cellat<N>(t, In, Jn, Kn);
-        if (idx >= 0) return x[idx];
-        return xc[cl(min(max(In >> sh, 0), n0 - 1), min(max(Jn >> sh, 0), n1 - 1), min(max(Kn >> sh, 0), n2 - 1), n0, n1)];
+        if (idx >= 0) return lf(x[idx]);
+        return lf(xc[cl(min(max(In >> sh, 0), n0 - 1), min(max(Jn >> sh, 0), n1 - 1), min(max(Kn >> sh, 0), n2 - 1), n0, n1)]);
     };
     size_t base = (size_t)(s - 1) * per;
     float c;
-    c = cx[base + ffn(N, 0, i, j, k)];     if (c > 0.0f) { diag += c; off += c * nv(I - 1, J, K); }
-    c = cx[base + ffn(N, 0, i + 1, j, k)]; if (c > 0.0f) { diag += c; off += c * nv(I + 1, J, K); }
-    c = cy[base + ffn(N, 1, i, j, k)];     if (c > 0.0f) { diag += c; off += c * nv(I, J - 1, K); }
-    c = cy[base + ffn(N, 1, i, j + 1, k)]; if (c > 0.0f) { diag += c; off += c * nv(I, J + 1, K); }
-    c = cz[base + ffn(N, 2, i, j, k)];     if (c > 0.0f) { diag += c; off += c * nv(I, J, K - 1); }
-    c = cz[base + ffn(N, 2, i, j, k + 1)]; if (c > 0.0f) { diag += c; off += c * nv(I, J, K + 1); }
+    c = lf(cf.c[0][base + ffn(N, 0, i, j, k)]);     if (c > 0.0f) { diag += c; off += c * nv(I - 1, J, K); }
+    c = lf(cf.c[0][base + ffn(N, 0, i + 1, j, k)]); if (c > 0.0f) { diag += c; off += c * nv(I + 1, J, K); }
+    c = lf(cf.c[1][base + ffn(N, 1, i, j, k)]);     if (c > 0.0f) { diag += c; off += c * nv(I, J - 1, K); }
+    c = lf(cf.c[1][base + ffn(N, 1, i, j + 1, k)]); if (c > 0.0f) { diag += c; off += c * nv(I, J + 1, K); }
+    c = lf(cf.c[2][base + ffn(N, 2, i, j, k)]);     if (c > 0.0f) { diag += c; off += c * nv(I, J, K - 1); }
+    c = lf(cf.c[2][base + ffn(N, 2, i, j, k + 1)]); if (c > 0.0f) { diag += c; off += c * nv(I, J, K + 1); }
 }
 
-struct CoarseCoef { const float* c[3]; };
-struct FineCoef { const float* c[3]; };
-
 // An uncovered coarse cell: coarse faces, or a covered neighbour's (N/4)^2 sub-faces.
-template <int N>
-__device__ inline void coarserow(const STopo& t, const float* __restrict__ xf, const FineCoef& cf, const float* __restrict__ xc,
-                                 const CoarseCoef& ccf, const float* __restrict__ dDc, int I, int J, int K, float& diag,
+template <int N, class T, class C>
+__device__ inline void coarserow(const STopo& t, const T* __restrict__ xf, const FineCoefT<C>& cf, const T* __restrict__ xc,
+                                 const CoarseCoefT<C>& ccf, const C* __restrict__ dDc, int I, int J, int K, float& diag,
                                  float& off) {
     constexpr int R = N / 4;
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
-    diag = dDc[cl(I, J, K, n0, n1)];
+    diag = lf(dDc[cl(I, J, K, n0, n1)]);
     off = 0.0f;
     for (int a = 0; a < 3; ++a) {
         int f0 = n0 + (a == 0), f1 = n1 + (a == 1);
@@ -325,34 +328,34 @@ __device__ inline void coarserow(const STopo& t, const float* __restrict__ xf, c
                         int PK = (a == 2 && sg < 0) ? FK + 1 : FK;
                         int32_t fidx = faceat<N>(t, a, PI, PJ, PK), cidx = cellat<N>(t, FI, FJ, FK);
                         if (fidx >= 0 && cidx >= 0) {
-                            float c = cf.c[a][fidx];
+                            float c = lf(cf.c[a][fidx]);
                             diag += c;
-                            off += c * xf[cidx];
+                            off += c * lf(xf[cidx]);
                         }
                     }
             } else {
-                float c = sg < 0 ? ccf.c[a][cl(I, J, K, f0, f1)] : ccf.c[a][cl(I + (a == 0), J + (a == 1), K + (a == 2), f0, f1)];
-                if (c > 0.0f) { diag += c; off += c * xc[cl(In, Jn, Kn, n0, n1)]; }
+                float c = lf(sg < 0 ? ccf.c[a][cl(I, J, K, f0, f1)] : ccf.c[a][cl(I + (a == 0), J + (a == 1), K + (a == 2), f0, f1)]);
+                if (c > 0.0f) { diag += c; off += c * lf(xc[cl(In, Jn, Kn, n0, n1)]); }
             }
         }
     }
 }
 
 // Level views: fine couplings and diagonal at N, the coarse grid's level-0 couplings.
-struct Comp {
-    FineCoef cf;
-    const float* dDf;
+template <class C> struct CompT {
+    FineCoefT<C> cf;
+    const C* dDf;
     const uint8_t* actf;
-    CoarseCoef cc;
-    const float* dDc;
+    CoarseCoefT<C> cc;
+    const C* dDc;
     const uint8_t* actc;
 };
 
 // sflip.jl:1218-1269: mode 0 Jacobi (out = x + w (b - A x)/diag), 1 residual (out = b - A x),
 // 2 apply (out = A x), on the fine cells of a level; inactive cells get 0.
-template <int N, int MODE>
-__global__ __launch_bounds__(256) void k_rowfine(float* __restrict__ out, const float* __restrict__ x, const float* __restrict__ b,
-                                                 const float* __restrict__ xc, Comp L, STopo t, int32_t nf) {
+template <int N, int MODE, class T, class C>
+__global__ __launch_bounds__(256) void k_rowfine(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
+                                                 const T* __restrict__ xc, CompT<C> L, STopo t, int32_t nf) {
     int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nf) return;
     constexpr int lg = N == 16 ? 12 : 9, msk = N * N * N - 1, l2 = N == 16 ? 4 : 3;
@@ -363,18 +366,18 @@ __global__ __launch_bounds__(256) void k_rowfine(float* __restrict__ out, const 
     float res = 0.0f;
     if (act) {
         float diag, off;
-        finerow<N>(t, x, xc, L.cf.c[0], L.cf.c[1], L.cf.c[2], L.dDf, s, c & (N - 1), (c >> l2) & (N - 1), c >> (2 * l2), w, diag,
-                   off);
-        float ax = diag * x[n] - off;
-        if (MODE == 0) res = diag > 0.0f ? x[n] + __fdiv_rn(OMEGA * (b[n] - ax), diag) : 0.0f;
-        else if (MODE == 1) res = b[n] - ax;
+        finerow<N>(t, x, xc, L.cf, L.dDf, s, c & (N - 1), (c >> l2) & (N - 1), c >> (2 * l2), w, diag, off);
+        float xn = lf(x[n]);
+        float ax = diag * xn - off;
+        if (MODE == 0) res = diag > 0.0f ? xn + __fdiv_rn(OMEGA * (lf(b[n]) - ax), diag) : 0.0f;
+        else if (MODE == 1) res = lf(b[n]) - ax;
         else res = ax;
     }
-    out[n] = res;
+    out[n] = sf<T>(res);
 }
-template <int N, int MODE>
-__global__ __launch_bounds__(256) void k_rowcoarse(float* __restrict__ out, const float* __restrict__ x,
-                                                   const float* __restrict__ b, const float* __restrict__ xf, Comp L, STopo t) {
+template <int N, int MODE, class T, class C>
+__global__ __launch_bounds__(256) void k_rowcoarse(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
+                                                   const T* __restrict__ xf, CompT<C> L, STopo t) {
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
@@ -383,37 +386,39 @@ __global__ __launch_bounds__(256) void k_rowcoarse(float* __restrict__ out, cons
     if (L.actc[l] != 0 && !covered(t, I, J, K)) {
         float diag, off;
         coarserow<N>(t, xf, L.cf, x, L.cc, L.dDc, I, J, K, diag, off);
-        float ax = diag * x[l] - off;
-        if (MODE == 0) res = diag > 0.0f ? x[l] + __fdiv_rn(OMEGA * (b[l] - ax), diag) : 0.0f;
-        else if (MODE == 1) res = b[l] - ax;
+        float xl = lf(x[l]);
+        float ax = diag * xl - off;
+        if (MODE == 0) res = diag > 0.0f ? xl + __fdiv_rn(OMEGA * (lf(b[l]) - ax), diag) : 0.0f;
+        else if (MODE == 1) res = lf(b[l]) - ax;
         else res = ax;
     }
-    out[l] = res;
+    out[l] = sf<T>(res);
 }
 
 // ─── dense levels (sflip.jl:1574-1657) ───────────────────────────────────────
 
-struct DView {
+template <class C> struct DViewT {
     int3 n;
-    const float* c[3];
-    const float* dD;
+    const C* c[3];
+    const C* dD;
     const uint8_t* act;
 };
-__device__ inline void denserow(const DView& L, const float* __restrict__ x, int i, int j, int k, float& diag, float& off) {
+template <class T, class C>
+__device__ inline void denserow(const DViewT<C>& L, const T* __restrict__ x, int i, int j, int k, float& diag, float& off) {
     int n0 = L.n.x, n1 = L.n.y, n2 = L.n.z;
-    diag = L.dD[cl(i, j, k, n0, n1)];
+    diag = lf(L.dD[cl(i, j, k, n0, n1)]);
     off = 0.0f;
     float c;
-    c = L.c[0][cl(i, j, k, n0 + 1, n1)];     if (c > 0.0f && i > 0) { diag += c; off += c * x[cl(i - 1, j, k, n0, n1)]; }
-    c = L.c[0][cl(i + 1, j, k, n0 + 1, n1)]; if (c > 0.0f && i < n0 - 1) { diag += c; off += c * x[cl(i + 1, j, k, n0, n1)]; }
-    c = L.c[1][cl(i, j, k, n0, n1 + 1)];     if (c > 0.0f && j > 0) { diag += c; off += c * x[cl(i, j - 1, k, n0, n1)]; }
-    c = L.c[1][cl(i, j + 1, k, n0, n1 + 1)]; if (c > 0.0f && j < n1 - 1) { diag += c; off += c * x[cl(i, j + 1, k, n0, n1)]; }
-    c = L.c[2][cl(i, j, k, n0, n1)];         if (c > 0.0f && k > 0) { diag += c; off += c * x[cl(i, j, k - 1, n0, n1)]; }
-    c = L.c[2][cl(i, j, k + 1, n0, n1)];     if (c > 0.0f && k < n2 - 1) { diag += c; off += c * x[cl(i, j, k + 1, n0, n1)]; }
+    c = lf(L.c[0][cl(i, j, k, n0 + 1, n1)]);     if (c > 0.0f && i > 0) { diag += c; off += c * lf(x[cl(i - 1, j, k, n0, n1)]); }
+    c = lf(L.c[0][cl(i + 1, j, k, n0 + 1, n1)]); if (c > 0.0f && i < n0 - 1) { diag += c; off += c * lf(x[cl(i + 1, j, k, n0, n1)]); }
+    c = lf(L.c[1][cl(i, j, k, n0, n1 + 1)]);     if (c > 0.0f && j > 0) { diag += c; off += c * lf(x[cl(i, j - 1, k, n0, n1)]); }
+    c = lf(L.c[1][cl(i, j + 1, k, n0, n1 + 1)]); if (c > 0.0f && j < n1 - 1) { diag += c; off += c * lf(x[cl(i, j + 1, k, n0, n1)]); }
+    c = lf(L.c[2][cl(i, j, k, n0, n1)]);         if (c > 0.0f && k > 0) { diag += c; off += c * lf(x[cl(i, j, k - 1, n0, n1)]); }
+    c = lf(L.c[2][cl(i, j, k + 1, n0, n1)]);     if (c > 0.0f && k < n2 - 1) { diag += c; off += c * lf(x[cl(i, j, k + 1, n0, n1)]); }
 }
-template <int MODE>
-__global__ __launch_bounds__(256) void k_rowdense(float* __restrict__ out, const float* __restrict__ x, const float* __restrict__ b,
-                                                  DView L) {
+template <int MODE, class T, class C>
+__global__ __launch_bounds__(256) void k_rowdense(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
+                                                  DViewT<C> L) {
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)L.n.x * L.n.y * L.n.z) return;
     int i = (int)(l % L.n.x), j = (int)((l / L.n.x) % L.n.y), k = (int)(l / ((size_t)L.n.x * L.n.y));
@@ -421,70 +426,78 @@ __global__ __launch_bounds__(256) void k_rowdense(float* __restrict__ out, const
     if (L.act[l] != 0) {
         float diag, off;
         denserow(L, x, i, j, k, diag, off);
-        float ax = diag * x[l] - off;
-        res = MODE == 0 ? (diag > 0.0f ? x[l] + __fdiv_rn(OMEGA * (b[l] - ax), diag) : 0.0f) : b[l] - ax;
+        float xl = lf(x[l]);
+        float ax = diag * xl - off;
+        res = MODE == 0 ? (diag > 0.0f ? xl + __fdiv_rn(OMEGA * (lf(b[l]) - ax), diag) : 0.0f) : lf(b[l]) - ax;
     }
-    out[l] = res;
+    out[l] = sf<T>(res);
 }
-__global__ void k_restrictdense(float* __restrict__ bc, const float* __restrict__ rf, int3 nc, int3 nf) {
+// The coarser level's right-hand side: the sum of the eight finer residuals, times
+// `scale` (1 in the reference's solve; the reduced-precision hierarchy carries each
+// dense level's equations divided by 4 per level, so its values stay in FP16's range).
+template <class T>
+__global__ void k_restrictdense(T* __restrict__ bc, const T* __restrict__ rf, int3 nc, int3 nf, float scale) {
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)nc.x * nc.y * nc.z) return;
     int I = (int)(l % nc.x), J = (int)((l / nc.x) % nc.y), K = (int)(l / ((size_t)nc.x * nc.y));
     float acc = 0.0f;
     for (int c = 0; c < 2; ++c)
         for (int b = 0; b < 2; ++b)
-            for (int a = 0; a < 2; ++a) acc += rf[cl(2 * I + a, 2 * J + b, 2 * K + c, nf.x, nf.y)];
-    bc[l] = acc;
+            for (int a = 0; a < 2; ++a) acc += lf(rf[cl(2 * I + a, 2 * J + b, 2 * K + c, nf.x, nf.y)]);
+    bc[l] = sf<T>(acc * scale);
 }
-__global__ void k_prolongdense(float* __restrict__ xf, const float* __restrict__ xc, const uint8_t* __restrict__ actf, int3 nf,
-                               int3 nc, float boost) {
+template <class T>
+__global__ void k_prolongdense(T* __restrict__ xf, const T* __restrict__ xc, const uint8_t* __restrict__ actf, int3 nf, int3 nc,
+                               float boost) {
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)nf.x * nf.y * nf.z) return;
     int i = (int)(l % nf.x), j = (int)((l / nf.x) % nf.y), k = (int)(l / ((size_t)nf.x * nf.y));
-    if (actf[l] != 0) xf[l] += boost * xc[cl(i >> 1, j >> 1, k >> 1, nc.x, nc.y)];
+    if (actf[l] != 0) xf[l] = sf<T>(lf(xf[l]) + boost * lf(xc[cl(i >> 1, j >> 1, k >> 1, nc.x, nc.y)]));
 }
-// sflip.jl:1623-1657 coarsest_kernel!: red-black Gauss-Seidel from zero in one workgroup.
-__global__ __launch_bounds__(1024) void k_scoarsest(float* __restrict__ x, const float* __restrict__ b, DView L, int sweeps) {
-    __shared__ float xs[1024];
+// sflip.jl:1623-1657 coarsest_kernel!: red-black Gauss-Seidel from zero in one workgroup,
+// in arithmetic A (FP64 in the mixed-precision solve).
+template <class T, class C, class A>
+__global__ __launch_bounds__(1024) void k_scoarsest(T* __restrict__ x, const T* __restrict__ b, DViewT<C> L, int sweeps) {
+    __shared__ A xs[1024];
     int t = threadIdx.x;
     int n0 = L.n.x, n1 = L.n.y, n2 = L.n.z, N = n0 * n1 * n2;
     bool inside = t < N;
     int i = t % n0, j = (t / n0) % n1, k = t / (n0 * n1);
     bool act = inside && L.act[t] != 0;
-    float bi = inside ? b[t] : 0.0f;
-    float dg = 0.0f, cw[6] = {0, 0, 0, 0, 0, 0};
+    A bi = inside ? (A)lf(b[t]) : A(0);
+    A dg = A(0), cw[6] = {0, 0, 0, 0, 0, 0};
     int nb[6] = {0, 0, 0, 0, 0, 0};
     if (act) {
-        dg = L.dD[t];
+        dg = (A)lf(L.dD[t]);
         float c;
-        c = L.c[0][cl(i, j, k, n0 + 1, n1)];     if (c > 0.0f && i > 0) { cw[0] = c; nb[0] = t - 1; }
-        c = L.c[0][cl(i + 1, j, k, n0 + 1, n1)]; if (c > 0.0f && i < n0 - 1) { cw[1] = c; nb[1] = t + 1; }
-        c = L.c[1][cl(i, j, k, n0, n1 + 1)];     if (c > 0.0f && j > 0) { cw[2] = c; nb[2] = t - n0; }
-        c = L.c[1][cl(i, j + 1, k, n0, n1 + 1)]; if (c > 0.0f && j < n1 - 1) { cw[3] = c; nb[3] = t + n0; }
-        c = L.c[2][cl(i, j, k, n0, n1)];         if (c > 0.0f && k > 0) { cw[4] = c; nb[4] = t - n0 * n1; }
-        c = L.c[2][cl(i, j, k + 1, n0, n1)];     if (c > 0.0f && k < n2 - 1) { cw[5] = c; nb[5] = t + n0 * n1; }
+        c = lf(L.c[0][cl(i, j, k, n0 + 1, n1)]);     if (c > 0.0f && i > 0) { cw[0] = c; nb[0] = t - 1; }
+        c = lf(L.c[0][cl(i + 1, j, k, n0 + 1, n1)]); if (c > 0.0f && i < n0 - 1) { cw[1] = c; nb[1] = t + 1; }
+        c = lf(L.c[1][cl(i, j, k, n0, n1 + 1)]);     if (c > 0.0f && j > 0) { cw[2] = c; nb[2] = t - n0; }
+        c = lf(L.c[1][cl(i, j + 1, k, n0, n1 + 1)]); if (c > 0.0f && j < n1 - 1) { cw[3] = c; nb[3] = t + n0; }
+        c = lf(L.c[2][cl(i, j, k, n0, n1)]);         if (c > 0.0f && k > 0) { cw[4] = c; nb[4] = t - n0 * n1; }
+        c = lf(L.c[2][cl(i, j, k + 1, n0, n1)]);     if (c > 0.0f && k < n2 - 1) { cw[5] = c; nb[5] = t + n0 * n1; }
     }
-    xs[t] = 0.0f;
+    xs[t] = A(0);
     __syncthreads();
     int par = (i + j + k + 1) & 1;
     for (int s = 1; s <= 2 * sweeps; ++s)
         for (int half = 0; half < 2; ++half) {
             int colour = s <= sweeps ? half : 1 - half;
             if (act && par == colour) {
-                float diag = dg, acc = bi;
+                A diag = dg, acc = bi;
 #pragma unroll
                 for (int m = 0; m < 6; ++m)
-                    if (cw[m] > 0.0f) { diag += cw[m]; acc += cw[m] * xs[nb[m]]; }
-                xs[t] = diag > 0.0f ? __fdiv_rn(acc, diag) : 0.0f;
+                    if (cw[m] > A(0)) { diag += cw[m]; acc += cw[m] * xs[nb[m]]; }
+                xs[t] = diag > A(0) ? acc / diag : A(0);
             }
             __syncthreads();
         }
-    if (inside) x[t] = xs[t];
+    if (inside) x[t] = sf<T>((float)xs[t]);
 }
 
 // ─── between the composite levels (sflip.jl:1661-1705) ───────────────────────
 
-__global__ void k_restrict01(float* __restrict__ b1, const float* __restrict__ r0, STopo t, int nslots) {
+template <class T> __global__ void k_restrict01(T* __restrict__ b1, const T* __restrict__ r0, STopo t, int nslots) {
     int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nslots * SBCELLS1) return;
     int s = (n >> 9) + 1, c = n & 511, i = c & 7, j = (c >> 3) & 7, k = c >> 6;
@@ -493,10 +506,11 @@ __global__ void k_restrict01(float* __restrict__ b1, const float* __restrict__ r
     for (int cz = 0; cz < 2; ++cz)
         for (int by = 0; by < 2; ++by)
             for (int ax = 0; ax < 2; ++ax)
-                acc += r0[(s - 1) * SBCELLS + (2 * i + ax) + 16 * ((2 * j + by) + 16 * (2 * k + cz))];
-    b1[n] = acc;
+                acc += lf(r0[(s - 1) * SBCELLS + (2 * i + ax) + 16 * ((2 * j + by) + 16 * (2 * k + cz))]);
+    b1[n] = sf<T>(acc);
 }
-__global__ void k_restrict12(float* __restrict__ b2, const float* __restrict__ r1f, const float* __restrict__ r1c, STopo t) {
+template <class T>
+__global__ void k_restrict12(T* __restrict__ b2, const T* __restrict__ r1f, const T* __restrict__ r1c, STopo t) {
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
@@ -505,36 +519,59 @@ __global__ void k_restrict12(float* __restrict__ b2, const float* __restrict__ r
         float acc = 0.0f;
         for (int c = 0; c < 2; ++c)
             for (int b = 0; b < 2; ++b)
-                for (int a = 0; a < 2; ++a) acc += r1f[cellat<8>(t, 2 * I + a, 2 * J + b, 2 * K + c)];
-        b2[l] = acc;
+                for (int a = 0; a < 2; ++a) acc += lf(r1f[cellat<8>(t, 2 * I + a, 2 * J + b, 2 * K + c)]);
+        b2[l] = sf<T>(acc);
     } else {
         b2[l] = r1c[l];
     }
 }
-__global__ void k_prolong21(float* __restrict__ x1f, const float* __restrict__ x2, const uint8_t* __restrict__ act1, STopo t,
+template <class T>
+__global__ void k_prolong21(T* __restrict__ x1f, const T* __restrict__ x2, const uint8_t* __restrict__ act1, STopo t,
                             float boost, int nslots) {
     int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nslots * SBCELLS1) return;
     int s = (n >> 9) + 1, c = n & 511, i = c & 7, j = (c >> 3) & 7, k = c >> 6;
     int4 w = t.where[s - 1];
     if (w.x < 0 || act1[n] == 0) return;
-    x1f[n] += boost * x2[cl((w.x * 8 + i) >> 1, (w.y * 8 + j) >> 1, (w.z * 8 + k) >> 1, t.g.nc(0), t.g.nc(1))];
+    x1f[n] = sf<T>(lf(x1f[n]) + boost * lf(x2[cl((w.x * 8 + i) >> 1, (w.y * 8 + j) >> 1, (w.z * 8 + k) >> 1, t.g.nc(0), t.g.nc(1))]));
 }
-__global__ void k_prolongcoarse(float* __restrict__ xc, const float* __restrict__ from, const uint8_t* __restrict__ act, STopo t,
+template <class T>
+__global__ void k_prolongcoarse(T* __restrict__ xc, const T* __restrict__ from, const uint8_t* __restrict__ act, STopo t,
                                 float boost) {
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
     int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
-    if (act[l] != 0 && !covered(t, I, J, K)) xc[l] += boost * from[l];
+    if (act[l] != 0 && !covered(t, I, J, K)) xc[l] = sf<T>(lf(xc[l]) + boost * lf(from[l]));
 }
-__global__ void k_prolong10(float* __restrict__ x0f, const float* __restrict__ x1f, const uint8_t* __restrict__ tf, STopo t,
-                            float boost, int32_t nf) {
+template <class T>
+__global__ void k_prolong10(T* __restrict__ x0f, const T* __restrict__ x1f, const uint8_t* __restrict__ tf, STopo t, float boost,
+                            int32_t nf) {
     int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nf) return;
     SCell c = finecell(t, n);
     if (c.w.x < 0 || tf[n] != LIQUID) return;
-    x0f[n] += boost * x1f[(c.s - 1) * SBCELLS1 + (c.i >> 1) + 8 * ((c.j >> 1) + 8 * (c.k >> 1))];
+    x0f[n] = sf<T>(lf(x0f[n]) + boost * lf(x1f[(c.s - 1) * SBCELLS1 + (c.i >> 1) + 8 * ((c.j >> 1) + 8 * (c.k >> 1))]));
+}
+
+// Conversions between the Krylov vectors (FP32) and the reduced-precision hierarchy:
+// y = x * mult * scal[slot] (or * mult when slot < 0), over the fine and the coarse range.
+template <class Y, class X>
+__global__ void k_convert(Y* __restrict__ yf, const X* __restrict__ xf, size_t nf, Y* __restrict__ yc, const X* __restrict__ xc,
+                          size_t nc, const double* __restrict__ scal, int slot, int invert) {
+    float c = 1.0f;
+    if (slot >= 0) c = invert ? (float)(1.0 / scal[slot]) : (float)scal[slot];
+    if (!(c == c) || c == 0.0f || !(fabsf(c) <= 3.0e38f)) c = 1.0f;
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < nf + nc; l += (size_t)gridDim.x * blockDim.x) {
+        if (l < nf) yf[l] = sf<Y>(lf(xf[l]) * c);
+        else yc[l - nf] = sf<Y>(lf(xc[l - nf]) * c);
+    }
+}
+// A coupling array in reduced precision, times `scale`.
+template <class Y>
+__global__ void k_castc(Y* __restrict__ y, const float* __restrict__ x, size_t n, float scale) {
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (size_t)gridDim.x * blockDim.x)
+        y[l] = sf<Y>(x[l] * scale);
 }
 
 // ─── vectors: dot products over the unknowns, axpy and friends (sflip.jl:1766-1822) ──
@@ -627,6 +664,104 @@ __global__ void k_saxpy(float* __restrict__ y, const float* __restrict__ x, size
         y[l] = XPBY ? x[l] + c * y[l] : y[l] + c * x[l];
 }
 
+// The unknowns' mask of the inner product (sflip.jl:1772-1805), for position l of the
+// concatenated (fine pool, coarse grid) range.
+__device__ inline bool unknown(size_t l, int32_t nf, const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc,
+                               const STopo& t) {
+    if (l < (size_t)nf) return tf[l] == LIQUID && t.where[l >> 12].x >= 0;
+    size_t m = l - nf;
+    int n0 = t.g.nc(0), n1 = t.g.nc(1);
+    int I = (int)(m % n0), J = (int)((m / n0) % n1), K = (int)(m / ((size_t)n0 * n1));
+    return tc[m] == LIQUID && !covered(t, I, J, K);
+}
+
+// sflip.jl:1889-1892 in one pass: P += alpha d, rold = r, r -= alpha q, and r.r per
+// workgroup (alpha from the device scalars).
+__global__ __launch_bounds__(256) void k_fcg_update(double* __restrict__ partials, float* __restrict__ Pf,
+                                                    float* __restrict__ Pc, float* __restrict__ rf, float* __restrict__ rc,
+                                                    float* __restrict__ roldf, float* __restrict__ roldc,
+                                                    const float* __restrict__ df, const float* __restrict__ dc,
+                                                    const float* __restrict__ qf, const float* __restrict__ qc,
+                                                    const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc, STopo t,
+                                                    int32_t nf, const double* __restrict__ scal, int aslot) {
+    __shared__ double lds[4];
+    float al = (float)scal[aslot];
+    size_t Cc = (size_t)t.g.nc(0) * t.g.nc(1) * t.g.nc(2);
+    double v = 0;
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < (size_t)nf + Cc; l += (size_t)gridDim.x * blockDim.x) {
+        bool fine = l < (size_t)nf;
+        size_t m = fine ? l : l - nf;
+        float* P = fine ? Pf : Pc;
+        float* r = fine ? rf : rc;
+        float* ro = fine ? roldf : roldc;
+        const float* d = fine ? df : dc;
+        const float* q = fine ? qf : qc;
+        P[m] = P[m] + al * d[m];
+        float rv = r[m];
+        ro[m] = rv;
+        rv = rv + -al * q[m];
+        r[m] = rv;
+        if (unknown(l, nf, tf, tc, t)) v += (double)rv * (double)rv;
+    }
+    double s = block_sum_red(v, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+// z.r and z.rold in one pass (sflip.jl:1894-1897): partials[b] and partials[B + b].
+__global__ __launch_bounds__(256) void k_sdot2(double* __restrict__ partials, const float* __restrict__ zf,
+                                               const float* __restrict__ zc, const float* __restrict__ rf,
+                                               const float* __restrict__ rc, const float* __restrict__ of,
+                                               const float* __restrict__ oc, const uint8_t* __restrict__ tf,
+                                               const uint8_t* __restrict__ tc, STopo t, int32_t nf) {
+    __shared__ double lds[4];
+    size_t Cc = (size_t)t.g.nc(0) * t.g.nc(1) * t.g.nc(2);
+    double v1 = 0, v2 = 0;
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < (size_t)nf + Cc; l += (size_t)gridDim.x * blockDim.x) {
+        if (!unknown(l, nf, tf, tc, t)) continue;
+        bool fine = l < (size_t)nf;
+        size_t m = fine ? l : l - nf;
+        double zv = fine ? zf[m] : zc[m];
+        v1 += zv * (double)(fine ? rf[m] : rc[m]);
+        v2 += zv * (double)(fine ? of[m] : oc[m]);
+    }
+    double s1 = block_sum_red(v1, lds);
+    __syncthreads();
+    double s2 = block_sum_red(v2, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s1;
+        partials[gridDim.x + blockIdx.x] = s2;
+    }
+}
+// beta from the two sums (sflip.jl:1895-1900), rho = rho_new.
+__global__ __launch_bounds__(256) void k_sfinish_beta(const double* __restrict__ partials, int np, double* __restrict__ scal) {
+    __shared__ double lds[4];
+    double v1 = 0, v2 = 0;
+    for (int l = threadIdx.x; l < np; l += blockDim.x) { v1 += partials[l]; v2 += partials[np + l]; }
+    double rn = block_sum_red(v1, lds);
+    __syncthreads();
+    double zo = block_sum_red(v2, lds);
+    if (threadIdx.x != 0) return;
+    long long flags = (long long)scal[SS_FLAG];
+    if (!(rn > 0)) flags |= EF_RHONEW_NONPOS;
+    double be = (double)fmaxf(0.0f, ((float)rn - (float)zo) / (float)scal[SS_RHO]);
+    if (!isfinite(be)) flags |= EF_BETA_NONFINITE;
+    scal[SS_RHONEW] = rn;
+    scal[SS_BETA] = be;
+    scal[SS_RHO] = rn;
+    scal[SS_FLAG] = (double)flags;
+}
+// y = x + c y (XPBY) or y += c x over the fine and the coarse range in one launch.
+template <int XPBY>
+__global__ void k_saxpy2(float* __restrict__ yf, const float* __restrict__ xf, size_t nf, float* __restrict__ yc,
+                         const float* __restrict__ xc, size_t nc, const double* __restrict__ scal, int slot, float mult) {
+    float c = slot >= 0 ? mult * (float)scal[slot] : mult;
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < nf + nc; l += (size_t)gridDim.x * blockDim.x) {
+        float* y = l < nf ? yf : yc;
+        const float* x = l < nf ? xf : xc;
+        size_t m = l < nf ? l : l - nf;
+        y[m] = XPBY ? x[m] + c * y[m] : y[m] + c * x[m];
+    }
+}
+
 // ─── projection (sflip.jl:1906-1962) ─────────────────────────────────────────
 
 __global__ void k_projectfine(float* __restrict__ u, uint8_t* __restrict__ valid, const float* __restrict__ P,
@@ -679,6 +814,10 @@ __global__ void k_projectcoarse(float* __restrict__ u, uint8_t* __restrict__ val
 
 // ─── the solver ──────────────────────────────────────────────────────────────
 
+template <class T> struct SVecT {   // a pressure-sized vector in storage T
+    T* f;
+    T* c;
+};
 struct SVec {   // a pressure-sized vector: fine pool part and coarse part
     float* f;
     float* c;
@@ -694,18 +833,6 @@ struct SparseSolver {
     explicit SparseSolver(SparseBasin& b) : s(b), st(b.st.s), t(b.topo()), nf((int32_t)b.finecells()), Cc(b.ncells()) {}
 
     SVec vec(SField<float>& f) { return SVec{f.fine.p, f.coarse.p}; }
-    Comp level0() {
-        return Comp{FineCoef{{s.cf[0].p, s.cf[1].p, s.cf[2].p}}, s.dD.fine.p, s.types.fine.p,
-                    CoarseCoef{{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p};
-    }
-    Comp level1() {
-        return Comp{FineCoef{{s.c1[0].p, s.c1[1].p, s.c1[2].p}}, s.dD1.p, s.act1.p,
-                    CoarseCoef{{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p};
-    }
-    DView dview(int l) {
-        SLevel& L = s.D[l];
-        return DView{make_int3(L.n[0], L.n[1], L.n[2]), {L.c[0].p, L.c[1].p, L.c[2].p}, L.dD.p, L.act.p};
-    }
     static int3 i3(const int n[3]) { return make_int3(n[0], n[1], n[2]); }
 
     void coefficients() {
@@ -747,83 +874,183 @@ struct SparseSolver {
         MAEL_LAUNCH_CHECK();
     }
 
+
+    // The hierarchy in one storage precision: its vectors and couplings.
+    template <class T, class C> struct Hier {
+        SVecT<T> q, tmp, X1x, X1b, X1r, X1t;
+        std::vector<T*> Dx, Db, Dr, Dt;
+        CompT<C> L0, L1;
+        std::vector<DViewT<C>> D;
+        float rscale = 1.0f;   // dense-level restriction scale
+    };
+    Hier<float, float> hier32() {
+        Hier<float, float> H;
+        H.q = {s.q.fine.p, s.q.coarse.p};
+        H.tmp = {s.tmp.fine.p, s.tmp.coarse.p};
+        H.X1x = {s.X1x.fine.p, s.X1x.coarse.p}; H.X1b = {s.X1b.fine.p, s.X1b.coarse.p};
+        H.X1r = {s.X1r.fine.p, s.X1r.coarse.p}; H.X1t = {s.X1t.fine.p, s.X1t.coarse.p};
+        for (auto& L : s.D) {
+            H.Dx.push_back(L.x.p); H.Db.push_back(L.b.p); H.Dr.push_back(L.r.p); H.Dt.push_back(L.t.p);
+            H.D.push_back(DViewT<float>{i3(L.n), {L.c[0].p, L.c[1].p, L.c[2].p}, L.dD.p, L.act.p});
+        }
+        H.L0 = CompT<float>{{{s.cf[0].p, s.cf[1].p, s.cf[2].p}}, s.dD.fine.p, s.types.fine.p,
+                            {{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p};
+        H.L1 = CompT<float>{{{s.c1[0].p, s.c1[1].p, s.c1[2].p}}, s.dD1.p, s.act1.p,
+                            {{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p};
+        return H;
+    }
+    // The reduced-precision hierarchy (storage in s.mp): vectors and couplings, the dense
+    // levels' equations divided by 4 per level below the coarse grid.
+    template <class T> Hier<T, T> hier16() {
+        MPStore& m = s.mp;
+        auto V = [&](DBuf<uint16_t>& f, DBuf<uint16_t>& c) { return SVecT<T>{(T*)f.p, (T*)c.p}; };
+        Hier<T, T> H;
+        H.q = V(m.q[0], m.q[1]); H.tmp = V(m.tmp[0], m.tmp[1]);
+        H.X1x = V(m.X1[0][0], m.X1[0][1]); H.X1b = V(m.X1[1][0], m.X1[1][1]);
+        H.X1r = V(m.X1[2][0], m.X1[2][1]); H.X1t = V(m.X1[3][0], m.X1[3][1]);
+        for (size_t l = 0; l < s.D.size(); ++l) {
+            H.Dx.push_back((T*)m.Dv[l][0].p); H.Db.push_back((T*)m.Dv[l][1].p);
+            H.Dr.push_back((T*)m.Dv[l][2].p); H.Dt.push_back((T*)m.Dv[l][3].p);
+            H.D.push_back(DViewT<T>{i3(s.D[l].n), {(T*)m.Dc[l][0].p, (T*)m.Dc[l][1].p, (T*)m.Dc[l][2].p}, (T*)m.Dc[l][3].p,
+                                    s.D[l].act.p});
+        }
+        H.L0 = CompT<T>{{{(T*)m.cf[0].p, (T*)m.cf[1].p, (T*)m.cf[2].p}}, (T*)m.dDf.p, s.types.fine.p,
+                        {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p};
+        H.L1 = CompT<T>{{{(T*)m.c1[0].p, (T*)m.c1[1].p, (T*)m.c1[2].p}}, (T*)m.dD1.p, s.act1.p,
+                        {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p};
+        H.rscale = 0.25f;
+        return H;
+    }
+    // Round the couplings of the hierarchy into the reduced-precision copies.
+    template <class T> void cast_couplings() {
+        MPStore& m = s.mp;
+        auto cast = [&](DBuf<uint16_t>& y, const float* x, size_t n, float scale) {
+            if (n) hipLaunchKernelGGL(k_castc<T>, dim3(std::min<unsigned>(cdiv(n, 256), 8192)), dim3(256), 0, st, (T*)y.p, x, n,
+                                      scale);
+        };
+        for (int a = 0; a < 3; ++a) {
+            cast(m.cf[a], s.cf[a].p, s.finefaces(), 1.0f);
+            cast(m.cc[a], s.cc[a].p, s.nfaces(a), 1.0f);
+            cast(m.c1[a], s.c1[a].p, (size_t)s.nslots * SBFACES1, 1.0f);
+        }
+        cast(m.dDf, s.dD.fine.p, s.finecells(), 1.0f);
+        cast(m.dDc, s.dD.coarse.p, Cc, 1.0f);
+        cast(m.dD1, s.dD1.p, (size_t)s.nslots * SBCELLS1, 1.0f);
+        float sc = 1.0f;
+        for (size_t l = 0; l < s.D.size(); ++l) {
+            for (int a = 0; a < 3; ++a) cast(m.Dc[l][a], s.D[l].c[a].p, s.D[l].faces(a), sc);
+            cast(m.Dc[l][3], s.D[l].dD.p, s.D[l].cells(), sc);
+            sc *= 0.25f;
+        }
+        MAEL_LAUNCH_CHECK();
+    }
+
+    template <class T> void zero_t(T* p, size_t n) {
+        if (!n) return;
+        if (sizeof(T) == 2) fill_async((uint16_t*)p, n, (uint16_t)0, st);
+        else fill_async((float*)p, n, 0.0f, st);
+    }
+    template <class T> void zero_t(SVecT<T> v, size_t nfine) { zero_t(v.f, nfine); zero_t(v.c, Cc); }
+    void zero(SVec v, size_t nfine) { zero_t(SVecT<float>{v.f, v.c}, nfine); }
+
     // One composite-level pass (MODE 0 Jacobi, 1 residual, 2 apply) over the fine part
     // (N^3 a slot) and the coarse part.
-    template <int N, int MODE> void rows(SVec out, SVec x, SVec b, const Comp& L) {
+    template <int N, int MODE, class T, class C> void rows(SVecT<T> out, SVecT<T> x, SVecT<T> b, const CompT<C>& L) {
         int32_t nfl = s.nslots * N * N * N;
         if (nfl) {
             if (N == 16 && MODE == 0) s.kt.begin(K_RBGS, st);
-            hipLaunchKernelGGL((k_rowfine<N, MODE>), dim3(cdiv(nfl, 256)), dim3(256), 0, st, out.f, x.f, b.f, x.c, L, t, nfl);
+            hipLaunchKernelGGL((k_rowfine<N, MODE, T, C>), dim3(cdiv(nfl, 256)), dim3(256), 0, st, out.f, x.f, b.f, x.c, L, t,
+                               nfl);
             if (N == 16 && MODE == 0) s.kt.end(K_RBGS, st);
         }
-        hipLaunchKernelGGL((k_rowcoarse<N, MODE>), dim3(cdiv(Cc, 256)), dim3(256), 0, st, out.c, x.c, b.c, x.f, L, t);
+        hipLaunchKernelGGL((k_rowcoarse<N, MODE, T, C>), dim3(cdiv(Cc, 256)), dim3(256), 0, st, out.c, x.c, b.c, x.f, L, t);
     }
-    template <int N> void smooth(SVec x, SVec b, SVec tmp, const Comp& L, int sweeps) {
+    template <int N, class T, class C> void smooth(SVecT<T> x, SVecT<T> b, SVecT<T> tmp, const CompT<C>& L, int sweeps) {
         for (int k = 0; k < sweeps; ++k) {
             rows<N, 0>(tmp, x, b, L);
             rows<N, 0>(x, tmp, b, L);
         }
     }
-    void zero(SVec v, size_t nfine) {
-        fill_async(v.f, nfine, 0.0f, st);
-        fill_async(v.c, Cc, 0.0f, st);
-    }
 
-    // sflip.jl:1722-1762 vcycle!: z = M^-1 r.
-    void vcycle(SVec z, SVec r, int sweeps, int coarse, float boost) {
-        Comp L0 = level0(), L1 = level1();
+    // sflip.jl:1722-1762 vcycle!: z = M^-1 r in the hierarchy H (coarsest in arithmetic A).
+    template <class T, class C, class A>
+    void vcycle_t(Hier<T, C>& H, SVecT<T> z, SVecT<T> r, int sweeps, int coarse, float boost) {
         int ns = s.nslots;
-        SVec q = vec(s.q), tmp = vec(s.tmp);
-        SVec X1x = vec(s.X1x), X1b = vec(s.X1b), X1r = vec(s.X1r), X1t = vec(s.X1t);
-        zero(z, (size_t)nf);
-        smooth<16>(z, r, tmp, L0, sweeps);
-        rows<16, 1>(q, z, r, L0);
-        if (ns) hipLaunchKernelGGL(k_restrict01, dim3(cdiv(ns * SBCELLS1, 256)), dim3(256), 0, st, X1b.f, q.f, t, ns);
-        MAEL_HIP(hipMemcpyAsync(X1b.c, q.c, Cc * sizeof(float), hipMemcpyDeviceToDevice, st));
-        zero(X1x, (size_t)ns * SBCELLS1);
-        smooth<8>(X1x, X1b, X1t, L1, sweeps);
-        rows<8, 1>(X1r, X1x, X1b, L1);
-        SLevel& D2 = s.D[0];
-        hipLaunchKernelGGL(k_restrict12, dim3(cdiv(Cc, 256)), dim3(256), 0, st, D2.b.p, X1r.f, X1r.c, t);
+        zero_t(z, (size_t)nf);
+        smooth<16>(z, r, H.tmp, H.L0, sweeps);
+        rows<16, 1>(H.q, z, r, H.L0);
+        if (ns) hipLaunchKernelGGL(k_restrict01<T>, dim3(cdiv(ns * SBCELLS1, 256)), dim3(256), 0, st, H.X1b.f, H.q.f, t, ns);
+        MAEL_HIP(hipMemcpyAsync(H.X1b.c, H.q.c, Cc * sizeof(T), hipMemcpyDeviceToDevice, st));
+        zero_t(H.X1x, (size_t)ns * SBCELLS1);
+        smooth<8>(H.X1x, H.X1b, H.X1t, H.L1, sweeps);
+        rows<8, 1>(H.X1r, H.X1x, H.X1b, H.L1);
+        hipLaunchKernelGGL(k_restrict12<T>, dim3(cdiv(Cc, 256)), dim3(256), 0, st, H.Db[0], H.X1r.f, H.X1r.c, t);
         int nl = (int)s.D.size();
         for (int l = 0; l + 1 < nl; ++l) {
             SLevel& Dl = s.D[l];
-            DView V = dview(l);
             size_t c = Dl.cells();
-            fill_async(Dl.x.p, c, 0.0f, st);
+            zero_t(H.Dx[l], c);
             for (int k = 0; k < sweeps; ++k) {
-                hipLaunchKernelGGL(k_rowdense<0>, dim3(cdiv(c, 256)), dim3(256), 0, st, Dl.t.p, Dl.x.p, Dl.b.p, V);
-                hipLaunchKernelGGL(k_rowdense<0>, dim3(cdiv(c, 256)), dim3(256), 0, st, Dl.x.p, Dl.t.p, Dl.b.p, V);
+                hipLaunchKernelGGL((k_rowdense<0, T, C>), dim3(cdiv(c, 256)), dim3(256), 0, st, H.Dt[l], H.Dx[l], H.Db[l], H.D[l]);
+                hipLaunchKernelGGL((k_rowdense<0, T, C>), dim3(cdiv(c, 256)), dim3(256), 0, st, H.Dx[l], H.Dt[l], H.Db[l], H.D[l]);
             }
-            hipLaunchKernelGGL(k_rowdense<1>, dim3(cdiv(c, 256)), dim3(256), 0, st, Dl.r.p, Dl.x.p, Dl.b.p, V);
+            hipLaunchKernelGGL((k_rowdense<1, T, C>), dim3(cdiv(c, 256)), dim3(256), 0, st, H.Dr[l], H.Dx[l], H.Db[l], H.D[l]);
             SLevel& Cn = s.D[l + 1];
-            hipLaunchKernelGGL(k_restrictdense, dim3(cdiv(Cn.cells(), 256)), dim3(256), 0, st, Cn.b.p, Dl.r.p, i3(Cn.n), i3(Dl.n));
+            hipLaunchKernelGGL(k_restrictdense<T>, dim3(cdiv(Cn.cells(), 256)), dim3(256), 0, st, H.Db[l + 1], H.Dr[l], i3(Cn.n),
+                               i3(Dl.n), H.rscale);
         }
-        SLevel& Dm = s.D[nl - 1];
-        hipLaunchKernelGGL(k_scoarsest, dim3(1), dim3(1024), 0, st, Dm.x.p, Dm.b.p, dview(nl - 1), coarse);
+        hipLaunchKernelGGL((k_scoarsest<T, C, A>), dim3(1), dim3(1024), 0, st, H.Dx[nl - 1], H.Db[nl - 1], H.D[nl - 1], coarse);
         for (int l = nl - 2; l >= 0; --l) {
             SLevel &Dl = s.D[l], &Cn = s.D[l + 1];
-            DView V = dview(l);
             size_t c = Dl.cells();
-            hipLaunchKernelGGL(k_prolongdense, dim3(cdiv(c, 256)), dim3(256), 0, st, Dl.x.p, Cn.x.p, Dl.act.p, i3(Dl.n), i3(Cn.n),
-                               boost);
+            hipLaunchKernelGGL(k_prolongdense<T>, dim3(cdiv(c, 256)), dim3(256), 0, st, H.Dx[l], H.Dx[l + 1], Dl.act.p, i3(Dl.n),
+                               i3(Cn.n), boost);
             for (int k = 0; k < sweeps; ++k) {
-                hipLaunchKernelGGL(k_rowdense<0>, dim3(cdiv(c, 256)), dim3(256), 0, st, Dl.t.p, Dl.x.p, Dl.b.p, V);
-                hipLaunchKernelGGL(k_rowdense<0>, dim3(cdiv(c, 256)), dim3(256), 0, st, Dl.x.p, Dl.t.p, Dl.b.p, V);
+                hipLaunchKernelGGL((k_rowdense<0, T, C>), dim3(cdiv(c, 256)), dim3(256), 0, st, H.Dt[l], H.Dx[l], H.Db[l], H.D[l]);
+                hipLaunchKernelGGL((k_rowdense<0, T, C>), dim3(cdiv(c, 256)), dim3(256), 0, st, H.Dx[l], H.Dt[l], H.Db[l], H.D[l]);
             }
         }
+        SLevel& D2 = s.D[0];
         if (ns)
-            hipLaunchKernelGGL(k_prolong21, dim3(cdiv(ns * SBCELLS1, 256)), dim3(256), 0, st, X1x.f, D2.x.p, s.act1.p, t, boost,
-                               ns);
-        hipLaunchKernelGGL(k_prolongcoarse, dim3(cdiv(Cc, 256)), dim3(256), 0, st, X1x.c, D2.x.p, D2.act.p, t, boost);
-        smooth<8>(X1x, X1b, X1t, L1, sweeps);
-        if (nf) hipLaunchKernelGGL(k_prolong10, dim3(cdiv(nf, 256)), dim3(256), 0, st, z.f, X1x.f, s.types.fine.p, t, boost, nf);
-        hipLaunchKernelGGL(k_prolongcoarse, dim3(cdiv(Cc, 256)), dim3(256), 0, st, z.c, X1x.c, D2.act.p, t, boost);
-        smooth<16>(z, r, tmp, L0, sweeps);
+            hipLaunchKernelGGL(k_prolong21<T>, dim3(cdiv(ns * SBCELLS1, 256)), dim3(256), 0, st, H.X1x.f, H.Dx[0], s.act1.p, t,
+                               boost, ns);
+        hipLaunchKernelGGL(k_prolongcoarse<T>, dim3(cdiv(Cc, 256)), dim3(256), 0, st, H.X1x.c, H.Dx[0], D2.act.p, t, boost);
+        smooth<8>(H.X1x, H.X1b, H.X1t, H.L1, sweeps);
+        if (nf)
+            hipLaunchKernelGGL(k_prolong10<T>, dim3(cdiv(nf, 256)), dim3(256), 0, st, z.f, H.X1x.f, s.types.fine.p, t, boost, nf);
+        hipLaunchKernelGGL(k_prolongcoarse<T>, dim3(cdiv(Cc, 256)), dim3(256), 0, st, z.c, H.X1x.c, D2.act.p, t, boost);
+        smooth<16>(z, r, H.tmp, H.L0, sweeps);
         MAEL_LAUNCH_CHECK();
     }
 
-    void apply(SVec q, SVec x) { rows<16, 2>(q, x, x, level0()); }
+    int precision = 0;   // 0 FP32, 1 FP16 storage, 2 BF16 storage
+    // z = M^-1 r. In reduced precision r is scaled by 1 / scal[nslot] on the way in (its
+    // norm, so its entries are at most 1) and z back on the way out.
+    void vcycle(SVec z, SVec r, int sweeps, int coarse, float boost, int nslot = -1) {
+        if (precision == 0) {
+            Hier<float, float> H = hier32();
+            vcycle_t<float, float, float>(H, SVecT<float>{z.f, z.c}, SVecT<float>{r.f, r.c}, sweeps, coarse, boost);
+            return;
+        }
+        if (precision == 1) vcycle_mp<__half>(z, r, sweeps, coarse, boost, nslot);
+        else vcycle_mp<bf16>(z, r, sweeps, coarse, boost, nslot);
+    }
+    template <class T> void vcycle_mp(SVec z, SVec r, int sweeps, int coarse, float boost, int nslot) {
+        Hier<T, T> H = hier16<T>();
+        MPStore& m = s.mp;
+        SVecT<T> zh{(T*)m.z[0].p, (T*)m.z[1].p}, rh{(T*)m.r[0].p, (T*)m.r[1].p};
+        unsigned g = std::min<unsigned>(cdiv((size_t)nf + Cc, 256), 8192);
+        hipLaunchKernelGGL((k_convert<T, float>), dim3(g), dim3(256), 0, st, rh.f, r.f, (size_t)nf, rh.c, r.c, Cc, s.scal.p, nslot,
+                           1);
+        vcycle_t<T, T, double>(H, zh, rh, sweeps, coarse, boost);
+        hipLaunchKernelGGL((k_convert<float, T>), dim3(g), dim3(256), 0, st, z.f, zh.f, (size_t)nf, z.c, zh.c, Cc, s.scal.p, nslot,
+                           0);
+    }
+
+    void apply(SVec q, SVec x) {
+        Hier<float, float> H = hier32();
+        rows<16, 2>(SVecT<float>{q.f, q.c}, SVecT<float>{x.f, x.c}, SVecT<float>{x.f, x.c}, H.L0);
+    }
     void dot(SVec a, SVec b, int op, int slot) {
         hipLaunchKernelGGL(k_sdot, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, a.f, b.f, a.c, b.c, s.types.fine.p,
                            s.types.coarse.p, t, nf);
@@ -832,13 +1059,13 @@ struct SparseSolver {
     }
     // y += c x (c = mult * scal[slot], or mult when slot < 0), over the used fine cells and the coarse part.
     void axpy(SVec y, SVec x, int slot, float mult) {
-        if (nf) hipLaunchKernelGGL(k_saxpy<0>, dim3(std::min<unsigned>(cdiv(nf, 256), 8192)), dim3(256), 0, st, y.f, x.f, (size_t)nf, s.scal.p, slot, mult);
-        hipLaunchKernelGGL(k_saxpy<0>, dim3(std::min<unsigned>(cdiv(Cc, 256), 8192)), dim3(256), 0, st, y.c, x.c, Cc, s.scal.p, slot, mult);
+        hipLaunchKernelGGL(k_saxpy2<0>, dim3(std::min<unsigned>(cdiv((size_t)nf + Cc, 256), 8192)), dim3(256), 0, st, y.f, x.f,
+                           (size_t)nf, y.c, x.c, Cc, s.scal.p, slot, mult);
     }
     // y = x + c y
     void xpby(SVec y, SVec x, int slot, float mult) {
-        if (nf) hipLaunchKernelGGL(k_saxpy<1>, dim3(std::min<unsigned>(cdiv(nf, 256), 8192)), dim3(256), 0, st, y.f, x.f, (size_t)nf, s.scal.p, slot, mult);
-        hipLaunchKernelGGL(k_saxpy<1>, dim3(std::min<unsigned>(cdiv(Cc, 256), 8192)), dim3(256), 0, st, y.c, x.c, Cc, s.scal.p, slot, mult);
+        hipLaunchKernelGGL(k_saxpy2<1>, dim3(std::min<unsigned>(cdiv((size_t)nf + Cc, 256), 8192)), dim3(256), 0, st, y.f, x.f,
+                           (size_t)nf, y.c, x.c, Cc, s.scal.p, slot, mult);
     }
     void copy(SVec y, SVec x) {
         if (nf) MAEL_HIP(hipMemcpyAsync(y.f, x.f, (size_t)nf * sizeof(float), hipMemcpyDeviceToDevice, st));
@@ -884,24 +1111,25 @@ struct SparseSolver {
         double rnorm = read(SS_RNORM);
         relres = rnorm / bnorm;
         if (rnorm <= target) { converged = true; return 0; }
-        vcycle(z, r, sweeps, coarse, boost);
+        vcycle(z, r, sweeps, coarse, boost, SS_RNORM);
         copy(d, z);
         dot(z, r, SF_RHO0, 0);
         for (int it = 1; it <= maxiter; ++it) {
             apply(q, d);
             dot(d, q, SF_ALPHA, 0);
-            axpy(P, d, SS_ALPHA, 1.0f);
-            copy(rold, r);
-            axpy(r, q, SS_ALPHA, -1.0f);
-            dot(r, r, SF_NORM, SS_RNORM);
+            hipLaunchKernelGGL(k_fcg_update, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, P.f, P.c, r.f, r.c, rold.f,
+                               rold.c, d.f, d.c, q.f, q.c, s.types.fine.p, s.types.coarse.p, t, nf, s.scal.p, (int)SS_ALPHA);
+            hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(256), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p, (int)SF_NORM,
+                               (int)SS_RNORM);
             double flags;
             read2(SS_RNORM, rnorm, flags);
             raise_flags((long long)flags);
             relres = rnorm / bnorm;
             if (rnorm <= target) { converged = true; return it; }
-            vcycle(z, r, sweeps, coarse, boost);
-            dot(z, r, SF_SET, SS_RHONEW);
-            dot(z, rold, SF_BETA, 0);
+            vcycle(z, r, sweeps, coarse, boost, SS_RNORM);
+            hipLaunchKernelGGL(k_sdot2, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, z.f, z.c, r.f, r.c, rold.f, rold.c,
+                               s.types.fine.p, s.types.coarse.p, t, nf);
+            hipLaunchKernelGGL(k_sfinish_beta, dim3(1), dim3(256), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p);
             xpby(d, z, SS_BETA, 1.0f);
         }
         double flags;
@@ -956,7 +1184,7 @@ struct SparseSolver {
                 vcycle(z, Vj, sweeps, coarse, boost);
                 // the diagonal part: one damped Jacobi sweep from zero (fgmres.jl:15-23)
                 zero(tmp, (size_t)nf);
-                rows<16, 0>(Zj, tmp, Vj, level0());
+                rows<16, 0>(SVecT<float>{Zj.f, Zj.c}, SVecT<float>{tmp.f, tmp.c}, SVecT<float>{Vj.f, Vj.c}, hier32().L0);
                 axpy(Zj, z, -1, 1.0f);
                 apply(q, Zj);
                 int k = 0;
@@ -1026,6 +1254,12 @@ void SparseBasin::project(float gamma, const SparseOpts& o, SparseStats& stats) 
                        u[1].coarse.p, u[2].coarse.p, t, gamma);
     MAEL_LAUNCH_CHECK();
     S.hierarchy();
+    S.precision = o.precision;
+    if (o.precision != 0) {
+        mp_alloc();
+        if (o.precision == 1) S.cast_couplings<__half>();
+        else S.cast_couplings<bf16>();
+    }
     double rr = 0;
     bool conv = false;
     if (o.policy == 0) {

@@ -84,6 +84,29 @@ SparseBasin::SparseBasin(const SGrid& grid, int nblk, int64_t cap, int32_t bnd, 
     st.sync();
 }
 
+void SparseBasin::mp_alloc() {
+    if (mp.ready) return;
+    size_t nf = (size_t)blocks * SBCELLS, nfa = (size_t)blocks * SBFACES, n1 = (size_t)blocks * SBCELLS1, Cc = ncells();
+    for (auto* v : {mp.z, mp.r, mp.q, mp.tmp}) { v[0].alloc(nf); v[1].alloc(Cc); }
+    for (auto& v : mp.X1) { v[0].alloc(n1); v[1].alloc(Cc); }
+    mp.Dv.resize(D.size());
+    mp.Dc.resize(D.size());
+    for (size_t l = 0; l < D.size(); ++l) {
+        for (auto& b : mp.Dv[l]) b.alloc(D[l].cells());
+        for (int a = 0; a < 3; ++a) mp.Dc[l][a].alloc(D[l].faces(a));
+        mp.Dc[l][3].alloc(D[l].cells());
+    }
+    for (int a = 0; a < 3; ++a) {
+        mp.cf[a].alloc(nfa); mp.cc[a].alloc(nfaces(a)); mp.c1[a].alloc((size_t)blocks * SBFACES1);
+    }
+    mp.dDf.alloc(nf); mp.dDc.alloc(Cc); mp.dD1.alloc(n1);
+    // every vector starts at zero: free slots are never written and must read as zero
+    for (auto* v : {mp.z, mp.r, mp.q, mp.tmp}) { fill_async(v[0].p, nf, (uint16_t)0, st.s); fill_async(v[1].p, Cc, (uint16_t)0, st.s); }
+    for (auto& v : mp.X1) { fill_async(v[0].p, n1, (uint16_t)0, st.s); fill_async(v[1].p, Cc, (uint16_t)0, st.s); }
+    st.sync();
+    mp.ready = true;
+}
+
 // ─── marks (sflip.jl:384-455) ────────────────────────────────────────────────
 
 __device__ inline size_t blockof(const SGrid& g, int I, int J, int K) {   // the block of coarse cell (I, J, K)

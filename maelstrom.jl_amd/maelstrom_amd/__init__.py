@@ -6,14 +6,14 @@
 ABI (include/maelstrom_hip.h).
 """
 from ._lib import MaelstromError, Context, default_context, LIBPATH, lib
-from .basin import Basin, Race, Stir, Wave, narutobasin, surface, oseen, balanced
+from .basin import Basin, Race, Stir, Wave, narutobasin, narutowaves, waves, surface, oseen, balanced
 from .dense import (FlipBasin, flipgrid, initialize, resample, particles2grid, grid2particles, project, step, advance,
                     volume, maxspeed, classify, redistance)
 from . import sparse
 from .sparse import SparseBasin, sparsegrid
 
 __all__ = ["MaelstromError", "Context", "default_context", "LIBPATH", "lib", "Basin", "Race", "Stir", "Wave",
-           "narutobasin", "surface",
+           "narutobasin", "narutowaves", "waves", "surface",
            "oseen", "balanced", "FlipBasin", "flipgrid", "initialize", "resample", "particles2grid",
            "grid2particles", "project", "step", "advance", "volume", "maxspeed", "classify", "redistance", "sparse",
            "SparseBasin", "sparsegrid"]

@@ -67,7 +67,7 @@ def surface(b, r):
     return balanced(b.Γ, b.core, b.g, r)
 
 
-__all__ = ["Basin", "narutobasin", "oseen", "balanced", "surface", "replace"]
+__all__ = ["Basin", "narutobasin", "oseen", "balanced", "surface", "replace", "Race", "Wave", "Stir", "waves", "narutowaves"]
 
 
 @dataclass
@@ -94,6 +94,33 @@ class Wave:
     a: float
     q: float
     phase: float
+
+
+def waves(n=48, wind=0.6, spread=1.1, hs=0.35, lmin=0.5, lmax=24.0, seed=3):
+    """sea.jl:22-32 `waves(; n, wind, spread, hs, λmin, λmax, seed)`: `n` Gerstner waves,
+    wavelengths log-spaced from `lmax` down to `lmin`, amplitudes as wavelength^1.25
+    scaled so the sea's significant height is about `hs`, steepness 0.55, deep-water
+    frequency sqrt(9.81 k), directions within `wind ± spread`. Each wave draws its
+    direction, then its phase; the draws come from numpy's PCG64 seeded with `seed`, not
+    from Julia's MersenneTwister, so the table has the reference's shape and spectrum but
+    not its exact angles and phases."""
+    rng = np.random.default_rng(seed)
+    lams = np.exp(np.linspace(np.log(lmax), np.log(lmin), n))
+    raw = lams ** 1.25
+    scale = hs / (4.0 * np.sqrt(np.sum(raw * raw) / 2.0))
+    out = []
+    for lam, r in zip(lams, raw):
+        th = wind + spread * (2.0 * rng.random() - 1.0)
+        k = 2.0 * np.pi / lam
+        ph = 2.0 * np.pi * rng.random()
+        f = lambda v: float(np.float32(v))
+        out.append(Wave((f(np.cos(th)), f(np.sin(th))), f(k), f(np.sqrt(9.81 * k)), f(scale * r), 0.55, f(ph)))
+    return out
+
+
+def narutowaves(hs=1.4, lmin=1.0, lmax=10.0, wind=0.4, spread=0.9, seed=5):
+    """sea.jl:35-36 `narutowaves`: a wind sea of 1.4 m significant height, 1-10 m long, 32 waves."""
+    return waves(n=32, wind=wind, spread=spread, hs=hs, lmin=lmin, lmax=lmax, seed=seed)
 
 
 @dataclass

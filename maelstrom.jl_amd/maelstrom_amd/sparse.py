@@ -10,6 +10,7 @@ mirrored from the library.
 """
 import ctypes as C
 import math
+import warnings
 
 import numpy as np
 
@@ -232,12 +233,29 @@ def gamma_for(s, dt, recover=0.5):
     return float(np.float32(min(max(loss / recover, -0.01 / dt), 0.01 / dt)))
 
 
+def pressure_defaults(s):
+    """The solve `pressure!(s)` dispatches to: with a surface stir, fgmres.jl:96 (tol 5e-4,
+    maxiter 240, boost 1.5, flexible CG for at most 80 iterations, then FGMRES); else
+    sflip.jl:1867 (tol 1e-4, maxiter 200, boost 1, flexible CG that warns when it stops
+    short)."""
+    if s.stir is not None and s.stir.surface:
+        return dict(tol=5e-4, maxiter=240, boost=1.5, policy=2)
+    return dict(tol=1e-4, maxiter=200, boost=1.0, policy=0)
+
+
 def project(s, dt, recover=0.5, gamma=None, **solve):
-    """sflip.jl:1998-2015 `project!(s, dt; recover)`; `solve` are the solver's options."""
+    """sflip.jl:1998-2015 `project!(s, dt; recover)`; `solve` overrides the options of
+    `pressure_defaults(s)`."""
     g = gamma_for(s, dt, recover) if gamma is None else float(gamma)
-    opts = solve_options(**solve)
+    o = pressure_defaults(s)
+    o.update(solve)
+    opts = solve_options(**o)
     st = SparseStats()
     s.ctx.check(_lib.lib().mael_sparse_project(s.h, g, C.byref(opts), C.byref(st)))
+    if opts.policy == 0 and st.rel_residual > opts.tol:
+        # sflip.jl:1902: the plain flexible CG warns and keeps its last iterate
+        warnings.warn("pressure solve did not converge: residual %.3g after %d iterations"
+                      % (st.rel_residual, st.iterations), RuntimeWarning, stacklevel=2)
     s.iterations.append(st.iterations)
     s.last_solve = st
     return st

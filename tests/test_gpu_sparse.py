@@ -249,6 +249,23 @@ def test_project_parity(M, policy):
         np.testing.assert_array_equal(s.get("F" + nm.upper() + "_C").ravel(), o.array("f" + nm + "_c"))
 
 
+@pytest.mark.parametrize("precision", [1, 2])
+def test_project_mixed_precision(M, precision):
+    """The FP16 / BF16 hierarchy (FP32 CG, FP64 sums and coarsest solve) reaches the same
+    1e-6 true residual and the same pressure as the FP64 oracle."""
+    print("case: sparse project precision %d" % precision, flush=True)
+    o = _scene()
+    _projection_inputs(o, seed=13)
+    s = _upload(M, o)
+    st = M.sparse.project(s, 0.1, gamma=0.005, tol=1e-6, maxiter=400, policy=2, boost=1.5, precision=precision)
+    assert st.rel_residual <= 1e-6 * 1.01, st.rel_residual
+    o.project(gamma=0.005, tol=1e-12, maxiter=600, policy=0)
+    ns = o.nslots
+    Pe = np.concatenate([o.array("P_f")[:ns * BCELLS], o.array("P_c")])
+    Pg = np.concatenate([s.get("P_F"), s.get("P_C").ravel()])
+    np.testing.assert_allclose(Pg, Pe, rtol=0, atol=2e-5 * np.abs(Pe).max())
+
+
 def test_project_errors(M):
     o = _scene()
     s = _upload(M, o)
@@ -266,13 +283,14 @@ def test_steps_keep_volume(M, surface):
     air, extrapolation): finite, the volume kept to a few percent."""
     print("case: sparse steps, surface stir %s" % surface, flush=True)
     b = _basin(M)
-    stir = M.Stir(b, race=M.Race(eddy=3.0), surface=surface)
+    stir = M.Stir(b, waves=M.narutowaves(), race=M.Race(eddy=3.0), surface=surface)
     s = M.SparseBasin(b, dx=DX, blocks=64, capacity=300000, cfl=4, stir=stir)
     M.sparse.initialize(s)
     v0 = s.volume0
+    tol = M.sparse.pressure_defaults(s)["tol"]   # 5e-4 with the surface stir, else 1e-4
     for _ in range(3):
-        st = M.sparse.step(s, 0.1, tol=1e-6)
-        assert st.rel_residual <= 1.01e-6
+        st = M.sparse.step(s, 0.1)
+        assert st.rel_residual <= tol * 1.01, st.rel_residual
     v = M.sparse.volume(s)
     assert 0.95 < v / v0 < 1.05
     assert np.isfinite(s.get("U_F")).all() and s.n > 0

@@ -190,7 +190,8 @@ def sparse_main(args, desc, kw, dx):
     umax = max(float(np.abs(s.get(f)).max()) for f in ("U_F", "V_F", "W_F", "U_C", "V_C", "W_C"))
     left = 1.0 / 24
     dt = left / math.ceil(left / (s.cfl * dx / max(umax, 1e-3)))
-    solve = dict(tol=args.tol, maxiter=240, restart=20, policy=2, boost=1.5, precision=args.precision)
+    solve = dict(tol=args.tol, maxiter=240, restart=20, policy=2, boost=1.5, precision=args.precision,
+                 refine=args.refine)
     for _ in range(args.warmup):
         sp.step(s, dt, **solve)
     s.sync()
@@ -224,7 +225,7 @@ def sparse_main(args, desc, kw, dx):
         "config": {"workload": desc, "blocks": list(int(x) for x in s.grid.nb), "coarse_cells": list(s.nc),
                    "live_blocks": live_slots, "fine_cells": live_slots * 4096, "particles": int(n_avg),
                    "dt_s": round(dt, 5), "tol": args.tol, "policy": "fcg-then-fgmres(20)",
-                   "precision": ["fp32", "fp16", "bf16"][args.precision],
+                   "precision": ["fp32", "fp16", "bf16"][args.precision], "refine": args.refine,
                    "stir": "surface stir, 32 seeded narutowaves-shaped waves, race eddy 3",
                    "phases": "two advection-reflection halves of refine, advect, p2g, redistance, classify, "
                              "trapped air, stir forces, project, kick, extrapolate, reflect, g2p, resample"},
@@ -249,7 +250,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="sparse5cm", choices=sorted(WORKLOADS))
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--precision", type=int, default=0, help="V-cycle storage: 0 FP32, 1 FP16, 2 BF16")
+    ap.add_argument("--refine", type=int, default=10,
+                    help="sparse solve: FP64 residual recomputation every k iterations (0: the plain FP32 solve)")
     ap.add_argument("--kernel", default="rbgs", help="the kernel the roofline object reports")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

@@ -250,8 +250,11 @@ typedef struct {
     int32_t coarse;      /* coarsest-level RBGS sweeps */
     int32_t restart;     /* FGMRES restart length (at most 40) */
     int32_t policy;      /* 0 flexible CG (sflip.jl:1867), 1 FGMRES (fgmres.jl:26), 2 CG then FGMRES (fgmres.jl:96) */
-    int32_t precision;   /* 0: FP32 hierarchy */
+    int32_t precision;   /* V-cycle storage: 0 FP32 (the reference's), 1 FP16, 2 BF16 (FP32 arithmetic, FP64 coarsest) */
     float boost;         /* coarse-correction scale */
+    int32_t refine;      /* 0: the reference's FP32 solve; k > 0: iterative refinement, the solution
+                          * kept in FP64 and its residual recomputed in FP64 every k iterations and
+                          * before convergence is accepted (no counterpart in the reference) */
 } mael_sparse_opts;
 typedef struct {
     int32_t iterations, fcg_iterations, fgmres_iterations, used_fgmres;
@@ -259,7 +262,7 @@ typedef struct {
     double ms;
 } mael_sparse_stats;
 /* The defaults of `pressure!(::SparseBasin{B,<:SurfaceStir})` (fgmres.jl:96): tol 5e-4,
- * maxiter 240, sweeps 1, coarse 40, restart 8, policy 2, boost 1.5. */
+ * maxiter 240, sweeps 1, coarse 40, restart 8, policy 2, boost 1.5; precision 0, refine 0. */
 void mael_sparse_defaults(mael_sparse_opts *o);
 /* `project!(s, dt)` (sflip.jl:1998-2015) after the caller's γ: couplings, rhs, clear,
  * the solve by `opts->policy`, the faces and flags. Errors carry the reference's texts. */

@@ -231,7 +231,7 @@ int mael_sparse_g2p(mael_sparse* s, float dt, float gravity) {
 void mael_sparse_defaults(mael_sparse_opts* o) {
     if (!o) return;
     o->tol = 5e-4; o->maxiter = 240; o->sweeps = 1; o->coarse = 40; o->restart = 8; o->policy = 2; o->precision = 0;
-    o->boost = 1.5f;
+    o->boost = 1.5f; o->refine = 0;
 }
 int mael_sparse_project(mael_sparse* s, float gamma, const mael_sparse_opts* opts, mael_sparse_stats* stats) {
     if (!s) return MAEL_E_ARG;
@@ -240,9 +240,10 @@ int mael_sparse_project(mael_sparse* s, float gamma, const mael_sparse_opts* opt
         if (opts) {
             o.tol = opts->tol; o.maxiter = opts->maxiter; o.sweeps = opts->sweeps; o.coarse = opts->coarse;
             o.restart = opts->restart; o.policy = opts->policy; o.precision = opts->precision; o.boost = opts->boost;
+            o.refine = opts->refine;
         }
         if (o.sweeps < 1 || o.coarse < 1 || o.maxiter < 0 || !(o.tol >= 0.0) || o.restart < 1 || o.restart > 40 ||
-            o.policy < 0 || o.policy > 2 || o.precision < 0 || o.precision > 2)
+            o.policy < 0 || o.policy > 2 || o.precision < 0 || o.precision > 2 || o.refine < 0)
             throw mael::Error(mael::E_ARG, "bad solver options");
         mael::SparseStats st;
         s->b->project(gamma, o, st);

@@ -12,6 +12,7 @@ using bf16 = __hip_bfloat16;
 __device__ inline float lf(float x) { return x; }
 __device__ inline float lf(__half x) { return __half2float(x); }
 __device__ inline float lf(bf16 x) { return __bfloat162float(x); }
+__device__ inline double lf(double x) { return x; }
 
 template <class T> __device__ inline T sf(float x);
 template <> __device__ inline float sf<float>(float x) { return x; }

@@ -286,6 +286,7 @@ struct SparseOpts {          // the sparse solve's options (sflip.jl:1867, fgmre
     float boost = 1.5f;
     int policy = 2;          // 0 flexible CG, 1 FGMRES, 2 CG (at most 80) then FGMRES
     int precision = 0;       // 0 FP32 hierarchy
+    int refine = 0;          // > 0: FP64 solution, FP64 residual every `refine` iterations
 };
 struct SparseStats {
     int iterations = 0, fcg_iterations = 0, fgmres_iterations = 0, used_fgmres = 0;
@@ -342,6 +343,10 @@ struct SparseBasin {
     std::vector<SField<float>> V, Z;          // FGMRES bases, allocated on first use
     MPStore mp;
     void mp_alloc();
+    SField<double> Pd;                        // the FP64 pressure of iterative refinement, on first use
+    void ir_alloc() {
+        if (!Pd.fine.p) Pd.alloc((size_t)blocks * SBCELLS, ncells());
+    }
     DBuf<double> partials, scal;
 
     // particles

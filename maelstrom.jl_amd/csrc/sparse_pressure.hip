@@ -279,39 +279,40 @@ __global__ void k_cellsdense(float* __restrict__ dDc, uint8_t* __restrict__ actc
 template <class C> struct FineCoefT { const C* c[3]; };
 template <class C> struct CoarseCoefT { const C* c[3]; };
 
-// A level's fine cell (N = 16 or 8): diag and the neighbours' sum, A x = diag x - off.
-template <int N, class T, class C>
+// A level's fine cell (N = 16 or 8): diag and the neighbours' sum, A x = diag x - off,
+// summed in A (FP32, or FP64 for iterative refinement's residual).
+template <int N, class T, class C, class A = float>
 __device__ inline void finerow(const STopo& t, const T* __restrict__ x, const T* __restrict__ xc, const FineCoefT<C>& cf,
-                               const C* __restrict__ dD, int s, int i, int j, int k, int4 w, float& diag, float& off) {
+                               const C* __restrict__ dD, int s, int i, int j, int k, int4 w, A& diag, A& off) {
     constexpr int sh = N == 16 ? 2 : 1, per = (N + 1) * N * N;
     int I = w.x * N + i, J = w.y * N + j, K = w.z * N + k;
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
-    diag = lf(dD[(s - 1) * (N * N * N) + i + N * (j + N * k)]);
-    off = 0.0f;
-    auto nv = [&](int In, int Jn, int Kn) -> float {
+    diag = (A)lf(dD[(s - 1) * (N * N * N) + i + N * (j + N * k)]);
+    off = (A)0;
+    auto nv = [&](int In, int Jn, int Kn) -> A {
         int32_t idx = cellat<N>(t, In, Jn, Kn);
-        if (idx >= 0) return lf(x[idx]);
-        return lf(xc[cl(min(max(In >> sh, 0), n0 - 1), min(max(Jn >> sh, 0), n1 - 1), min(max(Kn >> sh, 0), n2 - 1), n0, n1)]);
+        if (idx >= 0) return (A)lf(x[idx]);
+        return (A)lf(xc[cl(min(max(In >> sh, 0), n0 - 1), min(max(Jn >> sh, 0), n1 - 1), min(max(Kn >> sh, 0), n2 - 1), n0, n1)]);
     };
     size_t base = (size_t)(s - 1) * per;
-    float c;
-    c = lf(cf.c[0][base + ffn(N, 0, i, j, k)]);     if (c > 0.0f) { diag += c; off += c * nv(I - 1, J, K); }
-    c = lf(cf.c[0][base + ffn(N, 0, i + 1, j, k)]); if (c > 0.0f) { diag += c; off += c * nv(I + 1, J, K); }
-    c = lf(cf.c[1][base + ffn(N, 1, i, j, k)]);     if (c > 0.0f) { diag += c; off += c * nv(I, J - 1, K); }
-    c = lf(cf.c[1][base + ffn(N, 1, i, j + 1, k)]); if (c > 0.0f) { diag += c; off += c * nv(I, J + 1, K); }
-    c = lf(cf.c[2][base + ffn(N, 2, i, j, k)]);     if (c > 0.0f) { diag += c; off += c * nv(I, J, K - 1); }
-    c = lf(cf.c[2][base + ffn(N, 2, i, j, k + 1)]); if (c > 0.0f) { diag += c; off += c * nv(I, J, K + 1); }
+    A c;
+    c = (A)lf(cf.c[0][base + ffn(N, 0, i, j, k)]);     if (c > (A)0) { diag += c; off += c * nv(I - 1, J, K); }
+    c = (A)lf(cf.c[0][base + ffn(N, 0, i + 1, j, k)]); if (c > (A)0) { diag += c; off += c * nv(I + 1, J, K); }
+    c = (A)lf(cf.c[1][base + ffn(N, 1, i, j, k)]);     if (c > (A)0) { diag += c; off += c * nv(I, J - 1, K); }
+    c = (A)lf(cf.c[1][base + ffn(N, 1, i, j + 1, k)]); if (c > (A)0) { diag += c; off += c * nv(I, J + 1, K); }
+    c = (A)lf(cf.c[2][base + ffn(N, 2, i, j, k)]);     if (c > (A)0) { diag += c; off += c * nv(I, J, K - 1); }
+    c = (A)lf(cf.c[2][base + ffn(N, 2, i, j, k + 1)]); if (c > (A)0) { diag += c; off += c * nv(I, J, K + 1); }
 }
 
 // An uncovered coarse cell: coarse faces, or a covered neighbour's (N/4)^2 sub-faces.
-template <int N, class T, class C>
+template <int N, class T, class C, class A = float>
 __device__ inline void coarserow(const STopo& t, const T* __restrict__ xf, const FineCoefT<C>& cf, const T* __restrict__ xc,
-                                 const CoarseCoefT<C>& ccf, const C* __restrict__ dDc, int I, int J, int K, float& diag,
-                                 float& off) {
+                                 const CoarseCoefT<C>& ccf, const C* __restrict__ dDc, int I, int J, int K, A& diag,
+                                 A& off) {
     constexpr int R = N / 4;
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
-    diag = lf(dDc[cl(I, J, K, n0, n1)]);
-    off = 0.0f;
+    diag = (A)lf(dDc[cl(I, J, K, n0, n1)]);
+    off = (A)0;
     for (int a = 0; a < 3; ++a) {
         int f0 = n0 + (a == 0), f1 = n1 + (a == 1);
         for (int sg = -1; sg <= 1; sg += 2) {
@@ -328,14 +329,14 @@ __device__ inline void coarserow(const STopo& t, const T* __restrict__ xf, const
                         int PK = (a == 2 && sg < 0) ? FK + 1 : FK;
                         int32_t fidx = faceat<N>(t, a, PI, PJ, PK), cidx = cellat<N>(t, FI, FJ, FK);
                         if (fidx >= 0 && cidx >= 0) {
-                            float c = lf(cf.c[a][fidx]);
+                            A c = (A)lf(cf.c[a][fidx]);
                             diag += c;
-                            off += c * lf(xf[cidx]);
+                            off += c * (A)lf(xf[cidx]);
                         }
                     }
             } else {
-                float c = lf(sg < 0 ? ccf.c[a][cl(I, J, K, f0, f1)] : ccf.c[a][cl(I + (a == 0), J + (a == 1), K + (a == 2), f0, f1)]);
-                if (c > 0.0f) { diag += c; off += c * lf(xc[cl(In, Jn, Kn, n0, n1)]); }
+                A c = (A)lf(sg < 0 ? ccf.c[a][cl(I, J, K, f0, f1)] : ccf.c[a][cl(I + (a == 0), J + (a == 1), K + (a == 2), f0, f1)]);
+                if (c > (A)0) { diag += c; off += c * (A)lf(xc[cl(In, Jn, Kn, n0, n1)]); }
             }
         }
     }
@@ -762,10 +763,66 @@ __global__ void k_saxpy2(float* __restrict__ yf, const float* __restrict__ xf, s
     }
 }
 
+// ─── iterative refinement: the solution in FP64, FP32 corrections ────────────
+
+// r = b - A x on the unknowns (0 elsewhere), x in FP64 and each row summed in FP64, then
+// rounded to FP32: the residual the FP32 (or FP16) iterations correct.
+__global__ __launch_bounds__(256) void k_resid64fine(float* __restrict__ r, const double* __restrict__ x,
+                                                     const double* __restrict__ xc, const float* __restrict__ b,
+                                                     CompT<float> L, STopo t, int32_t nf) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nf) return;
+    int s = (n >> 12) + 1, c = n & 4095;
+    int4 w = t.where[s - 1];
+    if (w.x < 0) return;
+    double res = 0.0;
+    if (L.actf[n] == LIQUID) {
+        double diag, off;
+        finerow<16, double, float, double>(t, x, xc, L.cf, L.dDf, s, c & 15, (c >> 4) & 15, c >> 8, w, diag, off);
+        res = (double)b[n] - (diag * x[n] - off);
+    }
+    r[n] = (float)res;
+}
+__global__ __launch_bounds__(256) void k_resid64coarse(float* __restrict__ r, const double* __restrict__ x,
+                                                       const double* __restrict__ xf, const float* __restrict__ b,
+                                                       CompT<float> L, STopo t) {
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    double res = 0.0;
+    if (L.actc[l] != 0 && !covered(t, I, J, K)) {
+        double diag, off;
+        coarserow<16, double, float, double>(t, xf, L.cf, x, L.cc, L.dDc, I, J, K, diag, off);
+        res = (double)b[l] - (diag * x[l] - off);
+    }
+    r[l] = (float)res;
+}
+// The bookkeeping over the fine and the coarse range: mode 0 x64 = x32 and x32 = 0 (the
+// start), 1 x64 += x32 and x32 = 0 (a correction folded in), 2 x32 = x64 rounded (the end).
+__global__ void k_refine(double* __restrict__ x64f, double* __restrict__ x64c, float* __restrict__ x32f,
+                         float* __restrict__ x32c, size_t nf, size_t nc, int mode) {
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < nf + nc; l += (size_t)gridDim.x * blockDim.x) {
+        double* d = l < nf ? x64f + l : x64c + (l - nf);
+        float* x = l < nf ? x32f + l : x32c + (l - nf);
+        if (mode == 0) { *d = (double)*x; *x = 0.0f; }
+        else if (mode == 1) { *d += (double)*x; *x = 0.0f; }
+        else *x = (float)*d;
+    }
+}
+
 // ─── projection (sflip.jl:1906-1962) ─────────────────────────────────────────
 
-__global__ void k_projectfine(float* __restrict__ u, uint8_t* __restrict__ valid, const float* __restrict__ P,
-                              const float* __restrict__ Pc, const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc,
+// The pressure difference over a face, in FP32 from FP32 pressures (the reference's), or
+// taken in FP64 from the refined FP64 pressure and then rounded.
+__device__ inline float pdiff(float a, float b) { return a - b; }
+__device__ inline float pdiff(double a, double b) { return (float)(a - b); }
+__device__ inline float pdiv(float a, float b) { return __fdiv_rn(a, b); }
+__device__ inline float pdiv(double a, float b) { return (float)(a / (double)b); }
+
+template <class PT>
+__global__ void k_projectfine(float* __restrict__ u, uint8_t* __restrict__ valid, const PT* __restrict__ P,
+                              const PT* __restrict__ Pc, const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc,
                               const float* __restrict__ phi, const float* __restrict__ phic, STopo t, int a, int32_t nfa) {
     int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nfa) return;
@@ -778,16 +835,17 @@ __global__ void k_projectfine(float* __restrict__ u, uint8_t* __restrict__ valid
     bool ok = tp != SOLID && tm != SOLID && (tp == LIQUID || tm == LIQUID);
     if (ok) {
         float d = (fp && fm ? 1.0f : 2.5f) * t.g.dx;
-        float Pp = fineorcoarse(t, P, Pc, I, J, K), Pm = fineorcoarse(t, P, Pc, Im, Jm, Km);
+        PT Pp = fineorcoarse(t, P, Pc, I, J, K), Pm = fineorcoarse(t, P, Pc, Im, Jm, Km);
         float Dl;
-        if (tp == LIQUID && tm == LIQUID) Dl = Pp - Pm;
-        else if (tm == LIQUID) Dl = __fdiv_rn(-Pm, theta(fineorcoarse(t, phi, phic, Im, Jm, Km), fineorcoarse(t, phi, phic, I, J, K)));
-        else Dl = __fdiv_rn(Pp, theta(fineorcoarse(t, phi, phic, I, J, K), fineorcoarse(t, phi, phic, Im, Jm, Km)));
+        if (tp == LIQUID && tm == LIQUID) Dl = pdiff(Pp, Pm);
+        else if (tm == LIQUID) Dl = pdiv(-Pm, theta(fineorcoarse(t, phi, phic, Im, Jm, Km), fineorcoarse(t, phi, phic, I, J, K)));
+        else Dl = pdiv(Pp, theta(fineorcoarse(t, phi, phic, I, J, K), fineorcoarse(t, phi, phic, Im, Jm, Km)));
         u[n] -= __fdiv_rn(Dl, d);
     }
     valid[n] = ok ? 1 : 0;
 }
-__global__ void k_projectcoarse(float* __restrict__ u, uint8_t* __restrict__ valid, const float* __restrict__ Pc,
+template <class PT>
+__global__ void k_projectcoarse(float* __restrict__ u, uint8_t* __restrict__ valid, const PT* __restrict__ Pc,
                                 const uint8_t* __restrict__ tc, const float* __restrict__ phic, STopo t, int a) {
     int nc0 = t.g.nc(0), nc1 = t.g.nc(1), nc2 = t.g.nc(2);
     int n0 = nc0 + (a == 0), n1 = nc1 + (a == 1), n2 = nc2 + (a == 2);
@@ -801,9 +859,10 @@ __global__ void k_projectcoarse(float* __restrict__ u, uint8_t* __restrict__ val
         uint8_t tp = tc[cp], tm = tc[cm];
         bool ok = tp != SOLID && tm != SOLID && (tp == LIQUID || tm == LIQUID);
         if (ok) {
-            float Pp = Pc[cp], Pm = Pc[cm], fp = phic[cp], fm = phic[cm];
-            float Dl = (tp == LIQUID && tm == LIQUID) ? Pp - Pm
-                                                      : (tm == LIQUID ? __fdiv_rn(-Pm, theta(fm, fp)) : __fdiv_rn(Pp, theta(fp, fm)));
+            PT Pp = Pc[cp], Pm = Pc[cm];
+            float fp = phic[cp], fm = phic[cm];
+            float Dl = (tp == LIQUID && tm == LIQUID) ? pdiff(Pp, Pm)
+                                                      : (tm == LIQUID ? pdiv(-Pm, theta(fm, fp)) : pdiv(Pp, theta(fp, fm)));
             u[l] -= __fdiv_rn(Dl, t.g.cdx());
         }
         valid[l] = ok ? 1 : 0;
@@ -1093,22 +1152,66 @@ struct SparseSolver {
         if (f & EF_BETA_NONFINITE) throw Error(E_NONFINITE, "Nonfinite pressure direction");
     }
 
+    // Iterative refinement (refine > 0): the solution lives in s.Pd (FP64) and s.P holds
+    // the FP32 correction gathered since the last fold.
+    int refine = 0;
+    void ir(int mode) {
+        unsigned g = std::min<unsigned>(cdiv((size_t)nf + Cc, 256), 8192);
+        hipLaunchKernelGGL(k_refine, dim3(g), dim3(256), 0, st, s.Pd.fine.p, s.Pd.coarse.p, s.P.fine.p, s.P.coarse.p,
+                           (size_t)nf, Cc, mode);
+    }
+    // Fold the correction in, r = b - A Pd in FP64 rows (stored FP32); returns ||r||.
+    double resid64(SVec r) {
+        ir(1);
+        Hier<float, float> H = hier32();
+        if (nf)
+            hipLaunchKernelGGL(k_resid64fine, dim3(cdiv(nf, 256)), dim3(256), 0, st, r.f, s.Pd.fine.p, s.Pd.coarse.p,
+                               s.b.fine.p, H.L0, t, nf);
+        hipLaunchKernelGGL(k_resid64coarse, dim3(cdiv(Cc, 256)), dim3(256), 0, st, r.c, s.Pd.coarse.p, s.Pd.fine.p,
+                           s.b.coarse.p, H.L0, t);
+        dot(r, r, SF_NORM, SS_RNORM);
+        return read(SS_RNORM);
+    }
+    // ||b - A P|| of the current solution (into r, with q as scratch), in FP64 sums.
+    double true_residual(SVec r, SVec q) {
+        if (refine > 0) return resid64(r);
+        apply(q, vec(s.P));
+        copy(r, vec(s.b));
+        axpy(r, q, -1, -1.0f);
+        dot(r, r, SF_NORM, SS_RNORM);
+        return read(SS_RNORM);
+    }
+    void zero_solution() {
+        zero(vec(s.P), (size_t)nf);
+        if (refine > 0) {
+            fill_async(s.Pd.fine.p, (size_t)nf, 0.0, st);
+            fill_async(s.Pd.coarse.p, Cc, 0.0, st);
+        }
+    }
+
     // sflip.jl:1867-1904 pressure!: flexible CG from the pressure in s.P. Returns the
-    // iterations; throws the reference's errors; `converged` false at maxiter.
+    // iterations; throws the reference's errors; `converged` false at maxiter. With
+    // refinement the residual is recomputed from the FP64 solution every `refine`
+    // iterations and before convergence is accepted.
     int fcg(double tol, int maxiter, int sweeps, int coarse, float boost, bool& converged, double& relres) {
         SVec P = vec(s.P), b = vec(s.b), r = vec(s.r), z = vec(s.z), d = vec(s.d), q = vec(s.q), rold = vec(s.rold);
         converged = false;
         set(SS_FLAG, 0.0);
-        apply(r, P);
-        xpby(r, b, -1, -1.0f);
         dot(b, b, SF_NORM, SS_BB);
         double bnorm = read(SS_BB);
         relres = 0;
         if (!std::isfinite(bnorm)) throw Error(E_NONFINITE, "Nonfinite pressure right-hand side");
         if (bnorm == 0) { converged = true; return 0; }
         double target = (double)((float)tol * (float)bnorm);
-        dot(r, r, SF_NORM, SS_RNORM);
-        double rnorm = read(SS_RNORM);
+        double rnorm;
+        if (refine > 0) {
+            rnorm = resid64(r);
+        } else {
+            apply(r, P);
+            xpby(r, b, -1, -1.0f);
+            dot(r, r, SF_NORM, SS_RNORM);
+            rnorm = read(SS_RNORM);
+        }
         relres = rnorm / bnorm;
         if (rnorm <= target) { converged = true; return 0; }
         vcycle(z, r, sweeps, coarse, boost, SS_RNORM);
@@ -1124,6 +1227,7 @@ struct SparseSolver {
             double flags;
             read2(SS_RNORM, rnorm, flags);
             raise_flags((long long)flags);
+            if (refine > 0 && (rnorm <= target || it % refine == 0)) rnorm = resid64(r);
             relres = rnorm / bnorm;
             if (rnorm <= target) { converged = true; return it; }
             vcycle(z, r, sweeps, coarse, boost, SS_RNORM);
@@ -1159,15 +1263,9 @@ struct SparseSolver {
         dot(b, b, SF_NORM, SS_BB);
         double bnorm = read(SS_BB);
         if (!std::isfinite(bnorm)) throw Error(E_NONFINITE, "Nonfinite pressure RHS");
-        if (bnorm == 0) { zero(P, (size_t)nf); relres = 0; return 0; }
+        if (bnorm == 0) { zero_solution(); relres = 0; return 0; }
         double target = (double)((float)tol * (float)bnorm);
-        auto actual = [&]() {
-            apply(q, P);
-            copy(r, b);
-            axpy(r, q, -1, -1.0f);
-            dot(r, r, SF_NORM, SS_RNORM);
-            return read(SS_RNORM);
-        };
+        auto actual = [&]() { return true_residual(r, q); };
         double residual = actual();
         int iterations = 0;
         while (residual > target && iterations < maxiter) {
@@ -1255,6 +1353,11 @@ void SparseBasin::project(float gamma, const SparseOpts& o, SparseStats& stats) 
     MAEL_LAUNCH_CHECK();
     S.hierarchy();
     S.precision = o.precision;
+    S.refine = o.refine;
+    if (o.refine > 0) {   // the FP64 solution starts from the stored pressure, the correction from 0
+        ir_alloc();
+        S.ir(0);
+    }
     if (o.precision != 0) {
         mp_alloc();
         if (o.precision == 1) S.cast_couplings<__half>();
@@ -1275,37 +1378,45 @@ void SparseBasin::project(float gamma, const SparseOpts& o, SparseStats& stats) 
             std::string m = e.what();
             if (m.rfind("Nonfinite pressure", 0) != 0 && m.rfind("Pressure CG ", 0) != 0) throw;
         }
-        SVec Pv = S.vec(P), bv = S.vec(b), rv = S.vec(r), qv = S.vec(q);
-        S.apply(qv, Pv);
-        S.copy(rv, bv);
-        S.axpy(rv, qv, -1, -1.0f);
+        SVec bv = S.vec(b), rv = S.vec(r), qv = S.vec(q);
+        double rn = S.true_residual(rv, qv);
         S.dot(bv, bv, SF_NORM, SS_BB);
-        S.dot(rv, rv, SF_NORM, SS_RNORM);
-        double bn = S.read(SS_BB), rn = S.read(SS_RNORM);
+        double bn = S.read(SS_BB);
         if (!(rn <= (double)((float)o.tol * (float)bn))) {
-            if (!std::isfinite(rn) || rn > 2.0 * bn) S.zero(Pv, (size_t)S.nf);
+            if (!std::isfinite(rn) || rn > 2.0 * bn) S.zero_solution();
             stats.used_fgmres = 1;
             stats.fgmres_iterations = S.fgmres(o.tol, o.maxiter, o.restart, o.sweeps, o.coarse, o.boost, rr);
         }
     }
     stats.iterations = stats.fcg_iterations + stats.fgmres_iterations;
     {   // the true relative residual of the returned pressure, in FP64 sums
-        SVec Pv = S.vec(P), bv = S.vec(b), rv = S.vec(r), qv = S.vec(q);
-        S.apply(qv, Pv);
-        S.copy(rv, bv);
-        S.axpy(rv, qv, -1, -1.0f);
+        SVec bv = S.vec(b), rv = S.vec(r), qv = S.vec(q);
+        double rn = S.true_residual(rv, qv);
         S.dot(bv, bv, SF_NORM, SS_BB);
-        S.dot(rv, rv, SF_NORM, SS_RNORM);
-        double bn = S.read(SS_BB), rn = S.read(SS_RNORM);
+        double bn = S.read(SS_BB);
         stats.rel_residual = bn > 0 ? rn / bn : 0.0;
     }
+    // with refinement the stored pressure is the FP64 one rounded, and the faces take
+    // their differences from the FP64 one
+    if (S.refine > 0) S.ir(2);
     // sflip.jl:2008-2013 the faces, then the coarse faces under fine blocks from theirs
     for (int a = 0; a < 3; ++a) {
-        if (S.nf)
-            hipLaunchKernelGGL(k_projectfine, dim3(cdiv(finefaces(), 256)), dim3(256), 0, s, u[a].fine.p, fl[a].fine.p, P.fine.p,
-                               P.coarse.p, types.fine.p, types.coarse.p, phi.fine.p, phi.coarse.p, t, a, (int32_t)finefaces());
-        hipLaunchKernelGGL(k_projectcoarse, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, s, u[a].coarse.p, fl[a].coarse.p, P.coarse.p,
-                           types.coarse.p, phi.coarse.p, t, a);
+        if (S.nf) {
+            if (S.refine > 0)
+                hipLaunchKernelGGL(k_projectfine<double>, dim3(cdiv(finefaces(), 256)), dim3(256), 0, s, u[a].fine.p,
+                                   fl[a].fine.p, Pd.fine.p, Pd.coarse.p, types.fine.p, types.coarse.p, phi.fine.p, phi.coarse.p,
+                                   t, a, (int32_t)finefaces());
+            else
+                hipLaunchKernelGGL(k_projectfine<float>, dim3(cdiv(finefaces(), 256)), dim3(256), 0, s, u[a].fine.p,
+                                   fl[a].fine.p, P.fine.p, P.coarse.p, types.fine.p, types.coarse.p, phi.fine.p, phi.coarse.p,
+                                   t, a, (int32_t)finefaces());
+        }
+        if (S.refine > 0)
+            hipLaunchKernelGGL(k_projectcoarse<double>, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, s, u[a].coarse.p,
+                               fl[a].coarse.p, Pd.coarse.p, types.coarse.p, phi.coarse.p, t, a);
+        else
+            hipLaunchKernelGGL(k_projectcoarse<float>, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, s, u[a].coarse.p,
+                               fl[a].coarse.p, P.coarse.p, types.coarse.p, phi.coarse.p, t, a);
         launch_restrictfaces_f(*this, u[a].coarse.p, u[a].fine.p, a);
         launch_restrictfaces_u8(*this, fl[a].coarse.p, fl[a].fine.p, a);
     }

@@ -59,7 +59,8 @@ class SparseGrid(C.Structure):
 
 class SparseOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int32), ("sweeps", C.c_int32), ("coarse", C.c_int32),
-                ("restart", C.c_int32), ("policy", C.c_int32), ("precision", C.c_int32), ("boost", C.c_float)]
+                ("restart", C.c_int32), ("policy", C.c_int32), ("precision", C.c_int32), ("boost", C.c_float),
+                ("refine", C.c_int32)]
 
 
 class SparseStats(C.Structure):

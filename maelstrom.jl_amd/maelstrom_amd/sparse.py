@@ -212,12 +212,18 @@ def grid2particles(s, dt):
     return s
 
 
-def solve_options(tol=5e-4, maxiter=240, sweeps=1, coarse=40, restart=8, policy=2, precision=0, boost=1.5):
+def solve_options(tol=5e-4, maxiter=240, sweeps=1, coarse=40, restart=8, policy=2, precision=0, boost=1.5,
+                  refine=None):
     """`pressure!(::SparseBasin{B,<:SurfaceStir})`'s defaults (fgmres.jl:96); policy 0 is the
     plain flexible CG of sflip.jl:1867 (its defaults: tol 1e-4, maxiter 200, boost 1),
-    1 FGMRES alone (fgmres.jl:26)."""
+    1 FGMRES alone (fgmres.jl:26). `precision` 1 / 2 keeps the V-cycle in FP16 / BF16;
+    `refine` k > 0 keeps the solution in FP64 with its residual recomputed in FP64 every
+    k iterations (default: 10 with a reduced-precision V-cycle, else 0, the reference's
+    FP32 solve)."""
+    if refine is None:
+        refine = 10 if precision else 0
     return SparseOpts(float(tol), int(maxiter), int(sweeps), int(coarse), int(restart), int(policy), int(precision),
-                      float(boost))
+                      float(boost), int(refine))
 
 
 def volume(s):

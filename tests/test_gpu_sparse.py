@@ -249,15 +249,17 @@ def test_project_parity(M, policy):
         np.testing.assert_array_equal(s.get("F" + nm.upper() + "_C").ravel(), o.array("f" + nm + "_c"))
 
 
-@pytest.mark.parametrize("precision", [1, 2])
-def test_project_mixed_precision(M, precision):
-    """The FP16 / BF16 hierarchy (FP32 CG, FP64 sums and coarsest solve) reaches the same
-    1e-6 true residual and the same pressure as the FP64 oracle."""
-    print("case: sparse project precision %d" % precision, flush=True)
+@pytest.mark.parametrize("precision,refine", [(1, 0), (2, 0), (0, 10), (1, 10), (2, 10)])
+def test_project_mixed_precision(M, precision, refine):
+    """The FP16 / BF16 hierarchy (FP32 CG, FP64 sums and coarsest solve), with and without
+    FP64 iterative refinement, reaches the same 1e-6 true residual and the same pressure
+    as the FP64 oracle."""
+    print("case: sparse project precision %d refine %d" % (precision, refine), flush=True)
     o = _scene()
     _projection_inputs(o, seed=13)
     s = _upload(M, o)
-    st = M.sparse.project(s, 0.1, gamma=0.005, tol=1e-6, maxiter=400, policy=2, boost=1.5, precision=precision)
+    st = M.sparse.project(s, 0.1, gamma=0.005, tol=1e-6, maxiter=400, policy=2, boost=1.5, precision=precision,
+                          refine=refine)
     assert st.rel_residual <= 1e-6 * 1.01, st.rel_residual
     o.project(gamma=0.005, tol=1e-12, maxiter=600, policy=0)
     ns = o.nslots

@@ -40,13 +40,16 @@ WORKLOADS = {
 }
 
 
-def sparse_bytes(kernel, live_cells, n_particles):
-    """Compulsory bytes of one launch of `kernel` on the sparse basin, at the reference's
-    element sizes, over the live fine cells (the fine part of the level-0 kernels)."""
+def sparse_bytes(kernel, live_cells, n_particles, precision=0):
+    """Compulsory bytes of one launch of `kernel` on the sparse basin over the live fine
+    cells (the fine part of the level-0 kernels), at the element sizes the launch reads:
+    FP32 (the reference's), or 2 B for the reduced-precision V-cycle's vectors and
+    couplings."""
+    e = 4 if precision == 0 else 2
     return {
-        # x, b, diag (4 B each), three face couplings (each face shared by two cells), the
-        # type (1 B), and the result (4 B) per fine cell
-        "jacobi": 29 * live_cells,
+        # x, b, diag, three face couplings (each face shared by two cells), the result
+        # (e B each) and the type (1 B) per fine cell
+        "jacobi": (7 * e + 1) * live_cells,
         "g2p": 60 * n_particles,
         "p2g": 64 * n_particles,
         "scatter": 136 * n_particles,
@@ -159,11 +162,12 @@ def sparse_cpu_baseline(s, iters_per_step, dt, sample_particles=1_000_000):
     t2 = time.perf_counter()
     per_iter = (t2 - t1 - (t1 - t0)) / 2
     setup = max((t1 - t0) - per_iter, 0.0)
-    ms = 1e3 * ((t["p2g"] + t["g2p"] + t["resample"]) * scale + setup + per_iter * iters_per_step)
-    sample = ("oracle FP32 port, one core: P2G, G2P and resample on %d of the %d particles (scaled by %.1f); the "
-              "pressure set-up and 1 and 3 flexible-CG iterations on the whole grid, per-iteration cost scaled to the "
-              "GPU's %.1f iterations/step (%.1f s of CPU work)" % (m, n, scale, iters_per_step,
-                                                                    (t2 - t0) + t["p2g"] + t["g2p"] + t["resample"]))
+    halves = 2 if s.reflect else 1
+    ms = 1e3 * (halves * ((t["p2g"] + t["g2p"] + t["resample"]) * scale + setup) + per_iter * iters_per_step)
+    sample = ("oracle FP32 port, one core, the hot path only: P2G, G2P and resample on %d of the %d particles "
+              "(scaled by %.1f), the pressure set-up and 1 and 3 flexible-CG iterations on the whole grid, the "
+              "per-iteration cost scaled to the GPU's %.1f iterations a step, %d half-steps a step (%.1f s of CPU "
+              "work)" % (m, n, scale, iters_per_step, halves, (t2 - t0) + t["p2g"] + t["g2p"] + t["resample"]))
     return {"value": round(ms, 1), "unit": "ms/step", "cores": 1, "kind": "port", "sample": sample}
 
 
@@ -202,10 +206,11 @@ def sparse_main(args, desc, kw, dx):
     iters, fg, pressure_ms, nparts, live = [], [], [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        k0 = len(s.iterations)
         st = sp.step(s, dt, **solve)
-        iters.append(st.iterations)
+        iters.append(sum(s.iterations[k0:]))   # both half-steps' solves
         fg.append(st.used_fgmres)
-        pressure_ms.append(st.ms)
+        pressure_ms.append(sum(s.solve_ms[k0:]))
         nparts.append(s.n)
     s.sync()
     t1 = time.perf_counter()
@@ -216,7 +221,7 @@ def sparse_main(args, desc, kw, dx):
     ms_step = 1e3 * (t1 - t0) / args.steps
     avg_launch_ms = kms.value / max(kl.value, 1)
     n_avg = sum(nparts) / len(nparts)
-    abytes = sparse_bytes(kernel, live_slots * 4096, n_avg)
+    abytes = sparse_bytes(kernel, live_slots * 4096, n_avg, args.precision)
     achieved = abytes / (avg_launch_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": round(ms_step, 3), "unit": "ms/step", "n_gpus": 1, "steps": args.steps,

@@ -119,4 +119,72 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiled(PView ps, int64_t n, 
     }
 }
 
+// The particles' level-set splat (flip.jl:539-560) tiled the same way: a workgroup's 256
+// particles take the minimum of their order keys over a box of cells in LDS, then each
+// touched cell of the box takes one global atomic min. Minimum is order-independent, so
+// the result is the direct splat's bit for bit. CellMap: `index(I, J, K)` of a cell (< 0
+// off the grid or outside the fine blocks).
+constexpr int SPLAT_BOX = 2048;   // cells in LDS (8 KB)
+
+template <class CellMap>
+__global__ __launch_bounds__(P2G_THREADS) void k_splat_tiled(PView ps, int64_t n, CellMap map, float lo0, float lo1, float lo2,
+                                                            float dx, float dt, int32_t* __restrict__ phip) {
+    __shared__ int32_t box[SPLAT_BOX];
+    __shared__ int red[P2G_THREADS / 64];
+    int64_t p = (int64_t)blockIdx.x * P2G_THREADS + threadIdx.x;
+    bool live = p < n;
+    float xp = 0, yp = 0, zp = 0;
+    int ci = 0, cj = 0, ck = 0;
+    bool ok = false;
+    if (live) {
+        float vx = ps.f[3][p], vy = ps.f[4][p], vz = ps.f[5][p];
+        float sp = sqrtf(vx * vx + vy * vy + vz * vz);
+        float s = ps.f[15][p] + ahead(strength(sp, dt, dx)) * dt;
+        xp = ps.f[0][p] + s * vx; yp = ps.f[1][p] + s * vy; zp = ps.f[2][p] + s * vz;
+        float f0 = floorf(__fdiv_rn(xp - lo0, dx)), f1 = floorf(__fdiv_rn(yp - lo1, dx)), f2 = floorf(__fdiv_rn(zp - lo2, dx));
+        ci = (int)f0; cj = (int)f1; ck = (int)f2;
+        // finite, in-range cells only shape the box; others take the direct path
+        ok = fabsf(f0) < 16777216.0f && fabsf(f1) < 16777216.0f && fabsf(f2) < 16777216.0f;
+    }
+    const int BIG = 1 << 30;
+    int x0 = wg_min(ok ? ci : BIG, red), y0 = wg_min(ok ? cj : BIG, red), z0 = wg_min(ok ? ck : BIG, red);
+    int x1 = wg_max(ok ? ci : -BIG, red), y1 = wg_max(ok ? cj : -BIG, red), z1 = wg_max(ok ? ck : -BIG, red);
+    int ex = x1 - x0 + 3, ey = y1 - y0 + 3, ez = z1 - z0 + 3;
+    bool boxed = x0 != BIG && ex <= SPLAT_BOX && ey <= SPLAT_BOX && ez <= SPLAT_BOX && ex * ey * ez <= SPLAT_BOX;
+    x0 -= 1; y0 -= 1; z0 -= 1;
+    if (boxed) {
+        int V = ex * ey * ez;
+        for (int k = threadIdx.x; k < V; k += P2G_THREADS) box[k] = 0x7fffffff;
+        __syncthreads();
+    }
+    if (live) {
+        float radius = 0.5f * dx;
+        for (int dk = -1; dk <= 1; ++dk)
+            for (int dj = -1; dj <= 1; ++dj)
+                for (int di = -1; di <= 1; ++di) {
+                    int I = ci + di, J = cj + dj, K = ck + dk;
+                    float ex_ = lo0 + ((float)I + 0.5f) * dx - xp;
+                    float ey_ = lo1 + ((float)J + 0.5f) * dx - yp;
+                    float ez_ = lo2 + ((float)K + 0.5f) * dx - zp;
+                    float d = sqrtf(ex_ * ex_ + ey_ * ey_ + ez_ * ez_) - radius;
+                    if (boxed && ok) {
+                        atomicMin(&box[(I - x0) + ex * ((J - y0) + ey * (K - z0))], orderkey(d));
+                    } else {
+                        auto idx = map.index(I, J, K);
+                        if (idx >= 0) atomicMin(&phip[idx], orderkey(d));
+                    }
+                }
+    }
+    if (boxed) {
+        __syncthreads();
+        int V = ex * ey * ez;
+        for (int k = threadIdx.x; k < V; k += P2G_THREADS) {
+            int32_t v = box[k];
+            if (v == 0x7fffffff) continue;
+            auto idx = map.index(x0 + k % ex, y0 + (k / ex) % ey, z0 + k / (ex * ey));
+            if (idx >= 0) atomicMin(&phip[idx], v);
+        }
+    }
+}
+
 }  // namespace mael

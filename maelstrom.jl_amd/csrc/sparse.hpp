@@ -338,6 +338,7 @@ struct SparseBasin {
     DBuf<float> cf[3], cc[3];                 // level-0 face couplings, fine pools and coarse faces
     DBuf<float> c1[3], dD1;                   // level 1 (8^3 a block)
     DBuf<uint8_t> act1;
+    DBuf<uint8_t> cmask;                      // coarse cells' cover masks (k_covermask), per solve
     SField<float> X1x, X1b, X1r, X1t;         // level-1 vectors (fine 512 a slot, coarse part)
     std::vector<SLevel> D;                    // D[0] = the coarse grid as level 2, then halvings
     std::vector<SField<float>> V, Z;          // FGMRES bases, allocated on first use

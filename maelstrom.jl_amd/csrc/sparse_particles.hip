@@ -169,31 +169,10 @@ struct SparseNodes {
     STopo t;
     __device__ int32_t index(int a, int I, int J, int K) const { return finefaceat(t, a, I, J, K); }
 };
-
-// flip.jl:539-560 over the topology.
-__global__ __launch_bounds__(256) void k_sparse_splat(PView ps, int64_t n, STopo t, float dt, int32_t* __restrict__ phip) {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const SGrid& g = t.g;
-    float vx = ps.f[3][p], vy = ps.f[4][p], vz = ps.f[5][p];
-    float sp = sqrtf(vx * vx + vy * vy + vz * vz);
-    float s = ps.f[15][p] + ahead(strength(sp, dt, g.dx)) * dt;
-    float xp = ps.f[0][p] + s * vx, yp = ps.f[1][p] + s * vy, zp = ps.f[2][p] + s * vz;
-    int ci = (int)floorf(__fdiv_rn(xp - g.lo[0], g.dx)), cj = (int)floorf(__fdiv_rn(yp - g.lo[1], g.dx)),
-        ck = (int)floorf(__fdiv_rn(zp - g.lo[2], g.dx));
-    float radius = 0.5f * g.dx;
-    for (int dk = -1; dk <= 1; ++dk)
-        for (int dj = -1; dj <= 1; ++dj)
-            for (int di = -1; di <= 1; ++di) {
-                int32_t idx = fineat(t, ci + di, cj + dj, ck + dk);
-                if (idx < 0) continue;
-                float c[3];
-                finecentre(g, ci + di, cj + dj, ck + dk, c);
-                float ex = c[0] - xp, ey = c[1] - yp, ez = c[2] - zp;
-                float d = sqrtf(ex * ex + ey * ey + ez * ez) - radius;
-                atomicMin(&phip[idx], orderkey(d));
-            }
-}
+struct FineCellMap {
+    STopo t;
+    __device__ int32_t index(int I, int J, int K) const { return fineat(t, I, J, K); }
+};
 
 // sflip.jl:620-640 syncplanes_kernel!, on the fixed-point accumulators.
 __global__ void k_syncplanes(unsigned long long* __restrict__ A, STopo t, int a, int nslots) {
@@ -260,7 +239,8 @@ void SparseBasin::p2g(float dt) {
             hipLaunchKernelGGL(k_syncplanes, dim3(cdiv(nslots * 256, 256)), dim3(256), 0, s, wt[a].p, t, a, nslots);
         }
         kt.begin(K_SPLAT, s);
-        hipLaunchKernelGGL(k_sparse_splat, dim3(cdiv(n, 256)), dim3(256), 0, s, ps, n, t, dt, phip.p);
+        hipLaunchKernelGGL(k_splat_tiled<FineCellMap>, dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s, ps, n,
+                           FineCellMap{t}, g.lo[0], g.lo[1], g.lo[2], g.dx, dt, phip.p);
         kt.end(K_SPLAT, s);
         MAEL_LAUNCH_CHECK();
     }
@@ -283,21 +263,117 @@ void SparseBasin::p2g(float dt) {
 
 // ─── grid to particles (sflip.jl:2141-2144; flip.jl:1226-1244 over the topology) ──
 
+// One component's gather at a particle: the flag sum (finegather_value) and the APIC
+// value and affine row (finegather, sflip.jl:193-215) in one pass over the 27 nodes, in
+// the reference's order; `fetch(I, J, K, present, u, flag)` reads a node.
+template <class Fetch>
+__device__ inline void gather_component(const SGrid& g, int a, float px, float py, float pz, Fetch fetch, float& flag,
+                                        float& vel, float& a0, float& a1, float& a2) {
+    float q0 = ispace(px, g.lo[0], g.dx, a == 0 ? 0.0f : 0.5f);
+    float q1 = ispace(py, g.lo[1], g.dx, a == 1 ? 0.0f : 0.5f);
+    float q2 = ispace(pz, g.lo[2], g.dx, a == 2 ? 0.0f : 0.5f);
+    BSpline bx = bspline(q0), by = bspline(q1), bz = bspline(q2);
+    flag = 0.0f; vel = 0.0f; a0 = 0.0f; a1 = 0.0f; a2 = 0.0f;
+#pragma unroll
+    for (int mz = 0; mz < 3; ++mz) {
+        float wz = pick(bz, mz), dz = (float)(bz.b + mz) - q2;
+#pragma unroll
+        for (int my = 0; my < 3; ++my) {
+            float wy = pick(by, my), dy = (float)(by.b + my) - q1;
+#pragma unroll
+            for (int mx = 0; mx < 3; ++mx) {
+                bool pres;
+                float uv, fv;
+                fetch(bx.b + mx, by.b + my, bz.b + mz, pres, uv, fv);
+                if (pres) {
+                    float wgt = pick(bx, mx) * wy * wz;
+                    flag += wgt * fv;
+                    float tt = wgt * uv;
+                    vel += tt;
+                    a0 += (tt * ((float)(bx.b + mx) - q0)) * g.dx;
+                    a1 += (tt * dy) * g.dx;
+                    a2 += (tt * dz) * g.dx;
+                }
+            }
+        }
+    }
+    float c = __fdiv_rn(4.0f, g.dx * g.dx);
+    a0 *= c; a1 *= c; a2 *= c;
+}
+
+// flip.jl:1226-1244 g2p_kernel! over the topology, tiled: a workgroup's 256 particles
+// (cell-sorted, so close together) gather each component from a box of faces staged in
+// LDS (the value, and the flag with a presence bit), loaded once per box node instead of
+// once per particle and node; a workgroup spread over more nodes than the box holds
+// reads the faces directly. Both paths sum the same terms in the same order.
+constexpr int G2P_BOX = 1024;
 __global__ __launch_bounds__(256) void k_sparse_g2p(PView ps, int64_t n, STopo t, const float* __restrict__ u,
                                                     const float* __restrict__ v, const float* __restrict__ w,
                                                     const uint8_t* __restrict__ fu, const uint8_t* __restrict__ fv,
                                                     const uint8_t* __restrict__ fw, float dt, float gravity) {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    float xp = ps.f[0][p], yp = ps.f[1][p], zp = ps.f[2][p];
-    bool k1 = finegather_value(t, fu, 0, xp, yp, zp) > 0.999f;
-    bool k2 = finegather_value(t, fv, 1, xp, yp, zp) > 0.999f;
-    bool k3 = finegather_value(t, fw, 2, xp, yp, zp) > 0.999f;
+    __shared__ float sval[G2P_BOX];
+    __shared__ uint32_t sflag[G2P_BOX];   // bit 8: the face exists; low byte: its flag
+    __shared__ int red[4];
+    const SGrid& g = t.g;
+    int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    bool live = p < n;
+    float xp = 0, yp = 0, zp = 0;
+    if (live) { xp = ps.f[0][p]; yp = ps.f[1][p]; zp = ps.f[2][p]; }
+    const float* U[3] = {u, v, w};
+    const uint8_t* F[3] = {fu, fv, fw};
+    float fl[3], val[3], c0[3], c1[3], c2[3];
+    for (int a = 0; a < 3; ++a) {
+        float q0 = ispace(xp, g.lo[0], g.dx, a == 0 ? 0.0f : 0.5f);
+        float q1 = ispace(yp, g.lo[1], g.dx, a == 1 ? 0.0f : 0.5f);
+        float q2 = ispace(zp, g.lo[2], g.dx, a == 2 ? 0.0f : 0.5f);
+        BSpline bx = bspline(q0), by = bspline(q1), bz = bspline(q2);
+        bool ok = live && bx.b > -(1 << 24) && bx.b < (1 << 24) && by.b > -(1 << 24) && by.b < (1 << 24) &&
+                  bz.b > -(1 << 24) && bz.b < (1 << 24) && isfinite(q0) && isfinite(q1) && isfinite(q2);
+        const int BIG = 1 << 30;
+        int x0 = wg_min(ok ? bx.b : BIG, red), y0 = wg_min(ok ? by.b : BIG, red), z0 = wg_min(ok ? bz.b : BIG, red);
+        int x1 = wg_max(ok ? bx.b : -BIG, red), y1 = wg_max(ok ? by.b : -BIG, red), z1 = wg_max(ok ? bz.b : -BIG, red);
+        int ex = x1 - x0 + 3, ey = y1 - y0 + 3, ez = z1 - z0 + 3;
+        bool boxed = x0 != BIG && ex <= G2P_BOX && ey <= G2P_BOX && ez <= G2P_BOX && ex * ey * ez <= G2P_BOX;
+        const float* Ua = U[a];
+        const uint8_t* Fa = F[a];
+        if (boxed) {
+            int V = ex * ey * ez;
+            for (int k = threadIdx.x; k < V; k += 256) {
+                int32_t idx = finefaceat(t, a, x0 + k % ex, y0 + (k / ex) % ey, z0 + k / (ex * ey));
+                sval[k] = idx >= 0 ? Ua[idx] : 0.0f;
+                sflag[k] = idx >= 0 ? (256u | (uint32_t)Fa[idx]) : 0u;
+            }
+            __syncthreads();
+        }
+        if (live) {
+            if (boxed && ok) {
+                auto fetch = [&](int I, int J, int K, bool& pres, float& uv, float& fv_) {
+                    int l = (I - x0) + ex * ((J - y0) + ey * (K - z0));
+                    uint32_t f = sflag[l];
+                    pres = (f & 256u) != 0;
+                    uv = sval[l];
+                    fv_ = (float)(f & 255u);
+                };
+                gather_component(g, a, xp, yp, zp, fetch, fl[a], val[a], c0[a], c1[a], c2[a]);
+            } else {
+                auto fetch = [&](int I, int J, int K, bool& pres, float& uv, float& fv_) {
+                    int32_t idx = finefaceat(t, a, I, J, K);
+                    pres = idx >= 0;
+                    uv = pres ? Ua[idx] : 0.0f;
+                    fv_ = pres ? (float)Fa[idx] : 0.0f;
+                };
+                gather_component(g, a, xp, yp, zp, fetch, fl[a], val[a], c0[a], c1[a], c2[a]);
+            }
+        }
+        if (boxed) __syncthreads();   // the box is reused by the next component
+    }
+    if (!live) return;
+    bool k1 = fl[0] > 0.999f, k2 = fl[1] > 0.999f, k3 = fl[2] > 0.999f;
     float rec[12];   // stored unconditionally (see k_dense_spawn)
     if (k1 && k2 && k3) {
-        finegather(t, u, 0, xp, yp, zp, rec[0], rec[3], rec[4], rec[5]);
-        finegather(t, v, 1, xp, yp, zp, rec[1], rec[6], rec[7], rec[8]);
-        finegather(t, w, 2, xp, yp, zp, rec[2], rec[9], rec[10], rec[11]);
+        rec[0] = val[0]; rec[3] = c0[0]; rec[4] = c1[0]; rec[5] = c2[0];
+        rec[1] = val[1]; rec[6] = c0[1]; rec[7] = c1[1]; rec[8] = c2[1];
+        rec[2] = val[2]; rec[9] = c0[2]; rec[10] = c1[2]; rec[11] = c2[2];
     } else {
         rec[0] = ps.f[3][p];
         rec[1] = ps.f[4][p];

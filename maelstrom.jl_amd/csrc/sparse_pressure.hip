@@ -279,65 +279,127 @@ __global__ void k_cellsdense(float* __restrict__ dDc, uint8_t* __restrict__ actc
 template <class C> struct FineCoefT { const C* c[3]; };
 template <class C> struct CoarseCoefT { const C* c[3]; };
 
-// A level's fine cell (N = 16 or 8): diag and the neighbours' sum, A x = diag x - off,
-// summed in A (FP32, or FP64 for iterative refinement's residual).
-template <int N, class T, class C, class A = float>
-__device__ inline void finerow(const STopo& t, const T* __restrict__ x, const T* __restrict__ xc, const FineCoefT<C>& cf,
-                               const C* __restrict__ dD, int s, int i, int j, int k, int4 w, A& diag, A& off) {
-    constexpr int sh = N == 16 ? 2 : 1, per = (N + 1) * N * N;
-    int I = w.x * N + i, J = w.y * N + j, K = w.z * N + k;
-    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
-    diag = (A)lf(dD[(s - 1) * (N * N * N) + i + N * (j + N * k)]);
-    off = (A)0;
-    auto nv = [&](int In, int Jn, int Kn) -> A {
-        int32_t idx = cellat<N>(t, In, Jn, Kn);
-        if (idx >= 0) return (A)lf(x[idx]);
-        return (A)lf(xc[cl(min(max(In >> sh, 0), n0 - 1), min(max(Jn >> sh, 0), n1 - 1), min(max(Kn >> sh, 0), n2 - 1), n0, n1)]);
-    };
-    size_t base = (size_t)(s - 1) * per;
-    A c;
-    c = (A)lf(cf.c[0][base + ffn(N, 0, i, j, k)]);     if (c > (A)0) { diag += c; off += c * nv(I - 1, J, K); }
-    c = (A)lf(cf.c[0][base + ffn(N, 0, i + 1, j, k)]); if (c > (A)0) { diag += c; off += c * nv(I + 1, J, K); }
-    c = (A)lf(cf.c[1][base + ffn(N, 1, i, j, k)]);     if (c > (A)0) { diag += c; off += c * nv(I, J - 1, K); }
-    c = (A)lf(cf.c[1][base + ffn(N, 1, i, j + 1, k)]); if (c > (A)0) { diag += c; off += c * nv(I, J + 1, K); }
-    c = (A)lf(cf.c[2][base + ffn(N, 2, i, j, k)]);     if (c > (A)0) { diag += c; off += c * nv(I, J, K - 1); }
-    c = (A)lf(cf.c[2][base + ffn(N, 2, i, j, k + 1)]); if (c > (A)0) { diag += c; off += c * nv(I, J, K + 1); }
+// The slot of block w's face neighbour d (-x, +x, -y, +y, -z, +z), 0 for none.
+__device__ inline int32_t nbslot(const STopo& t, int4 w, int d) {
+    int ax = d >> 1, sg = (d & 1) ? 1 : -1;
+    int bi = w.x + (ax == 0 ? sg : 0), bj = w.y + (ax == 1 ? sg : 0), bk = w.z + (ax == 2 ? sg : 0);
+    return inblocks(t.g, bi, bj, bk) ? t.slot[blockid(t.g, bi, bj, bk)] : 0;
 }
 
-// An uncovered coarse cell: coarse faces, or a covered neighbour's (N/4)^2 sub-faces.
-template <int N, class T, class C, class A = float>
-__device__ inline void coarserow(const STopo& t, const T* __restrict__ xf, const FineCoefT<C>& cf, const T* __restrict__ xc,
-                                 const CoarseCoefT<C>& ccf, const C* __restrict__ dDc, int I, int J, int K, A& diag,
-                                 A& off) {
-    constexpr int R = N / 4;
+// A level's fine cell (N = 16 or 8) of slot s: diag and the neighbours' sum, A x = diag x
+// - off, summed in A (FP32, or FP64 for iterative refinement's residual). Neighbours in
+// the block index directly; across a face they are the neighbour block's (slots in nb)
+// or, without one, the coarse cell's. Every index is computed and all thirteen loads are
+// issued before any value is used (each index is in bounds whatever the couplings); a
+// coupling that is not positive adds nothing, in the reference's order.
+template <int N, class T, class C, class A = float, bool Z = false>
+__device__ inline void finerow(const STopo& t, const int32_t* nb, const T* __restrict__ x, const T* __restrict__ xc,
+                               const FineCoefT<C>& cf, const C* __restrict__ dD, int s, int i, int j, int k, int4 w, A& diag,
+                               A& off) {
+    constexpr int sh = N == 16 ? 2 : 1, V = N * N * N;
+    constexpr uint32_t per = (N + 1) * N * N;
+    int I = w.x * N + i, J = w.y * N + j, K = w.z * N + k;
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
-    diag = (A)lf(dDc[cl(I, J, K, n0, n1)]);
+    uint32_t own = (uint32_t)(s - 1) * V + (uint32_t)(i + N * (j + N * k));
+    auto nbr = [&](bool inside, int delta, int d, int li, int lj, int lk, int In, int Jn, int Kn) -> A {
+        if constexpr (Z) return (A)0;   // x = 0: nothing to read
+        int32_t sn = nb[d];
+        uint32_t in_block = own + (uint32_t)delta;
+        uint32_t in_nb = (uint32_t)(sn - 1) * V + (uint32_t)(li + N * (lj + N * lk));
+        int ci = min(max(In >> sh, 0), n0 - 1), cj = min(max(Jn >> sh, 0), n1 - 1), ck = min(max(Kn >> sh, 0), n2 - 1);
+        uint32_t in_coarse = (uint32_t)ci + (uint32_t)n0 * ((uint32_t)cj + (uint32_t)n1 * (uint32_t)ck);
+        bool fine = inside || sn != 0;
+        uint32_t idx = inside ? in_block : (sn != 0 ? in_nb : in_coarse);
+        const T* p = fine ? x : xc;
+        return (A)lf(p[idx]);
+    };
+    uint32_t base = (uint32_t)(s - 1) * per;
+    A c0 = (A)lf(cf.c[0][base + ffn(N, 0, i, j, k)]), c1 = (A)lf(cf.c[0][base + ffn(N, 0, i + 1, j, k)]);
+    A c2 = (A)lf(cf.c[1][base + ffn(N, 1, i, j, k)]), c3 = (A)lf(cf.c[1][base + ffn(N, 1, i, j + 1, k)]);
+    A c4 = (A)lf(cf.c[2][base + ffn(N, 2, i, j, k)]), c5 = (A)lf(cf.c[2][base + ffn(N, 2, i, j, k + 1)]);
+    A x0 = nbr(i > 0, -1, 0, N - 1, j, k, I - 1, J, K), x1 = nbr(i < N - 1, 1, 1, 0, j, k, I + 1, J, K);
+    A x2 = nbr(j > 0, -N, 2, i, N - 1, k, I, J - 1, K), x3 = nbr(j < N - 1, N, 3, i, 0, k, I, J + 1, K);
+    A x4 = nbr(k > 0, -N * N, 4, i, j, N - 1, I, J, K - 1), x5 = nbr(k < N - 1, N * N, 5, i, j, 0, I, J, K + 1);
+    diag = (A)lf(dD[own]);
     off = (A)0;
-    for (int a = 0; a < 3; ++a) {
+    if (c0 > (A)0) { diag += c0; off += c0 * x0; }
+    if (c1 > (A)0) { diag += c1; off += c1 * x1; }
+    if (c2 > (A)0) { diag += c2; off += c2 * x2; }
+    if (c3 > (A)0) { diag += c3; off += c3 * x3; }
+    if (c4 > (A)0) { diag += c4; off += c4 * x4; }
+    if (c5 > (A)0) { diag += c5; off += c5 * x5; }
+}
+
+// Per coarse cell, which of it and its six neighbours lie in fine blocks: bit 0 the cell,
+// bit 1 + d neighbour d (-x, +x, -y, +y, -z, +z; 0 off the grid). Fixed for a solve.
+__global__ void k_covermask(uint8_t* __restrict__ m, STopo t) {
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    unsigned v = covered(t, I, J, K) ? 1u : 0u;
+    for (int d = 0; d < 6; ++d) {
+        int ax = d >> 1, sg = (d & 1) ? 1 : -1;
+        int In = I + (ax == 0 ? sg : 0), Jn = J + (ax == 1 ? sg : 0), Kn = K + (ax == 2 ? sg : 0);
+        bool in = In >= 0 && Jn >= 0 && Kn >= 0 && In < n0 && Jn < n1 && Kn < n2;
+        if (in && covered(t, In, Jn, Kn)) v |= 2u << d;
+    }
+    m[l] = (uint8_t)v;
+}
+
+// An uncovered coarse cell (cover mask msk): coarse faces, or towards a covered
+// neighbour its (N/4)^2 sub-faces. The uncovered neighbours' couplings and values are
+// loaded first (an index off the grid reads the cell itself); a covered neighbour is
+// always at the edge of its block, so its sub-faces and cells index from one slot
+// lookup (the upper block's copy of a face, or the lower block's high plane). Sums in
+// the reference's order.
+template <int N, class T, class C, class A = float, bool Z = false>
+__device__ inline void coarserow(const STopo& t, unsigned msk, const T* __restrict__ xf, const FineCoefT<C>& cf,
+                                 const T* __restrict__ xc, const CoarseCoefT<C>& ccf, const C* __restrict__ dDc, int I, int J,
+                                 int K, A& diag, A& off) {
+    constexpr int R = N / 4, V = N * N * N, m = N - 1;
+    constexpr uint32_t per = (N + 1) * N * N;
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    size_t own = cl(I, J, K, n0, n1);
+    A cc[6], xv[6];
+    bool in[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        int a = d >> 1, sg = (d & 1) ? 1 : -1;
+        int In = I + sg * (a == 0), Jn = J + sg * (a == 1), Kn = K + sg * (a == 2);
+        in[d] = In >= 0 && Jn >= 0 && Kn >= 0 && In < n0 && Jn < n1 && Kn < n2;
         int f0 = n0 + (a == 0), f1 = n1 + (a == 1);
-        for (int sg = -1; sg <= 1; sg += 2) {
+        size_t fi = sg < 0 ? cl(I, J, K, f0, f1) : cl(I + (a == 0), J + (a == 1), K + (a == 2), f0, f1);
+        cc[d] = (A)lf(ccf.c[a][fi]);
+        if constexpr (Z) xv[d] = (A)0;
+        else xv[d] = (A)lf(xc[in[d] ? cl(In, Jn, Kn, n0, n1) : own]);
+    }
+    diag = (A)lf(dDc[own]);
+    off = (A)0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        if (!in[d]) continue;
+        int a = d >> 1, sg = (d & 1) ? 1 : -1;
+        if (msk & (2u << d)) {
             int In = I + sg * (a == 0), Jn = J + sg * (a == 1), Kn = K + sg * (a == 2);
-            if (In < 0 || Jn < 0 || Kn < 0 || In >= n0 || Jn >= n1 || Kn >= n2) continue;
-            if (covered(t, In, Jn, Kn)) {
-                for (int q = 0; q < R; ++q)
-                    for (int p = 0; p < R; ++p) {
-                        int FI = a == 0 ? (sg > 0 ? R * In : R * I - 1) : R * I + p;
-                        int FJ = a == 1 ? (sg > 0 ? R * Jn : R * J - 1) : R * J + (a == 0 ? p : q);
-                        int FK = a == 2 ? (sg > 0 ? R * Kn : R * K - 1) : R * K + q;
-                        int PI = (a == 0 && sg < 0) ? FI + 1 : FI;
-                        int PJ = (a == 1 && sg < 0) ? FJ + 1 : FJ;
-                        int PK = (a == 2 && sg < 0) ? FK + 1 : FK;
-                        int32_t fidx = faceat<N>(t, a, PI, PJ, PK), cidx = cellat<N>(t, FI, FJ, FK);
-                        if (fidx >= 0 && cidx >= 0) {
-                            A c = (A)lf(cf.c[a][fidx]);
-                            diag += c;
-                            off += c * (A)lf(xf[cidx]);
-                        }
-                    }
-            } else {
-                A c = (A)lf(sg < 0 ? ccf.c[a][cl(I, J, K, f0, f1)] : ccf.c[a][cl(I + (a == 0), J + (a == 1), K + (a == 2), f0, f1)]);
-                if (c > (A)0) { diag += c; off += c * (A)lf(xc[cl(In, Jn, Kn, n0, n1)]); }
-            }
+            int32_t sn = t.slot[blockid(t.g, In >> 2, Jn >> 2, Kn >> 2)];
+            uint32_t cbase = (uint32_t)(sn - 1) * V, fbase = (uint32_t)(sn - 1) * per;
+            for (int q = 0; q < R; ++q)
+                for (int p = 0; p < R; ++p) {
+                    int FI = a == 0 ? (sg > 0 ? R * In : R * I - 1) : R * I + p;
+                    int FJ = a == 1 ? (sg > 0 ? R * Jn : R * J - 1) : R * J + (a == 0 ? p : q);
+                    int FK = a == 2 ? (sg > 0 ? R * Kn : R * K - 1) : R * K + q;
+                    int li = FI & m, lj = FJ & m, lk = FK & m;
+                    uint32_t cidx = cbase + (uint32_t)(li + N * (lj + N * lk));
+                    uint32_t fidx = fbase + (uint32_t)(sg > 0 ? ffn(N, a, li, lj, lk)
+                                                              : ffn(N, a, a == 0 ? N : li, a == 1 ? N : lj, a == 2 ? N : lk));
+                    A c = (A)lf(cf.c[a][fidx]);
+                    diag += c;
+                    off += c * (Z ? (A)0 : (A)lf(xf[cidx]));
+                }
+        } else if (cc[d] > (A)0) {
+            diag += cc[d];
+            off += cc[d] * xv[d];
         }
     }
 }
@@ -350,27 +412,39 @@ template <class C> struct CompT {
     CoarseCoefT<C> cc;
     const C* dDc;
     const uint8_t* actc;
+    const uint8_t* cmask;   // k_covermask
 };
 
 // sflip.jl:1218-1269: mode 0 Jacobi (out = x + w (b - A x)/diag), 1 residual (out = b - A x),
-// 2 apply (out = A x), on the fine cells of a level; inactive cells get 0.
+// 2 apply (out = A x), 3 Jacobi from x = 0 (x not read), on the fine cells of a level;
+// inactive cells and the cells of free slots get 0.
 template <int N, int MODE, class T, class C>
 __global__ __launch_bounds__(256) void k_rowfine(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
                                                  const T* __restrict__ xc, CompT<C> L, STopo t, int32_t nf) {
-    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nf) return;
+    // launched with 256 threads a workgroup: its cells lie in one slot (256 divides N^3),
+    // so the slot, its place and its neighbours are uniform (scalar loads), and as nf is
+    // a multiple of N^3 both returns below are uniform over the workgroup
     constexpr int lg = N == 16 ? 12 : 9, msk = N * N * N - 1, l2 = N == 16 ? 4 : 3;
-    int s = (n >> lg) + 1, c = n & msk;
+    int32_t n = blockIdx.x * 256 + threadIdx.x;
+    if ((int32_t)(blockIdx.x * 256) >= nf) return;
+    int s = (int)((blockIdx.x * 256u) >> lg) + 1, c = n & msk;
     int4 w = t.where[s - 1];
-    if (w.x < 0) return;
+    if (w.x < 0) {
+        out[n] = sf<T>(0.0f);
+        return;
+    }
+    int32_t nb[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) nb[d] = nbslot(t, w, d);
     bool act = MODE == 2 ? L.actf[n] == LIQUID : L.actf[n] != 0;
     float res = 0.0f;
     if (act) {
         float diag, off;
-        finerow<N>(t, x, xc, L.cf, L.dDf, s, c & (N - 1), (c >> l2) & (N - 1), c >> (2 * l2), w, diag, off);
-        float xn = lf(x[n]);
+        finerow<N, T, C, float, MODE == 3>(t, nb, x, xc, L.cf, L.dDf, s, c & (N - 1), (c >> l2) & (N - 1), c >> (2 * l2), w,
+                                           diag, off);
+        float xn = MODE == 3 ? 0.0f : lf(x[n]);
         float ax = diag * xn - off;
-        if (MODE == 0) res = diag > 0.0f ? xn + __fdiv_rn(OMEGA * (lf(b[n]) - ax), diag) : 0.0f;
+        if (MODE == 0 || MODE == 3) res = diag > 0.0f ? xn + __fdiv_rn(OMEGA * (lf(b[n]) - ax), diag) : 0.0f;
         else if (MODE == 1) res = lf(b[n]) - ax;
         else res = ax;
     }
@@ -384,12 +458,13 @@ __global__ __launch_bounds__(256) void k_rowcoarse(T* __restrict__ out, const T*
     if (l >= (size_t)n0 * n1 * n2) return;
     int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
     float res = 0.0f;
-    if (L.actc[l] != 0 && !covered(t, I, J, K)) {
+    unsigned msk = L.cmask[l];
+    if (L.actc[l] != 0 && !(msk & 1u)) {
         float diag, off;
-        coarserow<N>(t, xf, L.cf, x, L.cc, L.dDc, I, J, K, diag, off);
-        float xl = lf(x[l]);
+        coarserow<N, T, C, float, MODE == 3>(t, msk, xf, L.cf, x, L.cc, L.dDc, I, J, K, diag, off);
+        float xl = MODE == 3 ? 0.0f : lf(x[l]);
         float ax = diag * xl - off;
-        if (MODE == 0) res = diag > 0.0f ? xl + __fdiv_rn(OMEGA * (lf(b[l]) - ax), diag) : 0.0f;
+        if (MODE == 0 || MODE == 3) res = diag > 0.0f ? xl + __fdiv_rn(OMEGA * (lf(b[l]) - ax), diag) : 0.0f;
         else if (MODE == 1) res = lf(b[l]) - ax;
         else res = ax;
     }
@@ -404,19 +479,32 @@ template <class C> struct DViewT {
     const C* dD;
     const uint8_t* act;
 };
-template <class T, class C>
+// A dense level's cell: diag and the neighbours' sum; all loads issued before any use
+// (an index off the grid reads the cell itself, and its coupling adds nothing).
+template <class T, class C, bool Z = false>
 __device__ inline void denserow(const DViewT<C>& L, const T* __restrict__ x, int i, int j, int k, float& diag, float& off) {
     int n0 = L.n.x, n1 = L.n.y, n2 = L.n.z;
-    diag = lf(L.dD[cl(i, j, k, n0, n1)]);
+    size_t own = cl(i, j, k, n0, n1), sy = (size_t)n0, sz = (size_t)n0 * n1;
+    float c0 = lf(L.c[0][cl(i, j, k, n0 + 1, n1)]), c1 = lf(L.c[0][cl(i + 1, j, k, n0 + 1, n1)]);
+    float c2 = lf(L.c[1][cl(i, j, k, n0, n1 + 1)]), c3 = lf(L.c[1][cl(i, j + 1, k, n0, n1 + 1)]);
+    float c4 = lf(L.c[2][cl(i, j, k, n0, n1)]), c5 = lf(L.c[2][cl(i, j, k + 1, n0, n1)]);
+    auto xv = [&](bool ok, size_t idx) -> float {
+        if constexpr (Z) return 0.0f;   // x = 0: nothing to read
+        return lf(x[ok ? idx : own]);
+    };
+    bool b0 = i > 0, b1 = i < n0 - 1, b2 = j > 0, b3 = j < n1 - 1, b4 = k > 0, b5 = k < n2 - 1;
+    float x0 = xv(b0, own - 1), x1 = xv(b1, own + 1), x2 = xv(b2, own - sy), x3 = xv(b3, own + sy);
+    float x4 = xv(b4, own - sz), x5 = xv(b5, own + sz);
+    diag = lf(L.dD[own]);
     off = 0.0f;
-    float c;
-    c = lf(L.c[0][cl(i, j, k, n0 + 1, n1)]);     if (c > 0.0f && i > 0) { diag += c; off += c * lf(x[cl(i - 1, j, k, n0, n1)]); }
-    c = lf(L.c[0][cl(i + 1, j, k, n0 + 1, n1)]); if (c > 0.0f && i < n0 - 1) { diag += c; off += c * lf(x[cl(i + 1, j, k, n0, n1)]); }
-    c = lf(L.c[1][cl(i, j, k, n0, n1 + 1)]);     if (c > 0.0f && j > 0) { diag += c; off += c * lf(x[cl(i, j - 1, k, n0, n1)]); }
-    c = lf(L.c[1][cl(i, j + 1, k, n0, n1 + 1)]); if (c > 0.0f && j < n1 - 1) { diag += c; off += c * lf(x[cl(i, j + 1, k, n0, n1)]); }
-    c = lf(L.c[2][cl(i, j, k, n0, n1)]);         if (c > 0.0f && k > 0) { diag += c; off += c * lf(x[cl(i, j, k - 1, n0, n1)]); }
-    c = lf(L.c[2][cl(i, j, k + 1, n0, n1)]);     if (c > 0.0f && k < n2 - 1) { diag += c; off += c * lf(x[cl(i, j, k + 1, n0, n1)]); }
+    if (c0 > 0.0f && b0) { diag += c0; off += c0 * x0; }
+    if (c1 > 0.0f && b1) { diag += c1; off += c1 * x1; }
+    if (c2 > 0.0f && b2) { diag += c2; off += c2 * x2; }
+    if (c3 > 0.0f && b3) { diag += c3; off += c3 * x3; }
+    if (c4 > 0.0f && b4) { diag += c4; off += c4 * x4; }
+    if (c5 > 0.0f && b5) { diag += c5; off += c5 * x5; }
 }
+// mode 0 Jacobi, 1 residual, 3 Jacobi from x = 0 (x not read)
 template <int MODE, class T, class C>
 __global__ __launch_bounds__(256) void k_rowdense(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
                                                   DViewT<C> L) {
@@ -426,10 +514,10 @@ __global__ __launch_bounds__(256) void k_rowdense(T* __restrict__ out, const T* 
     float res = 0.0f;
     if (L.act[l] != 0) {
         float diag, off;
-        denserow(L, x, i, j, k, diag, off);
-        float xl = lf(x[l]);
+        denserow<T, C, MODE == 3>(L, x, i, j, k, diag, off);
+        float xl = MODE == 3 ? 0.0f : lf(x[l]);
         float ax = diag * xl - off;
-        res = MODE == 0 ? (diag > 0.0f ? xl + __fdiv_rn(OMEGA * (lf(b[l]) - ax), diag) : 0.0f) : lf(b[l]) - ax;
+        res = MODE != 1 ? (diag > 0.0f ? xl + __fdiv_rn(OMEGA * (lf(b[l]) - ax), diag) : 0.0f) : lf(b[l]) - ax;
     }
     out[l] = sf<T>(res);
 }
@@ -600,18 +688,17 @@ __device__ inline double block_sum_red(double v, double* lds) {
 __global__ __launch_bounds__(256) void k_sdot(double* __restrict__ partials, const float* __restrict__ af,
                                               const float* __restrict__ bf, const float* __restrict__ ac,
                                               const float* __restrict__ bc, const uint8_t* __restrict__ tf,
-                                              const uint8_t* __restrict__ tc, STopo t, int32_t nf) {
+                                              const uint8_t* __restrict__ tc, const uint8_t* __restrict__ cm, STopo t,
+                                              int32_t nf) {
     __shared__ double lds[4];
-    int n0 = t.g.nc(0), n1 = t.g.nc(1);
-    size_t Cc = (size_t)n0 * n1 * t.g.nc(2);
+    size_t Cc = (size_t)t.g.nc(0) * t.g.nc(1) * t.g.nc(2);
     double v = 0;
     for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < (size_t)nf + Cc; l += (size_t)gridDim.x * blockDim.x) {
         if (l < (size_t)nf) {
             if (tf[l] == LIQUID && t.where[l >> 12].x >= 0) v += (double)af[l] * (double)bf[l];
         } else {
             size_t m = l - nf;
-            int I = (int)(m % n0), J = (int)((m / n0) % n1), K = (int)(m / ((size_t)n0 * n1));
-            if (tc[m] == LIQUID && !covered(t, I, J, K)) v += (double)ac[m] * (double)bc[m];
+            if (tc[m] == LIQUID && !(cm[m] & 1)) v += (double)ac[m] * (double)bc[m];
         }
     }
     double s = block_sum_red(v, lds);
@@ -668,12 +755,10 @@ __global__ void k_saxpy(float* __restrict__ y, const float* __restrict__ x, size
 // The unknowns' mask of the inner product (sflip.jl:1772-1805), for position l of the
 // concatenated (fine pool, coarse grid) range.
 __device__ inline bool unknown(size_t l, int32_t nf, const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc,
-                               const STopo& t) {
+                               const uint8_t* __restrict__ cm, const STopo& t) {
     if (l < (size_t)nf) return tf[l] == LIQUID && t.where[l >> 12].x >= 0;
     size_t m = l - nf;
-    int n0 = t.g.nc(0), n1 = t.g.nc(1);
-    int I = (int)(m % n0), J = (int)((m / n0) % n1), K = (int)(m / ((size_t)n0 * n1));
-    return tc[m] == LIQUID && !covered(t, I, J, K);
+    return tc[m] == LIQUID && !(cm[m] & 1);
 }
 
 // sflip.jl:1889-1892 in one pass: P += alpha d, rold = r, r -= alpha q, and r.r per
@@ -683,8 +768,9 @@ __global__ __launch_bounds__(256) void k_fcg_update(double* __restrict__ partial
                                                     float* __restrict__ roldf, float* __restrict__ roldc,
                                                     const float* __restrict__ df, const float* __restrict__ dc,
                                                     const float* __restrict__ qf, const float* __restrict__ qc,
-                                                    const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc, STopo t,
-                                                    int32_t nf, const double* __restrict__ scal, int aslot) {
+                                                    const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc,
+                                                    const uint8_t* __restrict__ cm, STopo t, int32_t nf,
+                                                    const double* __restrict__ scal, int aslot) {
     __shared__ double lds[4];
     float al = (float)scal[aslot];
     size_t Cc = (size_t)t.g.nc(0) * t.g.nc(1) * t.g.nc(2);
@@ -702,7 +788,7 @@ __global__ __launch_bounds__(256) void k_fcg_update(double* __restrict__ partial
         ro[m] = rv;
         rv = rv + -al * q[m];
         r[m] = rv;
-        if (unknown(l, nf, tf, tc, t)) v += (double)rv * (double)rv;
+        if (unknown(l, nf, tf, tc, cm, t)) v += (double)rv * (double)rv;
     }
     double s = block_sum_red(v, lds);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
@@ -712,12 +798,13 @@ __global__ __launch_bounds__(256) void k_sdot2(double* __restrict__ partials, co
                                                const float* __restrict__ zc, const float* __restrict__ rf,
                                                const float* __restrict__ rc, const float* __restrict__ of,
                                                const float* __restrict__ oc, const uint8_t* __restrict__ tf,
-                                               const uint8_t* __restrict__ tc, STopo t, int32_t nf) {
+                                               const uint8_t* __restrict__ tc, const uint8_t* __restrict__ cm, STopo t,
+                                               int32_t nf) {
     __shared__ double lds[4];
     size_t Cc = (size_t)t.g.nc(0) * t.g.nc(1) * t.g.nc(2);
     double v1 = 0, v2 = 0;
     for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < (size_t)nf + Cc; l += (size_t)gridDim.x * blockDim.x) {
-        if (!unknown(l, nf, tf, tc, t)) continue;
+        if (!unknown(l, nf, tf, tc, cm, t)) continue;
         bool fine = l < (size_t)nf;
         size_t m = fine ? l : l - nf;
         double zv = fine ? zf[m] : zc[m];
@@ -770,15 +857,18 @@ __global__ void k_saxpy2(float* __restrict__ yf, const float* __restrict__ xf, s
 __global__ __launch_bounds__(256) void k_resid64fine(float* __restrict__ r, const double* __restrict__ x,
                                                      const double* __restrict__ xc, const float* __restrict__ b,
                                                      CompT<float> L, STopo t, int32_t nf) {
-    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nf) return;
-    int s = (n >> 12) + 1, c = n & 4095;
+    int32_t n = blockIdx.x * 256 + threadIdx.x;   // 256 threads a workgroup, as k_rowfine
+    if ((int32_t)(blockIdx.x * 256) >= nf) return;
+    int s = (int)((blockIdx.x * 256u) >> 12) + 1, c = n & 4095;
     int4 w = t.where[s - 1];
     if (w.x < 0) return;
+    int32_t nb[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) nb[d] = nbslot(t, w, d);
     double res = 0.0;
     if (L.actf[n] == LIQUID) {
         double diag, off;
-        finerow<16, double, float, double>(t, x, xc, L.cf, L.dDf, s, c & 15, (c >> 4) & 15, c >> 8, w, diag, off);
+        finerow<16, double, float, double>(t, nb, x, xc, L.cf, L.dDf, s, c & 15, (c >> 4) & 15, c >> 8, w, diag, off);
         res = (double)b[n] - (diag * x[n] - off);
     }
     r[n] = (float)res;
@@ -791,9 +881,10 @@ __global__ __launch_bounds__(256) void k_resid64coarse(float* __restrict__ r, co
     if (l >= (size_t)n0 * n1 * n2) return;
     int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
     double res = 0.0;
-    if (L.actc[l] != 0 && !covered(t, I, J, K)) {
+    unsigned msk = L.cmask[l];
+    if (L.actc[l] != 0 && !(msk & 1u)) {
         double diag, off;
-        coarserow<16, double, float, double>(t, xf, L.cf, x, L.cc, L.dDc, I, J, K, diag, off);
+        coarserow<16, double, float, double>(t, msk, xf, L.cf, x, L.cc, L.dDc, I, J, K, diag, off);
         res = (double)b[l] - (diag * x[l] - off);
     }
     r[l] = (float)res;
@@ -907,6 +998,8 @@ struct SparseSolver {
         hipLaunchKernelGGL(k_coarsedirichlet, dim3(cdiv(Cc, 256)), dim3(256), 0, st, s.dD.coarse.p, s.types.fine.p,
                            s.types.coarse.p, s.phi.fine.p, s.phi.coarse.p, t);
         hipLaunchKernelGGL(k_activecoarse, dim3(cdiv(Cc, 256)), dim3(256), 0, st, s.D[0].act.p, s.types.coarse.p, t);
+        if (!s.cmask.p) s.cmask.alloc(Cc);
+        hipLaunchKernelGGL(k_covermask, dim3(cdiv(Cc, 256)), dim3(256), 0, st, s.cmask.p, t);
         MAEL_LAUNCH_CHECK();
     }
     void hierarchy() {
@@ -953,9 +1046,9 @@ struct SparseSolver {
             H.D.push_back(DViewT<float>{i3(L.n), {L.c[0].p, L.c[1].p, L.c[2].p}, L.dD.p, L.act.p});
         }
         H.L0 = CompT<float>{{{s.cf[0].p, s.cf[1].p, s.cf[2].p}}, s.dD.fine.p, s.types.fine.p,
-                            {{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p};
+                            {{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p, s.cmask.p};
         H.L1 = CompT<float>{{{s.c1[0].p, s.c1[1].p, s.c1[2].p}}, s.dD1.p, s.act1.p,
-                            {{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p};
+                            {{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p, s.cmask.p};
         return H;
     }
     // The reduced-precision hierarchy (storage in s.mp): vectors and couplings, the dense
@@ -974,9 +1067,9 @@ struct SparseSolver {
                                     s.D[l].act.p});
         }
         H.L0 = CompT<T>{{{(T*)m.cf[0].p, (T*)m.cf[1].p, (T*)m.cf[2].p}}, (T*)m.dDf.p, s.types.fine.p,
-                        {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p};
+                        {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p, s.cmask.p};
         H.L1 = CompT<T>{{{(T*)m.c1[0].p, (T*)m.c1[1].p, (T*)m.c1[2].p}}, (T*)m.dD1.p, s.act1.p,
-                        {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p};
+                        {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p, s.cmask.p};
         H.rscale = 0.25f;
         return H;
     }
@@ -1030,27 +1123,39 @@ struct SparseSolver {
             rows<N, 0>(x, tmp, b, L);
         }
     }
+    // The same from x = 0: the first sweep reads no x, and x needs no clearing (every cell
+    // of the level's slots and coarse part is written).
+    template <int N, class T, class C>
+    void smooth0(SVecT<T> x, SVecT<T> b, SVecT<T> tmp, const CompT<C>& L, int sweeps, size_t nfine) {
+        if (sweeps < 1) { zero_t(x, nfine); return; }
+        rows<N, 3>(tmp, x, b, L);
+        rows<N, 0>(x, tmp, b, L);
+        smooth<N>(x, b, tmp, L, sweeps - 1);
+    }
 
     // sflip.jl:1722-1762 vcycle!: z = M^-1 r in the hierarchy H (coarsest in arithmetic A).
     template <class T, class C, class A>
     void vcycle_t(Hier<T, C>& H, SVecT<T> z, SVecT<T> r, int sweeps, int coarse, float boost) {
         int ns = s.nslots;
-        zero_t(z, (size_t)nf);
-        smooth<16>(z, r, H.tmp, H.L0, sweeps);
+        smooth0<16>(z, r, H.tmp, H.L0, sweeps, (size_t)nf);
         rows<16, 1>(H.q, z, r, H.L0);
         if (ns) hipLaunchKernelGGL(k_restrict01<T>, dim3(cdiv(ns * SBCELLS1, 256)), dim3(256), 0, st, H.X1b.f, H.q.f, t, ns);
         MAEL_HIP(hipMemcpyAsync(H.X1b.c, H.q.c, Cc * sizeof(T), hipMemcpyDeviceToDevice, st));
-        zero_t(H.X1x, (size_t)ns * SBCELLS1);
-        smooth<8>(H.X1x, H.X1b, H.X1t, H.L1, sweeps);
+        smooth0<8>(H.X1x, H.X1b, H.X1t, H.L1, sweeps, (size_t)ns * SBCELLS1);
         rows<8, 1>(H.X1r, H.X1x, H.X1b, H.L1);
         hipLaunchKernelGGL(k_restrict12<T>, dim3(cdiv(Cc, 256)), dim3(256), 0, st, H.Db[0], H.X1r.f, H.X1r.c, t);
         int nl = (int)s.D.size();
         for (int l = 0; l + 1 < nl; ++l) {
             SLevel& Dl = s.D[l];
             size_t c = Dl.cells();
-            zero_t(H.Dx[l], c);
-            for (int k = 0; k < sweeps; ++k) {
-                hipLaunchKernelGGL((k_rowdense<0, T, C>), dim3(cdiv(c, 256)), dim3(256), 0, st, H.Dt[l], H.Dx[l], H.Db[l], H.D[l]);
+            if (sweeps < 1) zero_t(H.Dx[l], c);
+            for (int k = 0; k < sweeps; ++k) {   // from x = 0: the first sweep reads no x
+                if (k == 0)
+                    hipLaunchKernelGGL((k_rowdense<3, T, C>), dim3(cdiv(c, 256)), dim3(256), 0, st, H.Dt[l], H.Dx[l], H.Db[l],
+                                       H.D[l]);
+                else
+                    hipLaunchKernelGGL((k_rowdense<0, T, C>), dim3(cdiv(c, 256)), dim3(256), 0, st, H.Dt[l], H.Dx[l], H.Db[l],
+                                       H.D[l]);
                 hipLaunchKernelGGL((k_rowdense<0, T, C>), dim3(cdiv(c, 256)), dim3(256), 0, st, H.Dx[l], H.Dt[l], H.Db[l], H.D[l]);
             }
             hipLaunchKernelGGL((k_rowdense<1, T, C>), dim3(cdiv(c, 256)), dim3(256), 0, st, H.Dr[l], H.Dx[l], H.Db[l], H.D[l]);
@@ -1112,7 +1217,7 @@ struct SparseSolver {
     }
     void dot(SVec a, SVec b, int op, int slot) {
         hipLaunchKernelGGL(k_sdot, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, a.f, b.f, a.c, b.c, s.types.fine.p,
-                           s.types.coarse.p, t, nf);
+                           s.types.coarse.p, s.cmask.p, t, nf);
         hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(256), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p, op, slot);
         MAEL_LAUNCH_CHECK();
     }
@@ -1221,7 +1326,8 @@ struct SparseSolver {
             apply(q, d);
             dot(d, q, SF_ALPHA, 0);
             hipLaunchKernelGGL(k_fcg_update, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, P.f, P.c, r.f, r.c, rold.f,
-                               rold.c, d.f, d.c, q.f, q.c, s.types.fine.p, s.types.coarse.p, t, nf, s.scal.p, (int)SS_ALPHA);
+                               rold.c, d.f, d.c, q.f, q.c, s.types.fine.p, s.types.coarse.p, s.cmask.p, t, nf, s.scal.p,
+                               (int)SS_ALPHA);
             hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(256), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p, (int)SF_NORM,
                                (int)SS_RNORM);
             double flags;
@@ -1232,7 +1338,7 @@ struct SparseSolver {
             if (rnorm <= target) { converged = true; return it; }
             vcycle(z, r, sweeps, coarse, boost, SS_RNORM);
             hipLaunchKernelGGL(k_sdot2, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, z.f, z.c, r.f, r.c, rold.f, rold.c,
-                               s.types.fine.p, s.types.coarse.p, t, nf);
+                               s.types.fine.p, s.types.coarse.p, s.cmask.p, t, nf);
             hipLaunchKernelGGL(k_sfinish_beta, dim3(1), dim3(256), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p);
             xpby(d, z, SS_BETA, 1.0f);
         }
@@ -1281,8 +1387,7 @@ struct SparseSolver {
                 SVec Vj = vfield(s.V, j), Zj = vfield(s.Z, j);
                 vcycle(z, Vj, sweeps, coarse, boost);
                 // the diagonal part: one damped Jacobi sweep from zero (fgmres.jl:15-23)
-                zero(tmp, (size_t)nf);
-                rows<16, 0>(SVecT<float>{Zj.f, Zj.c}, SVecT<float>{tmp.f, tmp.c}, SVecT<float>{Vj.f, Vj.c}, hier32().L0);
+                rows<16, 3>(SVecT<float>{Zj.f, Zj.c}, SVecT<float>{tmp.f, tmp.c}, SVecT<float>{Vj.f, Vj.c}, hier32().L0);
                 axpy(Zj, z, -1, 1.0f);
                 apply(q, Zj);
                 int k = 0;

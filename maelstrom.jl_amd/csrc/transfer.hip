@@ -22,29 +22,13 @@ struct DenseNodes {
     }
 };
 
-// flip.jl:539-560 splat_kernel!: the particles' level set by atomic min over order keys.
-__global__ __launch_bounds__(256) void k_dense_splat(PView ps, int64_t n, DGrid g, float dt, int32_t* __restrict__ phip) {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    float vx = ps.f[3][p], vy = ps.f[4][p], vz = ps.f[5][p];
-    float sp = sqrtf(vx * vx + vy * vy + vz * vz);
-    float s = ps.f[15][p] + ahead(strength(sp, dt, g.dx)) * dt;
-    float xp = ps.f[0][p] + s * vx, yp = ps.f[1][p] + s * vy, zp = ps.f[2][p] + s * vz;
-    int ci = (int)floorf(__fdiv_rn(xp - g.lo[0], g.dx)), cj = (int)floorf(__fdiv_rn(yp - g.lo[1], g.dx)),
-        ck = (int)floorf(__fdiv_rn(zp - g.lo[2], g.dx));
-    float radius = 0.5f * g.dx;
-    for (int dk = -1; dk <= 1; ++dk)
-        for (int dj = -1; dj <= 1; ++dj)
-            for (int di = -1; di <= 1; ++di) {
-                int I = ci + di, J = cj + dj, K = ck + dk;
-                if (I < 0 || I >= g.n[0] || J < 0 || J >= g.n[1] || K < 0 || K >= g.n[2]) continue;
-                float ex = g.lo[0] + ((float)I + 0.5f) * g.dx - xp;
-                float ey = g.lo[1] + ((float)J + 0.5f) * g.dx - yp;
-                float ez = g.lo[2] + ((float)K + 0.5f) * g.dx - zp;
-                float d = sqrtf(ex * ex + ey * ey + ez * ez) - radius;
-                atomicMin(&phip[(size_t)I + (size_t)g.n[0] * ((size_t)J + (size_t)g.n[1] * K)], orderkey(d));
-            }
-}
+struct DenseCells {
+    int n[3];
+    __device__ int64_t index(int I, int J, int K) const {
+        if (I < 0 || I >= n[0] || J < 0 || J >= n[1] || K < 0 || K >= n[2]) return -1;
+        return (int64_t)I + (int64_t)n[0] * ((int64_t)J + (int64_t)n[1] * K);
+    }
+};
 
 // flip.jl:562-565 union_kernel!.
 __global__ void k_union(size_t C, float* __restrict__ phi, const float* __restrict__ phi0, const int32_t* __restrict__ phip,
@@ -99,7 +83,8 @@ void DenseBasin::p2g(float dt) {
         hipLaunchKernelGGL(k_p2g_tiled<DenseNodes>, dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s, ps, n, map, A);
         kt.end(K_P2G, s);
         kt.begin(K_SPLAT, s);
-        hipLaunchKernelGGL(k_dense_splat, dim3(cdiv(n, 256)), dim3(256), 0, s, ps, n, g, dt, phip.p);
+        hipLaunchKernelGGL(k_splat_tiled<DenseCells>, dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s, ps, n,
+                           DenseCells{{g.n[0], g.n[1], g.n[2]}}, g.lo[0], g.lo[1], g.lo[2], g.dx, dt, phip.p);
         kt.end(K_SPLAT, s);
         MAEL_LAUNCH_CHECK();
     }

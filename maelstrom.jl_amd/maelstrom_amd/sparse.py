@@ -51,7 +51,8 @@ class SparseBasin:
         self.volume0 = 0.0
         self.t = 0.0
         self.steps = 0
-        self.iterations = []
+        self.iterations = []   # each solve's iterations, as the reference's s.iterations
+        self.solve_ms = []     # each solve's wall time (ms), set-up and projection of the faces included
         self.last_solve = None
         set_stir(self, Stir(basin) if stir is None else stir)
 
@@ -263,6 +264,7 @@ def project(s, dt, recover=0.5, gamma=None, **solve):
         warnings.warn("pressure solve did not converge: residual %.3g after %d iterations"
                       % (st.rel_residual, st.iterations), RuntimeWarning, stacklevel=2)
     s.iterations.append(st.iterations)
+    s.solve_ms.append(st.ms)
     s.last_solve = st
     return st
 

@@ -194,33 +194,41 @@ def sparse_main(args, desc, kw, dx):
     umax = max(float(np.abs(s.get(f)).max()) for f in ("U_F", "V_F", "W_F", "U_C", "V_C", "W_C"))
     left = 1.0 / 24
     dt = left / math.ceil(left / (s.cfl * dx / max(umax, 1e-3)))
-    solve = dict(tol=args.tol, maxiter=240, restart=20, policy=2, boost=1.5, precision=args.precision,
-                 refine=args.refine)
-    for _ in range(args.warmup):
-        sp.step(s, dt, **solve)
-    s.sync()
+    def solve_for(precision):
+        return dict(tol=args.tol, maxiter=240, restart=20, policy=2, boost=1.5, precision=precision, refine=args.refine)
+
+    def timed(precision, steps, kernel_id):
+        """`steps` whole steps between two synchronisations; the kernel timer on `kernel_id`."""
+        solve = solve_for(precision)
+        s.sync()
+        s.ctx.check(L.mael_sparse_ktimer(s.h, kernel_id, 64 * 1024))
+        iters, fg, pms, nparts = [], [], [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            k0 = len(s.iterations)
+            st = sp.step(s, dt, **solve)
+            iters.append(sum(s.iterations[k0:]))   # both half-steps' solves
+            fg.append(st.used_fgmres)
+            pms.append(sum(s.solve_ms[k0:]))
+            nparts.append(s.n)
+        s.sync()
+        t1 = time.perf_counter()
+        kms, kl = C.c_double(), C.c_int32()
+        s.ctx.check(L.mael_sparse_ktime(s.h, C.byref(kms), C.byref(kl)))
+        return dict(ms=1e3 * (t1 - t0) / steps, pressure_ms=sum(pms) / steps, iterations=sum(iters) / steps,
+                    fgmres=int(sum(fg)), particles=sum(nparts) / steps, kms=kms.value, launches=kl.value)
+
     L = m.lib()
+    for _ in range(args.warmup):
+        sp.step(s, dt, **solve_for(args.precision))
     kernel = "jacobi" if args.kernel in ("rbgs", "jacobi") else args.kernel
     kid = KERNEL_ID["rbgs" if kernel == "jacobi" else kernel]
-    s.ctx.check(L.mael_sparse_ktimer(s.h, kid, 64 * 1024))
-    iters, fg, pressure_ms, nparts, live = [], [], [], [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        k0 = len(s.iterations)
-        st = sp.step(s, dt, **solve)
-        iters.append(sum(s.iterations[k0:]))   # both half-steps' solves
-        fg.append(st.used_fgmres)
-        pressure_ms.append(sum(s.solve_ms[k0:]))
-        nparts.append(s.n)
-    s.sync()
-    t1 = time.perf_counter()
+    R = timed(args.precision, args.steps, kid)
     slot, where, ns, free = s.topology()
     live_slots = int((where[:ns, 0] >= 0).sum())
-    kms, kl = C.c_double(), C.c_int32()
-    s.ctx.check(L.mael_sparse_ktime(s.h, C.byref(kms), C.byref(kl)))
-    ms_step = 1e3 * (t1 - t0) / args.steps
-    avg_launch_ms = kms.value / max(kl.value, 1)
-    n_avg = sum(nparts) / len(nparts)
+    ms_step = R["ms"]
+    avg_launch_ms = R["kms"] / max(R["launches"], 1)
+    n_avg = R["particles"]
     abytes = sparse_bytes(kernel, live_slots * 4096, n_avg, args.precision)
     achieved = abytes / (avg_launch_ms * 1e-3) / 1e9
     out = {
@@ -234,15 +242,24 @@ def sparse_main(args, desc, kw, dx):
                    "stir": "surface stir, 32 seeded narutowaves-shaped waves, race eddy 3",
                    "phases": "two advection-reflection halves of refine, advect, p2g, redistance, classify, "
                              "trapped air, stir forces, project, kick, extrapolate, reflect, g2p, resample"},
-        "pressure_ms": round(sum(pressure_ms) / len(pressure_ms), 3),
-        "iterations": round(sum(iters) / len(iters), 2),
-        "fgmres_steps": int(sum(fg)),
+        "pressure_ms": round(R["pressure_ms"], 3),
+        "iterations": round(R["iterations"], 2),
+        "fgmres_steps": R["fgmres"],
         "mparticles_per_s": round(n_avg / (ms_step * 1e-3) / 1e6, 2),
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "launches": kl.value, "avg_launch_us": round(avg_launch_ms * 1e3, 3),
+                     "launches": R["launches"], "avg_launch_us": round(avg_launch_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": int(abytes)},
     }
+    if args.precision != 0 and not args.no_fp32:
+        # the same steps with the FP32 V-cycle (the same FP64 refinement), from the state the
+        # mixed-precision steps left: the comparison the metric names
+        F = timed(0, args.steps, kid)
+        out["fp32"] = {"ms_per_step": round(F["ms"], 3), "pressure_ms": round(F["pressure_ms"], 3),
+                       "iterations": round(F["iterations"], 2),
+                       "jacobi_avg_launch_us": round(1e3 * F["kms"] / max(F["launches"], 1), 3)}
+        out["speedup_vs_fp32"] = {"step": round(F["ms"] / ms_step, 3),
+                                  "pressure": round(F["pressure_ms"] / max(R["pressure_ms"], 1e-9), 3)}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = sparse_cpu_baseline(s, out["iterations"], dt)
     print(json.dumps(out), flush=True)
@@ -255,11 +272,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="sparse5cm", choices=sorted(WORKLOADS))
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--precision", type=int, default=0, help="V-cycle storage: 0 FP32, 1 FP16, 2 BF16")
+    ap.add_argument("--precision", type=int, default=None,
+                    help="V-cycle storage: 0 FP32, 1 FP16, 2 BF16 (default: 1 on the sparse workloads, 0 on the dense)")
     ap.add_argument("--refine", type=int, default=10,
                     help="sparse solve: FP64 residual recomputation every k iterations (0: the plain FP32 solve)")
     ap.add_argument("--kernel", default="rbgs", help="the kernel the roofline object reports")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="sparse: skip the FP32 V-cycle comparison steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's hot path runs on a single MI355X (--gpus %d not supported)"
@@ -267,6 +286,8 @@ def main():
         sys.exit(2)
 
     kw, dx, desc = WORKLOADS[args.workload]
+    if args.precision is None:
+        args.precision = 1 if args.workload.startswith("sparse") else 0
     if args.workload.startswith("sparse"):
         return sparse_main(args, desc, kw, dx)
     import maelstrom_amd as m

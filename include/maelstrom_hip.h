@@ -97,6 +97,11 @@ enum {
     MAEL_COUNTS, MAEL_TOTALS, MAEL_OFFSETS, /* last resample: per cell (offsets inclusive) */
     MAEL_MOM_U, MAEL_MOM_V, MAEL_MOM_W,   /* last P2G: momentum accumulators (int64 fixed point, 2^-32) */
     MAEL_WT_U, MAEL_WT_V, MAEL_WT_W,      /* last P2G: weight accumulators (int64 fixed point, 2^-32) */
+    MAEL_UB_U, MAEL_UB_V, MAEL_UB_W,      /* adjoints: cotangents of the face velocities (float) */
+    MAEL_UB0_U, MAEL_UB0_V, MAEL_UB0_W,   /* adjoints: cotangents of u0 (float) */
+    MAEL_PB_VX, MAEL_PB_VY, MAEL_PB_VZ,   /* adjoints: cotangents of the particles' v (float, capacity) */
+    MAEL_PB_C0, MAEL_PB_C1, MAEL_PB_C2, MAEL_PB_C3, MAEL_PB_C4, MAEL_PB_C5, MAEL_PB_C6, MAEL_PB_C7,
+    MAEL_PB_C8,                           /* ... and of their APIC rows */
     MAEL_NARRAYS
 };
 /* Device pointer, element count and element size of array `which`. */
@@ -120,6 +125,20 @@ int mael_dense_p2g(mael_dense *d, float dt);
 
 /* `grid2particles!(s, dt)` (flip.jl:1246-1252). */
 int mael_dense_g2p(mael_dense *d, float dt, float gravity);
+
+/* Adjoints of the hot path's linear maps, for reverse-mode gradients (a ChainRulesCore
+ * `rrule` on the Julia side). They have no counterpart in the reference, which has no AD.
+ * Each is the transpose of its phase's map at the state that phase left (positions, lags,
+ * level set, types, flags and deposit weights held fixed); cotangents live in the
+ * MAEL_UB_*, MAEL_UB0_* and MAEL_PB_* arrays.
+ *   g2p_adjoint:     UB = G2P^T (PB); PB becomes the cotangent of G2P's input particles
+ *                    (v passes through where the grid was not reached, rows never do).
+ *   project_adjoint: UB <- UB + D^T A^-1 (-G^T UB) in place (the transposed MG-PCG solve;
+ *                    A is symmetric), P is left holding the adjoint pressure.
+ *   p2g_adjoint:     PB += P2G^T (UB) for the particles' v and rows (added to what
+ *                    g2p_adjoint passed through), UB0 = the part for u0. */
+int mael_dense_g2p_adjoint(mael_dense *d, float dt);
+int mael_dense_p2g_adjoint(mael_dense *d, float dt);
 
 typedef struct {
     double tol;          /* relative residual target (the reference's default 1e-4) */
@@ -146,6 +165,8 @@ void mael_solve_defaults(mael_solve_opts *o);
 /* `project!(s, dt)` (flip.jl:1144-1156) after the caller's γ (the volume control,
  * flip.jl:1146-1147): rhs, clear air, `pressure!`, project the faces and flag them. */
 int mael_dense_project(mael_dense *d, float dt, float gamma, const mael_solve_opts *opts, mael_solve_stats *stats);
+/* The transpose of `project!`'s linear part (see mael_dense_g2p_adjoint). */
+int mael_dense_project_adjoint(mael_dense *d, const mael_solve_opts *opts, mael_solve_stats *stats);
 
 /* Support phases around the hot path (not part of it): */
 /* `classify!(s)` (flip.jl:674-677). */

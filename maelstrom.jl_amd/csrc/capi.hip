@@ -40,7 +40,17 @@ ArrRef dense_array(mael::DenseBasin& b, int which) {
             return {b.mom[which - MAEL_MOM_U].p, (int64_t)b.faces(which - MAEL_MOM_U), 8};
         case MAEL_WT_U: case MAEL_WT_V: case MAEL_WT_W:
             return {b.wt[which - MAEL_WT_U].p, (int64_t)b.faces(which - MAEL_WT_U), 8};
+        case MAEL_UB_U: case MAEL_UB_V: case MAEL_UB_W:
+            b.adj_alloc();
+            return {b.ub[which - MAEL_UB_U].p, (int64_t)b.faces(which - MAEL_UB_U), 4};
+        case MAEL_UB0_U: case MAEL_UB0_V: case MAEL_UB0_W:
+            b.adj_alloc();
+            return {b.ub0[which - MAEL_UB0_U].p, (int64_t)b.faces(which - MAEL_UB0_U), 4};
         default:
+            if (which >= MAEL_PB_VX && which <= MAEL_PB_C8) {
+                b.adj_alloc();
+                return {b.pb[which - MAEL_PB_VX].p, b.capacity, 4};
+            }
             if (which >= MAEL_PX && which <= MAEL_LAG) return {b.pbuf[c][which - MAEL_PX].p, b.capacity, 4};
     }
     throw mael::Error(mael::E_ARG, "unknown array " + std::to_string(which));
@@ -200,6 +210,35 @@ int mael_dense_project(mael_dense* d, float dt, float gamma, const mael_solve_op
             stats->rel_residual_rec = s.rel_residual_recursive; stats->ms = s.ms_total;
         }
     });
+}
+static mael::SolveOpts solve_opts(const mael_solve_opts* opts) {
+    mael::SolveOpts o;
+    if (opts) {
+        o.tol = opts->tol; o.maxiter = opts->maxiter; o.sweeps = opts->sweeps; o.coarse = opts->coarse;
+        o.precision = opts->precision; o.refine = opts->refine; o.warm = opts->warm;
+    }
+    if (o.sweeps < 1 || o.coarse < 1 || o.maxiter < 0 || !(o.tol >= 0.0)) throw mael::Error(mael::E_ARG, "bad solver options");
+    return o;
+}
+int mael_dense_g2p_adjoint(mael_dense* d, float dt) {
+    if (!d) return MAEL_E_ARG;
+    return guard(d->ctx, [&] { d->b->g2p_adjoint(dt); });
+}
+int mael_dense_project_adjoint(mael_dense* d, const mael_solve_opts* opts, mael_solve_stats* stats) {
+    if (!d) return MAEL_E_ARG;
+    return guard(d->ctx, [&] {
+        mael::SolveOpts o = solve_opts(opts);
+        mael::SolveStats s;
+        d->b->project_adjoint(o, s);
+        if (stats) {
+            stats->iterations = s.iterations; stats->restarts = s.restarts; stats->rel_residual = s.rel_residual;
+            stats->rel_residual_rec = s.rel_residual_recursive; stats->ms = s.ms_total;
+        }
+    });
+}
+int mael_dense_p2g_adjoint(mael_dense* d, float dt) {
+    if (!d) return MAEL_E_ARG;
+    return guard(d->ctx, [&] { d->b->p2g_adjoint(dt); });
 }
 int mael_dense_classify(mael_dense* d) {
     if (!d) return MAEL_E_ARG;

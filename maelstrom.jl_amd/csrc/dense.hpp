@@ -152,6 +152,17 @@ struct DenseBasin {
     void p2g(float dt);                                      // flip.jl:587-606
     void g2p(float dt, float gravity);                       // flip.jl:1246-1252
     void project(float dt, float gamma, const SolveOpts& o, SolveStats& s);   // flip.jl:1144-1156
+    void operator_setup();                                   // coarse types and the fine diagonal
+    void solve_pressure(const SolveOpts& o, SolveStats& s);  // A P = b from P
+    // adjoints of the hot path's linear maps (no counterpart in the reference): cotangents
+    // of the face velocities in ub (u0's in ub0) and of the particles' v, c in pb
+    DBuf<float> ub[3], ub0[3], pb[12];
+    DBuf<unsigned long long> adjacc[3];
+    DBuf<uint8_t> reached;
+    void adj_alloc();
+    void g2p_adjoint(float dt);
+    void project_adjoint(const SolveOpts& o, SolveStats& s);
+    void p2g_adjoint(float dt);
     // support phases for initialisation and a whole step
     void classify();                                         // flip.jl:674-677
     void redistance(int iterations);                         // flip.jl:672-679

@@ -1,20 +1,23 @@
-// The APIC deposit (flip.jl:513-537) as a workgroup-tiled reduction.
+// The APIC deposit (flip.jl:513-537) as a tiled reduction.
 //
 // A workgroup takes 256 consecutive particles (resampling leaves them in cell order, so
 // they are close in space) and, one velocity component at a time, accumulates their
 // 27-node contributions into a box of nodes in LDS with 64-bit integer adds, then adds
-// each touched node of the box to HBM once. Contributions are rounded to 2^-32 fixed
-// point exactly as the per-particle deposit does and integer addition is associative,
-// so the result is bit-identical to depositing every contribution with its own global
-// atomic, in any order. A workgroup whose particles spread over more nodes than the box
-// holds deposits directly.
+// each touched node of the box to HBM once. When the workgroup's particles spread over
+// more nodes than its box holds (typically when its 256 particles straddle two blocks
+// that are far apart in the pool), each wavefront takes a quarter of the LDS for a box of
+// its own 64 particles, and only a wavefront that does not fit either deposits directly.
+// Contributions are rounded to 2^-32 fixed point exactly as the per-particle deposit does
+// and integer addition is associative, so the result is bit-identical to depositing every
+// contribution with its own global atomic, in any order.
 #pragma once
 #include "common.hpp"
 
 namespace mael {
 
 constexpr int P2G_THREADS = 256;
-constexpr int P2G_BOX = 1024;   // nodes per component in LDS (mom + wt: 16 KB)
+constexpr int P2G_BOX = 2048;                             // nodes per component in LDS (mom + wt: 32 KB)
+constexpr int P2G_WBOX = P2G_BOX / (P2G_THREADS / 64);    // a wavefront's share
 
 __device__ inline int wg_min(int v, int* red) {
 #pragma unroll
@@ -28,19 +31,73 @@ __device__ inline int wg_min(int v, int* red) {
     return r;
 }
 __device__ inline int wg_max(int v, int* red) { return -wg_min(-v, red); }
+__device__ inline int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline int wave_max(int v) { return -wave_min(-v); }
+
+// A box of nodes: origin, extents, and where it starts in LDS.
+struct TileBox {
+    int x0, y0, z0, ex, ey, ez, base;
+    bool use;
+    __device__ int at(int I, int J, int K) const { return base + (I - x0) + ex * ((J - y0) + ey * (K - z0)); }
+    __device__ int size() const { return ex * ey * ez; }
+};
+
+// The box of the workgroup (if its particles' nodes fit in `cap`), else of the thread's
+// wavefront (if they fit in `cap` / 4, in the wavefront's quarter). `ok` lanes are the
+// ones whose base nodes (b0, b1, b2) shape it; nodes span b .. b + `span` - 1.
+__device__ inline TileBox tile_box(bool ok, int b0, int b1, int b2, int span, int cap, int* red, bool& wgbox) {
+    const int BIG = 1 << 30;
+    TileBox t;
+    int x0 = wg_min(ok ? b0 : BIG, red), y0 = wg_min(ok ? b1 : BIG, red), z0 = wg_min(ok ? b2 : BIG, red);
+    int x1 = wg_max(ok ? b0 : -BIG, red), y1 = wg_max(ok ? b1 : -BIG, red), z1 = wg_max(ok ? b2 : -BIG, red);
+    int ex = x1 - x0 + span, ey = y1 - y0 + span, ez = z1 - z0 + span;
+    wgbox = x0 != BIG && ex <= cap && ey <= cap && ez <= cap && ex * ey * ez <= cap;
+    if (wgbox) {
+        t.x0 = x0; t.y0 = y0; t.z0 = z0; t.ex = ex; t.ey = ey; t.ez = ez; t.base = 0; t.use = true;
+        return t;
+    }
+    int wcap = cap / (P2G_THREADS / 64);
+    x0 = wave_min(ok ? b0 : BIG); y0 = wave_min(ok ? b1 : BIG); z0 = wave_min(ok ? b2 : BIG);
+    x1 = wave_max(ok ? b0 : -BIG); y1 = wave_max(ok ? b1 : -BIG); z1 = wave_max(ok ? b2 : -BIG);
+    ex = x1 - x0 + span; ey = y1 - y0 + span; ez = z1 - z0 + span;
+    t.use = x0 != BIG && ex <= wcap && ey <= wcap && ez <= wcap && ex * ey * ez <= wcap;
+    t.x0 = x0; t.y0 = y0; t.z0 = z0; t.ex = ex; t.ey = ey; t.ez = ez;
+    t.base = (threadIdx.x >> 6) * wcap;
+    return t;
+}
+// The lanes that clear and flush box t: the workgroup's threads, or the wavefront's lanes.
+__device__ inline int tile_first(bool wgbox) { return wgbox ? threadIdx.x : (threadIdx.x & 63); }
+__device__ inline int tile_stride(bool wgbox) { return wgbox ? P2G_THREADS : 64; }
 
 struct P2GArgs {
     float lo[3];
     float dx;
-    float dt;
     unsigned long long* mom[3];
-    unsigned long long* wt[3];
+    unsigned long long* wt[3];   // null: no weights accumulated
+    float scale;                 // every momentum contribution is multiplied by it before rounding
+};
+
+// The particle side of the deposit: the temporal weight of particle p, and the value and
+// affine row it deposits for component a.
+struct P2GSource {   // flip.jl:513-537: v_a and the APIC row c_a, weighted by temporal(lag, dt)
+    PView ps;
+    float dt;
+    __device__ float weight(int64_t p) const { return temporal(ps.f[15][p], dt); }
+    __device__ void comp(int64_t p, int a, float& v, float& c0, float& c1, float& c2) const {
+        v = ps.f[3 + a][p];
+        c0 = ps.f[6 + 3 * a][p]; c1 = ps.f[7 + 3 * a][p]; c2 = ps.f[8 + 3 * a][p];
+    }
 };
 
 // NodeMap: `int32_t/size_t index(int a, int I, int J, int K)` of a face node, < 0 off the
-// grid (the dense grid's array bounds, or no fine block holding the face).
-template <class NodeMap>
-__global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiled(PView ps, int64_t n, NodeMap map, P2GArgs A) {
+// grid (the dense grid's array bounds, or no fine block holding the face). `ps` gives the
+// positions (fields 0-2).
+template <class NodeMap, class Source>
+__global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiled(PView ps, int64_t n, NodeMap map, Source src, P2GArgs A) {
     __shared__ unsigned long long smom[P2G_BOX], swt[P2G_BOX];
     __shared__ int red[P2G_THREADS / 64];
     int64_t p = (int64_t)blockIdx.x * P2G_THREADS + threadIdx.x;
@@ -48,7 +105,7 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiled(PView ps, int64_t n, 
     float xp = 0, yp = 0, zp = 0, wT = 0;
     if (live) {
         xp = ps.f[0][p]; yp = ps.f[1][p]; zp = ps.f[2][p];
-        wT = temporal(ps.f[15][p], A.dt);
+        wT = src.weight(p);
     }
     for (int a = 0; a < 3; ++a) {
         float q0 = ispace(xp, A.lo[0], A.dx, a == 0 ? 0.0f : 0.5f);
@@ -58,21 +115,14 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiled(PView ps, int64_t n, 
         // finite, in-range bases only shape the box; others take the direct path
         bool ok = live && bx.b > -(1 << 24) && bx.b < (1 << 24) && by.b > -(1 << 24) && by.b < (1 << 24) &&
                   bz.b > -(1 << 24) && bz.b < (1 << 24) && isfinite(q0) && isfinite(q1) && isfinite(q2);
-        const int BIG = 1 << 30;
-        int x0 = wg_min(ok ? bx.b : BIG, red), y0 = wg_min(ok ? by.b : BIG, red), z0 = wg_min(ok ? bz.b : BIG, red);
-        int x1 = wg_max(ok ? bx.b : -BIG, red), y1 = wg_max(ok ? by.b : -BIG, red), z1 = wg_max(ok ? bz.b : -BIG, red);
-        int ex = x1 - x0 + 3, ey = y1 - y0 + 3, ez = z1 - z0 + 3;
-        bool boxed = x0 != BIG && ex <= P2G_BOX && ey <= P2G_BOX && ez <= P2G_BOX && ex * ey * ez <= P2G_BOX;
+        bool wgbox;
+        TileBox T = tile_box(ok, bx.b, by.b, bz.b, 3, P2G_BOX, red, wgbox);
+        int first = tile_first(wgbox), stride = tile_stride(wgbox);
         float vp = 0, c0 = 0, c1 = 0, c2 = 0;
-        if (live) {
-            vp = ps.f[3 + a][p];
-            c0 = ps.f[6 + 3 * a][p]; c1 = ps.f[7 + 3 * a][p]; c2 = ps.f[8 + 3 * a][p];
-        }
-        if (boxed) {
-            int V = ex * ey * ez;
-            for (int k = threadIdx.x; k < V; k += P2G_THREADS) { smom[k] = 0ull; swt[k] = 0ull; }
-            __syncthreads();
-        }
+        if (live) src.comp(p, a, vp, c0, c1, c2);
+        if (T.use)
+            for (int k = first; k < T.size(); k += stride) { smom[T.base + k] = 0ull; swt[T.base + k] = 0ull; }
+        __syncthreads();
         if (live) {
             for (int mz = 0; mz < 3; ++mz) {
                 int K = bz.b + mz;
@@ -85,37 +135,36 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiled(PView ps, int64_t n, 
                         int I = bx.b + mx;
                         float wgt = wT * pick(bx, mx) * wy * wz;
                         float d0 = ((float)I - q0) * A.dx;
-                        unsigned long long m = tofix(wgt * (vp + (c0 * d0 + c1 * d1 + c2 * d2))), w = tofix(wgt);
-                        if (boxed && ok) {
-                            int l = (I - x0) + ex * ((J - y0) + ey * (K - z0));
+                        unsigned long long m = tofix((wgt * (vp + (c0 * d0 + c1 * d1 + c2 * d2))) * A.scale),
+                                           w = tofix(wgt);
+                        if (T.use && ok) {
+                            int l = T.at(I, J, K);
                             atomicAdd(&smom[l], m);
                             atomicAdd(&swt[l], w);
                         } else {
                             auto idx = map.index(a, I, J, K);
                             if (idx >= 0) {
                                 atomicAdd(&A.mom[a][idx], m);
-                                atomicAdd(&A.wt[a][idx], w);
+                                if (A.wt[a]) atomicAdd(&A.wt[a][idx], w);
                             }
                         }
                     }
                 }
             }
         }
-        if (boxed) {
-            __syncthreads();
-            int V = ex * ey * ez;
-            for (int k = threadIdx.x; k < V; k += P2G_THREADS) {
-                unsigned long long m = smom[k], w = swt[k];
+        __syncthreads();
+        if (T.use)
+            for (int k = first; k < T.size(); k += stride) {
+                unsigned long long m = smom[T.base + k], w = swt[T.base + k];
                 if (m == 0ull && w == 0ull) continue;
-                int I = x0 + k % ex, J = y0 + (k / ex) % ey, K = z0 + k / (ex * ey);
+                int I = T.x0 + k % T.ex, J = T.y0 + (k / T.ex) % T.ey, K = T.z0 + k / (T.ex * T.ey);
                 auto idx = map.index(a, I, J, K);
                 if (idx >= 0) {
                     if (m) atomicAdd(&A.mom[a][idx], m);
-                    if (w) atomicAdd(&A.wt[a][idx], w);
+                    if (w && A.wt[a]) atomicAdd(&A.wt[a][idx], w);
                 }
             }
-            __syncthreads();   // the box is reused by the next component
-        }
+        __syncthreads();   // the boxes are reused by the next component
     }
 }
 

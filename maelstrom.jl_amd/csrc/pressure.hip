@@ -480,24 +480,88 @@ struct DenseSolver {
     }
 };
 
-void DenseBasin::project(float dt, float gamma, const SolveOpts& o, SolveStats& stats) {
-    (void)dt;
-    auto t0 = std::chrono::steady_clock::now();
+// ─── the projection's adjoint (no counterpart in the reference) ───────────────
+// project! maps u to u - G A^-1 (D u + dx^2 gamma) (D: the scaled divergence of k_rhs, G:
+// the face differences of k_project); with the types and the level set fixed its linear
+// part's transpose is ub -> ub + D^T A^-1 (-G^T ub), A being symmetric: the same solver
+// on the adjoint right-hand side.
+
+// b = -G^T ub per cell: each valid face of the cell (k_project's test) contributes its
+// cotangent times -dD/dP_c / dx (LIQUID both sides: +-1; ghost fluid: 1/theta on the
+// LIQUID side).
+__global__ void k_adj_gradT(float* __restrict__ b, const float* __restrict__ u, const float* __restrict__ v,
+                            const float* __restrict__ w, const uint8_t* __restrict__ t, const float* __restrict__ phi, int3 n,
+                            float dx) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= n.x || j >= n.y || k >= n.z) return;
+    size_t c = lin(i, j, k, n.x, n.y);
+    float acc = 0.0f;
+    if (t[c] == LIQUID) {
+        const float* U[3] = {u, v, w};
+        for (int A = 0; A < 3; ++A) {
+            int n1 = n.x + (A == 0), n2 = n.y + (A == 1);
+            int ei = A == 0, ej = A == 1, ek = A == 2;
+            // the low face: this cell is its upper cell (c1)
+            {
+                uint8_t t0 = typeat(t, i - ei, j - ej, k - ek, n.x, n.y, n.z);
+                if (t0 != SOLID) {
+                    float ubf = U[A][lin(i, j, k, n1, n2)], dD;
+                    if (t0 == LIQUID) dD = 1.0f;
+                    else dD = 1.0f / theta(phi[c], phi[lin(i - ei, j - ej, k - ek, n.x, n.y)]);
+                    acc += -ubf * dD / dx;
+                }
+            }
+            // the high face: this cell is its lower cell (c0)
+            {
+                uint8_t t1 = typeat(t, i + ei, j + ej, k + ek, n.x, n.y, n.z);
+                if (t1 != SOLID) {
+                    float ubf = U[A][lin(i + ei, j + ej, k + ek, n1, n2)], dD;
+                    if (t1 == LIQUID) dD = -1.0f;
+                    else dD = -1.0f / theta(phi[c], phi[lin(i + ei, j + ej, k + ek, n.x, n.y)]);
+                    acc += -ubf * dD / dx;
+                }
+            }
+        }
+    }
+    b[c] = acc;
+}
+// ub += D^T lambda per face: the face is the high face of its lower cell (db/du = -dx)
+// and the low face of its upper cell (db/du = +dx), for LIQUID cells (k_rhs).
+template <int A>
+__global__ void k_adj_divT(float* __restrict__ ub, const float* __restrict__ lam, const uint8_t* __restrict__ t, int3 n,
+                           float dx) {
+    int n1 = n.x + (A == 0), n2 = n.y + (A == 1), n3 = n.z + (A == 2);
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= n1 || j >= n2 || k >= n3) return;
+    int a = i - (A == 0), b = j - (A == 1), c = k - (A == 2);
+    uint8_t t0 = typeat(t, a, b, c, n.x, n.y, n.z), t1 = typeat(t, i, j, k, n.x, n.y, n.z);
+    float add = 0.0f;
+    if (t0 == LIQUID) add += -dx * lam[lin(a, b, c, n.x, n.y)];
+    if (t1 == LIQUID) add += dx * lam[lin(i, j, k, n.x, n.y)];
+    ub[lin(i, j, k, n1, n2)] += add;
+}
+
+// flip.jl:1084-1088: coarse types and the fine diagonal of the operator.
+void DenseBasin::operator_setup() {
+    hipStream_t s = st.s;
+    DenseSolver S(*this);
+    int3 n = i3(g.n);
+    for (int l = 1; l < levels; ++l) {
+        int3 nc = i3(lv[l].n), nf = i3(lv[l - 1].n);
+        hipLaunchKernelGGL(k_coarsen, grid3(nc.x, nc.y, nc.z), dim3(BX, BY), 0, s, lv[l].types.p, S.types(l - 1), nc, nf);
+    }
+    hipLaunchKernelGGL(k_diag, grid3(n.x, n.y, n.z), dim3(BX, BY), 0, s, diag.p, types.p, phi.p, n);
+    MAEL_LAUNCH_CHECK();
+}
+
+// flip.jl:1075-1111 pressure!: A P = b from the pressure in P (the caller's start), by
+// MG-preconditioned CG on the FP32 correction with FP64 restarts (SolveOpts::refine).
+void DenseBasin::solve_pressure(const SolveOpts& o, SolveStats& stats) {
     hipStream_t s = st.s;
     DenseSolver S(*this);
     int3 n = i3(g.n);
     dim3 gr = grid3(n.x, n.y, n.z);
     size_t N = cells();
-    if (!o.warm) fill_async(P.p, N, 0.0f, s);
-    hipLaunchKernelGGL(k_rhs, gr, dim3(BX, BY), 0, s, b.p, P.p, u[0].p, u[1].p, u[2].p, types.p, n, g.dx, gamma);
-    // flip.jl:1084-1088: coarse types and the fine diagonal.
-    for (int l = 1; l < levels; ++l) {
-        int3 nc = i3(lv[l].n), nf = i3(lv[l - 1].n);
-        hipLaunchKernelGGL(k_coarsen, grid3(nc.x, nc.y, nc.z), dim3(BX, BY), 0, s, lv[l].types.p, S.types(l - 1), nc, nf);
-    }
-    hipLaunchKernelGGL(k_diag, gr, dim3(BX, BY), 0, s, diag.p, types.p, phi.p, n);
-    MAEL_LAUNCH_CHECK();
-
     // FP64 pressure (warm start from P) and its true residual.
     double* P64 = this->P64.p;
     hipLaunchKernelGGL(k_f2d, dim3(cdiv(N, 256)), dim3(256), 0, s, P64, P.p, N);
@@ -542,7 +606,38 @@ void DenseBasin::project(float dt, float gamma, const SolveOpts& o, SolveStats& 
     }
     stats.iterations = it;
     stats.rel_residual = bnorm == 0.0 ? 0.0 : rnorm / bnorm;
+}
 
+// The transpose of project!'s linear part, in place on ub; P is left holding the adjoint
+// pressure lambda = A^-1 (-G^T ub) (solved from zero).
+void DenseBasin::project_adjoint(const SolveOpts& o, SolveStats& stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    adj_alloc();
+    hipStream_t s = st.s;
+    int3 n = i3(g.n);
+    dim3 gr = grid3(n.x, n.y, n.z);
+    hipLaunchKernelGGL(k_adj_gradT, gr, dim3(BX, BY), 0, s, b.p, ub[0].p, ub[1].p, ub[2].p, types.p, phi.p, n, g.dx);
+    fill_async(P.p, cells(), 0.0f, s);
+    operator_setup();
+    solve_pressure(o, stats);
+    hipLaunchKernelGGL(k_adj_divT<0>, grid3(n.x + 1, n.y, n.z), dim3(BX, BY), 0, s, ub[0].p, P.p, types.p, n, g.dx);
+    hipLaunchKernelGGL(k_adj_divT<1>, grid3(n.x, n.y + 1, n.z), dim3(BX, BY), 0, s, ub[1].p, P.p, types.p, n, g.dx);
+    hipLaunchKernelGGL(k_adj_divT<2>, grid3(n.x, n.y, n.z + 1), dim3(BX, BY), 0, s, ub[2].p, P.p, types.p, n, g.dx);
+    MAEL_LAUNCH_CHECK();
+    st.sync();
+    stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void DenseBasin::project(float dt, float gamma, const SolveOpts& o, SolveStats& stats) {
+    (void)dt;
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = st.s;
+    int3 n = i3(g.n);
+    dim3 gr = grid3(n.x, n.y, n.z);
+    if (!o.warm) fill_async(P.p, cells(), 0.0f, s);
+    hipLaunchKernelGGL(k_rhs, gr, dim3(BX, BY), 0, s, b.p, P.p, u[0].p, u[1].p, u[2].p, types.p, n, g.dx, gamma);
+    operator_setup();
+    solve_pressure(o, stats);
     // flip.jl:1152-1154 the velocity update and the valid-face flags.
     hipLaunchKernelGGL(k_project<0>, grid3(n.x + 1, n.y, n.z), dim3(BX, BY), 0, s, u[0].p, flag[0].p, P.p, types.p, phi.p, n, g.dx);
     hipLaunchKernelGGL(k_project<1>, grid3(n.x, n.y + 1, n.z), dim3(BX, BY), 0, s, u[1].p, flag[1].p, P.p, types.p, phi.p, n, g.dx);

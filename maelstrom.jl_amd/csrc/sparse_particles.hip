@@ -231,8 +231,9 @@ void SparseBasin::p2g(float dt) {
     if (n > 0) {
         PView ps = view(cur);
         kt.begin(K_P2G, s);
-        P2GArgs A{{g.lo[0], g.lo[1], g.lo[2]}, g.dx, dt, {mom[0].p, mom[1].p, mom[2].p}, {wt[0].p, wt[1].p, wt[2].p}};
-        hipLaunchKernelGGL(k_p2g_tiled<SparseNodes>, dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s, ps, n, SparseNodes{t}, A);
+        P2GArgs A{{g.lo[0], g.lo[1], g.lo[2]}, g.dx, {mom[0].p, mom[1].p, mom[2].p}, {wt[0].p, wt[1].p, wt[2].p}, 1.0f};
+        hipLaunchKernelGGL((k_p2g_tiled<SparseNodes, P2GSource>), dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s, ps, n,
+                           SparseNodes{t}, P2GSource{ps, dt}, A);
         kt.end(K_P2G, s);
         for (int a = 0; a < 3; ++a) {
             hipLaunchKernelGGL(k_syncplanes, dim3(cdiv(nslots * 256, 256)), dim3(256), 0, s, mom[a].p, t, a, nslots);

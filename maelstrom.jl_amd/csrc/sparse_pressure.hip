@@ -413,6 +413,7 @@ template <class C> struct CompT {
     const C* dDc;
     const uint8_t* actc;
     const uint8_t* cmask;   // k_covermask
+    const uint8_t* tc;      // the coarse cell types (level 0's matrix-free couplings)
 };
 
 // sflip.jl:1218-1269: mode 0 Jacobi (out = x + w (b - A x)/diag), 1 residual (out = b - A x),
@@ -450,6 +451,75 @@ __global__ __launch_bounds__(256) void k_rowfine(T* __restrict__ out, const T* _
     }
     out[n] = sf<T>(res);
 }
+// The same on level 0, matrix-free: level 0's couplings are 1 between LIQUID fine cells,
+// 0.4 between a LIQUID fine cell and a LIQUID coarse one, else 0 (k_finecoef,
+// sflip.jl:1323-1342), so they are taken from the cell types the row reads at the same
+// places as x, rather than from the coupling arrays: the same values, summed in
+// finerow's order. Every index is computed and every load issued before any is used.
+// Launched with 256 threads a workgroup (one slot's z plane); free slots get 0.
+template <int MODE, class T, class C>
+__global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
+                                                   const T* __restrict__ xc, CompT<C> L, STopo t, int32_t nf) {
+    constexpr bool Z = MODE == 3;
+    int32_t n = blockIdx.x * 256 + threadIdx.x;
+    if ((int32_t)(blockIdx.x * 256) >= nf) return;
+    const int s = (int)((blockIdx.x * 256u) >> 12) + 1, c = n & 4095;
+    const int i = c & 15, j = (c >> 4) & 15, k = c >> 8;
+    const uint8_t* __restrict__ tf = L.actf;   // level 0: the fine cell types
+    int4 w = t.where[s - 1];
+    if (w.x < 0) {
+        out[n] = sf<T>(0.0f);
+        return;
+    }
+    int32_t nb[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) nb[d] = nbslot(t, w, d);
+    uint8_t t0 = tf[n];
+    bool act = MODE == 2 ? t0 == LIQUID : t0 != 0;
+    float res = 0.0f;
+    if (act) {
+        const int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+        const int I = w.x * 16 + i, J = w.y * 16 + j, K = w.z * 16 + k;
+        float xv[6], cf[6];
+        uint8_t ty[6];
+        bool fine[6], cin[6];
+        uint32_t fidx[6];
+        size_t cidx[6];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            int ax = d >> 1, sg = (d & 1) ? 1 : -1;
+            int li = i + (ax == 0 ? sg : 0), lj = j + (ax == 1 ? sg : 0), lk = k + (ax == 2 ? sg : 0);
+            bool inside = li >= 0 && li < 16 && lj >= 0 && lj < 16 && lk >= 0 && lk < 16;
+            int32_t sn = nb[d];
+            fine[d] = inside || sn != 0;
+            uint32_t slot0 = inside ? (uint32_t)(s - 1) : (uint32_t)(sn - 1);
+            fidx[d] = slot0 * 4096u + (uint32_t)((li & 15) + 16 * ((lj & 15) + 16 * (lk & 15)));
+            int ci = (I + (ax == 0 ? sg : 0)) >> 2, cj = (J + (ax == 1 ? sg : 0)) >> 2, ck = (K + (ax == 2 ? sg : 0)) >> 2;
+            cin[d] = ci >= 0 && cj >= 0 && ck >= 0 && ci < n0 && cj < n1 && ck < n2;
+            cidx[d] = cl(min(max(ci, 0), n0 - 1), min(max(cj, 0), n1 - 1), min(max(ck, 0), n2 - 1), n0, n1);
+        }
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            ty[d] = fine[d] ? tf[fidx[d]] : L.tc[cidx[d]];
+            xv[d] = Z ? 0.0f : (fine[d] ? lf(x[fidx[d]]) : lf(xc[cidx[d]]));
+        }
+        float diag = lf(L.dDf[n]), off = 0.0f;
+        float x0 = Z ? 0.0f : lf(x[n]);
+        float bn = MODE == 2 ? 0.0f : lf(b[n]);
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            uint8_t tn = (fine[d] || cin[d]) ? ty[d] : SOLID;
+            cf[d] = (t0 == LIQUID && tn == LIQUID) ? (fine[d] ? FINE_FINE : FINE_COARSE) : 0.0f;
+            if (cf[d] > 0.0f) { diag += cf[d]; off += cf[d] * xv[d]; }
+        }
+        float ax = diag * x0 - off;
+        if (MODE == 0 || MODE == 3) res = diag > 0.0f ? x0 + __fdiv_rn(OMEGA * (bn - ax), diag) : 0.0f;
+        else if (MODE == 1) res = bn - ax;
+        else res = ax;
+    }
+    out[n] = sf<T>(res);
+}
+
 template <int N, int MODE, class T, class C>
 __global__ __launch_bounds__(256) void k_rowcoarse(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
                                                    const T* __restrict__ xf, CompT<C> L, STopo t) {
@@ -670,7 +740,7 @@ enum { SS_BB = 0, SS_RR, SS_RHO, SS_DEN, SS_ALPHA, SS_RHONEW, SS_ZROLD, SS_BETA,
 enum { EF_RHO_NONFINITE = 1, EF_RHO_NONPOS = 2, EF_DEN_NONPOS = 4, EF_ALPHA_NONFINITE = 8, EF_RHONEW_NONPOS = 16,
        EF_BETA_NONFINITE = 32 };
 
-constexpr int DOT_BLOCKS = 1024;
+constexpr int DOT_BLOCKS = 2048;   // partials: 2 x DOT_BLOCKS doubles at most
 
 __device__ inline double block_sum_red(double v, double* lds) {
 #pragma unroll
@@ -684,22 +754,51 @@ __device__ inline double block_sum_red(double v, double* lds) {
     return s;
 }
 
-// sflip.jl:1772-1805 inner: LIQUID fine cells of live slots, LIQUID uncovered coarse cells.
+// The vector kernels run over the concatenated (fine pool, coarse grid) range in groups of
+// four consecutive cells (float4 / uchar4 loads): the fine part is whole slots and the
+// coarse grid's sides are multiples of 4, so a group never straddles the two parts or two
+// slots. Group g's mask of unknowns (sflip.jl:1772-1805: LIQUID fine cells of live slots,
+// LIQUID uncovered coarse cells), one bit per cell.
+__device__ inline unsigned unknown4(size_t g, size_t nf4, const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc,
+                                    const uint8_t* __restrict__ cm, const STopo& t) {
+    if (g < nf4) {
+        if (t.where[g >> 10].x < 0) return 0u;
+        uchar4 ty = reinterpret_cast<const uchar4*>(tf)[g];
+        return (ty.x == LIQUID) | ((ty.y == LIQUID) << 1) | ((ty.z == LIQUID) << 2) | ((ty.w == LIQUID) << 3);
+    }
+    size_t h = g - nf4;
+    uchar4 ty = reinterpret_cast<const uchar4*>(tc)[h], c = reinterpret_cast<const uchar4*>(cm)[h];
+    return (ty.x == LIQUID && !(c.x & 1)) | ((ty.y == LIQUID && !(c.y & 1)) << 1) | ((ty.z == LIQUID && !(c.z & 1)) << 2) |
+           ((ty.w == LIQUID && !(c.w & 1)) << 3);
+}
+// Group g of a pressure-sized vector (fine part, coarse part).
+__device__ inline float4& grp(float* f, float* c, size_t g, size_t nf4) {
+    return g < nf4 ? reinterpret_cast<float4*>(f)[g] : reinterpret_cast<float4*>(c)[g - nf4];
+}
+__device__ inline float4 grp(const float* f, const float* c, size_t g, size_t nf4) {
+    return g < nf4 ? reinterpret_cast<const float4*>(f)[g] : reinterpret_cast<const float4*>(c)[g - nf4];
+}
+__device__ inline double dot4(unsigned m, float4 a, float4 b) {
+    double v = 0;
+    if (m & 1) v += (double)a.x * (double)b.x;
+    if (m & 2) v += (double)a.y * (double)b.y;
+    if (m & 4) v += (double)a.z * (double)b.z;
+    if (m & 8) v += (double)a.w * (double)b.w;
+    return v;
+}
+
+// sflip.jl:1772-1805 inner: the sum over the unknowns, per workgroup.
 __global__ __launch_bounds__(256) void k_sdot(double* __restrict__ partials, const float* __restrict__ af,
                                               const float* __restrict__ bf, const float* __restrict__ ac,
                                               const float* __restrict__ bc, const uint8_t* __restrict__ tf,
                                               const uint8_t* __restrict__ tc, const uint8_t* __restrict__ cm, STopo t,
                                               int32_t nf) {
     __shared__ double lds[4];
-    size_t Cc = (size_t)t.g.nc(0) * t.g.nc(1) * t.g.nc(2);
+    size_t nf4 = (size_t)nf / 4, ng = nf4 + (size_t)t.g.nc(0) * t.g.nc(1) * t.g.nc(2) / 4;
     double v = 0;
-    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < (size_t)nf + Cc; l += (size_t)gridDim.x * blockDim.x) {
-        if (l < (size_t)nf) {
-            if (tf[l] == LIQUID && t.where[l >> 12].x >= 0) v += (double)af[l] * (double)bf[l];
-        } else {
-            size_t m = l - nf;
-            if (tc[m] == LIQUID && !(cm[m] & 1)) v += (double)ac[m] * (double)bc[m];
-        }
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += (size_t)gridDim.x * blockDim.x) {
+        unsigned m = unknown4(g, nf4, tf, tc, cm, t);
+        if (m) v += dot4(m, grp(af, ac, g, nf4), grp(bf, bc, g, nf4));
     }
     double s = block_sum_red(v, lds);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
@@ -773,22 +872,20 @@ __global__ __launch_bounds__(256) void k_fcg_update(double* __restrict__ partial
                                                     const double* __restrict__ scal, int aslot) {
     __shared__ double lds[4];
     float al = (float)scal[aslot];
-    size_t Cc = (size_t)t.g.nc(0) * t.g.nc(1) * t.g.nc(2);
+    size_t nf4 = (size_t)nf / 4, ng = nf4 + (size_t)t.g.nc(0) * t.g.nc(1) * t.g.nc(2) / 4;
     double v = 0;
-    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < (size_t)nf + Cc; l += (size_t)gridDim.x * blockDim.x) {
-        bool fine = l < (size_t)nf;
-        size_t m = fine ? l : l - nf;
-        float* P = fine ? Pf : Pc;
-        float* r = fine ? rf : rc;
-        float* ro = fine ? roldf : roldc;
-        const float* d = fine ? df : dc;
-        const float* q = fine ? qf : qc;
-        P[m] = P[m] + al * d[m];
-        float rv = r[m];
-        ro[m] = rv;
-        rv = rv + -al * q[m];
-        r[m] = rv;
-        if (unknown(l, nf, tf, tc, cm, t)) v += (double)rv * (double)rv;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += (size_t)gridDim.x * blockDim.x) {
+        float4 d = grp(df, dc, g, nf4), q = grp(qf, qc, g, nf4);
+        float4& P = grp(Pf, Pc, g, nf4);
+        float4& r = grp(rf, rc, g, nf4);
+        float4 p = P, rv = r;
+        p.x = p.x + al * d.x; p.y = p.y + al * d.y; p.z = p.z + al * d.z; p.w = p.w + al * d.w;
+        P = p;
+        grp(roldf, roldc, g, nf4) = rv;
+        rv.x = rv.x + -al * q.x; rv.y = rv.y + -al * q.y; rv.z = rv.z + -al * q.z; rv.w = rv.w + -al * q.w;
+        r = rv;
+        unsigned m = unknown4(g, nf4, tf, tc, cm, t);
+        if (m) v += dot4(m, rv, rv);
     }
     double s = block_sum_red(v, lds);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
@@ -801,15 +898,14 @@ __global__ __launch_bounds__(256) void k_sdot2(double* __restrict__ partials, co
                                                const uint8_t* __restrict__ tc, const uint8_t* __restrict__ cm, STopo t,
                                                int32_t nf) {
     __shared__ double lds[4];
-    size_t Cc = (size_t)t.g.nc(0) * t.g.nc(1) * t.g.nc(2);
+    size_t nf4 = (size_t)nf / 4, ng = nf4 + (size_t)t.g.nc(0) * t.g.nc(1) * t.g.nc(2) / 4;
     double v1 = 0, v2 = 0;
-    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < (size_t)nf + Cc; l += (size_t)gridDim.x * blockDim.x) {
-        if (!unknown(l, nf, tf, tc, cm, t)) continue;
-        bool fine = l < (size_t)nf;
-        size_t m = fine ? l : l - nf;
-        double zv = fine ? zf[m] : zc[m];
-        v1 += zv * (double)(fine ? rf[m] : rc[m]);
-        v2 += zv * (double)(fine ? of[m] : oc[m]);
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += (size_t)gridDim.x * blockDim.x) {
+        unsigned m = unknown4(g, nf4, tf, tc, cm, t);
+        if (!m) continue;
+        float4 z = grp(zf, zc, g, nf4);
+        v1 += dot4(m, z, grp(rf, rc, g, nf4));
+        v2 += dot4(m, z, grp(of, oc, g, nf4));
     }
     double s1 = block_sum_red(v1, lds);
     __syncthreads();
@@ -842,11 +938,14 @@ template <int XPBY>
 __global__ void k_saxpy2(float* __restrict__ yf, const float* __restrict__ xf, size_t nf, float* __restrict__ yc,
                          const float* __restrict__ xc, size_t nc, const double* __restrict__ scal, int slot, float mult) {
     float c = slot >= 0 ? mult * (float)scal[slot] : mult;
-    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < nf + nc; l += (size_t)gridDim.x * blockDim.x) {
-        float* y = l < nf ? yf : yc;
-        const float* x = l < nf ? xf : xc;
-        size_t m = l < nf ? l : l - nf;
-        y[m] = XPBY ? x[m] + c * y[m] : y[m] + c * x[m];
+    size_t nf4 = nf / 4, ng = nf4 + nc / 4;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += (size_t)gridDim.x * blockDim.x) {
+        float4 x = grp(xf, xc, g, nf4);
+        float4& yr = grp(yf, yc, g, nf4);
+        float4 y = yr;
+        if (XPBY) { y.x = x.x + c * y.x; y.y = x.y + c * y.y; y.z = x.z + c * y.z; y.w = x.w + c * y.w; }
+        else { y.x = y.x + c * x.x; y.y = y.y + c * x.y; y.z = y.z + c * x.z; y.w = y.w + c * x.w; }
+        yr = y;
     }
 }
 
@@ -1046,9 +1145,9 @@ struct SparseSolver {
             H.D.push_back(DViewT<float>{i3(L.n), {L.c[0].p, L.c[1].p, L.c[2].p}, L.dD.p, L.act.p});
         }
         H.L0 = CompT<float>{{{s.cf[0].p, s.cf[1].p, s.cf[2].p}}, s.dD.fine.p, s.types.fine.p,
-                            {{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p, s.cmask.p};
+                            {{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p, s.cmask.p, s.types.coarse.p};
         H.L1 = CompT<float>{{{s.c1[0].p, s.c1[1].p, s.c1[2].p}}, s.dD1.p, s.act1.p,
-                            {{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p, s.cmask.p};
+                            {{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p, s.cmask.p, s.types.coarse.p};
         return H;
     }
     // The reduced-precision hierarchy (storage in s.mp): vectors and couplings, the dense
@@ -1067,9 +1166,9 @@ struct SparseSolver {
                                     s.D[l].act.p});
         }
         H.L0 = CompT<T>{{{(T*)m.cf[0].p, (T*)m.cf[1].p, (T*)m.cf[2].p}}, (T*)m.dDf.p, s.types.fine.p,
-                        {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p, s.cmask.p};
+                        {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p, s.cmask.p, s.types.coarse.p};
         H.L1 = CompT<T>{{{(T*)m.c1[0].p, (T*)m.c1[1].p, (T*)m.c1[2].p}}, (T*)m.dD1.p, s.act1.p,
-                        {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p, s.cmask.p};
+                        {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p, s.cmask.p, s.types.coarse.p};
         H.rscale = 0.25f;
         return H;
     }
@@ -1111,8 +1210,12 @@ struct SparseSolver {
         int32_t nfl = s.nslots * N * N * N;
         if (nfl) {
             if (N == 16 && MODE == 0) s.kt.begin(K_RBGS, st);
-            hipLaunchKernelGGL((k_rowfine<N, MODE, T, C>), dim3(cdiv(nfl, 256)), dim3(256), 0, st, out.f, x.f, b.f, x.c, L, t,
-                               nfl);
+            if constexpr (N == 16)
+                hipLaunchKernelGGL((k_rowfine16<MODE, T, C>), dim3(cdiv(nfl, 256)), dim3(256), 0, st, out.f, x.f, b.f, x.c, L, t,
+                                   nfl);
+            else
+                hipLaunchKernelGGL((k_rowfine<N, MODE, T, C>), dim3(cdiv(nfl, 256)), dim3(256), 0, st, out.f, x.f, b.f, x.c, L,
+                                   t, nfl);
             if (N == 16 && MODE == 0) s.kt.end(K_RBGS, st);
         }
         hipLaunchKernelGGL((k_rowcoarse<N, MODE, T, C>), dim3(cdiv(Cc, 256)), dim3(256), 0, st, out.c, x.c, b.c, x.f, L, t);

@@ -9,6 +9,8 @@
 #include "dense.hpp"
 #include "scan.hpp"
 #include "p2g_tiled.hpp"
+#include <cmath>
+#include <cstring>
 
 namespace mael {
 
@@ -79,8 +81,9 @@ void DenseBasin::p2g(float dt) {
         PView ps = view(cur);
         kt.begin(K_P2G, s);
         DenseNodes map{{g.n[0], g.n[1], g.n[2]}};
-        P2GArgs A{{g.lo[0], g.lo[1], g.lo[2]}, g.dx, dt, {mom[0].p, mom[1].p, mom[2].p}, {wt[0].p, wt[1].p, wt[2].p}};
-        hipLaunchKernelGGL(k_p2g_tiled<DenseNodes>, dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s, ps, n, map, A);
+        P2GArgs A{{g.lo[0], g.lo[1], g.lo[2]}, g.dx, {mom[0].p, mom[1].p, mom[2].p}, {wt[0].p, wt[1].p, wt[2].p}, 1.0f};
+        hipLaunchKernelGGL((k_p2g_tiled<DenseNodes, P2GSource>), dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s, ps, n,
+                           map, P2GSource{ps, dt}, A);
         kt.end(K_P2G, s);
         kt.begin(K_SPLAT, s);
         hipLaunchKernelGGL(k_splat_tiled<DenseCells>, dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s, ps, n,
@@ -167,6 +170,234 @@ void DenseBasin::g2p(float dt, float grav) {
     hipLaunchKernelGGL(k_dense_g2p, dim3(cdiv(n, 256)), dim3(256), 0, st.s, view(cur), n, g, u[0].p, u[1].p, u[2].p,
                        flag[0].p, flag[1].p, flag[2].p, dt, grav);
     kt.end(K_G2P, st.s);
+    MAEL_LAUNCH_CHECK();
+}
+
+}  // namespace mael
+
+namespace mael {
+
+// ─── adjoints of the transfers (no counterpart in the reference) ─────────────
+// With the particles' positions and time lags and the level set fixed, P2G is linear in
+// the particles' velocities and APIC rows (and in u0), and G2P is linear in the face
+// velocities; these are their transposes, for reverse-mode gradients. The G2P adjoint is
+// a deposit (the tiled fixed-point scatter, so it is deterministic), the P2G adjoint a
+// gather.
+
+// The gather's face nodes: indices clamped to the array (flip.jl:1210-1224).
+struct DenseClampNodes {
+    int n[3];
+    __device__ int64_t index(int a, int I, int J, int K) const {
+        int n1 = n[0] + (a == 0), n2 = n[1] + (a == 1), n3 = n[2] + (a == 2);
+        I = min(max(I, 0), n1 - 1); J = min(max(J, 0), n2 - 1); K = min(max(K, 0), n3 - 1);
+        return (int64_t)I + (int64_t)n1 * ((int64_t)J + (int64_t)n2 * K);
+    }
+};
+
+// The G2P adjoint's particle side: the output cotangents, the rows times the gather's
+// 4 / dx^2, for the particles that took the grid's velocity (the others deposit 0).
+struct G2PAdjSource {
+    const float* pb[12];
+    const uint8_t* reached;
+    float cs;
+    __device__ float weight(int64_t p) const { return reached[p] ? 1.0f : 0.0f; }
+    __device__ void comp(int64_t p, int a, float& v, float& c0, float& c1, float& c2) const {
+        if (!reached[p]) { v = c0 = c1 = c2 = 0.0f; return; }
+        v = pb[a][p];
+        c0 = pb[3 + 3 * a][p] * cs; c1 = pb[4 + 3 * a][p] * cs; c2 = pb[5 + 3 * a][p] * cs;
+    }
+};
+struct PB12 { float* f[12]; };
+
+// Which particles took the grid's velocity in G2P (the flags' gathers, as k_dense_g2p),
+// and the largest contribution one of them makes to a node, as float bits (for the scale).
+__global__ __launch_bounds__(256) void k_g2p_adj_reached(PView ps, int64_t n, DGrid g, const uint8_t* __restrict__ fu,
+                                                         const uint8_t* __restrict__ fv, const uint8_t* __restrict__ fw,
+                                                         uint8_t* __restrict__ reached, PB12 pb, int* __restrict__ bound) {
+    __shared__ int red[4];
+    int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    float m = 0.0f;
+    if (p < n) {
+        float xp = ps.f[0][p], yp = ps.f[1][p], zp = ps.f[2][p];
+        int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
+        bool k = gather_value(fu, n1 + 1, n2, n3, g, 0, xp, yp, zp) > 0.999f &&
+                 gather_value(fv, n1, n2 + 1, n3, g, 1, xp, yp, zp) > 0.999f &&
+                 gather_value(fw, n1, n2, n3 + 1, g, 2, xp, yp, zp) > 0.999f;
+        reached[p] = k ? 1 : 0;
+        if (k) {
+            float cd = 6.0f / g.dx;   // |row . d| <= (4 / dx^2)(1.5 dx) sum |row|
+            for (int a = 0; a < 3; ++a) {
+                float c = fabsf(pb.f[a][p]) + cd * (fabsf(pb.f[3 + 3 * a][p]) + fabsf(pb.f[4 + 3 * a][p]) + fabsf(pb.f[5 + 3 * a][p]));
+                m = fmaxf(m, c);
+            }
+            if (!(m <= 3.0e38f)) m = 3.0e38f;
+        }
+    }
+    int v = __float_as_int(m);   // non-negative: integer order is float order
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(bound, max(max(red[0], red[1]), max(red[2], red[3])));
+}
+// u-bar = the accumulated deposit / scale.
+__global__ void k_adj_fromfix(float* __restrict__ ub, const unsigned long long* __restrict__ acc, size_t n, double inv) {
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) ub[l] = (float)((double)(long long)acc[l] * FIX_INV * inv);
+}
+// The particles' input cotangents: v passes through G2P where the grid was not reached
+// (flip.jl:1239-1242), the rows never do.
+__global__ void k_g2p_adj_post(PB12 pb, const uint8_t* __restrict__ reached, int64_t n) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    bool k = reached[p] != 0;
+    for (int f = 0; f < 12; ++f)
+        if (k || f >= 3) pb.f[f][p] = 0.0f;
+}
+
+void DenseBasin::adj_alloc() {
+    if (ub[0].p) return;
+    hipStream_t s = st.s;
+    for (int a = 0; a < 3; ++a) {
+        ub[a].alloc(faces(a)); ub0[a].alloc(faces(a)); adjacc[a].alloc(faces(a));
+        fill_async(ub[a].p, faces(a), 0.0f, s);
+        fill_async(ub0[a].p, faces(a), 0.0f, s);
+    }
+    for (int f = 0; f < 12; ++f) {
+        pb[f].alloc(capacity);
+        fill_async(pb[f].p, (size_t)capacity, 0.0f, s);
+    }
+    reached.alloc(capacity);
+    MAEL_LAUNCH_CHECK();
+}
+
+// The transpose of grid2particles! (flip.jl:1226-1252) in u: ub = G2P^T (pb); pb becomes
+// the cotangent of the input particle velocities.
+void DenseBasin::g2p_adjoint(float dt) {
+    (void)dt;
+    adj_alloc();
+    hipStream_t s = st.s;
+    for (int a = 0; a < 3; ++a) fill_async(adjacc[a].p, faces(a), 0ull, s);
+    if (n == 0) {
+        for (int a = 0; a < 3; ++a) fill_async(ub[a].p, faces(a), 0.0f, s);
+        return;
+    }
+    PB12 P;
+    for (int f = 0; f < 12; ++f) P.f[f] = pb[f].p;
+    int* bound = (int*)partials.p;
+    fill_async(bound, (size_t)1, 0, s);
+    hipLaunchKernelGGL(k_g2p_adj_reached, dim3(cdiv(n, 256)), dim3(256), 0, s, view(cur), n, g, flag[0].p, flag[1].p,
+                       flag[2].p, reached.p, P, bound);
+    int bits = 0;
+    MAEL_HIP(hipMemcpyAsync(st.pinned, bound, sizeof(int), hipMemcpyDeviceToHost, s));
+    st.sync();
+    std::memcpy(&bits, st.pinned, sizeof(int));
+    float m;
+    std::memcpy(&m, &bits, sizeof(float));
+    // a node takes at most 27 cells' particles (2 nmin each): keep the sum within 2^30
+    double lim = std::ldexp(1.0, 30) / (27.0 * 2.0 * std::max(nmin, 1)), scale = 1.0;
+    if (m > 0.0f) scale = std::ldexp(1.0, std::min(60, std::max(-60, (int)std::floor(std::log2(lim / (double)m)))));
+    G2PAdjSource src{{P.f[0], P.f[1], P.f[2], P.f[3], P.f[4], P.f[5], P.f[6], P.f[7], P.f[8], P.f[9], P.f[10], P.f[11]},
+                     reached.p, 4.0f / (g.dx * g.dx)};
+    P2GArgs A{{g.lo[0], g.lo[1], g.lo[2]}, g.dx, {adjacc[0].p, adjacc[1].p, adjacc[2].p}, {nullptr, nullptr, nullptr},
+              (float)scale};
+    hipLaunchKernelGGL((k_p2g_tiled<DenseClampNodes, G2PAdjSource>), dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s,
+                       view(cur), n, DenseClampNodes{{g.n[0], g.n[1], g.n[2]}}, src, A);
+    for (int a = 0; a < 3; ++a)
+        hipLaunchKernelGGL(k_adj_fromfix, dim3(cdiv(faces(a), 256)), dim3(256), 0, s, ub[a].p, adjacc[a].p, faces(a),
+                           1.0 / scale);
+    hipLaunchKernelGGL(k_g2p_adj_post, dim3(cdiv(n, 256)), dim3(256), 0, s, P, reached.p, n);
+    MAEL_LAUNCH_CHECK();
+}
+
+// The P2G adjoint's face part: the cotangent of a face's momentum sum (s = ub / max(nmin,
+// W) in the band, else 0) and of u0 (flip.jl:567-585 combine, differentiated).
+template <int A>
+__global__ __launch_bounds__(256) void k_p2g_adj_faces(float* __restrict__ sface, float* __restrict__ ub0,
+                                                       const float* __restrict__ ub, const unsigned long long* __restrict__ wt,
+                                                       const float* __restrict__ phi, int3 n, float r, float full) {
+    int n1 = n.x + (A == 0), n2 = n.y + (A == 1), n3 = n.z + (A == 2);
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= n1 || j >= n2 || k >= n3) return;
+    int bi = i - (A == 0), bj = j - (A == 1), bk = k - (A == 2);
+    auto at = [&](int a, int b, int c) { return phi[(size_t)a + (size_t)n.x * ((size_t)b + (size_t)n.y * c)]; };
+    bool inb = bi >= 0 && bj >= 0 && bk >= 0;
+    bool ina = i < n.x && j < n.y && k < n.z;
+    float fa = inb ? at(bi, bj, bk) : at(min(i, n.x - 1), min(j, n.y - 1), min(k, n.z - 1));
+    float fb = ina ? at(i, j, k) : fa;
+    float phif = 0.5f * (fa + fb);
+    size_t idx = (size_t)i + (size_t)n1 * ((size_t)j + (size_t)n2 * k);
+    float u = ub[idx];
+    if (phif >= -r) {
+        float W = fromfix(wt[idx]), den = jmax(full, W);
+        sface[idx] = u / den;
+        ub0[idx] = u * (jmax(full - W, 0.0f) / den);
+    } else {
+        sface[idx] = 0.0f;
+        ub0[idx] = u;
+    }
+}
+// The P2G adjoint's particle part: each particle gathers the momentum cotangents of the
+// nodes it deposited to, with the deposit's weights and offsets.
+__global__ __launch_bounds__(256) void k_p2g_adj_gather(PView ps, int64_t n, DGrid g, float dt, const float* __restrict__ s0,
+                                                        const float* __restrict__ s1, const float* __restrict__ s2, PB12 pb) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float xp = ps.f[0][p], yp = ps.f[1][p], zp = ps.f[2][p];
+    float wT = temporal(ps.f[15][p], dt);
+    const float* S[3] = {s0, s1, s2};
+    DenseNodes map{{g.n[0], g.n[1], g.n[2]}};
+    for (int a = 0; a < 3; ++a) {
+        float q0 = ispace(xp, g.lo[0], g.dx, a == 0 ? 0.0f : 0.5f);
+        float q1 = ispace(yp, g.lo[1], g.dx, a == 1 ? 0.0f : 0.5f);
+        float q2 = ispace(zp, g.lo[2], g.dx, a == 2 ? 0.0f : 0.5f);
+        BSpline bx = bspline(q0), by = bspline(q1), bz = bspline(q2);
+        float v = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+        for (int mz = 0; mz < 3; ++mz) {
+            int K = bz.b + mz;
+            float wz = pick(bz, mz), d2 = ((float)K - q2) * g.dx;
+            for (int my = 0; my < 3; ++my) {
+                int J = by.b + my;
+                float wy = pick(by, my), d1 = ((float)J - q1) * g.dx;
+                for (int mx = 0; mx < 3; ++mx) {
+                    int I = bx.b + mx;
+                    int64_t idx = map.index(a, I, J, K);
+                    if (idx < 0) continue;
+                    float t = wT * pick(bx, mx) * wy * wz * S[a][idx];
+                    v += t;
+                    c0 += t * (((float)I - q0) * g.dx);
+                    c1 += t * d1;
+                    c2 += t * d2;
+                }
+            }
+        }
+        pb.f[a][p] += v;
+        pb.f[3 + 3 * a][p] += c0; pb.f[4 + 3 * a][p] += c1; pb.f[5 + 3 * a][p] += c2;
+    }
+}
+
+// The transpose of particles2grid! (flip.jl:587-606) in the particles' v and APIC rows and
+// in u0, at the positions, lags, level set and weights of the last deposit: pb += P2G^T
+// (ub) for the particles (added to what G2P's adjoint passed through), ub0 = its part for
+// u0.
+void DenseBasin::p2g_adjoint(float dt) {
+    adj_alloc();
+    hipStream_t s = st.s;
+    float r = (float)(band - 1) * g.dx, full = (float)nmin;
+    int3 nn = make_int3(g.n[0], g.n[1], g.n[2]);
+    dim3 blk(64, 4, 1);
+    float* sf[3];
+    for (int a = 0; a < 3; ++a) sf[a] = reinterpret_cast<float*>(adjacc[a].p);   // scratch, faces(a) floats
+    hipLaunchKernelGGL(k_p2g_adj_faces<0>, dim3(cdiv(g.n[0] + 1, 64), cdiv(g.n[1], 4), g.n[2]), blk, 0, s, sf[0], ub0[0].p,
+                       ub[0].p, wt[0].p, phi.p, nn, r, full);
+    hipLaunchKernelGGL(k_p2g_adj_faces<1>, dim3(cdiv(g.n[0], 64), cdiv(g.n[1] + 1, 4), g.n[2]), blk, 0, s, sf[1], ub0[1].p,
+                       ub[1].p, wt[1].p, phi.p, nn, r, full);
+    hipLaunchKernelGGL(k_p2g_adj_faces<2>, dim3(cdiv(g.n[0], 64), cdiv(g.n[1], 4), g.n[2] + 1), blk, 0, s, sf[2], ub0[2].p,
+                       ub[2].p, wt[2].p, phi.p, nn, r, full);
+    if (n > 0) {
+        PB12 P;
+        for (int f = 0; f < 12; ++f) P.f[f] = pb[f].p;
+        hipLaunchKernelGGL(k_p2g_adj_gather, dim3(cdiv(n, 256)), dim3(256), 0, s, view(cur), n, g, dt, sf[0], sf[1], sf[2], P);
+    }
     MAEL_LAUNCH_CHECK();
 }
 

@@ -17,7 +17,9 @@ OK, E_ARG, E_CAPACITY, E_BLOCKS, E_NONFINITE, E_CG_INDEFINITE, E_NOCONVERGE, E_B
 ARRAYS = ["U", "V", "W", "U0", "V0", "W0", "PHI", "PHI0", "TYPES", "P", "FU", "FV", "FW", "RHS",
           "PX", "PY", "PZ", "VX", "VY", "VZ", "C0", "C1", "C2", "C3", "C4", "C5", "C6", "C7", "C8", "LAG",
           "KEY", "RANK", "COUNTS", "TOTALS", "OFFSETS",
-          "MOM_U", "MOM_V", "MOM_W", "WT_U", "WT_V", "WT_W"]
+          "MOM_U", "MOM_V", "MOM_W", "WT_U", "WT_V", "WT_W",
+          "UB_U", "UB_V", "UB_W", "UB0_U", "UB0_V", "UB0_W",
+          "PB_VX", "PB_VY", "PB_VZ"] + ["PB_C%d" % k for k in range(9)]
 ARRAY_ID = {n: i for i, n in enumerate(ARRAYS)}
 PARTICLE_FIELDS = ARRAYS[ARRAY_ID["PX"]:ARRAY_ID["LAG"] + 1]
 
@@ -116,6 +118,9 @@ def lib():
         "mael_dense_p2g": (C.c_int, [p, f32]),
         "mael_dense_g2p": (C.c_int, [p, f32, f32]),
         "mael_dense_project": (C.c_int, [p, f32, f32, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+        "mael_dense_g2p_adjoint": (C.c_int, [p, f32]),
+        "mael_dense_project_adjoint": (C.c_int, [p, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+        "mael_dense_p2g_adjoint": (C.c_int, [p, f32]),
         "mael_dense_classify": (C.c_int, [p]),
         "mael_dense_volume": (C.c_int, [p, C.POINTER(f64)]),
         "mael_dense_redistance": (C.c_int, [p, i32]),

@@ -57,11 +57,11 @@ class FlipBasin:
     def shape(self, name):
         """numpy shape (z, y, x) of a grid array, or (capacity,) of a particle array."""
         n1, n2, n3 = self.n_cells
-        if name in ("U", "U0", "FU", "MOM_U", "WT_U"):
+        if name in ("U", "U0", "FU", "MOM_U", "WT_U", "UB_U", "UB0_U"):
             return (n3, n2, n1 + 1)
-        if name in ("V", "V0", "FV", "MOM_V", "WT_V"):
+        if name in ("V", "V0", "FV", "MOM_V", "WT_V", "UB_V", "UB0_V"):
             return (n3, n2 + 1, n1)
-        if name in ("W", "W0", "FW", "MOM_W", "WT_W"):
+        if name in ("W", "W0", "FW", "MOM_W", "WT_W", "UB_W", "UB0_W"):
             return (n3 + 1, n2, n1)
         if name in ("PHI", "PHI0", "TYPES", "P", "RHS", "COUNTS", "TOTALS", "OFFSETS"):
             return (n3, n2, n1)
@@ -76,7 +76,7 @@ class FlipBasin:
     def get(self, name, count=None):
         """Download array `name` (`Array(s.x)`); particle arrays are cut to `count` (default n)."""
         _, cnt, dt = self._info(name)
-        if name in PARTICLE_FIELDS or name in ("KEY", "RANK"):
+        if name in PARTICLE_FIELDS or name in ("KEY", "RANK") or name.startswith("PB_"):
             cnt = self.n if count is None else count
             out = np.empty(cnt, dtype=dt)
         else:
@@ -182,6 +182,36 @@ def project(s, dt, recover=0.5, gamma=None, **solve):
     s.iterations.append(st.iterations)
     s.last_solve = st
     return st
+
+
+# ── adjoints (no counterpart in the reference, which has no AD) ──────────────
+# The transposes of the hot path's linear maps at the state the forward phase left, for
+# reverse-mode gradients: cotangents of the face velocities in UB_U/V/W (of u0 in
+# UB0_U/V/W) and of the particles' v and APIC rows in PB_VX ... PB_C8.
+
+def grid2particles_adjoint(s, dt):
+    """Transpose of `grid2particles!` in u: UB = G2P^T(PB); PB becomes the cotangent of
+    G2P's input particles (v passes through where the grid was not reached)."""
+    s.ctx.check(_lib.lib().mael_dense_g2p_adjoint(s.h, float(dt)))
+    return s
+
+
+def project_adjoint(s, **solve):
+    """Transpose of `project!`'s linear part, in place on UB (a transposed MG-PCG solve;
+    P is left holding the adjoint pressure). Returns the solve's stats."""
+    solve.setdefault("warm", 0)
+    opts = solve_options(**solve)
+    st = SolveStats()
+    s.ctx.check(_lib.lib().mael_dense_project_adjoint(s.h, C.byref(opts), C.byref(st)))
+    return st
+
+
+def particles2grid_adjoint(s, dt):
+    """Transpose of `particles2grid!` in the particles' v and APIC rows (PB += P2G^T(UB),
+    added to what `grid2particles_adjoint` passed through) and in u0 (UB0 =), at the
+    positions, lags, level set and weights of the last deposit."""
+    s.ctx.check(_lib.lib().mael_dense_p2g_adjoint(s.h, float(dt)))
+    return s
 
 
 def classify(s):

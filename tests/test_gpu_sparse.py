@@ -184,16 +184,42 @@ def test_p2g_parity(M):
                                    atol=1e-5 * max(1.0, np.abs(exp).max()))
 
 
+def _orders(n, seed=5):
+    """Particle orders that drive the tiled transfers down each of their paths: cell order
+    (a box per workgroup), 128-particle runs from the two halves interleaved (every
+    workgroup straddles two regions, so a box per wavefront), and a random order (mostly
+    the direct path)."""
+    half = n // 2 // 128 * 128
+    runs = []
+    for k in range(0, half, 128):
+        runs += [np.arange(k, k + 128), np.arange(half + k, half + k + 128)]
+    inter = np.concatenate(runs + [np.arange(2 * half, n)])
+    rnd = np.random.default_rng(seed).permutation(n)
+    return {"sorted": np.arange(n), "interleaved": inter, "random": rnd}
+
+
+def _permute(M, s, o, order):
+    for nm in PNAMES:
+        s.set(nm.upper(), o.array(nm)[:o.n][order])
+
+
 def test_p2g_deterministic(M):
+    """The deposit and the splat give the same bits whatever the particle order (every
+    path of the tiled kernels), and on repeated runs."""
+    print("case: sparse p2g orders", flush=True)
     o = _scene()
-    s = _upload(M, o)
-    M.sparse.particles2grid(s, 0.1)
-    a = [s.get(x) for x in ("U_F", "V_F", "W_F", "PHI_F")]
-    s2 = _upload(M, o)
-    M.sparse.particles2grid(s2, 0.1)
-    b = [s2.get(x) for x in ("U_F", "V_F", "W_F", "PHI_F")]
-    for x, y in zip(a, b):
-        np.testing.assert_array_equal(x, y)
+    ref = None
+    for name, order in _orders(o.n).items():
+        for rep in range(2 if name == "sorted" else 1):
+            s = _upload(M, o)
+            _permute(M, s, o, order)
+            M.sparse.particles2grid(s, 0.1)
+            got = [s.get(x) for x in ("U_F", "V_F", "W_F", "PHI_F")]
+            if ref is None:
+                ref = got
+                continue
+            for nm, x, y in zip(("U_F", "V_F", "W_F", "PHI_F"), ref, got):
+                np.testing.assert_array_equal(x, y, err_msg="%s order %s" % (nm, name))
 
 
 def test_g2p_bitexact(M):
@@ -207,10 +233,20 @@ def test_g2p_bitexact(M):
     fw[rng.random(fw.size) < 0.01] = 0
     s = _upload(M, o)
     dt = np.float32(0.08)
+    orders = _orders(o.n)
     M.sparse.grid2particles(s, dt)
+    before = {nm: o.array(nm)[:o.n].copy() for nm in PNAMES}
     o.g2p(dt, 9.81)
     for nm in PNAMES:
         np.testing.assert_array_equal(s.get(nm.upper()), o.array(nm)[:o.n], err_msg=nm)
+    for name in ("interleaved", "random"):   # the wavefront and direct paths of the tiled gather
+        order = orders[name]
+        s = _upload(M, o)
+        for nm in PNAMES:
+            s.set(nm.upper(), before[nm][order])
+        M.sparse.grid2particles(s, dt)
+        for nm in PNAMES:
+            np.testing.assert_array_equal(s.get(nm.upper()), o.array(nm)[:o.n][order], err_msg="%s order %s" % (nm, name))
 
 
 # ── projection ───────────────────────────────────────────────────────────────

@@ -47,9 +47,9 @@ def sparse_bytes(kernel, live_cells, n_particles, precision=0):
     couplings."""
     e = 4 if precision == 0 else 2
     return {
-        # x, b, diag, three face couplings (each face shared by two cells), the result
-        # (e B each) and the type (1 B) per fine cell
-        "jacobi": (7 * e + 1) * live_cells,
+        # x, b, the Dirichlet diagonal, the result (e B each) and the type (1 B) per fine
+        # cell: level 0's couplings follow from the types (the row is matrix-free)
+        "jacobi": (4 * e + 1) * live_cells,
         "g2p": 60 * n_particles,
         "p2g": 64 * n_particles,
         "scatter": 136 * n_particles,
@@ -189,7 +189,8 @@ def sparse_main(args, desc, kw, dx):
     b = m.Basin(**kw)
     # README.md:86-90: SurfaceStir(b, narutowaves(); race = Race(; eddy = 3)), cfl 4
     stir = m.Stir(b, waves=m.narutowaves(), race=m.Race(eddy=3.0), surface=True)
-    s = m.SparseBasin(b, dx=dx, stir=stir, band=4, nmin=8, cfl=4.0, capacity=30_000_000, blocks=8000)
+    # a 12000-block pool: the stirred surface's band outgrows the default 8000 within ~20 steps
+    s = m.SparseBasin(b, dx=dx, stir=stir, band=4, nmin=8, cfl=4.0, capacity=30_000_000, blocks=12000)
     sp.initialize(s)
     umax = max(float(np.abs(s.get(f)).max()) for f in ("U_F", "V_F", "W_F", "U_C", "V_C", "W_C"))
     left = 1.0 / 24
@@ -236,7 +237,7 @@ def sparse_main(args, desc, kw, dx):
         "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": desc, "blocks": list(int(x) for x in s.grid.nb), "coarse_cells": list(s.nc),
-                   "live_blocks": live_slots, "fine_cells": live_slots * 4096, "particles": int(n_avg),
+                   "live_blocks": live_slots, "slots": int(ns), "fine_cells": live_slots * 4096, "particles": int(n_avg),
                    "dt_s": round(dt, 5), "tol": args.tol, "policy": "fcg-then-fgmres(20)",
                    "precision": ["fp32", "fp16", "bf16"][args.precision], "refine": args.refine,
                    "stir": "surface stir, 32 seeded narutowaves-shaped waves, race eddy 3",

@@ -211,6 +211,15 @@ struct PView {                 // device pointers of one particle buffer (SoA)
     float* f[PFIELDS];
 };
 
+// MI355X deals a launch's workgroups round-robin over its 8 XCDs, each with its own L2.
+// Stencil kernels whose neighbouring workgroups share data take their tile from this
+// bijection of the workgroup id instead, so that consecutive tiles run on one XCD:
+// XCD x (= b mod 8) gets the x-th contiguous run of tile ids.
+__device__ inline uint32_t xcd_tile(uint32_t b, uint32_t G) {
+    uint32_t q = G / 8, r = G % 8, x = b % 8, i = b / 8;
+    return x * q + min(x, r) + i;
+}
+
 // ─── fixed-point accumulation (deterministic P2G) ────────────────────────────
 // A contribution c is accumulated as round(c * 2^32) in a 64-bit integer: integer
 // addition is associative, so the sum does not depend on the order the particles

@@ -100,7 +100,7 @@ template <class NodeMap, class Source>
 __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiled(PView ps, int64_t n, NodeMap map, Source src, P2GArgs A) {
     __shared__ unsigned long long smom[P2G_BOX], swt[P2G_BOX];
     __shared__ int red[P2G_THREADS / 64];
-    int64_t p = (int64_t)blockIdx.x * P2G_THREADS + threadIdx.x;
+    int64_t p = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * P2G_THREADS + threadIdx.x;   // neighbouring tiles on one XCD
     bool live = p < n;
     float xp = 0, yp = 0, zp = 0, wT = 0;
     if (live) {
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(P2G_THREADS) void k_splat_tiled(PView ps, int64_t n
                                                             float dx, float dt, int32_t* __restrict__ phip) {
     __shared__ int32_t box[SPLAT_BOX];
     __shared__ int red[P2G_THREADS / 64];
-    int64_t p = (int64_t)blockIdx.x * P2G_THREADS + threadIdx.x;
+    int64_t p = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * P2G_THREADS + threadIdx.x;   // neighbouring tiles on one XCD
     bool live = p < n;
     float xp = 0, yp = 0, zp = 0;
     int ci = 0, cj = 0, ck = 0;

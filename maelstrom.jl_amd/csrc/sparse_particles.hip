@@ -316,7 +316,7 @@ __global__ __launch_bounds__(256) void k_sparse_g2p(PView ps, int64_t n, STopo t
     __shared__ uint32_t sflag[G2P_BOX];   // bit 8: the face exists; low byte: its flag
     __shared__ int red[4];
     const SGrid& g = t.g;
-    int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int64_t p = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;   // neighbouring tiles on one XCD
     bool live = p < n;
     float xp = 0, yp = 0, zp = 0;
     if (live) { xp = ps.f[0][p]; yp = ps.f[1][p]; zp = ps.f[2][p]; }

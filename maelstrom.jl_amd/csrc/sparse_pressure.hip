@@ -353,10 +353,10 @@ __global__ void k_covermask(uint8_t* __restrict__ m, STopo t) {
 // always at the edge of its block, so its sub-faces and cells index from one slot
 // lookup (the upper block's copy of a face, or the lower block's high plane). Sums in
 // the reference's order.
-template <int N, class T, class C, class A = float, bool Z = false>
-__device__ inline void coarserow(const STopo& t, unsigned msk, const T* __restrict__ xf, const FineCoefT<C>& cf,
-                                 const T* __restrict__ xc, const CoarseCoefT<C>& ccf, const C* __restrict__ dDc, int I, int J,
-                                 int K, A& diag, A& off) {
+template <int N, class C, class A, class FX, class CX>
+__device__ inline void coarserow_acc(const STopo& t, unsigned msk, FX fx, const FineCoefT<C>& cf, CX cx,
+                                     const CoarseCoefT<C>& ccf, const C* __restrict__ dDc, int I, int J, int K, A& diag,
+                                     A& off) {
     constexpr int R = N / 4, V = N * N * N, m = N - 1;
     constexpr uint32_t per = (N + 1) * N * N;
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
@@ -371,8 +371,7 @@ __device__ inline void coarserow(const STopo& t, unsigned msk, const T* __restri
         int f0 = n0 + (a == 0), f1 = n1 + (a == 1);
         size_t fi = sg < 0 ? cl(I, J, K, f0, f1) : cl(I + (a == 0), J + (a == 1), K + (a == 2), f0, f1);
         cc[d] = (A)lf(ccf.c[a][fi]);
-        if constexpr (Z) xv[d] = (A)0;
-        else xv[d] = (A)lf(xc[in[d] ? cl(In, Jn, Kn, n0, n1) : own]);
+        xv[d] = cx(in[d] ? cl(In, Jn, Kn, n0, n1) : own);
     }
     diag = (A)lf(dDc[own]);
     off = (A)0;
@@ -395,13 +394,39 @@ __device__ inline void coarserow(const STopo& t, unsigned msk, const T* __restri
                                                               : ffn(N, a, a == 0 ? N : li, a == 1 ? N : lj, a == 2 ? N : lk));
                     A c = (A)lf(cf.c[a][fidx]);
                     diag += c;
-                    off += c * (Z ? (A)0 : (A)lf(xf[cidx]));
+                    off += c * fx(cidx);
                 }
         } else if (cc[d] > (A)0) {
             diag += cc[d];
             off += cc[d] * xv[d];
         }
     }
+}
+
+// The same reading x from arrays (Z: x = 0, nothing read).
+template <int N, class T, class C, class A = float, bool Z = false>
+__device__ inline void coarserow(const STopo& t, unsigned msk, const T* __restrict__ xf, const FineCoefT<C>& cf,
+                                 const T* __restrict__ xc, const CoarseCoefT<C>& ccf, const C* __restrict__ dDc, int I, int J,
+                                 int K, A& diag, A& off) {
+    auto fx = [&](uint32_t i) -> A { if constexpr (Z) return (A)0; else return (A)lf(xf[i]); };
+    auto cx = [&](size_t i) -> A { if constexpr (Z) return (A)0; else return (A)lf(xc[i]); };
+    coarserow_acc<N, C, A>(t, msk, fx, cf, cx, ccf, dDc, I, J, K, diag, off);
+}
+
+// The device scalars of the Krylov loops (slots of SparseBasin::scal).
+enum { SS_BB = 0, SS_RR, SS_RHO, SS_DEN, SS_ALPHA, SS_RHONEW, SS_ZROLD, SS_BETA, SS_RNORM, SS_FLAG, SS_TMP, SS_H0 = 16 };
+
+// A workgroup's sum (thread 0 holds it); every thread must call it.
+__device__ inline double block_sum_red(double v, double* lds) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += lds[i];
+    return s;
 }
 
 // Level views: fine couplings and diagonal at N, the coarse grid's level-0 couplings.
@@ -416,129 +441,253 @@ template <class C> struct CompT {
     const uint8_t* tc;      // the coarse cell types (level 0's matrix-free couplings)
 };
 
-// sflip.jl:1218-1269: mode 0 Jacobi (out = x + w (b - A x)/diag), 1 residual (out = b - A x),
-// 2 apply (out = A x), 3 Jacobi from x = 0 (x not read), on the fine cells of a level;
-// inactive cells and the cells of free slots get 0.
-template <int N, int MODE, class T, class C>
-__global__ __launch_bounds__(256) void k_rowfine(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
-                                                 const T* __restrict__ xc, CompT<C> L, STopo t, int32_t nf) {
-    // launched with 256 threads a workgroup: its cells lie in one slot (256 divides N^3),
-    // so the slot, its place and its neighbours are uniform (scalar loads), and as nf is
-    // a multiple of N^3 both returns below are uniform over the workgroup
-    constexpr int lg = N == 16 ? 12 : 9, msk = N * N * N - 1, l2 = N == 16 ? 4 : 3;
-    int32_t n = blockIdx.x * 256 + threadIdx.x;
-    if ((int32_t)(blockIdx.x * 256) >= nf) return;
-    int s = (int)((blockIdx.x * 256u) >> lg) + 1, c = n & msk;
-    int4 w = t.where[s - 1];
-    if (w.x < 0) {
-        out[n] = sf<T>(0.0f);
-        return;
-    }
-    int32_t nb[6];
-#pragma unroll
-    for (int d = 0; d < 6; ++d) nb[d] = nbslot(t, w, d);
-    bool act = MODE == 2 ? L.actf[n] == LIQUID : L.actf[n] != 0;
-    float res = 0.0f;
-    if (act) {
-        float diag, off;
-        finerow<N, T, C, float, MODE == 3>(t, nb, x, xc, L.cf, L.dDf, s, c & (N - 1), (c >> l2) & (N - 1), c >> (2 * l2), w,
-                                           diag, off);
-        float xn = MODE == 3 ? 0.0f : lf(x[n]);
-        float ax = diag * xn - off;
-        if (MODE == 0 || MODE == 3) res = diag > 0.0f ? xn + __fdiv_rn(OMEGA * (lf(b[n]) - ax), diag) : 0.0f;
-        else if (MODE == 1) res = lf(b[n]) - ax;
-        else res = ax;
-    }
-    out[n] = sf<T>(res);
-}
-// The same on level 0, matrix-free: level 0's couplings are 1 between LIQUID fine cells,
-// 0.4 between a LIQUID fine cell and a LIQUID coarse one, else 0 (k_finecoef,
-// sflip.jl:1323-1342), so they are taken from the cell types the row reads at the same
-// places as x, rather than from the coupling arrays: the same values, summed in
-// finerow's order. Every index is computed and every load issued before any is used.
-// Launched with 256 threads a workgroup (one slot's z plane); free slots get 0.
+// Flexible CG's fused step (mode 4 of the level-0 rows, sflip.jl:1884-1886 with
+// 1903): the new direction d = z + beta d_old (x is z) is applied and written, and
+// d.q is summed per workgroup over the unknowns.
+struct SearchArgs {
+    const float* dof;   // d_old, fine pool and coarse grid
+    const float* doc;
+    float* dnf;         // d_new
+    float* dnc;
+    double* partials;   // [4 nslots fine quarter tiles | coarse workgroups]
+    int32_t coarse0;    // where the coarse workgroups' sums start
+    const double* scal;
+};
+
+// sflip.jl:1218-1269 on the fine cells of a level: mode 0 Jacobi (out = x + w (b - A x) /
+// diag), 1 residual (out = b - A x), 2 apply (out = A x), 3 Jacobi from x = 0 (x not
+// read); inactive cells and the cells of free slots get 0.
+// Level 0, matrix-free and staged in LDS: a workgroup takes a quarter of a
+// slot (4 z planes of its 16^3 block) and stages x and the cell types of those planes and
+// of the layers around them (the block's other planes, the face neighbours' slots, or
+// else the coarse cells: the value clamped to the grid, the type SOLID off it, flagged
+// coarse); each thread then computes 4 cells from LDS, with its cells' diagonal and
+// right-hand side loaded before the staging barrier. Level 0's couplings are 1 between
+// LIQUID fine cells, 0.4 between a LIQUID fine cell and a LIQUID coarse one, else 0
+// (k_finecoef, sflip.jl:1323-1342), so they follow from the staged types: the same values
+// as the coupling arrays, summed in finerow's order. Free slots get 0.
+constexpr int HX = 18, HZ = 6, HV = HX * HX * HZ;   // a staged quarter block with its halo
+constexpr uint8_t COARSE_FLAG = 0x80;
 template <int MODE, class T, class C>
 __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
-                                                   const T* __restrict__ xc, CompT<C> L, STopo t, int32_t nf) {
-    constexpr bool Z = MODE == 3;
-    int32_t n = blockIdx.x * 256 + threadIdx.x;
-    if ((int32_t)(blockIdx.x * 256) >= nf) return;
-    const int s = (int)((blockIdx.x * 256u) >> 12) + 1, c = n & 4095;
-    const int i = c & 15, j = (c >> 4) & 15, k = c >> 8;
+                                                   const T* __restrict__ xc, CompT<C> L, STopo t, SearchArgs S) {
+    constexpr bool Z = MODE == 3, SEARCH = MODE == 4;
+    __shared__ T sx[Z ? 1 : HV];
+    __shared__ uint8_t sty[HV];
+    __shared__ double lds[4];
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int s = (int)(tile >> 2) + 1, k0 = (int)(tile & 3) * 4;
+    const uint32_t cb = (uint32_t)(s - 1) * 4096u;
     const uint8_t* __restrict__ tf = L.actf;   // level 0: the fine cell types
+    const float beta = SEARCH ? (float)S.scal[SS_BETA] : 0.0f;
+    // x, or in the fused search the direction z + beta d_old
+    auto xval = [&](const T* __restrict__ xs, const float* __restrict__ ds, size_t i) -> T {
+        if constexpr (SEARCH) return sf<T>(lf(xs[i]) + beta * ds[i]);
+        else return xs[i];
+    };
     int4 w = t.where[s - 1];
     if (w.x < 0) {
-        out[n] = sf<T>(0.0f);
+        for (int c = threadIdx.x; c < 1024; c += 256) {
+            uint32_t n = cb + 256 * k0 + c;
+            out[n] = sf<T>(0.0f);
+            if constexpr (SEARCH) S.dnf[n] = lf(xval(x, S.dof, n));
+        }
+        if constexpr (SEARCH)
+            if (threadIdx.x == 0) S.partials[tile] = 0.0;
         return;
+    }
+    // this thread's cells: c = threadIdx.x + 256 e, e = 0..3, i.e. plane k0 + e
+    float dd[4], bb[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        uint32_t n = cb + 256 * (k0 + e) + threadIdx.x;
+        dd[e] = lf(L.dDf[n]);
+        bb[e] = (MODE == 2 || MODE == 4) ? 0.0f : lf(b[n]);
     }
     int32_t nb[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) nb[d] = nbslot(t, w, d);
-    uint8_t t0 = tf[n];
-    bool act = MODE == 2 ? t0 == LIQUID : t0 != 0;
-    float res = 0.0f;
-    if (act) {
-        const int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
-        const int I = w.x * 16 + i, J = w.y * 16 + j, K = w.z * 16 + k;
-        float xv[6], cf[6];
-        uint8_t ty[6];
-        bool fine[6], cin[6];
-        uint32_t fidx[6];
-        size_t cidx[6];
-#pragma unroll
-        for (int d = 0; d < 6; ++d) {
-            int ax = d >> 1, sg = (d & 1) ? 1 : -1;
-            int li = i + (ax == 0 ? sg : 0), lj = j + (ax == 1 ? sg : 0), lk = k + (ax == 2 ? sg : 0);
-            bool inside = li >= 0 && li < 16 && lj >= 0 && lj < 16 && lk >= 0 && lk < 16;
-            int32_t sn = nb[d];
-            fine[d] = inside || sn != 0;
-            uint32_t slot0 = inside ? (uint32_t)(s - 1) : (uint32_t)(sn - 1);
-            fidx[d] = slot0 * 4096u + (uint32_t)((li & 15) + 16 * ((lj & 15) + 16 * (lk & 15)));
-            int ci = (I + (ax == 0 ? sg : 0)) >> 2, cj = (J + (ax == 1 ? sg : 0)) >> 2, ck = (K + (ax == 2 ? sg : 0)) >> 2;
-            cin[d] = ci >= 0 && cj >= 0 && ck >= 0 && ci < n0 && cj < n1 && ck < n2;
-            cidx[d] = cl(min(max(ci, 0), n0 - 1), min(max(cj, 0), n1 - 1), min(max(ck, 0), n2 - 1), n0, n1);
+    const int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    // stage cell (li, lj, lk) of the block's frame (-1..16) at LDS place h
+    auto stage = [&](int li, int lj, int lk, int h) {
+        bool inside = li >= 0 && li < 16 && lj >= 0 && lj < 16 && lk >= 0 && lk < 16;
+        int d = li < 0 ? 0 : li > 15 ? 1 : lj < 0 ? 2 : lj > 15 ? 3 : lk < 0 ? 4 : 5;
+        int32_t sn = inside ? s : nb[d];
+        if (sn != 0) {
+            uint32_t idx = (uint32_t)(sn - 1) * 4096u + (uint32_t)((li & 15) + 16 * ((lj & 15) + 16 * (lk & 15)));
+            if constexpr (!Z) sx[h] = xval(x, S.dof, idx);
+            sty[h] = tf[idx];
+        } else {
+            int ci = (w.x * 16 + li) >> 2, cj = (w.y * 16 + lj) >> 2, ck = (w.z * 16 + lk) >> 2;
+            bool in = ci >= 0 && cj >= 0 && ck >= 0 && ci < n0 && cj < n1 && ck < n2;
+            size_t cidx = cl(min(max(ci, 0), n0 - 1), min(max(cj, 0), n1 - 1), min(max(ck, 0), n2 - 1), n0, n1);
+            if constexpr (!Z) sx[h] = xval(xc, S.doc, cidx);
+            sty[h] = (uint8_t)((in ? L.tc[cidx] : SOLID) | COARSE_FLAG);
         }
-#pragma unroll
-        for (int d = 0; d < 6; ++d) {
-            ty[d] = fine[d] ? tf[fidx[d]] : L.tc[cidx[d]];
-            xv[d] = Z ? 0.0f : (fine[d] ? lf(x[fidx[d]]) : lf(xc[cidx[d]]));
+    };
+    // the 4 planes (1024 cells), then the two z layers (512), then the x and y sides (256)
+    for (int c = threadIdx.x; c < 1024; c += 256) {
+        int i = c & 15, j = (c >> 4) & 15, kk = c >> 8;
+        uint32_t idx = cb + 256 * (k0 + kk) + (c & 255);
+        int h = (i + 1) + HX * ((j + 1) + HX * (kk + 1));
+        if constexpr (!Z) {
+            T v = xval(x, S.dof, idx);
+            sx[h] = v;
+            if constexpr (SEARCH) S.dnf[idx] = lf(v);
         }
-        float diag = lf(L.dDf[n]), off = 0.0f;
-        float x0 = Z ? 0.0f : lf(x[n]);
-        float bn = MODE == 2 ? 0.0f : lf(b[n]);
-#pragma unroll
-        for (int d = 0; d < 6; ++d) {
-            uint8_t tn = (fine[d] || cin[d]) ? ty[d] : SOLID;
-            cf[d] = (t0 == LIQUID && tn == LIQUID) ? (fine[d] ? FINE_FINE : FINE_COARSE) : 0.0f;
-            if (cf[d] > 0.0f) { diag += cf[d]; off += cf[d] * xv[d]; }
-        }
-        float ax = diag * x0 - off;
-        if (MODE == 0 || MODE == 3) res = diag > 0.0f ? x0 + __fdiv_rn(OMEGA * (bn - ax), diag) : 0.0f;
-        else if (MODE == 1) res = bn - ax;
-        else res = ax;
+        sty[h] = tf[idx];
     }
-    out[n] = sf<T>(res);
+    for (int e = threadIdx.x; e < 512; e += 256) {
+        int i = e & 15, j = (e >> 4) & 15, top = e >> 8, kk = top ? 4 : -1;
+        stage(i, j, k0 + kk, (i + 1) + HX * ((j + 1) + HX * (kk + 1)));
+    }
+    {
+        int e = threadIdx.x, side = e >> 6, p = e & 15, kk = (e >> 4) & 3;
+        int li = side == 0 ? -1 : side == 1 ? 16 : p, lj = side == 2 ? -1 : side == 3 ? 16 : p;
+        stage(li, lj, k0 + kk, (li + 1) + HX * ((lj + 1) + HX * (kk + 1)));
+    }
+    __syncthreads();
+    double pq = 0.0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        int c = threadIdx.x, i = c & 15, j = c >> 4;
+        int h = (i + 1) + HX * ((j + 1) + HX * (e + 1));
+        uint32_t n = cb + 256 * (k0 + e) + c;
+        uint8_t t0 = sty[h];
+        bool act = (MODE == 2 || SEARCH) ? t0 == LIQUID : t0 != 0;
+        float res = 0.0f;
+        if (act) {
+            const int off6[6] = {-1, 1, -HX, HX, -HX * HX, HX * HX};
+            float diag = dd[e], off = 0.0f;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+                uint8_t tn = sty[h + off6[d]];
+                float cf = (t0 == LIQUID && (tn & 0x7f) == LIQUID) ? ((tn & COARSE_FLAG) ? FINE_COARSE : FINE_FINE) : 0.0f;
+                if (cf > 0.0f) {
+                    diag += cf;
+                    off += cf * (Z ? 0.0f : lf(sx[Z ? 0 : h + off6[d]]));
+                }
+            }
+            float x0 = Z ? 0.0f : lf(sx[Z ? 0 : h]);
+            float ax = diag * x0 - off;
+            if (MODE == 0 || MODE == 3) res = diag > 0.0f ? x0 + __fdiv_rn(OMEGA * (bb[e] - ax), diag) : 0.0f;
+            else if (MODE == 1) res = bb[e] - ax;
+            else res = ax;
+            if constexpr (SEARCH) pq += (double)x0 * (double)res;
+        }
+        out[n] = sf<T>(res);
+    }
+    if constexpr (SEARCH) {
+        double sum = block_sum_red(pq, lds);
+        if (threadIdx.x == 0) S.partials[tile] = sum;
+    }
+}
+
+// Level 1 (8^3 a slot), staged in LDS: a workgroup per slot stages x of the block and its
+// face layers (face neighbours' slots, else the coarse cells, clamped), then each thread
+// computes 2 cells with finerow's couplings and order.
+template <int MODE, class T, class C>
+__global__ __launch_bounds__(256) void k_rowfine8(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
+                                                  const T* __restrict__ xc, CompT<C> L, STopo t) {
+    constexpr bool Z = MODE == 3;
+    constexpr int H8 = 10;
+    __shared__ T sx[Z ? 1 : H8 * H8 * H8];
+    const int s = (int)xcd_tile(blockIdx.x, gridDim.x) + 1;
+    const uint32_t cb = (uint32_t)(s - 1) * 512u, fb = (uint32_t)(s - 1) * 576u;
+    int4 w = t.where[s - 1];
+    if (w.x < 0) {
+        for (int c = threadIdx.x; c < 512; c += 256) out[cb + c] = sf<T>(0.0f);
+        return;
+    }
+    if constexpr (!Z) {
+        int32_t nb[6];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) nb[d] = nbslot(t, w, d);
+        const int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+        for (int c = threadIdx.x; c < 512; c += 256) {
+            int i = c & 7, j = (c >> 3) & 7, k = c >> 6;
+            sx[(i + 1) + H8 * ((j + 1) + H8 * (k + 1))] = x[cb + c];
+        }
+        for (int e = threadIdx.x; e < 6 * 64; e += 256) {
+            int d = e >> 6, p = e & 7, q = (e >> 3) & 7, ax = d >> 1, hi = d & 1;
+            int li = ax == 0 ? (hi ? 8 : -1) : p, lj = ax == 1 ? (hi ? 8 : -1) : (ax == 0 ? p : q), lk = ax == 2 ? (hi ? 8 : -1) : q;
+            int h = (li + 1) + H8 * ((lj + 1) + H8 * (lk + 1));
+            int32_t sn = nb[d];
+            if (sn != 0) {
+                sx[h] = x[(uint32_t)(sn - 1) * 512u + (uint32_t)((li & 7) + 8 * ((lj & 7) + 8 * (lk & 7)))];
+            } else {
+                int ci = (w.x * 8 + li) >> 1, cj = (w.y * 8 + lj) >> 1, ck = (w.z * 8 + lk) >> 1;
+                sx[h] = xc[cl(min(max(ci, 0), n0 - 1), min(max(cj, 0), n1 - 1), min(max(ck, 0), n2 - 1), n0, n1)];
+            }
+        }
+        __syncthreads();
+    }
+    for (int c = threadIdx.x; c < 512; c += 256) {
+        int i = c & 7, j = (c >> 3) & 7, k = c >> 6;
+        int h = (i + 1) + H8 * ((j + 1) + H8 * (k + 1));
+        uint32_t n = cb + c;
+        bool act = MODE == 2 ? L.actf[n] == LIQUID : L.actf[n] != 0;
+        float res = 0.0f;
+        if (act) {
+            float cf[6] = {lf(L.cf.c[0][fb + ffn(8, 0, i, j, k)]), lf(L.cf.c[0][fb + ffn(8, 0, i + 1, j, k)]),
+                           lf(L.cf.c[1][fb + ffn(8, 1, i, j, k)]), lf(L.cf.c[1][fb + ffn(8, 1, i, j + 1, k)]),
+                           lf(L.cf.c[2][fb + ffn(8, 2, i, j, k)]), lf(L.cf.c[2][fb + ffn(8, 2, i, j, k + 1)])};
+            const int off6[6] = {-1, 1, -H8, H8, -H8 * H8, H8 * H8};
+            float diag = lf(L.dDf[n]), off = 0.0f;
+#pragma unroll
+            for (int d = 0; d < 6; ++d)
+                if (cf[d] > 0.0f) {
+                    diag += cf[d];
+                    off += cf[d] * (Z ? 0.0f : lf(sx[Z ? 0 : h + off6[d]]));
+                }
+            float x0 = Z ? 0.0f : lf(sx[Z ? 0 : h]);
+            float ax = diag * x0 - off;
+            if (MODE == 0 || MODE == 3) res = diag > 0.0f ? x0 + __fdiv_rn(OMEGA * (lf(b[n]) - ax), diag) : 0.0f;
+            else if (MODE == 1) res = lf(b[n]) - ax;
+            else res = ax;
+        }
+        out[n] = sf<T>(res);
+    }
 }
 
 template <int N, int MODE, class T, class C>
 __global__ __launch_bounds__(256) void k_rowcoarse(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
-                                                   const T* __restrict__ xf, CompT<C> L, STopo t) {
+                                                   const T* __restrict__ xf, CompT<C> L, STopo t, SearchArgs S) {
+    constexpr bool SEARCH = MODE == 4;
+    __shared__ double lds[4];
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
-    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= (size_t)n0 * n1 * n2) return;
+    size_t l = (size_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;   // neighbouring rows on one XCD
+    bool live = l < (size_t)n0 * n1 * n2;
+    if (!SEARCH && !live) return;
+    const float beta = SEARCH ? (float)S.scal[SS_BETA] : 0.0f;
+    double pq = 0.0;
+    if (live) {
     int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
     float res = 0.0f;
     unsigned msk = L.cmask[l];
+    if constexpr (SEARCH) S.dnc[l] = lf(x[l]) + beta * S.doc[l];
     if (L.actc[l] != 0 && !(msk & 1u)) {
         float diag, off;
-        coarserow<N, T, C, float, MODE == 3>(t, msk, xf, L.cf, x, L.cc, L.dDc, I, J, K, diag, off);
-        float xl = MODE == 3 ? 0.0f : lf(x[l]);
+        if constexpr (SEARCH) {
+            auto fx = [&](uint32_t i) -> float { return lf(xf[i]) + beta * S.dof[i]; };
+            auto cx = [&](size_t i) -> float { return lf(x[i]) + beta * S.doc[i]; };
+            coarserow_acc<N, C, float>(t, msk, fx, L.cf, cx, L.cc, L.dDc, I, J, K, diag, off);
+        } else {
+            coarserow<N, T, C, float, MODE == 3>(t, msk, xf, L.cf, x, L.cc, L.dDc, I, J, K, diag, off);
+        }
+        float xl = MODE == 3 ? 0.0f : (SEARCH ? lf(x[l]) + beta * S.doc[l] : lf(x[l]));
         float ax = diag * xl - off;
         if (MODE == 0 || MODE == 3) res = diag > 0.0f ? xl + __fdiv_rn(OMEGA * (lf(b[l]) - ax), diag) : 0.0f;
         else if (MODE == 1) res = lf(b[l]) - ax;
         else res = ax;
+        if constexpr (SEARCH) pq = (double)xl * (double)res;
     }
     out[l] = sf<T>(res);
+    }
+    if constexpr (SEARCH) {
+        double sum = block_sum_red(pq, lds);
+        if (threadIdx.x == 0) S.partials[S.coarse0 + blockIdx.x] = sum;
+    }
 }
 
 // ─── dense levels (sflip.jl:1574-1657) ───────────────────────────────────────
@@ -578,7 +727,7 @@ __device__ inline void denserow(const DViewT<C>& L, const T* __restrict__ x, int
 template <int MODE, class T, class C>
 __global__ __launch_bounds__(256) void k_rowdense(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
                                                   DViewT<C> L) {
-    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t l = (size_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;   // neighbouring rows on one XCD
     if (l >= (size_t)L.n.x * L.n.y * L.n.z) return;
     int i = (int)(l % L.n.x), j = (int)((l / L.n.x) % L.n.y), k = (int)(l / ((size_t)L.n.x * L.n.y));
     float res = 0.0f;
@@ -703,27 +852,47 @@ __global__ void k_prolongcoarse(T* __restrict__ xc, const T* __restrict__ from, 
     int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
     if (act[l] != 0 && !covered(t, I, J, K)) xc[l] = sf<T>(lf(xc[l]) + boost * lf(from[l]));
 }
+// Four cells of an x row a thread (two level-1 parents); a workgroup's 1024 cells lie in
+// one slot, so the slot's place is uniform.
 template <class T>
-__global__ void k_prolong10(T* __restrict__ x0f, const T* __restrict__ x1f, const uint8_t* __restrict__ tf, STopo t, float boost,
-                            int32_t nf) {
-    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nf) return;
-    SCell c = finecell(t, n);
-    if (c.w.x < 0 || tf[n] != LIQUID) return;
-    x0f[n] = sf<T>(lf(x0f[n]) + boost * lf(x1f[(c.s - 1) * SBCELLS1 + (c.i >> 1) + 8 * ((c.j >> 1) + 8 * (c.k >> 1))]));
+__global__ __launch_bounds__(256) void k_prolong10(T* __restrict__ x0f, const T* __restrict__ x1f, const uint8_t* __restrict__ tf,
+                                                   STopo t, float boost, int32_t nf) {
+    uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    int32_t n = (int32_t)(tile * 1024u) + 4 * (int32_t)threadIdx.x;
+    if ((int32_t)(tile * 1024u) >= nf) return;
+    int s = (int)((tile * 1024u) >> 12) + 1, c = n & 4095;
+    if (t.where[s - 1].x < 0) return;
+    int i = c & 15, j = (c >> 4) & 15, k = c >> 8;
+    uchar4 ty = *reinterpret_cast<const uchar4*>(tf + n);
+    if (ty.x != LIQUID && ty.y != LIQUID && ty.z != LIQUID && ty.w != LIQUID) return;
+    const T* par = x1f + (size_t)(s - 1) * SBCELLS1 + (i >> 1) + 8 * ((j >> 1) + 8 * (k >> 1));
+    float p0 = boost * lf(par[0]), p1 = boost * lf(par[1]);
+    T* xo = x0f + n;
+    if (ty.x == LIQUID) xo[0] = sf<T>(lf(xo[0]) + p0);
+    if (ty.y == LIQUID) xo[1] = sf<T>(lf(xo[1]) + p0);
+    if (ty.z == LIQUID) xo[2] = sf<T>(lf(xo[2]) + p1);
+    if (ty.w == LIQUID) xo[3] = sf<T>(lf(xo[3]) + p1);
 }
 
 // Conversions between the Krylov vectors (FP32) and the reduced-precision hierarchy:
-// y = x * mult * scal[slot] (or * mult when slot < 0), over the fine and the coarse range.
+// y = x * mult * scal[slot] (or * mult when slot < 0), over the fine and the coarse range,
+// four values a thread (both parts are multiples of 4 long).
+template <class T> struct alignas(4 * sizeof(T)) Four { T v[4]; };
 template <class Y, class X>
 __global__ void k_convert(Y* __restrict__ yf, const X* __restrict__ xf, size_t nf, Y* __restrict__ yc, const X* __restrict__ xc,
                           size_t nc, const double* __restrict__ scal, int slot, int invert) {
     float c = 1.0f;
     if (slot >= 0) c = invert ? (float)(1.0 / scal[slot]) : (float)scal[slot];
     if (!(c == c) || c == 0.0f || !(fabsf(c) <= 3.0e38f)) c = 1.0f;
-    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < nf + nc; l += (size_t)gridDim.x * blockDim.x) {
-        if (l < nf) yf[l] = sf<Y>(lf(xf[l]) * c);
-        else yc[l - nf] = sf<Y>(lf(xc[l - nf]) * c);
+    size_t nf4 = nf / 4, ng = nf4 + nc / 4;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += (size_t)gridDim.x * blockDim.x) {
+        const Four<X>* xs = g < nf4 ? reinterpret_cast<const Four<X>*>(xf) + g : reinterpret_cast<const Four<X>*>(xc) + (g - nf4);
+        Four<Y>* ys = g < nf4 ? reinterpret_cast<Four<Y>*>(yf) + g : reinterpret_cast<Four<Y>*>(yc) + (g - nf4);
+        Four<X> a = *xs;
+        Four<Y> o;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) o.v[e] = sf<Y>(lf(a.v[e]) * c);
+        *ys = o;
     }
 }
 // A coupling array in reduced precision, times `scale`.
@@ -735,24 +904,12 @@ __global__ void k_castc(Y* __restrict__ y, const float* __restrict__ x, size_t n
 
 // ─── vectors: dot products over the unknowns, axpy and friends (sflip.jl:1766-1822) ──
 
-enum { SS_BB = 0, SS_RR, SS_RHO, SS_DEN, SS_ALPHA, SS_RHONEW, SS_ZROLD, SS_BETA, SS_RNORM, SS_FLAG, SS_TMP, SS_H0 = 16 };
 // error flags (the reference's messages, sflip.jl:1880-1900)
 enum { EF_RHO_NONFINITE = 1, EF_RHO_NONPOS = 2, EF_DEN_NONPOS = 4, EF_ALPHA_NONFINITE = 8, EF_RHONEW_NONPOS = 16,
        EF_BETA_NONFINITE = 32 };
 
 constexpr int DOT_BLOCKS = 2048;   // partials: 2 x DOT_BLOCKS doubles at most
 
-__device__ inline double block_sum_red(double v, double* lds) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) lds[w] = v;
-    __syncthreads();
-    double s = 0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += lds[i];
-    return s;
-}
 
 // The vector kernels run over the concatenated (fine pool, coarse grid) range in groups of
 // four consecutive cells (float4 / uchar4 loads): the fine part is whole slots and the
@@ -956,9 +1113,10 @@ __global__ void k_saxpy2(float* __restrict__ yf, const float* __restrict__ xf, s
 __global__ __launch_bounds__(256) void k_resid64fine(float* __restrict__ r, const double* __restrict__ x,
                                                      const double* __restrict__ xc, const float* __restrict__ b,
                                                      CompT<float> L, STopo t, int32_t nf) {
-    int32_t n = blockIdx.x * 256 + threadIdx.x;   // 256 threads a workgroup, as k_rowfine
-    if ((int32_t)(blockIdx.x * 256) >= nf) return;
-    int s = (int)((blockIdx.x * 256u) >> 12) + 1, c = n & 4095;
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);   // 256 threads a workgroup, as k_rowfine
+    int32_t n = tile * 256 + threadIdx.x;
+    if ((int32_t)(tile * 256) >= nf) return;
+    int s = (int)((tile * 256u) >> 12) + 1, c = n & 4095;
     int4 w = t.where[s - 1];
     if (w.x < 0) return;
     int32_t nb[6];
@@ -1206,19 +1364,31 @@ struct SparseSolver {
 
     // One composite-level pass (MODE 0 Jacobi, 1 residual, 2 apply) over the fine part
     // (N^3 a slot) and the coarse part.
-    template <int N, int MODE, class T, class C> void rows(SVecT<T> out, SVecT<T> x, SVecT<T> b, const CompT<C>& L) {
+    template <int N, int MODE, class T, class C>
+    void rows(SVecT<T> out, SVecT<T> x, SVecT<T> b, const CompT<C>& L, SearchArgs S = SearchArgs{}) {
         int32_t nfl = s.nslots * N * N * N;
         if (nfl) {
             if (N == 16 && MODE == 0) s.kt.begin(K_RBGS, st);
             if constexpr (N == 16)
-                hipLaunchKernelGGL((k_rowfine16<MODE, T, C>), dim3(cdiv(nfl, 256)), dim3(256), 0, st, out.f, x.f, b.f, x.c, L, t,
-                                   nfl);
+                hipLaunchKernelGGL((k_rowfine16<MODE, T, C>), dim3(4 * s.nslots), dim3(256), 0, st, out.f, x.f, b.f, x.c, L, t, S);
             else
-                hipLaunchKernelGGL((k_rowfine<N, MODE, T, C>), dim3(cdiv(nfl, 256)), dim3(256), 0, st, out.f, x.f, b.f, x.c, L,
-                                   t, nfl);
+                hipLaunchKernelGGL((k_rowfine8<MODE, T, C>), dim3(s.nslots), dim3(256), 0, st, out.f, x.f, b.f, x.c, L, t);
             if (N == 16 && MODE == 0) s.kt.end(K_RBGS, st);
+        } else if (MODE == 4) {
+            fill_async(S.partials, (size_t)1, 0.0, st);   // no fine workgroups: the finish reads one zero
         }
-        hipLaunchKernelGGL((k_rowcoarse<N, MODE, T, C>), dim3(cdiv(Cc, 256)), dim3(256), 0, st, out.c, x.c, b.c, x.f, L, t);
+        hipLaunchKernelGGL((k_rowcoarse<N, MODE, T, C>), dim3(cdiv(Cc, 256)), dim3(256), 0, st, out.c, x.c, b.c, x.f, L, t, S);
+    }
+    // flexible CG's fused step: dn = z + beta d (beta on the device), q = A dn, and d.q
+    // (with the reference's checks) into alpha
+    void search(SVec q, SVec dn, SVec z, SVec d) {
+        Hier<float, float> H = hier32();
+        int32_t fine_groups = s.nslots ? 4 * s.nslots : 1;
+        SearchArgs S{d.f, d.c, dn.f, dn.c, s.partials.p, fine_groups, s.scal.p};
+        rows<16, 4>(SVecT<float>{q.f, q.c}, SVecT<float>{z.f, z.c}, SVecT<float>{z.f, z.c}, H.L0, S);
+        hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(256), 0, st, s.partials.p, fine_groups + (int)cdiv(Cc, 256), s.scal.p,
+                           (int)SF_ALPHA, 0);
+        MAEL_LAUNCH_CHECK();
     }
     template <int N, class T, class C> void smooth(SVecT<T> x, SVecT<T> b, SVecT<T> tmp, const CompT<C>& L, int sweeps) {
         for (int k = 0; k < sweeps; ++k) {
@@ -1284,7 +1454,8 @@ struct SparseSolver {
         hipLaunchKernelGGL(k_prolongcoarse<T>, dim3(cdiv(Cc, 256)), dim3(256), 0, st, H.X1x.c, H.Dx[0], D2.act.p, t, boost);
         smooth<8>(H.X1x, H.X1b, H.X1t, H.L1, sweeps);
         if (nf)
-            hipLaunchKernelGGL(k_prolong10<T>, dim3(cdiv(nf, 256)), dim3(256), 0, st, z.f, H.X1x.f, s.types.fine.p, t, boost, nf);
+            hipLaunchKernelGGL(k_prolong10<T>, dim3(cdiv(nf, 1024)), dim3(256), 0, st, z.f, H.X1x.f, s.types.fine.p, t, boost,
+                               nf);
         hipLaunchKernelGGL(k_prolongcoarse<T>, dim3(cdiv(Cc, 256)), dim3(256), 0, st, z.c, H.X1x.c, D2.act.p, t, boost);
         smooth<16>(z, r, H.tmp, H.L0, sweeps);
         MAEL_LAUNCH_CHECK();
@@ -1425,9 +1596,18 @@ struct SparseSolver {
         vcycle(z, r, sweeps, coarse, boost, SS_RNORM);
         copy(d, z);
         dot(z, r, SF_RHO0, 0);
+        if (!s.d2.fine.p) s.d2.alloc((size_t)s.blocks * SBCELLS, Cc);
+        SVec dbuf[2] = {d, vec(s.d2)};
+        int cur = 0;
         for (int it = 1; it <= maxiter; ++it) {
-            apply(q, d);
-            dot(d, q, SF_ALPHA, 0);
+            if (it == 1) {
+                apply(q, d);
+                dot(d, q, SF_ALPHA, 0);
+            } else {   // d = z + beta d fused into q = A d and d.q
+                search(q, dbuf[cur ^ 1], z, dbuf[cur]);
+                cur ^= 1;
+                d = dbuf[cur];
+            }
             hipLaunchKernelGGL(k_fcg_update, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, P.f, P.c, r.f, r.c, rold.f,
                                rold.c, d.f, d.c, q.f, q.c, s.types.fine.p, s.types.coarse.p, s.cmask.p, t, nf, s.scal.p,
                                (int)SS_ALPHA);
@@ -1443,7 +1623,6 @@ struct SparseSolver {
             hipLaunchKernelGGL(k_sdot2, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, z.f, z.c, r.f, r.c, rold.f, rold.c,
                                s.types.fine.p, s.types.coarse.p, s.cmask.p, t, nf);
             hipLaunchKernelGGL(k_sfinish_beta, dim3(1), dim3(256), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p);
-            xpby(d, z, SS_BETA, 1.0f);
         }
         double flags;
         read2(SS_RNORM, rnorm, flags);

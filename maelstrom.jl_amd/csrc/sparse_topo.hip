@@ -49,7 +49,9 @@ SparseBasin::SparseBasin(const SGrid& grid, int nblk, int64_t cap, int32_t bnd, 
                                 std::to_string(g.nc(2)) + ") does not halve to at most 1024 cells");
     reach.alloc(Cc); changed.alloc(1);
     stir_waves.alloc(1); stir_heights.alloc(2);
-    partials.alloc(4096);
+    // per-workgroup sums: the vector kernels' (2 x 2048) or the fused search's (a quarter
+    // slot's workgroup each, then the coarse grid's)
+    partials.alloc(std::max<size_t>(4096, 4 * (size_t)blocks + (Cc + 255) / 256 + 64));
     scal.alloc(128);
     for (int w = 0; w < 2; ++w)
         for (int f = 0; f < PFIELDS; ++f) pbuf[w][f].alloc(std::max<int64_t>(cap, 1));

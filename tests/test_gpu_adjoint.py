@@ -193,4 +193,6 @@ def test_chain_gradient_finite_differences(mael):
     dpg = {f: (base[f] / G).astype(np.float32) for f in base}
     d0g = {f: (base0[f] / G).astype(np.float32) for f in base0}
     want = sum(_dot(gv[f], dpg[f]) for f in base) + sum(_dot(g0[f], d0g[f]) for f in base0)
-    assert _rel(fd(dpg, d0g, 1e-3 * G), want) < 1e-4
+    # a relative step of 1e-2: the chain is FP32, and at 1e-3 the difference of two losses
+    # of order 100 is 0.2, within FP32's rounding noise of 1e-4 relative
+    assert _rel(fd(dpg, d0g, 1e-2 * G), want) < 1e-4

@@ -56,6 +56,18 @@ def sparse_bytes(kernel, live_cells, n_particles, precision=0):
     }[kernel]
 
 
+def measured_traffic(key, units):
+    """HBM bytes per launch from a counter measurement committed under profiles/: the line
+    `<key>: <bytes per unit>` (FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for
+    kernels of known traffic), times this launch's units; None when no such line exists."""
+    for name in sorted(os.listdir(os.path.join(ROOT, "profiles"))):
+        with open(os.path.join(ROOT, "profiles", name), encoding="utf-8") as f:
+            for line in f:
+                if line.startswith(key + ":"):
+                    return int(round(float(line.split(":", 1)[1]) * units))
+    return None
+
+
 def algorithmic_bytes(kernel, s, n_particles):
     """Compulsory bytes of one launch of `kernel` at the reference's element sizes."""
     n1, n2, n3 = s.n_cells
@@ -171,6 +183,74 @@ def sparse_cpu_baseline(s, iters_per_step, dt, sample_particles=1_000_000):
     return {"value": round(ms, 1), "unit": "ms/step", "cores": 1, "kind": "port", "sample": sample}
 
 
+def sparse_mp_error(s, dt, tol, refine, oracle_max_cells=4_000_000):
+    """The error of the GPU solves (FP32, FP16 and BF16 V-cycles, all with the FP64
+    refinement) against a tight solve of the same system: the state of a first half-step's
+    projection, the same pressure to start from, gamma = 0. The tight solve is the FP64
+    oracle's (to 1e-10) up to `oracle_max_cells` live fine cells; above that, where the
+    oracle's single-threaded solve would take many minutes, the GPU's FP32 V-cycle with the
+    FP64 refinement every iteration, to 1e-10 (the oracle comparison at the smaller sizes
+    bounds that solve's error). Reports the relative L2 error of P over the unknowns and
+    the relative divergence left after each projection (||b(u)|| / ||b(u_before)|| over
+    the unknowns, b the oracle's FP64 right-hand side, evaluated on the host)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as orc
+    from maelstrom_amd import sparse as sp
+    sp.before_projection(s, dt / 2, reflect=True)
+    keep = {f: s.get(f) for f in ("U_F", "V_F", "W_F", "U_C", "V_C", "W_C", "P_F", "P_C")}
+    slot, where, ns, free = s.topology()
+    o = orc.SparseOracle(s.basin.radius, s.basin.depth, s.basin.rimwall, s.grid.dx, 2, max(ns, 1), 1, s.band, s.nmin)
+    o.array("slot")[:] = slot
+    o.array("where")[:3 * ns] = where[:ns].ravel()
+    _set_nslots(o, ns, free)
+    for gname, oname in [("PHI", "phi"), ("TYPES", "types"), ("U", "u"), ("V", "v"), ("W", "w"), ("P", "P")]:
+        o.array(oname + "_f")[:] = s.get(gname + "_F")
+        o.array(oname + "_c")[:] = s.get(gname + "_C").ravel()
+    # the unknowns: LIQUID fine cells of live slots, LIQUID coarse cells outside the blocks
+    nb0, nb1, nb2 = (int(x) for x in s.grid.nb)
+    covered = (slot.reshape(nb2, nb1, nb0) != 0).repeat(4, 0).repeat(4, 1).repeat(4, 2).ravel()
+    uf = (o.array("types_f")[:ns * 4096] == 1) & np.repeat(where[:ns, 0] >= 0, 4096)
+    uc = (o.array("types_c") == 1) & ~covered
+
+    def bnorm():
+        bf, bc = o.rhs(0.0)
+        return float(np.sqrt(np.sum(bf[uf] ** 2) + np.sum(bc[uc] ** 2)))
+
+    def gpu_solve(p, tol_, refine_, maxiter):
+        for f, v in keep.items():
+            s.set(f, v)
+        st = sp.project(s, dt / 2, gamma=0.0, tol=tol_, maxiter=maxiter, restart=20, policy=2, boost=1.5, precision=p,
+                        refine=refine_)
+        Pg = np.concatenate([s.get("P_F")[uf].astype(np.float64), s.get("P_C").ravel()[uc].astype(np.float64)])
+        for gname, oname in [("U", "u"), ("V", "v"), ("W", "w")]:
+            o.array(oname + "_f")[:] = s.get(gname + "_F")
+            o.array(oname + "_c")[:] = s.get(gname + "_C").ravel()
+        return st, Pg
+
+    b0 = bnorm()
+    t0 = time.perf_counter()
+    if int((where[:ns, 0] >= 0).sum()) * 4096 <= oracle_max_cells:
+        ref = o.project(gamma=0.0, tol=1e-10, maxiter=300, policy=0)
+        P64 = np.concatenate([o.array("P_f")[:ns * 4096][uf].astype(np.float64), o.array("P_c")[uc].astype(np.float64)])
+        rep = {"reference": "oracle FP64 MGPCG to 1e-10",
+               "tight": {"iterations": ref["iterations"], "rel_residual": ref["rel_residual"]}}
+    else:
+        st, P64 = gpu_solve(0, 1e-10, 1, 400)
+        rep = {"reference": "GPU FP32 V-cycle, FP64 refinement every iteration, to 1e-10",
+               "tight": {"iterations": st.iterations, "rel_residual": st.rel_residual}}
+    rep["tight"].update(seconds=round(time.perf_counter() - t0, 1), div_rel=bnorm() / b0)
+    print("mp_error: reference solve done", flush=True)
+    for p, name in ((0, "fp32"), (1, "fp16"), (2, "bf16")):
+        st, Pg = gpu_solve(p, tol, refine, 240)
+        rep[name] = {"iterations": st.iterations, "rel_residual": st.rel_residual,
+                     "P_rel_l2": float(np.linalg.norm(Pg - P64) / np.linalg.norm(P64)), "div_rel": bnorm() / b0}
+        print("mp_error: %s done" % name, flush=True)
+    for f, v in keep.items():
+        s.set(f, v)
+    return rep
+
+
 def _set_nslots(o, ns, free):
     """Give the oracle handle the GPU's slot count and free list (its where/slot arrays are
     already filled)."""
@@ -231,7 +311,9 @@ def sparse_main(args, desc, kw, dx):
     avg_launch_ms = R["kms"] / max(R["launches"], 1)
     n_avg = R["particles"]
     abytes = sparse_bytes(kernel, live_slots * 4096, n_avg, args.precision)
-    achieved = abytes / (avg_launch_ms * 1e-3) / 1e9
+    achieved = abytes / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
+    traffic = (measured_traffic("traffic_bytes_per_fine_cell_fp16_jacobi", live_slots * 4096)
+               if kernel == "jacobi" and args.precision == 1 else None)
     out = {
         "metric": METRIC, "value": round(ms_step, 3), "unit": "ms/step", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": False, "scaling": "weak",
@@ -248,7 +330,7 @@ def sparse_main(args, desc, kw, dx):
         "fgmres_steps": R["fgmres"],
         "mparticles_per_s": round(n_avg / (ms_step * 1e-3) / 1e6, 2),
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "launches": R["launches"], "avg_launch_us": round(avg_launch_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": int(abytes)},
     }
@@ -261,6 +343,8 @@ def sparse_main(args, desc, kw, dx):
                        "jacobi_avg_launch_us": round(1e3 * F["kms"] / max(F["launches"], 1), 3)}
         out["speedup_vs_fp32"] = {"step": round(F["ms"] / ms_step, 3),
                                   "pressure": round(F["pressure_ms"] / max(R["pressure_ms"], 1e-9), 3)}
+    if args.mp_error:
+        out["mp_error"] = sparse_mp_error(s, dt, args.tol, args.refine)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = sparse_cpu_baseline(s, out["iterations"], dt)
     print(json.dumps(out), flush=True)
@@ -280,6 +364,8 @@ def main():
     ap.add_argument("--kernel", default="rbgs", help="the kernel the roofline object reports")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="sparse: skip the FP32 V-cycle comparison steps")
+    ap.add_argument("--mp-error", action="store_true",
+                    help="sparse: report the solves' error against an FP64 oracle solve of the same system (minutes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's hot path runs on a single MI355X (--gpus %d not supported)"

@@ -68,28 +68,50 @@ __global__ void k_sparse_bin(PView ps, int64_t n, const float* __restrict__ phi,
     rank[p] = idx >= 0 ? atomicAdd(&counts[idx], 1) : -1;
 }
 
-// sflip.jl:2184-2218 sparsespawn_kernel!.
-__global__ void k_sparse_spawn(PView dst, const int32_t* __restrict__ counts, const int32_t* __restrict__ totals,
-                               const int32_t* __restrict__ offsets, const float* __restrict__ phi, const float* __restrict__ phic,
-                               const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w, STopo t,
-                               int32_t nmin, float deep, int seeding, uint32_t seed, int32_t nf) {
-    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nf) return;
-    SCell c = finecell(t, n);
+// sflip.jl:2184-2218 sparsespawn_kernel!, in two passes so that the threads of a launch
+// each spawn one particle: the cells that add particles are listed (in any order: where a
+// particle goes is fixed by its cell's offsets), then nmin threads a listed cell each take
+// one particle m < add. The records are the reference's, bit for bit.
+__device__ inline void spawn_counts(const STopo& t, const int32_t* __restrict__ counts, const float* __restrict__ phi, int32_t n,
+                                    int32_t nmin, float deep, int seeding, SCell& c, int32_t& kept, int32_t& add) {
+    c = finecell(t, n);
+    kept = 0; add = 0;
     if (c.w.x < 0) return;
     const SGrid& g = t.g;
-    int I = c.w.x * 16 + c.i, J = c.w.y * 16 + c.j, K = c.w.z * 16 + c.k;
     float pc[3];
-    finecentre(g, I, J, K, pc);
-    int32_t kept, add;
+    finecentre(g, c.w.x * 16 + c.i, c.w.y * 16 + c.j, c.w.z * 16 + c.k, pc);
     scellcounts(counts[n], phi[n], sissolid(g, pc[0], pc[1], pc[2]), nmin, deep, g.dx, seeding != 0, kept, add);
-    if (add <= 0) return;
-    int64_t base = (int64_t)offsets[n] - totals[n] + kept;
-    float cx = g.lo[0] + (float)I * g.dx, cy = g.lo[1] + (float)J * g.dx, cz = g.lo[2] + (float)K * g.dx;
-    uint32_t n1 = (uint32_t)n + 1u;   // the reference's 1-based index
-    int32_t sd = (int32_t)seed;
-    float half = 0.5f * g.dx;
-    for (int32_t m = 0; m < add; ++m) {
+}
+__global__ void k_spawn_list(const int32_t* __restrict__ counts, const float* __restrict__ phi, STopo t, int32_t nmin, float deep,
+                             int seeding, int32_t nf, int32_t* __restrict__ list, int* __restrict__ nlist) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nf) return;
+    SCell c;
+    int32_t kept, add;
+    spawn_counts(t, counts, phi, n, nmin, deep, seeding, c, kept, add);
+    if (add > 0) list[atomicAdd(nlist, 1)] = n;
+}
+__global__ __launch_bounds__(256) void k_sparse_spawn(PView dst, const int32_t* __restrict__ counts,
+                                                      const int32_t* __restrict__ totals, const int32_t* __restrict__ offsets,
+                                                      const float* __restrict__ phi, const float* __restrict__ phic,
+                                                      const float* __restrict__ u, const float* __restrict__ v,
+                                                      const float* __restrict__ w, STopo t, int32_t nmin, float deep, int seeding,
+                                                      uint32_t seed, const int32_t* __restrict__ list,
+                                                      const int* __restrict__ nlist) {
+    const SGrid& g = t.g;
+    int64_t total = (int64_t)(*nlist) * nmin;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        int32_t n = list[e / nmin], m = (int32_t)(e % nmin);
+        SCell c;
+        int32_t kept, add;
+        spawn_counts(t, counts, phi, n, nmin, deep, seeding, c, kept, add);
+        if (m >= add) continue;
+        int64_t base = (int64_t)offsets[n] - totals[n] + kept;
+        int I = c.w.x * 16 + c.i, J = c.w.y * 16 + c.j, K = c.w.z * 16 + c.k;
+        float cx = g.lo[0] + (float)I * g.dx, cy = g.lo[1] + (float)J * g.dx, cz = g.lo[2] + (float)K * g.dx;
+        uint32_t n1 = (uint32_t)n + 1u;   // the reference's 1-based index
+        int32_t sd = (int32_t)seed;
+        float half = 0.5f * g.dx;
         int32_t h = (int32_t)(n1 * 31u + (uint32_t)m);
         float jx = hash01(h, sd + 1), jy = hash01(h, sd + 2), jz = hash01(h, sd + 3);
         int32_t o = m % 8;
@@ -150,9 +172,13 @@ int64_t SparseBasin::resample(bool seeding, uint32_t seed) {
     if (C) {
         const SField<float>* fv = reflect_flow ? ur : u;   // the flow new particles take (sflip.jl:2312-2316)
         kt.begin(K_SPAWN, s);
-        hipLaunchKernelGGL(k_sparse_spawn, dim3(cdiv(C, 256)), dim3(256), 0, s, dst, counts.p, totals.p, offsets.p, phi.fine.p,
+        int* nlist = changed.p;   // a device counter, free during resampling
+        fill_async(nlist, (size_t)1, 0, s);
+        hipLaunchKernelGGL(k_spawn_list, dim3(cdiv(C, 256)), dim3(256), 0, s, counts.p, phi.fine.p, t, nmin, deep,
+                           seeding ? 1 : 0, (int32_t)C, cstart.p, nlist);
+        hipLaunchKernelGGL(k_sparse_spawn, dim3(4096), dim3(256), 0, s, dst, counts.p, totals.p, offsets.p, phi.fine.p,
                            phi.coarse.p, fv[0].fine.p, fv[1].fine.p, fv[2].fine.p, t, nmin, deep, seeding ? 1 : 0, seed,
-                           (int32_t)C);
+                           cstart.p, nlist);
         kt.end(K_SPAWN, s);
         MAEL_LAUNCH_CHECK();
     }

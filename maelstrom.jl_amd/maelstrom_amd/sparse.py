@@ -352,10 +352,10 @@ def maxspeed(s):
     return v.value
 
 
-def substep(s, dt, reflect=False, carried_reflected=False, **solve):
-    """sflip.jl:2295-2318 `substep!(s, dt; reflect, carried)`: one step `dt` carrying the
-    velocity (or the reflected one); with `reflect` the reflected velocity is left in
-    ur, vr, wr for the particles and the next half-step. Returns the solve's stats."""
+def before_projection(s, dt, reflect=False, carried_reflected=False):
+    """The phases of `substep!` before its projection (sflip.jl:2295-2306): refine, advect,
+    particles to grid, redistance, classify, trapped air, forces, and with `reflect` the
+    pre-projection velocity kept for the reflection."""
     L = _lib.lib()
     refine(s)
     s.ctx.check(L.mael_sparse_advect(s.h, float(dt), 1 if carried_reflected else 0, float(s.basin.g), int(s.steps)))
@@ -366,6 +366,14 @@ def substep(s, dt, reflect=False, carried_reflected=False, **solve):
     s.ctx.check(L.mael_sparse_forces(s.h, float(dt), float(s.t), float(s.basin.g)))
     if reflect:
         s.ctx.check(L.mael_sparse_save_reflect(s.h))
+
+
+def substep(s, dt, reflect=False, carried_reflected=False, **solve):
+    """sflip.jl:2295-2318 `substep!(s, dt; reflect, carried)`: one step `dt` carrying the
+    velocity (or the reflected one); with `reflect` the reflected velocity is left in
+    ur, vr, wr for the particles and the next half-step. Returns the solve's stats."""
+    L = _lib.lib()
+    before_projection(s, dt, reflect, carried_reflected)
     st = project(s, dt, **solve)
     if reflect:
         s.ctx.check(L.mael_sparse_kick(s.h))

@@ -118,6 +118,16 @@ __device__ inline void facepos(const SGrid& g, int a, int I, int J, int K, float
     p[2] = g.lo[2] + ((float)K + (a == 2 ? 0.0f : 0.5f)) * h;
 }
 
+// The slot (1-based) of the fine block that holds all of cells (or low faces) I..I+1,
+// J..J+1, K..K+1, or 0 when they straddle blocks or the block is not allocated. Inside
+// one allocated block, fineat and finefaceat return that block's entries (the upper
+// block's copy comes first), so one slot read stands for the eight.
+__device__ inline int32_t oneblock(const STopo& t, int I, int J, int K) {
+    int bi = I >> 4, bj = J >> 4, bk = K >> 4;
+    bool in = ((I & 15) != 15) & ((J & 15) != 15) & ((K & 15) != 15) & inblocks(t.g, bi, bj, bk);
+    return in ? t.slot[blockid(t.g, bi, bj, bk)] : 0;
+}
+
 // sflip.jl:132-134 lerp8.
 __device__ inline float lerp8(const float c[8], float fx, float fy, float fz) {
     float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
@@ -134,6 +144,14 @@ __device__ inline float cellsample(const STopo& t, const V* __restrict__ fine, c
           q2 = __fdiv_rn(pz - g.lo[2], g.dx) - 0.5f;
     float x = floorf(q0), y = floorf(q1), z = floorf(q2);
     int I = (int)x, J = (int)y, K = (int)z;
+    int32_t s1 = oneblock(t, I, J, K);
+    if (s1 != 0) {   // the 8 taps in one allocated block: the indices fineat would give
+        int32_t b = (s1 - 1) * SBCELLS + (I & 15) + 16 * ((J & 15) + 16 * (K & 15));
+        float c[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) c[m] = (float)fine[b + (m & 1) + 16 * ((m >> 1) & 1) + 256 * (m >> 2)];
+        return lerp8(c, q0 - x, q1 - y, q2 - z);
+    }
     int32_t a[8];
     bool all = true;
 #pragma unroll
@@ -161,6 +179,15 @@ __device__ inline float facesample(const STopo& t, const V* __restrict__ fine, c
     float q0 = __fdiv_rn(px - g.lo[0], g.dx) - o0, q1 = __fdiv_rn(py - g.lo[1], g.dx) - o1, q2 = __fdiv_rn(pz - g.lo[2], g.dx) - o2;
     float x = floorf(q0), y = floorf(q1), z = floorf(q2);
     int I = (int)x, J = (int)y, K = (int)z;
+    int32_t s1 = oneblock(t, I, J, K);
+    if (s1 != 0) {   // the 8 taps in one allocated block, which holds its own copy of each
+        int32_t b = (s1 - 1) * SBFACES + ffn(16, a, I & 15, J & 15, K & 15);
+        int dy = a == 0 ? 17 : 16, dz = a == 2 ? 256 : 272;
+        float c[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) c[m] = (float)fine[b + (m & 1) + dy * ((m >> 1) & 1) + dz * (m >> 2)];
+        return lerp8(c, q0 - x, q1 - y, q2 - z);
+    }
     int32_t ix[8];
     bool all = true;
 #pragma unroll

@@ -65,10 +65,11 @@ __global__ __launch_bounds__(256) void k_sadvect_particles(PView ps, int64_t n, 
     svelocity(flow, y0, y1, y2, vf);
     int m = substeps(__fdiv_rn(jmax(speed, sqrtf(vf[0] * vf[0] + vf[1] * vf[1] + vf[2] * vf[2])) * act, h0));
     float h = __fdiv_rn(act, (float)m);
+    const float vs0 = vf[0], vs1 = vf[1], vs2 = vf[2];   // the flow at the start point
     for (int s = 0; s < m; ++s) {
         if (sreached(flow, y0, y1, y2)) {
-            float v1[3];
-            svelocity(flow, y0, y1, y2, v1);
+            float v1[3] = {vs0, vs1, vs2};
+            if (s > 0) svelocity(flow, y0, y1, y2, v1);
             float hh = 0.5f * h;
             svelocity(flow, y0 + hh * v1[0], y1 + hh * v1[1], y2 + hh * v1[2], vf);
             b0 = vf[0]; b1 = vf[1]; b2 = vf[2];
@@ -90,8 +91,8 @@ __device__ inline void backtrace(const SFlow& f, float& px, float& py, float& pz
     int m = substeps(__fdiv_rn(sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * dt, 4.0f * h));
     float k = __fdiv_rn(dt, (float)m);
     for (int s = 0; s < m; ++s) {
-        float v1[3], v2[3];
-        svelocity(f, px, py, pz, v1);
+        float v1[3] = {v[0], v[1], v[2]}, v2[3];   // the first sub-step starts where v was taken
+        if (s > 0) svelocity(f, px, py, pz, v1);
         float hk = 0.5f * k;
         svelocity(f, px - hk * v1[0], py - hk * v1[1], pz - hk * v1[2], v2);
         px = px - k * v2[0]; py = py - k * v2[1]; pz = pz - k * v2[2];
@@ -295,54 +296,58 @@ __device__ inline float boils(const StirParams& P, float x, float y, float t) {
                       __fdiv_rn(0.6f * (y - P.current[1] * t), P.eddyscale) + __fdiv_rn(0.2f * t, P.eddytime));
     return P.boil * (__fdiv_rn(jmax(nn - 0.6f, 0.0f), 0.4f) - 0.15f);
 }
-// basin.jl:139-151 orbital: the waves' velocity at sea coordinates (x, y), depth z, time t.
-__device__ inline void orbital(const StirParams& P, float x, float y, float z, float t, float u[2], float& w) {
-    u[0] = 0.0f; u[1] = 0.0f; w = 0.0f;
+// basin.jl:139-151 orbital: component A of the waves' velocity (A = 0, 1 horizontal, 2
+// vertical) at sea coordinates (x, y), depth z, time t.
+template <int A>
+__device__ inline float orbital(const StirParams& P, float x, float y, float z, float t) {
+    float o = 0.0f;
     for (int k = 0; k < P.nwaves; ++k) {
         const float* W = P.waves + 7 * k;   // d0 d1 k omega a q phase
         float th = W[2] * (W[0] * x + W[1] * y) - W[3] * t + W[6];
-        float s = sinf(th), c = cosf(th);
         float e = W[4] * W[3] * expf(W[2] * jmin(z, 0.0f));
-        u[0] += (e * s) * W[0];
-        u[1] += (e * s) * W[1];
-        w -= e * c;
+        if (A < 2) o += (e * sinf(th)) * W[A];
+        else o -= e * cosf(th);
     }
+    return o;
 }
 __device__ inline float swirl_dev(const StirParams& P, float r) { return oseen_dev(P.Gamma, P.core, r); }
 
-// The stirring acceleration at (x, y, z) for the water's velocity vel (basin.jl:153-181
-// `Stir`, or basin.jl:208-235 `SurfaceStir` when P.surface).
-__device__ inline void stir(const StirParams& P, float x, float y, float z, const float vel[3], float t, float acc[3]) {
+// Component A of the stirring acceleration at (x, y, z) for the water's velocity vel
+// along A (basin.jl:153-181 `Stir`, or basin.jl:208-235 `SurfaceStir` when P.surface).
+// Each component reads only the water's velocity along its own axis, so a face of axis
+// A needs that one sample; the horizontal parts need the eddies, the vertical the boils.
+template <int A>
+__device__ inline float stir(const StirParams& P, float x, float y, float z, float vel, float t) {
     float r = sqrtf(x * x + y * y);
     float k = jclamp(__fdiv_rn(r - (P.radius - P.band), P.band), 0.0f, 1.0f);
     k = k * k * (3.0f - 2.0f * k);
     float v = swirl_dev(P, r);
     float turn = r > 1e-4f ? __fdiv_rn(v, r) : 0.0f;
-    float eddy[2];
-    eddies(P, x, y, t, eddy);
+    float eddy[2] = {0.0f, 0.0f};
+    if (A < 2) eddies(P, x, y, t, eddy);
+    float acc;
     if (!P.surface) {
-        float core = jclamp(2.0f - __fdiv_rn(r, P.core), 0.0f, 1.0f);
-        float hold = (1.0f - k) * jmax(jclamp(__fdiv_rn(-z - P.free, P.free), 0.0f, 1.0f), core);
         float churn = jclamp(__fdiv_rn(r - 2.0f, 2.0f), 0.0f, 1.0f);
-        float aim0 = -turn * y + P.current[0] + churn * eddy[0], aim1 = turn * x + P.current[1] + churn * eddy[1];
-        float gh = __fdiv_rn(hold, P.guide);
-        float guide0 = gh * (aim0 - vel[0]), guide1 = gh * (aim1 - vel[1]);
-        float top = (1.0f - k) * churn * jclamp(1.0f + __fdiv_rn(z, P.free), 0.0f, 1.0f);
-        float pf = P.eddy > 0.0f ? __fdiv_rn(top * P.turbulence, P.eddy) : 0.0f;
-        float push0 = P.eddy > 0.0f ? pf * eddy[0] : 0.0f, push1 = P.eddy > 0.0f ? pf * eddy[1] : 0.0f;
-        float rising = (1.0f - k) * expf(__fdiv_rn(jmin(z, 0.0f), P.reach));
-        acc[0] = guide0 + push0;
-        acc[1] = guide1 + push1;
-        acc[2] = rising * churn * boils(P, x, y, t);
-        if (k == 0.0f) return;
-        float sea[2], up;
-        orbital(P, P.origin[0] + x, P.origin[1] + y, z, t - P.offset, sea, up);
-        float t0 = __fdiv_rn(-v * y, r) + P.current[0] + sea[0], t1 = __fdiv_rn(v * x, r) + P.current[1] + sea[1], t2 = up;
+        if (A < 2) {
+            float core = jclamp(2.0f - __fdiv_rn(r, P.core), 0.0f, 1.0f);
+            float hold = (1.0f - k) * jmax(jclamp(__fdiv_rn(-z - P.free, P.free), 0.0f, 1.0f), core);
+            float aim = A == 0 ? -turn * y + P.current[0] + churn * eddy[0] : turn * x + P.current[1] + churn * eddy[1];
+            float gh = __fdiv_rn(hold, P.guide);
+            float guide = gh * (aim - vel);
+            float top = (1.0f - k) * churn * jclamp(1.0f + __fdiv_rn(z, P.free), 0.0f, 1.0f);
+            float pf = P.eddy > 0.0f ? __fdiv_rn(top * P.turbulence, P.eddy) : 0.0f;
+            float push = P.eddy > 0.0f ? pf * eddy[A] : 0.0f;
+            acc = guide + push;
+        } else {
+            float rising = (1.0f - k) * expf(__fdiv_rn(jmin(z, 0.0f), P.reach));
+            acc = rising * churn * boils(P, x, y, t);
+        }
+        if (k == 0.0f) return acc;
+        float sea = orbital<A>(P, P.origin[0] + x, P.origin[1] + y, z, t - P.offset);
+        float target = A == 0 ? __fdiv_rn(-v * y, r) + P.current[0] + sea
+                              : (A == 1 ? __fdiv_rn(v * x, r) + P.current[1] + sea : sea);
         float kr = __fdiv_rn(k, P.relax);
-        acc[0] = acc[0] + kr * (t0 - vel[0]);
-        acc[1] = acc[1] + kr * (t1 - vel[1]);
-        acc[2] = acc[2] + kr * (t2 - vel[2]);
-        return;
+        return acc + kr * (target - vel);
     }
     int nh = P.nheights;
     float hh = jclamp(__fdiv_rn(r, P.hradius) * (float)(nh - 1), 0.0f, (float)(nh - 1));
@@ -353,55 +358,62 @@ __device__ inline void stir(const StirParams& P, float x, float y, float z, cons
     onset = onset * onset * (3.0f - 2.0f * onset);
     float churn = jclamp(__fdiv_rn(r, 1.3f * P.core), 0.0f, 1.0f);
     churn = churn * churn * (3.0f - 2.0f * churn);
-    float aim0 = -turn * y + onset * churn * eddy[0], aim1 = turn * x + onset * churn * eddy[1];
-    float hold = (1.0f - k) * jclamp(__fdiv_rn(-relative - P.free, P.free), 0.0f, 1.0f);
-    float gh = __fdiv_rn(hold, P.guide);
-    float guide0 = gh * (aim0 - vel[0]), guide1 = gh * (aim1 - vel[1]);
-    float top = (1.0f - k) * churn * expf(__fdiv_rn(jmin(relative, 0.0f), P.free)) * onset;
-    float pf = P.eddy > 0.0f ? __fdiv_rn(top * P.turbulence, P.eddy) : 0.0f;
-    float push0 = P.eddy > 0.0f ? pf * eddy[0] : 0.0f, push1 = P.eddy > 0.0f ? pf * eddy[1] : 0.0f;
-    float rising = (1.0f - k) * churn * expf(__fdiv_rn(jmin(relative, 0.0f), P.reach)) * onset;
-    acc[0] = guide0 + push0;
-    acc[1] = guide1 + push1;
-    acc[2] = rising * boils(P, x, y, t);
-    if (k == 0.0f) return;
-    float sea[2], up;
-    orbital(P, P.origin[0] + x, P.origin[1] + y, relative, t - P.offset, sea, up);
-    float t0 = -turn * y + sea[0], t1 = turn * x + sea[1], t2 = up;
+    if (A < 2) {
+        float aim = A == 0 ? -turn * y + onset * churn * eddy[0] : turn * x + onset * churn * eddy[1];
+        float hold = (1.0f - k) * jclamp(__fdiv_rn(-relative - P.free, P.free), 0.0f, 1.0f);
+        float gh = __fdiv_rn(hold, P.guide);
+        float guide = gh * (aim - vel);
+        float top = (1.0f - k) * churn * expf(__fdiv_rn(jmin(relative, 0.0f), P.free)) * onset;
+        float pf = P.eddy > 0.0f ? __fdiv_rn(top * P.turbulence, P.eddy) : 0.0f;
+        float push = P.eddy > 0.0f ? pf * eddy[A] : 0.0f;
+        acc = guide + push;
+    } else {
+        float rising = (1.0f - k) * churn * expf(__fdiv_rn(jmin(relative, 0.0f), P.reach)) * onset;
+        acc = rising * boils(P, x, y, t);
+    }
+    if (k == 0.0f) return acc;
+    float sea = orbital<A>(P, P.origin[0] + x, P.origin[1] + y, relative, t - P.offset);
+    float target = A == 0 ? -turn * y + sea : (A == 1 ? turn * x + sea : sea);
     float kr = __fdiv_rn(k, P.relax);
-    acc[0] = acc[0] + kr * (t0 - vel[0]);
-    acc[1] = acc[1] + kr * (t1 - vel[1]);
-    acc[2] = acc[2] + kr * (t2 - vel[2]);
+    return acc + kr * (target - vel);
 }
 
 // sflip.jl:992-1016: u += dt (stir - g z) on the faces out of the walls, the flow read from `old`.
-__global__ void k_sforcesfine(float* __restrict__ u, SFlow old, StirParams P, int a, float dt, float time, float gravity,
+template <int A>
+__global__ void k_sforcesfine(float* __restrict__ u, SFlow old, StirParams P, float dt, float time, float gravity,
                               int32_t nfa) {
     int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nfa) return;
-    SCell c = fineface(old.t, a, n);
+    SCell c = fineface(old.t, A, n);
     if (c.w.x < 0) return;
     float p[3];
-    facepos(old.t.g, a, c.w.x * 16 + c.i, c.w.y * 16 + c.j, c.w.z * 16 + c.k, old.t.g.dx, p);
+    facepos(old.t.g, A, c.w.x * 16 + c.i, c.w.y * 16 + c.j, c.w.z * 16 + c.k, old.t.g.dx, p);
     if (sissolid(old.t.g, p[0], p[1], p[2])) return;
-    float vel[3], acc[3];
-    svelocity(old, p[0], p[1], p[2], vel);
-    stir(P, p[0], p[1], p[2], vel, time, acc);
-    u[n] += dt * (a == 0 ? acc[0] : (a == 1 ? acc[1] : acc[2] - gravity));
+    float vel = facesample(old.t, old.uf[A], old.uc[A], A, p[0], p[1], p[2]);
+    float acc = stir<A>(P, p[0], p[1], p[2], vel, time);
+    u[n] += dt * (A == 2 ? acc - gravity : acc);
 }
-__global__ void k_sforcescoarse(float* __restrict__ u, SFlow old, StirParams P, int a, float dt, float time, float gravity) {
+template <int A>
+__global__ void k_sforcescoarse(float* __restrict__ u, SFlow old, StirParams P, float dt, float time, float gravity) {
     const SGrid& g = old.t.g;
-    int n0 = g.nc(0) + (a == 0), n1 = g.nc(1) + (a == 1), n2 = g.nc(2) + (a == 2);
+    int n0 = g.nc(0) + (A == 0), n1 = g.nc(1) + (A == 1), n2 = g.nc(2) + (A == 2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
     int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
     float p[3];
-    facepos(g, a, I, J, K, g.cdx(), p);
+    facepos(g, A, I, J, K, g.cdx(), p);
     if (sissolid(g, p[0], p[1], p[2])) return;
-    float vel[3], acc[3];
-    svelocity(old, p[0], p[1], p[2], vel);
-    stir(P, p[0], p[1], p[2], vel, time, acc);
-    u[l] += dt * (a == 0 ? acc[0] : (a == 1 ? acc[1] : acc[2] - gravity));
+    float vel = facesample(old.t, old.uf[A], old.uc[A], A, p[0], p[1], p[2]);
+    float acc = stir<A>(P, p[0], p[1], p[2], vel, time);
+    u[l] += dt * (A == 2 ? acc - gravity : acc);
+}
+template <int A>
+static void launch_forces(SparseBasin& b, const SFlow& old, const StirParams& P, float dt, float time, float gacc) {
+    if (b.nslots)
+        hipLaunchKernelGGL(k_sforcesfine<A>, dim3(cdiv(b.finefaces(), 256)), dim3(256), 0, b.st.s, b.u[A].fine.p, old, P, dt,
+                           time, gacc, (int32_t)b.finefaces());
+    hipLaunchKernelGGL(k_sforcescoarse<A>, dim3(cdiv(b.nfaces(A), 256)), dim3(256), 0, b.st.s, b.u[A].coarse.p, old, P, dt,
+                       time, gacc);
 }
 
 void SparseBasin::forces(float dt, float time, float gacc) {
@@ -419,13 +431,9 @@ void SparseBasin::forces(float dt, float time, float gacc) {
     StirParams P = stirp;
     P.waves = stir_waves.p;
     P.heights = stir_heights.p;
-    for (int a = 0; a < 3; ++a) {
-        if (nslots)
-            hipLaunchKernelGGL(k_sforcesfine, dim3(cdiv(finefaces(), 256)), dim3(256), 0, s, u[a].fine.p, old, P, a, dt, time,
-                               gacc, (int32_t)finefaces());
-        hipLaunchKernelGGL(k_sforcescoarse, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, s, u[a].coarse.p, old, P, a, dt, time,
-                           gacc);
-    }
+    launch_forces<0>(*this, old, P, dt, time, gacc);
+    launch_forces<1>(*this, old, P, dt, time, gacc);
+    launch_forces<2>(*this, old, P, dt, time, gacc);
     MAEL_LAUNCH_CHECK();
     solidify();
     restrict_velocity();
@@ -466,9 +474,14 @@ __global__ void k_sextrapfine(float* __restrict__ out, uint8_t* __restrict__ fou
     int m = 0;
     float p[3];
     facepos(t.g, a, I, J, K, t.g.dx, p);
+    // away from the block's sides the six neighbours are this block's own faces
+    const bool inner = (unsigned)(c.i - 1) < 14u && (unsigned)(c.j - 1) < 14u && (unsigned)(c.k - 1) < 14u;
+    const int dy = a == 0 ? 17 : 16, dz = a == 2 ? 256 : 272;
     if (!walledface_dev(t.g, p, a, t.g.dx)) {
+#pragma unroll
         for (int q = 0; q < 6; ++q) {
-            int32_t nidx = finefaceat(t, a, I + NBX[q][0], J + NBX[q][1], K + NBX[q][2]);
+            const int step = (q & 1 ? -1 : 1) * (q < 2 ? 1 : (q < 4 ? dy : dz));
+            int32_t nidx = inner ? n + step : finefaceat(t, a, I + NBX[q][0], J + NBX[q][1], K + NBX[q][2]);
             if (nidx >= 0 && fin[nidx]) {
                 acc += src[nidx];
                 if (out2) acc2 += src2[nidx];

@@ -611,15 +611,20 @@ __global__ void k_sreinit(float* __restrict__ out, const float* __restrict__ phi
     int I = c.w.x * 16 + c.i, J = c.w.y * 16 + c.j, K = c.w.z * 16 + c.k;
     float cv = phi[n], h = g.dx, p[3];
     finecentre(g, I, J, K, p);
+    // away from the block's faces the six neighbours are this block's own cells
+    const bool inner = (unsigned)(c.i - 1) < 14u && (unsigned)(c.j - 1) < 14u && (unsigned)(c.k - 1) < 14u;
+    constexpr int step[6] = {1, -1, 16, -16, 256, -256};
+    auto nb = [&](int m, int a, int b, int cc) { return inner ? phi[n + step[m]] : nphi(t, phi, phic, a, b, cc); };
     float res;
     if (sissolid(g, p[0], p[1], p[2])) {
         float lowest = cv;
         bool found = false;
+#pragma unroll
         for (int m = 0; m < 6; ++m) {
             int a = I + NBS[m][0], b = J + NBS[m][1], cc = K + NBS[m][2];
             finecentre(g, a, b, cc, p);
             if (!sissolid(g, p[0], p[1], p[2])) {
-                float f = nphi(t, phi, phic, a, b, cc);
+                float f = nb(m, a, b, cc);
                 lowest = found ? jmin(lowest, f) : f;
                 found = true;
             }
@@ -628,11 +633,12 @@ __global__ void k_sreinit(float* __restrict__ out, const float* __restrict__ phi
     } else {
         bool inside = cv < 0.0f, frozen = false;
         float ax = phimax, ay = phimax, az = phimax;
+#pragma unroll
         for (int m = 0; m < 6; ++m) {
             int a = I + NBS[m][0], b = J + NBS[m][1], cc = K + NBS[m][2];
             finecentre(g, a, b, cc, p);
             if (!sissolid(g, p[0], p[1], p[2])) {
-                float f = nphi(t, phi, phic, a, b, cc);
+                float f = nb(m, a, b, cc);
                 if ((f < 0.0f) != inside) frozen = true;
                 float d = fabsf(f);
                 if (m < 2) ax = jmin(ax, d);

@@ -457,126 +457,211 @@ struct SearchArgs {
 // sflip.jl:1218-1269 on the fine cells of a level: mode 0 Jacobi (out = x + w (b - A x) /
 // diag), 1 residual (out = b - A x), 2 apply (out = A x), 3 Jacobi from x = 0 (x not
 // read); inactive cells and the cells of free slots get 0.
-// Level 0, matrix-free and staged in LDS: a workgroup takes a quarter of a
-// slot (4 z planes of its 16^3 block) and stages x and the cell types of those planes and
-// of the layers around them (the block's other planes, the face neighbours' slots, or
-// else the coarse cells: the value clamped to the grid, the type SOLID off it, flagged
-// coarse); each thread then computes 4 cells from LDS, with its cells' diagonal and
-// right-hand side loaded before the staging barrier. Level 0's couplings are 1 between
-// LIQUID fine cells, 0.4 between a LIQUID fine cell and a LIQUID coarse one, else 0
-// (k_finecoef, sflip.jl:1323-1342), so they follow from the staged types: the same values
-// as the coupling arrays, summed in finerow's order. Free slots get 0.
-constexpr int HX = 18, HZ = 6, HV = HX * HX * HZ;   // a staged quarter block with its halo
-constexpr uint8_t COARSE_FLAG = 0x80;
+// Level 0, matrix-free and staged in LDS. A workgroup takes a quarter of a slot (4 z
+// planes of its 16^3 block), a thread 4 consecutive cells along x of one row. Level 0's
+// coupling of a LIQUID fine cell to a neighbour is 1 when the neighbour is a LIQUID fine
+// cell, 0.4 when it is a LIQUID coarse cell, else 0 (k_finecoef, sflip.jl:1323-1342), so
+// it depends on the neighbour alone: the workgroup stages, for its cells and the layers
+// around them (the block's other planes, the face neighbours' slots, or else the coarse
+// cells: the value clamped to the grid, the type SOLID off it), each cell's coupling w and
+// the product w x (0 where w is 0), once per cell. A LIQUID cell's row then adds its six
+// neighbours' w into the diagonal and their w x into the off-diagonal sum in finerow's
+// order; a zero coupling adds exactly nothing, so the sums equal those over the coupling
+// arrays bit for bit. Loads and stores of a thread's own cells are 4-wide.
+template <class T> struct alignas(4 * sizeof(T)) Four { T v[4]; };
+template <class T> __device__ inline Four<T> load4(const T* __restrict__ p, uint32_t i) {
+    return *reinterpret_cast<const Four<T>*>(p + i);
+}
+constexpr int RS = 24, PS = 18 * RS, HV = 6 * PS;   // LDS rows of 24: cell li (-1..16) at 4 + li
 template <int MODE, class T, class C>
 __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
                                                    const T* __restrict__ xc, CompT<C> L, STopo t, SearchArgs S) {
     constexpr bool Z = MODE == 3, SEARCH = MODE == 4;
-    __shared__ T sx[Z ? 1 : HV];
-    __shared__ uint8_t sty[HV];
+    __shared__ float4 sw4[HV / 4];             // the couplings
+    __shared__ float4 sp4[Z ? 1 : HV / 4];     // coupling times x
     __shared__ double lds[4];
+    float* sw = reinterpret_cast<float*>(sw4);
+    float* sp = reinterpret_cast<float*>(sp4);
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int s = (int)(tile >> 2) + 1, k0 = (int)(tile & 3) * 4;
     const uint32_t cb = (uint32_t)(s - 1) * 4096u;
     const uint8_t* __restrict__ tf = L.actf;   // level 0: the fine cell types
     const float beta = SEARCH ? (float)S.scal[SS_BETA] : 0.0f;
-    // x, or in the fused search the direction z + beta d_old
-    auto xval = [&](const T* __restrict__ xs, const float* __restrict__ ds, size_t i) -> T {
-        if constexpr (SEARCH) return sf<T>(lf(xs[i]) + beta * ds[i]);
-        else return xs[i];
+    // x as the row reads it: in the fused search the direction z + beta d_old, rounded to T
+    auto xat = [&](const T* __restrict__ xs, const float* __restrict__ ds, size_t i) -> float {
+        if constexpr (SEARCH) return lf(sf<T>(lf(xs[i]) + beta * ds[i]));
+        else return lf(xs[i]);
     };
+    const int tid = threadIdx.x, i4 = tid & 3, j = (tid >> 2) & 15, kk = tid >> 6;
+    const uint32_t n = cb + 256u * (uint32_t)(k0 + kk) + 16u * (uint32_t)j + 4u * (uint32_t)i4;
     int4 w = t.where[s - 1];
     if (w.x < 0) {
-        for (int c = threadIdx.x; c < 1024; c += 256) {
-            uint32_t n = cb + 256 * k0 + c;
-            out[n] = sf<T>(0.0f);
-            if constexpr (SEARCH) S.dnf[n] = lf(xval(x, S.dof, n));
-        }
-        if constexpr (SEARCH)
+        Four<T> z4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) z4.v[e] = sf<T>(0.0f);
+        *reinterpret_cast<Four<T>*>(out + n) = z4;
+        if constexpr (SEARCH) {
+            float4 d4;
+            d4.x = xat(x, S.dof, n); d4.y = xat(x, S.dof, n + 1); d4.z = xat(x, S.dof, n + 2); d4.w = xat(x, S.dof, n + 3);
+            *reinterpret_cast<float4*>(S.dnf + n) = d4;
             if (threadIdx.x == 0) S.partials[tile] = 0.0;
+        }
         return;
     }
-    // this thread's cells: c = threadIdx.x + 256 e, e = 0..3, i.e. plane k0 + e
-    float dd[4], bb[4];
+    // the thread's own cells: types, diagonal, right-hand side, x
+    const uint32_t tq = *reinterpret_cast<const uint32_t*>(tf + n);
+    const Four<C> dq = load4(L.dDf, n);
+    Four<T> bq;
+    if constexpr (MODE == 2 || MODE == 4) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bq.v[e] = sf<T>(0.0f);
+    } else {
+        bq = load4(b, n);
+    }
+    float x0[4], w0[4], p0[4];
+    if constexpr (!Z) {
+        const Four<T> xq = load4(x, n);
+        if constexpr (SEARCH) {
+            const float4 dold = *reinterpret_cast<const float4*>(S.dof + n);
+            const float dv[4] = {dold.x, dold.y, dold.z, dold.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) x0[e] = lf(sf<T>(lf(xq.v[e]) + beta * dv[e]));
+            *reinterpret_cast<float4*>(S.dnf + n) = make_float4(x0[0], x0[1], x0[2], x0[3]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) x0[e] = lf(xq.v[e]);
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) x0[e] = 0.0f;
+    }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        uint32_t n = cb + 256 * (k0 + e) + threadIdx.x;
-        dd[e] = lf(L.dDf[n]);
-        bb[e] = (MODE == 2 || MODE == 4) ? 0.0f : lf(b[n]);
+        w0[e] = ((tq >> (8 * e)) & 0xffu) == LIQUID ? FINE_FINE : 0.0f;
+        p0[e] = w0[e] > 0.0f ? w0[e] * x0[e] : 0.0f;
     }
+    const int hq = 4 + 4 * i4 + RS * (j + 1) + PS * (kk + 1);   // LDS place of the first cell
+    sw4[hq >> 2] = make_float4(w0[0], w0[1], w0[2], w0[3]);
+    if constexpr (!Z) sp4[hq >> 2] = make_float4(p0[0], p0[1], p0[2], p0[3]);
+
     int32_t nb[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) nb[d] = nbslot(t, w, d);
     const int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
-    // stage cell (li, lj, lk) of the block's frame (-1..16) at LDS place h
-    auto stage = [&](int li, int lj, int lk, int h) {
-        bool inside = li >= 0 && li < 16 && lj >= 0 && lj < 16 && lk >= 0 && lk < 16;
-        int d = li < 0 ? 0 : li > 15 ? 1 : lj < 0 ? 2 : lj > 15 ? 3 : lk < 0 ? 4 : 5;
-        int32_t sn = inside ? s : nb[d];
-        if (sn != 0) {
-            uint32_t idx = (uint32_t)(sn - 1) * 4096u + (uint32_t)((li & 15) + 16 * ((lj & 15) + 16 * (lk & 15)));
-            if constexpr (!Z) sx[h] = xval(x, S.dof, idx);
-            sty[h] = tf[idx];
-        } else {
-            int ci = (w.x * 16 + li) >> 2, cj = (w.y * 16 + lj) >> 2, ck = (w.z * 16 + lk) >> 2;
-            bool in = ci >= 0 && cj >= 0 && ck >= 0 && ci < n0 && cj < n1 && ck < n2;
-            size_t cidx = cl(min(max(ci, 0), n0 - 1), min(max(cj, 0), n1 - 1), min(max(ck, 0), n2 - 1), n0, n1);
-            if constexpr (!Z) sx[h] = xval(xc, S.doc, cidx);
-            sty[h] = (uint8_t)((in ? L.tc[cidx] : SOLID) | COARSE_FLAG);
-        }
+    // a staged cell outside the slot with no fine block there: the coarse cell, clamped
+    auto coarse = [&](int li, int lj, int lk, float& wc, float& pc) {
+        int ci = (w.x * 16 + li) >> 2, cj = (w.y * 16 + lj) >> 2, ck = (w.z * 16 + lk) >> 2;
+        bool in = ci >= 0 && cj >= 0 && ck >= 0 && ci < n0 && cj < n1 && ck < n2;
+        size_t cidx = cl(min(max(ci, 0), n0 - 1), min(max(cj, 0), n1 - 1), min(max(ck, 0), n2 - 1), n0, n1);
+        wc = (in ? L.tc[cidx] : SOLID) == LIQUID ? FINE_COARSE : 0.0f;
+        float xv = Z ? 0.0f : xat(xc, S.doc, cidx);
+        pc = wc > 0.0f ? wc * xv : 0.0f;
     };
-    // the 4 planes (1024 cells), then the two z layers (512), then the x and y sides (256)
-    for (int c = threadIdx.x; c < 1024; c += 256) {
-        int i = c & 15, j = (c >> 4) & 15, kk = c >> 8;
-        uint32_t idx = cb + 256 * (k0 + kk) + (c & 255);
-        int h = (i + 1) + HX * ((j + 1) + HX * (kk + 1));
-        if constexpr (!Z) {
-            T v = xval(x, S.dof, idx);
-            sx[h] = v;
-            if constexpr (SEARCH) S.dnf[idx] = lf(v);
-        }
-        sty[h] = tf[idx];
-    }
-    for (int e = threadIdx.x; e < 512; e += 256) {
-        int i = e & 15, j = (e >> 4) & 15, top = e >> 8, kk = top ? 4 : -1;
-        stage(i, j, k0 + kk, (i + 1) + HX * ((j + 1) + HX * (kk + 1)));
-    }
-    {
-        int e = threadIdx.x, side = e >> 6, p = e & 15, kk = (e >> 4) & 3;
-        int li = side == 0 ? -1 : side == 1 ? 16 : p, lj = side == 2 ? -1 : side == 3 ? 16 : p;
-        stage(li, lj, k0 + kk, (li + 1) + HX * ((lj + 1) + HX * (kk + 1)));
-    }
-    __syncthreads();
-    double pq = 0.0;
+    // 4 cells along x from li0 (a multiple of 4) at row lj, plane lk, in this slot or else
+    // in the face neighbour's slot `across` (0: none), staged at LDS place h
+    auto stage4 = [&](int li0, int lj, int lk, int32_t across, int h) {
+        bool inside = lj >= 0 && lj < 16 && lk >= 0 && lk < 16;
+        int32_t sn = inside ? s : across;
+        float wv[4], pv[4];
+        if (sn != 0) {
+            uint32_t idx = (uint32_t)(sn - 1) * 4096u + (uint32_t)(li0 + 16 * ((lj & 15) + 16 * (lk & 15)));
+            uint32_t ty = *reinterpret_cast<const uint32_t*>(tf + idx);
+            float xv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if constexpr (!Z) {
+                const Four<T> x4 = load4(x, idx);
+                if constexpr (SEARCH) {
+                    const float4 d4 = *reinterpret_cast<const float4*>(S.dof + idx);
+                    const float dv[4] = {d4.x, d4.y, d4.z, d4.w};
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        int c = threadIdx.x, i = c & 15, j = c >> 4;
-        int h = (i + 1) + HX * ((j + 1) + HX * (e + 1));
-        uint32_t n = cb + 256 * (k0 + e) + c;
-        uint8_t t0 = sty[h];
-        bool act = (MODE == 2 || SEARCH) ? t0 == LIQUID : t0 != 0;
-        float res = 0.0f;
-        if (act) {
-            const int off6[6] = {-1, 1, -HX, HX, -HX * HX, HX * HX};
-            float diag = dd[e], off = 0.0f;
+                    for (int e = 0; e < 4; ++e) xv[e] = lf(sf<T>(lf(x4.v[e]) + beta * dv[e]));
+                } else {
 #pragma unroll
-            for (int d = 0; d < 6; ++d) {
-                uint8_t tn = sty[h + off6[d]];
-                float cf = (t0 == LIQUID && (tn & 0x7f) == LIQUID) ? ((tn & COARSE_FLAG) ? FINE_COARSE : FINE_FINE) : 0.0f;
-                if (cf > 0.0f) {
-                    diag += cf;
-                    off += cf * (Z ? 0.0f : lf(sx[Z ? 0 : h + off6[d]]));
+                    for (int e = 0; e < 4; ++e) xv[e] = lf(x4.v[e]);
                 }
             }
-            float x0 = Z ? 0.0f : lf(sx[Z ? 0 : h]);
-            float ax = diag * x0 - off;
-            if (MODE == 0 || MODE == 3) res = diag > 0.0f ? x0 + __fdiv_rn(OMEGA * (bb[e] - ax), diag) : 0.0f;
-            else if (MODE == 1) res = bb[e] - ax;
-            else res = ax;
-            if constexpr (SEARCH) pq += (double)x0 * (double)res;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                wv[e] = ((ty >> (8 * e)) & 0xffu) == LIQUID ? FINE_FINE : 0.0f;
+                pv[e] = wv[e] > 0.0f ? wv[e] * xv[e] : 0.0f;
+            }
+        } else {
+            float wc, pc;
+            coarse(li0, lj, lk, wc, pc);   // the 4 cells share one coarse cell
+#pragma unroll
+            for (int e = 0; e < 4; ++e) { wv[e] = wc; pv[e] = pc; }
         }
-        out[n] = sf<T>(res);
+        sw4[h >> 2] = make_float4(wv[0], wv[1], wv[2], wv[3]);
+        if constexpr (!Z) sp4[h >> 2] = make_float4(pv[0], pv[1], pv[2], pv[3]);
+    };
+    if (tid < 128) {   // the z layers below and above the quarter: 2 x 16 rows x 4 quads
+        int top = tid >> 6, lj = (tid >> 2) & 15, q = tid & 3;
+        int lk = top ? k0 + 4 : k0 - 1;
+        stage4(4 * q, lj, lk, top ? nb[5] : nb[4], 4 + 4 * q + RS * (lj + 1) + PS * (top ? 5 : 0));
+    } else {           // the x sides: 2 x 16 rows x 4 planes, one cell each
+        int e = tid - 128, side = e >> 6, lj = (e >> 2) & 15, p = e & 3;
+        int li = side ? 16 : -1, lk = k0 + p;
+        int h = 4 + li + RS * (lj + 1) + PS * (p + 1);
+        int32_t sn = side ? nb[1] : nb[0];
+        float wv, pv;
+        if (sn != 0) {
+            uint32_t idx = (uint32_t)(sn - 1) * 4096u + (uint32_t)((li & 15) + 16 * (lj + 16 * lk));
+            wv = tf[idx] == LIQUID ? FINE_FINE : 0.0f;
+            float xv = Z ? 0.0f : xat(x, S.dof, idx);
+            pv = wv > 0.0f ? wv * xv : 0.0f;
+        } else {
+            coarse(li, lj, lk, wv, pv);
+        }
+        sw[h] = wv;
+        if constexpr (!Z) sp[h] = pv;
     }
+    if (tid < 32) {    // the y sides: 2 x 4 planes x 4 quads
+        int side = tid >> 4, p = (tid >> 2) & 3, q = tid & 3;
+        int lj = side ? 16 : -1;
+        stage4(4 * q, lj, k0 + p, side ? nb[3] : nb[2], 4 + 4 * q + RS * (lj + 1) + PS * (p + 1));
+    }
+    __syncthreads();
+
+    // the neighbours' couplings and products: x within the quad from registers, the
+    // quad's x ends, y and z from LDS; order -x, +x, -y, +y, -z, +z
+    const float4 wym = sw4[(hq - RS) >> 2], wyp = sw4[(hq + RS) >> 2], wzm = sw4[(hq - PS) >> 2], wzp = sw4[(hq + PS) >> 2];
+    const float wl = sw[hq - 1], wr = sw[hq + 4];
+    float4 pym = make_float4(0, 0, 0, 0), pyp = pym, pzm = pym, pzp = pym;
+    float pl = 0.0f, pr = 0.0f;
+    if constexpr (!Z) {
+        pym = sp4[(hq - RS) >> 2]; pyp = sp4[(hq + RS) >> 2]; pzm = sp4[(hq - PS) >> 2]; pzp = sp4[(hq + PS) >> 2];
+        pl = sp[hq - 1]; pr = sp[hq + 4];
+    }
+    const float wy0[4] = {wym.x, wym.y, wym.z, wym.w}, wy1[4] = {wyp.x, wyp.y, wyp.z, wyp.w};
+    const float wz0[4] = {wzm.x, wzm.y, wzm.z, wzm.w}, wz1[4] = {wzp.x, wzp.y, wzp.z, wzp.w};
+    const float py0[4] = {pym.x, pym.y, pym.z, pym.w}, py1[4] = {pyp.x, pyp.y, pyp.z, pyp.w};
+    const float pz0[4] = {pzm.x, pzm.y, pzm.z, pzm.w}, pz1[4] = {pzp.x, pzp.y, pzp.z, pzp.w};
+    double pq = 0.0;
+    Four<T> oq;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint32_t t0 = (tq >> (8 * e)) & 0xffu;
+        const float dd = lf(dq.v[e]), bb = lf(bq.v[e]);
+        const float wn[6] = {e == 0 ? wl : w0[e - 1], e == 3 ? wr : w0[e + 1], wy0[e], wy1[e], wz0[e], wz1[e]};
+        const float pn[6] = {e == 0 ? pl : p0[e - 1], e == 3 ? pr : p0[e + 1], py0[e], py1[e], pz0[e], pz1[e]};
+        float diag = dd, off = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            diag = diag + wn[d];
+            off = off + pn[d];
+        }
+        if (t0 != LIQUID) { diag = dd; off = 0.0f; }   // only a LIQUID cell couples
+        const bool act = (MODE == 2 || SEARCH) ? t0 == LIQUID : t0 != 0;
+        float res = 0.0f;
+        if (act) {
+            float ax = diag * x0[e] - off;
+            if (MODE == 0 || MODE == 3) res = diag > 0.0f ? x0[e] + __fdiv_rn(OMEGA * (bb - ax), diag) : 0.0f;
+            else if (MODE == 1) res = bb - ax;
+            else res = ax;
+            if constexpr (SEARCH) pq += (double)x0[e] * (double)res;
+        }
+        oq.v[e] = sf<T>(res);
+    }
+    *reinterpret_cast<Four<T>*>(out + n) = oq;
     if constexpr (SEARCH) {
         double sum = block_sum_red(pq, lds);
         if (threadIdx.x == 0) S.partials[tile] = sum;
@@ -877,7 +962,6 @@ __global__ __launch_bounds__(256) void k_prolong10(T* __restrict__ x0f, const T*
 // Conversions between the Krylov vectors (FP32) and the reduced-precision hierarchy:
 // y = x * mult * scal[slot] (or * mult when slot < 0), over the fine and the coarse range,
 // four values a thread (both parts are multiples of 4 long).
-template <class T> struct alignas(4 * sizeof(T)) Four { T v[4]; };
 template <class Y, class X>
 __global__ void k_convert(Y* __restrict__ yf, const X* __restrict__ xf, size_t nf, Y* __restrict__ yc, const X* __restrict__ xc,
                           size_t nc, const double* __restrict__ scal, int slot, int invert) {

@@ -118,14 +118,29 @@ __device__ inline void facepos(const SGrid& g, int a, int I, int J, int K, float
     p[2] = g.lo[2] + ((float)K + (a == 2 ? 0.0f : 0.5f)) * h;
 }
 
-// The slot (1-based) of the fine block that holds all of cells (or low faces) I..I+1,
-// J..J+1, K..K+1, or 0 when they straddle blocks or the block is not allocated. Inside
-// one allocated block, fineat and finefaceat return that block's entries (the upper
-// block's copy comes first), so one slot read stands for the eight.
-__device__ inline int32_t oneblock(const STopo& t, int I, int J, int K) {
-    int bi = I >> 4, bj = J >> 4, bk = K >> 4;
-    bool in = ((I & 15) != 15) & ((J & 15) != 15) & ((K & 15) != 15) & inblocks(t.g, bi, bj, bk);
-    return in ? t.slot[blockid(t.g, bi, bj, bk)] : 0;
+// The slots (1-based, 0 for none) of the fine blocks that hold the eight trilinear taps
+// I..I+1, J..J+1, K..K+1 (tap m at +(m & 1, m >> 1 & 1, m >> 2)): the lower corner's block,
+// and across a block side for the taps past it. When all eight are allocated, fineat and
+// finefaceat return each tap's entry in its own block (the upper block's copy of a face
+// comes first), so the taps index from these slots with no further table reads.
+__device__ inline bool tapslots(const STopo& t, int I, int J, int K, int32_t s[8]) {
+    const SGrid& g = t.g;
+    const int bx = I >> 4, by = J >> 4, bz = K >> 4;
+    const int sx = (I & 15) == 15, sy = (J & 15) == 15, sz = (K & 15) == 15;
+    const bool inx[2] = {bx >= 0 && bx < g.nb[0], bx + sx >= 0 && bx + sx < g.nb[0]};
+    const bool iny[2] = {by >= 0 && by < g.nb[1], by + sy >= 0 && by + sy < g.nb[1]};
+    const bool inz[2] = {bz >= 0 && bz < g.nb[2], bz + sz >= 0 && bz + sz < g.nb[2]};
+    const int nx = g.nb[0], nxy = g.nb[0] * g.nb[1];
+    const int id0 = bx + nx * by + nxy * bz;
+    bool all = true;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int dx = m & 1, dy = (m >> 1) & 1, dz = m >> 2;
+        const bool in = inx[dx] && iny[dy] && inz[dz];
+        s[m] = in ? t.slot[id0 + dx * sx + nx * (dy * sy) + nxy * (dz * sz)] : 0;
+        all = all && s[m] != 0;
+    }
+    return all;
 }
 
 // sflip.jl:132-134 lerp8.
@@ -144,25 +159,13 @@ __device__ inline float cellsample(const STopo& t, const V* __restrict__ fine, c
           q2 = __fdiv_rn(pz - g.lo[2], g.dx) - 0.5f;
     float x = floorf(q0), y = floorf(q1), z = floorf(q2);
     int I = (int)x, J = (int)y, K = (int)z;
-    int32_t s1 = oneblock(t, I, J, K);
-    if (s1 != 0) {   // the 8 taps in one allocated block: the indices fineat would give
-        int32_t b = (s1 - 1) * SBCELLS + (I & 15) + 16 * ((J & 15) + 16 * (K & 15));
+    int32_t s8[8];
+    if (tapslots(t, I, J, K, s8)) {   // the 8 taps in allocated blocks: fineat's indices
         float c[8];
 #pragma unroll
-        for (int m = 0; m < 8; ++m) c[m] = (float)fine[b + (m & 1) + 16 * ((m >> 1) & 1) + 256 * (m >> 2)];
-        return lerp8(c, q0 - x, q1 - y, q2 - z);
-    }
-    int32_t a[8];
-    bool all = true;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        a[m] = fineat(t, I + (m & 1), J + ((m >> 1) & 1), K + ((m >> 2) & 1));
-        all = all && a[m] >= 0;
-    }
-    if (all) {
-        float c[8];
-#pragma unroll
-        for (int m = 0; m < 8; ++m) c[m] = (float)fine[a[m]];
+        for (int m = 0; m < 8; ++m)
+            c[m] = (float)fine[(s8[m] - 1) * SBCELLS + ((I + (m & 1)) & 15) +
+                               16 * (((J + ((m >> 1) & 1)) & 15) + 16 * ((K + (m >> 2)) & 15))];
         return lerp8(c, q0 - x, q1 - y, q2 - z);
     }
     float cd = g.cdx();
@@ -170,30 +173,45 @@ __device__ inline float cellsample(const STopo& t, const V* __restrict__ fine, c
                      __fdiv_rn(pz - g.lo[2], cd) - 0.5f);
 }
 
-// sflip.jl:157-175 facesample.
-template <class V>
-__device__ inline float facesample(const STopo& t, const V* __restrict__ fine, const V* __restrict__ coarse, int a, float px,
-                                   float py, float pz) {
+// sflip.jl:157-175 facesample, for faces along A. A tap whose own block is not allocated
+// can still be fine: the top face of the block below along A, held there at coordinate 16
+// (finefaceat's second choice). That block is the one of the tap's neighbour below along
+// A when the taps straddle the plane, else one table read.
+template <int A, class V>
+__device__ inline float facesample_a(const STopo& t, const V* __restrict__ fine, const V* __restrict__ coarse, float px,
+                                     float py, float pz) {
     const SGrid& g = t.g;
-    float o0 = a == 0 ? 0.0f : 0.5f, o1 = a == 1 ? 0.0f : 0.5f, o2 = a == 2 ? 0.0f : 0.5f;
+    constexpr float o0 = A == 0 ? 0.0f : 0.5f, o1 = A == 1 ? 0.0f : 0.5f, o2 = A == 2 ? 0.0f : 0.5f;
     float q0 = __fdiv_rn(px - g.lo[0], g.dx) - o0, q1 = __fdiv_rn(py - g.lo[1], g.dx) - o1, q2 = __fdiv_rn(pz - g.lo[2], g.dx) - o2;
     float x = floorf(q0), y = floorf(q1), z = floorf(q2);
     int I = (int)x, J = (int)y, K = (int)z;
-    int32_t s1 = oneblock(t, I, J, K);
-    if (s1 != 0) {   // the 8 taps in one allocated block, which holds its own copy of each
-        int32_t b = (s1 - 1) * SBFACES + ffn(16, a, I & 15, J & 15, K & 15);
-        int dy = a == 0 ? 17 : 16, dz = a == 2 ? 256 : 272;
-        float c[8];
-#pragma unroll
-        for (int m = 0; m < 8; ++m) c[m] = (float)fine[b + (m & 1) + dy * ((m >> 1) & 1) + dz * (m >> 2)];
-        return lerp8(c, q0 - x, q1 - y, q2 - z);
-    }
+    int32_t s8[8];
+    bool all = tapslots(t, I, J, K, s8);
     int32_t ix[8];
-    bool all = true;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        ix[m] = finefaceat(t, a, I + (m & 1), J + ((m >> 1) & 1), K + ((m >> 2) & 1));
-        all = all && ix[m] >= 0;
+    for (int m = 0; m < 8; ++m)
+        ix[m] = (s8[m] - 1) * SBFACES + ffn(16, A, (I + (m & 1)) & 15, (J + ((m >> 1) & 1)) & 15, (K + (m >> 2)) & 15);
+    if (!all) {
+        all = true;
+        const int Ic = A == 0 ? I : (A == 1 ? J : K);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            if (s8[m] != 0) continue;
+            const int dc = (m >> A) & 1;
+            int32_t lower = 0;
+            if (((Ic + dc) & 15) == 0) {
+                if (dc == 1) {
+                    lower = s8[m ^ (1 << A)];
+                } else {
+                    int bx = (I + (m & 1)) >> 4, by = (J + ((m >> 1) & 1)) >> 4, bz = (K + (m >> 2)) >> 4;
+                    bx -= A == 0; by -= A == 1; bz -= A == 2;
+                    lower = inblocks(g, bx, by, bz) ? t.slot[blockid(g, bx, by, bz)] : 0;
+                }
+            }
+            const int li = (I + (m & 1)) & 15, lj = (J + ((m >> 1) & 1)) & 15, lk = (K + (m >> 2)) & 15;
+            ix[m] = (lower - 1) * SBFACES + ffn(16, A, A == 0 ? 16 : li, A == 1 ? 16 : lj, A == 2 ? 16 : lk);
+            all = all && lower != 0;
+        }
     }
     if (all) {
         float c[8];
@@ -202,8 +220,14 @@ __device__ inline float facesample(const STopo& t, const V* __restrict__ fine, c
         return lerp8(c, q0 - x, q1 - y, q2 - z);
     }
     float cd = g.cdx();
-    return trilinear(coarse, g.nc(0) + (a == 0), g.nc(1) + (a == 1), g.nc(2) + (a == 2), __fdiv_rn(px - g.lo[0], cd) - o0,
+    return trilinear(coarse, g.nc(0) + (A == 0), g.nc(1) + (A == 1), g.nc(2) + (A == 2), __fdiv_rn(px - g.lo[0], cd) - o0,
                      __fdiv_rn(py - g.lo[1], cd) - o1, __fdiv_rn(pz - g.lo[2], cd) - o2);
+}
+template <class V>
+__device__ inline float facesample(const STopo& t, const V* __restrict__ fine, const V* __restrict__ coarse, int a, float px,
+                                   float py, float pz) {
+    return a == 0 ? facesample_a<0>(t, fine, coarse, px, py, pz)
+                  : (a == 1 ? facesample_a<1>(t, fine, coarse, px, py, pz) : facesample_a<2>(t, fine, coarse, px, py, pz));
 }
 
 // sflip.jl:193-215 finegather: APIC gather over the fine faces only, with the weight found.

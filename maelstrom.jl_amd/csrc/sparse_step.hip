@@ -112,16 +112,17 @@ __global__ void k_sadvect_finecells(float* __restrict__ out, const float* __rest
     backtrace(f, p[0], p[1], p[2], dt, f.t.g.dx);
     out[n] = cellsample(f.t, src, srcc, p[0], p[1], p[2]);
 }
+template <int A>
 __global__ void k_sadvect_finefaces(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ srcc,
-                                    SFlow f, int a, float dt, int32_t nfa) {
+                                    SFlow f, float dt, int32_t nfa) {
     int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nfa) return;
-    SCell c = fineface(f.t, a, n);
+    SCell c = fineface(f.t, A, n);
     if (c.w.x < 0) return;
     float p[3];
-    facepos(f.t.g, a, c.w.x * 16 + c.i, c.w.y * 16 + c.j, c.w.z * 16 + c.k, f.t.g.dx, p);
+    facepos(f.t.g, A, c.w.x * 16 + c.i, c.w.y * 16 + c.j, c.w.z * 16 + c.k, f.t.g.dx, p);
     backtrace(f, p[0], p[1], p[2], dt, f.t.g.dx);
-    out[n] = facesample(f.t, src, srcc, a, p[0], p[1], p[2]);
+    out[n] = facesample(f.t, src, srcc, A, p[0], p[1], p[2]);
 }
 __global__ void k_sadvect_coarsecells(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ srcc,
                                       SFlow f, float dt) {
@@ -172,9 +173,11 @@ void SparseBasin::advect(float dt, bool carry_reflected, float gacc, int32_t see
                        phi.coarse.p, f, dt);
     for (int a = 0; a < 3; ++a) {
         const SField<float>& c = carry_reflected ? ur[a] : u[a];
-        if (nslots)
-            hipLaunchKernelGGL(k_sadvect_finefaces, dim3(cdiv(finefaces(), 256)), dim3(256), 0, s, u0[a].fine.p, c.fine.p,
-                               c.coarse.p, f, a, dt, (int32_t)finefaces());
+        if (nslots) {
+            auto kern = a == 0 ? k_sadvect_finefaces<0> : (a == 1 ? k_sadvect_finefaces<1> : k_sadvect_finefaces<2>);
+            hipLaunchKernelGGL(kern, dim3(cdiv(finefaces(), 256)), dim3(256), 0, s, u0[a].fine.p, c.fine.p, c.coarse.p, f, dt,
+                               (int32_t)finefaces());
+        }
         hipLaunchKernelGGL(k_sadvect_coarsefaces, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, s, u0[a].coarse.p, c.fine.p,
                            c.coarse.p, f, a, dt);
     }

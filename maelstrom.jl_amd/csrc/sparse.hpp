@@ -284,6 +284,29 @@ __device__ inline float finegather_value(const STopo& t, const V* __restrict__ u
     return vel;
 }
 
+// Whether no point of the box of block w's cells widened by two cells is solid (the fluid
+// region, inside the wall's cylinder and above the floor, is convex: its eight corners
+// decide, and the margin covers the rounding of any cell centre or face position in it).
+__device__ inline bool blockclear(const SGrid& g, int4 w) {
+    bool clear = true;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        float px = g.lo[0] + (float)(w.x * 16 + (m & 1 ? 18 : -2)) * g.dx;
+        float py = g.lo[1] + (float)(w.y * 16 + (m & 2 ? 18 : -2)) * g.dx;
+        float pz = g.lo[2] + (float)(w.z * 16 + (m & 4 ? 18 : -2)) * g.dx;
+        clear = clear && !sissolid(g, px, py, pz);
+    }
+    return clear;
+}
+
+// The slots of the 27 blocks around block w (0 off the grid or not allocated), into LDS.
+__device__ inline void stage27(const STopo& t, int4 w, int32_t* s27) {
+    if (threadIdx.x < 27) {
+        int bi = w.x + (int)(threadIdx.x % 3) - 1, bj = w.y + (int)((threadIdx.x / 3) % 3) - 1, bk = w.z + (int)(threadIdx.x / 9) - 1;
+        s27[threadIdx.x] = inblocks(t.g, bi, bj, bk) ? t.slot[blockid(t.g, bi, bj, bk)] : 0;
+    }
+}
+
 // sflip.jl:1030-1031 walledface: a wall on either side of the face at p (cells h wide).
 __device__ inline bool walledface_dev(const SGrid& g, const float p[3], int a, float h) {
     float e = 0.5f * h;

@@ -458,43 +458,82 @@ void SparseBasin::set_stir(const StirParams& p, const float* waves, int nwaves, 
 // ─── kick, extrapolation, reflection (sflip.jl:2017-2139) ────────────────────
 
 // One layer: an unflagged face takes the mean of its flagged neighbours (fine faces;
-// neighbours through finefaceat).
-__global__ void k_sextrapfine(float* __restrict__ out, uint8_t* __restrict__ fout, const float* __restrict__ src,
-                              const uint8_t* __restrict__ fin, float* __restrict__ out2, const float* __restrict__ src2, STopo t,
-                              int a, int32_t nfa) {
-    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nfa) return;
-    SCell c = fineface(t, a, n);
-    if (c.w.x < 0) return;
-    if (fin[n]) {
+// neighbours through finefaceat's rule, the blocks read from a table of the 27 around
+// the workgroup's block). A workgroup's 256 faces lie in one slot (4352 = 17 x 256). The
+// output arrays hold the layer before the input's (or, with `fresh`, nothing yet): a
+// flagged face keeps its value from layer to layer, so a face flagged in both is already
+// this layer's there and is skipped. A face's neighbours are loaded before any is summed.
+template <int A>
+__global__ __launch_bounds__(256) void k_sextrapfine(float* __restrict__ out, uint8_t* __restrict__ fout,
+                                                     const float* __restrict__ src, const uint8_t* __restrict__ fin,
+                                                     float* __restrict__ out2, const float* __restrict__ src2, STopo t,
+                                                     int fresh) {
+    __shared__ int32_t s27[27];
+    const int32_t n = (int32_t)(blockIdx.x * 256 + threadIdx.x);
+    const int s = (int)(blockIdx.x / 17) + 1;
+    const int4 w = t.where[s - 1];
+    if (w.x < 0) return;   // the whole workgroup
+    stage27(t, w, s27);
+    const bool clear = blockclear(t.g, w);
+    __syncthreads();
+    const uint8_t f0 = fin[n];
+    if (f0) {
+        if (!fresh && fout[n]) return;
         out[n] = src[n];
         if (out2) out2[n] = src2[n];
         fout[n] = 1;
         return;
     }
-    int I = c.w.x * 16 + c.i, J = c.w.y * 16 + c.j, K = c.w.z * 16 + c.k;
-    float acc = 0.0f, acc2 = 0.0f;
-    int m = 0;
-    float p[3];
-    facepos(t.g, a, I, J, K, t.g.dx, p);
-    // away from the block's sides the six neighbours are this block's own faces
-    const bool inner = (unsigned)(c.i - 1) < 14u && (unsigned)(c.j - 1) < 14u && (unsigned)(c.k - 1) < 14u;
-    const int dy = a == 0 ? 17 : 16, dz = a == 2 ? 256 : 272;
-    if (!walledface_dev(t.g, p, a, t.g.dx)) {
+    const int f = n - (s - 1) * SBFACES;
+    int i, j, k;
+    if (A == 0) { i = f % 17; int q = f / 17; j = q & 15; k = q >> 4; }
+    else if (A == 1) { i = f & 15; int q = f >> 4; j = q % 17; k = q / 17; }
+    else { i = f & 15; j = (f >> 4) & 15; k = f >> 8; }
+    const int I = w.x * 16 + i, J = w.y * 16 + j, K = w.z * 16 + k;
+    // finefaceat for a neighbour at global (GI, GJ, GK): its own block's copy, else the
+    // top face of the block below along A when it lies on that plane
+    auto face = [&](int GI, int GJ, int GK) -> int32_t {
+        const int di = (GI >> 4) - w.x + 1, dj = (GJ >> 4) - w.y + 1, dk = (GK >> 4) - w.z + 1;
+        const int32_t s1 = s27[di + 3 * dj + 9 * dk];
+        if (s1 != 0) return (s1 - 1) * SBFACES + ffn(16, A, GI & 15, GJ & 15, GK & 15);
+        const int GA = A == 0 ? GI : (A == 1 ? GJ : GK);
+        if ((GA & 15) != 0) return -1;
+        const int32_t s2 = s27[(di - (A == 0)) + 3 * (dj - (A == 1)) + 9 * (dk - (A == 2))];
+        if (s2 == 0) return -1;
+        return (s2 - 1) * SBFACES + ffn(16, A, A == 0 ? 16 : (GI & 15), A == 1 ? 16 : (GJ & 15), A == 2 ? 16 : (GK & 15));
+    };
+    int32_t nidx[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const int step = (q & 1 ? -1 : 1) * (q < 2 ? 1 : (q < 4 ? dy : dz));
-            int32_t nidx = inner ? n + step : finefaceat(t, a, I + NBX[q][0], J + NBX[q][1], K + NBX[q][2]);
-            if (nidx >= 0 && fin[nidx]) {
-                acc += src[nidx];
-                if (out2) acc2 += src2[nidx];
-                ++m;
-            }
-        }
+    for (int q = 0; q < 6; ++q) nidx[q] = face(I + NBX[q][0], J + NBX[q][1], K + NBX[q][2]);
+    uint8_t nf[6];
+    float nv[6], nv2[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int32_t m = nidx[q] >= 0 ? nidx[q] : n;
+        nf[q] = nidx[q] >= 0 ? fin[m] : 0;
+        nv[q] = src[m];
+        nv2[q] = out2 ? src2[m] : 0.0f;
     }
-    out[n] = m > 0 ? __fdiv_rn(acc, (float)m) : src[n];
-    if (out2) out2[n] = m > 0 ? __fdiv_rn(acc2, (float)m) : src2[n];
-    fout[n] = m > 0 ? 1 : 0;
+    bool walled = false;
+    if (!clear) {
+        float p[3];
+        facepos(t.g, A, I, J, K, t.g.dx, p);
+        walled = walledface_dev(t.g, p, A, t.g.dx);
+    }
+    float acc = 0.0f, acc2 = 0.0f;
+    int cnt = 0;
+    if (!walled) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            if (nf[q]) {
+                acc += nv[q];
+                acc2 += nv2[q];
+                ++cnt;
+            }
+    }
+    out[n] = cnt > 0 ? __fdiv_rn(acc, (float)cnt) : src[n];
+    if (out2) out2[n] = cnt > 0 ? __fdiv_rn(acc2, (float)cnt) : src2[n];
+    fout[n] = cnt > 0 ? 1 : 0;
 }
 __global__ void k_sextrapcoarse(float* __restrict__ out, uint8_t* __restrict__ fout, const float* __restrict__ src,
                                 const uint8_t* __restrict__ fin, float* __restrict__ out2, const float* __restrict__ src2,
@@ -575,9 +614,10 @@ void SparseBasin::extrapolate(int layers, int coarselayers, bool withkick) {
     for (int a = 0; a < 3; ++a) {
         float *f = u[a].fine.p, *tmp = u0[a].fine.p, *k = withkick ? ur[a].fine.p : nullptr, *ktmp = withkick ? ur0[a].fine.p : nullptr;
         uint8_t *fin = fl[a].fine.p, *fout = fl2[a].fine.p;
-        for (int l = 0; l < layers; ++l) {
+        auto kern = a == 0 ? k_sextrapfine<0> : (a == 1 ? k_sextrapfine<1> : k_sextrapfine<2>);
+        for (int l = 0; l < layers; ++l) {   // layer 1 writes over the input of layer 0
             if (F)
-                hipLaunchKernelGGL(k_sextrapfine, dim3(cdiv(F, 256)), dim3(256), 0, s, tmp, fout, f, fin, ktmp, k, t, a, (int32_t)F);
+                hipLaunchKernelGGL(kern, dim3((unsigned)(F / 256)), dim3(256), 0, s, tmp, fout, f, fin, ktmp, k, t, l == 0 ? 1 : 0);
             std::swap(f, tmp); std::swap(k, ktmp); std::swap(fin, fout);
         }
         if (f != u[a].fine.p) {   // an odd count: copy back

@@ -594,37 +594,54 @@ void SparseBasin::gravity(float dt, float gacc) {
 
 // ─── redistancing (sflip.jl:833-919) ─────────────────────────────────────────
 
-__device__ inline float nphi(const STopo& t, const float* __restrict__ fine, const float* __restrict__ coarse, int I, int J, int K) {
-    int32_t idx = fineat(t, I, J, K);
-    if (idx >= 0) return fine[idx];
-    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
-    return coarse[cl(min(max(I >> 2, 0), n0 - 1), min(max(J >> 2, 0), n1 - 1), min(max(K >> 2, 0), n2 - 1), n0, n1)];
-}
-
-__global__ void k_sreinit(float* __restrict__ out, const float* __restrict__ phi, const float* __restrict__ phic, STopo t,
-                          float phimax, int32_t nf) {
-    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nf) return;
+// A workgroup's 256 cells lie in one slot (4096 = 16 x 256); the neighbours across its
+// sides come from the table of the blocks around it, and a block clear of the walls skips
+// the solid tests.
+__global__ __launch_bounds__(256) void k_sreinit(float* __restrict__ out, const float* __restrict__ phi,
+                                                 const float* __restrict__ phic, STopo t, float phimax) {
+    __shared__ int32_t s27[27];
+    const int32_t n = (int32_t)(blockIdx.x * 256 + threadIdx.x);
     SCell c = finecell(t, n);
-    if (c.w.x < 0) return;
+    if (c.w.x < 0) return;   // the whole workgroup
+    stage27(t, c.w, s27);
+    const bool clear = blockclear(t.g, c.w);
+    __syncthreads();
     const SGrid& g = t.g;
     int I = c.w.x * 16 + c.i, J = c.w.y * 16 + c.j, K = c.w.z * 16 + c.k;
     float cv = phi[n], h = g.dx, p[3];
-    finecentre(g, I, J, K, p);
-    // away from the block's faces the six neighbours are this block's own cells
-    const bool inner = (unsigned)(c.i - 1) < 14u && (unsigned)(c.j - 1) < 14u && (unsigned)(c.k - 1) < 14u;
-    constexpr int step[6] = {1, -1, 16, -16, 256, -256};
-    auto nb = [&](int m, int a, int b, int cc) { return inner ? phi[n + step[m]] : nphi(t, phi, phic, a, b, cc); };
+    const int n0 = g.nc(0), n1 = g.nc(1), n2 = g.nc(2);
+    // nphi of the neighbour m: this block's cell, the face neighbour block's, or the coarse cell
+    float nv[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        const int li = c.i + NBS[m][0], lj = c.j + NBS[m][1], lk = c.k + NBS[m][2];
+        const int di = (li >> 4) + 1, dj = (lj >> 4) + 1, dk = (lk >> 4) + 1;
+        const int32_t sn = s27[di + 3 * dj + 9 * dk];
+        if (sn != 0) {
+            nv[m] = phi[(sn - 1) * 4096 + (li & 15) + 16 * ((lj & 15) + 16 * (lk & 15))];
+        } else {
+            const int a = I + NBS[m][0], b = J + NBS[m][1], cc = K + NBS[m][2];
+            nv[m] = phic[cl(min(max(a >> 2, 0), n0 - 1), min(max(b >> 2, 0), n1 - 1), min(max(cc >> 2, 0), n2 - 1), n0, n1)];
+        }
+    }
+    bool solid = false, nsolid[6] = {false, false, false, false, false, false};
+    if (!clear) {
+        finecentre(g, I, J, K, p);
+        solid = sissolid(g, p[0], p[1], p[2]);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            finecentre(g, I + NBS[m][0], J + NBS[m][1], K + NBS[m][2], p);
+            nsolid[m] = sissolid(g, p[0], p[1], p[2]);
+        }
+    }
     float res;
-    if (sissolid(g, p[0], p[1], p[2])) {
+    if (solid) {
         float lowest = cv;
         bool found = false;
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
-            int a = I + NBS[m][0], b = J + NBS[m][1], cc = K + NBS[m][2];
-            finecentre(g, a, b, cc, p);
-            if (!sissolid(g, p[0], p[1], p[2])) {
-                float f = nb(m, a, b, cc);
+            if (!nsolid[m]) {
+                float f = nv[m];
                 lowest = found ? jmin(lowest, f) : f;
                 found = true;
             }
@@ -635,10 +652,8 @@ __global__ void k_sreinit(float* __restrict__ out, const float* __restrict__ phi
         float ax = phimax, ay = phimax, az = phimax;
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
-            int a = I + NBS[m][0], b = J + NBS[m][1], cc = K + NBS[m][2];
-            finecentre(g, a, b, cc, p);
-            if (!sissolid(g, p[0], p[1], p[2])) {
-                float f = nb(m, a, b, cc);
+            if (!nsolid[m]) {
+                float f = nv[m];
                 if ((f < 0.0f) != inside) frozen = true;
                 float d = fabsf(f);
                 if (m < 2) ax = jmin(ax, d);
@@ -682,8 +697,8 @@ void SparseBasin::redistance(int iterations) {
     float phimax = (float)(iterations + 1) * g.dx;
     hipStream_t s = st.s;
     for (int it = 0; it < iterations && nslots > 0; ++it) {
-        hipLaunchKernelGGL(k_sreinit, dim3(cdiv(finecells(), 256)), dim3(256), 0, s, phi0.fine.p, phi.fine.p, phi.coarse.p,
-                           topo(), phimax, (int32_t)finecells());
+        hipLaunchKernelGGL(k_sreinit, dim3((unsigned)(finecells() / 256)), dim3(256), 0, s, phi0.fine.p, phi.fine.p,
+                           phi.coarse.p, topo(), phimax);
         std::swap(phi.fine, phi0.fine);
     }
     hipLaunchKernelGGL(k_coarsewall, dim3(cdiv(ncells(), 256)), dim3(256), 0, s, phi0.coarse.p, phi.coarse.p, topo());

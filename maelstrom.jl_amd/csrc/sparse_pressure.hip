@@ -286,6 +286,12 @@ __device__ inline int32_t nbslot(const STopo& t, int4 w, int d) {
     return inblocks(t.g, bi, bj, bk) ? t.slot[blockid(t.g, bi, bj, bk)] : 0;
 }
 
+// Level 0's couplings follow from the cell types (k_finecoef, k_coarsecoef,
+// sflip.jl:1323-1353): for a LIQUID uncovered cell, 4 to an uncovered LIQUID coarse
+// neighbour and 0.4 to a LIQUID fine cell across a sub-face, else 0. With `ty` set
+// (level 0), a coarse row takes them from the types (ty->tf fine, ty->tc coarse) instead of
+// the coupling arrays: the same values as the FP32 arrays hold.
+struct LevelTypes { const uint8_t* tf; const uint8_t* tc; };
 // A level's fine cell (N = 16 or 8) of slot s: diag and the neighbours' sum, A x = diag x
 // - off, summed in A (FP32, or FP64 for iterative refinement's residual). Neighbours in
 // the block index directly; across a face they are the neighbour block's (slots in nb)
@@ -356,7 +362,7 @@ __global__ void k_covermask(uint8_t* __restrict__ m, STopo t) {
 template <int N, class C, class A, class FX, class CX>
 __device__ inline void coarserow_acc(const STopo& t, unsigned msk, FX fx, const FineCoefT<C>& cf, CX cx,
                                      const CoarseCoefT<C>& ccf, const C* __restrict__ dDc, int I, int J, int K, A& diag,
-                                     A& off) {
+                                     A& off, const LevelTypes* ty = nullptr) {
     constexpr int R = N / 4, V = N * N * N, m = N - 1;
     constexpr uint32_t per = (N + 1) * N * N;
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
@@ -368,10 +374,15 @@ __device__ inline void coarserow_acc(const STopo& t, unsigned msk, FX fx, const 
         int a = d >> 1, sg = (d & 1) ? 1 : -1;
         int In = I + sg * (a == 0), Jn = J + sg * (a == 1), Kn = K + sg * (a == 2);
         in[d] = In >= 0 && Jn >= 0 && Kn >= 0 && In < n0 && Jn < n1 && Kn < n2;
-        int f0 = n0 + (a == 0), f1 = n1 + (a == 1);
-        size_t fi = sg < 0 ? cl(I, J, K, f0, f1) : cl(I + (a == 0), J + (a == 1), K + (a == 2), f0, f1);
-        cc[d] = (A)lf(ccf.c[a][fi]);
-        xv[d] = cx(in[d] ? cl(In, Jn, Kn, n0, n1) : own);
+        size_t nidx = in[d] ? cl(In, Jn, Kn, n0, n1) : own;
+        if (ty) {
+            cc[d] = (A)(ty->tc[nidx] == LIQUID ? COARSE_COARSE : 0.0f);
+        } else {
+            int f0 = n0 + (a == 0), f1 = n1 + (a == 1);
+            size_t fi = sg < 0 ? cl(I, J, K, f0, f1) : cl(I + (a == 0), J + (a == 1), K + (a == 2), f0, f1);
+            cc[d] = (A)lf(ccf.c[a][fi]);
+        }
+        xv[d] = cx(nidx);
     }
     diag = (A)lf(dDc[own]);
     off = (A)0;
@@ -392,7 +403,7 @@ __device__ inline void coarserow_acc(const STopo& t, unsigned msk, FX fx, const 
                     uint32_t cidx = cbase + (uint32_t)(li + N * (lj + N * lk));
                     uint32_t fidx = fbase + (uint32_t)(sg > 0 ? ffn(N, a, li, lj, lk)
                                                               : ffn(N, a, a == 0 ? N : li, a == 1 ? N : lj, a == 2 ? N : lk));
-                    A c = (A)lf(cf.c[a][fidx]);
+                    A c = ty ? (A)(ty->tf[cidx] == LIQUID ? FINE_COARSE : 0.0f) : (A)lf(cf.c[a][fidx]);
                     diag += c;
                     off += c * fx(cidx);
                 }
@@ -407,10 +418,10 @@ __device__ inline void coarserow_acc(const STopo& t, unsigned msk, FX fx, const 
 template <int N, class T, class C, class A = float, bool Z = false>
 __device__ inline void coarserow(const STopo& t, unsigned msk, const T* __restrict__ xf, const FineCoefT<C>& cf,
                                  const T* __restrict__ xc, const CoarseCoefT<C>& ccf, const C* __restrict__ dDc, int I, int J,
-                                 int K, A& diag, A& off) {
+                                 int K, A& diag, A& off, const LevelTypes* ty = nullptr) {
     auto fx = [&](uint32_t i) -> A { if constexpr (Z) return (A)0; else return (A)lf(xf[i]); };
     auto cx = [&](size_t i) -> A { if constexpr (Z) return (A)0; else return (A)lf(xc[i]); };
-    coarserow_acc<N, C, A>(t, msk, fx, cf, cx, ccf, dDc, I, J, K, diag, off);
+    coarserow_acc<N, C, A>(t, msk, fx, cf, cx, ccf, dDc, I, J, K, diag, off, ty);
 }
 
 // The device scalars of the Krylov loops (slots of SparseBasin::scal).
@@ -753,12 +764,14 @@ __global__ __launch_bounds__(256) void k_rowcoarse(T* __restrict__ out, const T*
     if constexpr (SEARCH) S.dnc[l] = lf(x[l]) + beta * S.doc[l];
     if (L.actc[l] != 0 && !(msk & 1u)) {
         float diag, off;
+        const LevelTypes ty{L.actf, L.tc};   // level 0 (N = 16): the couplings from the types
+        const LevelTypes* typ = N == 16 ? &ty : nullptr;
         if constexpr (SEARCH) {
             auto fx = [&](uint32_t i) -> float { return lf(xf[i]) + beta * S.dof[i]; };
             auto cx = [&](size_t i) -> float { return lf(x[i]) + beta * S.doc[i]; };
-            coarserow_acc<N, C, float>(t, msk, fx, L.cf, cx, L.cc, L.dDc, I, J, K, diag, off);
+            coarserow_acc<N, C, float>(t, msk, fx, L.cf, cx, L.cc, L.dDc, I, J, K, diag, off, typ);
         } else {
-            coarserow<N, T, C, float, MODE == 3>(t, msk, xf, L.cf, x, L.cc, L.dDc, I, J, K, diag, off);
+            coarserow<N, T, C, float, MODE == 3>(t, msk, xf, L.cf, x, L.cc, L.dDc, I, J, K, diag, off, typ);
         }
         float xl = MODE == 3 ? 0.0f : (SEARCH ? lf(x[l]) + beta * S.doc[l] : lf(x[l]));
         float ax = diag * xl - off;
@@ -1225,7 +1238,8 @@ __global__ __launch_bounds__(256) void k_resid64coarse(float* __restrict__ r, co
     unsigned msk = L.cmask[l];
     if (L.actc[l] != 0 && !(msk & 1u)) {
         double diag, off;
-        coarserow<16, double, float, double>(t, msk, xf, L.cf, x, L.cc, L.dDc, I, J, K, diag, off);
+        const LevelTypes ty{L.actf, L.tc};
+        coarserow<16, double, float, double>(t, msk, xf, L.cf, x, L.cc, L.dDc, I, J, K, diag, off, &ty);
         res = (double)b[l] - (diag * x[l] - off);
     }
     r[l] = (float)res;
@@ -1407,7 +1421,8 @@ struct SparseSolver {
             H.D.push_back(DViewT<T>{i3(s.D[l].n), {(T*)m.Dc[l][0].p, (T*)m.Dc[l][1].p, (T*)m.Dc[l][2].p}, (T*)m.Dc[l][3].p,
                                     s.D[l].act.p});
         }
-        H.L0 = CompT<T>{{{(T*)m.cf[0].p, (T*)m.cf[1].p, (T*)m.cf[2].p}}, (T*)m.dDf.p, s.types.fine.p,
+        // level 0's rows take their couplings from the cell types: no fine coupling copies
+        H.L0 = CompT<T>{{{nullptr, nullptr, nullptr}}, (T*)m.dDf.p, s.types.fine.p,
                         {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p, s.cmask.p, s.types.coarse.p};
         H.L1 = CompT<T>{{{(T*)m.c1[0].p, (T*)m.c1[1].p, (T*)m.c1[2].p}}, (T*)m.dD1.p, s.act1.p,
                         {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p, s.cmask.p, s.types.coarse.p};
@@ -1422,7 +1437,6 @@ struct SparseSolver {
                                       scale);
         };
         for (int a = 0; a < 3; ++a) {
-            cast(m.cf[a], s.cf[a].p, s.finefaces(), 1.0f);
             cast(m.cc[a], s.cc[a].p, s.nfaces(a), 1.0f);
             cast(m.c1[a], s.c1[a].p, (size_t)s.nslots * SBFACES1, 1.0f);
         }

@@ -99,7 +99,7 @@ void SparseBasin::mp_alloc() {
         mp.Dc[l][3].alloc(D[l].cells());
     }
     for (int a = 0; a < 3; ++a) {
-        mp.cf[a].alloc(nfa); mp.cc[a].alloc(nfaces(a)); mp.c1[a].alloc((size_t)blocks * SBFACES1);
+        mp.cc[a].alloc(nfaces(a)); mp.c1[a].alloc((size_t)blocks * SBFACES1);
     }
     mp.dDf.alloc(nf); mp.dDc.alloc(Cc); mp.dD1.alloc(n1);
     // every vector starts at zero: free slots are never written and must read as zero

@@ -345,6 +345,7 @@ struct MPStore {
     DBuf<uint16_t> X1[4][2];                        // level-1 x, b, r, t
     std::vector<std::array<DBuf<uint16_t>, 4>> Dv;  // dense levels: x, b, r, t
     DBuf<uint16_t> cc[3], c1[3], dDf, dDc, dD1;   // level 0's couplings follow from the types
+    DBuf<uint16_t> rwf;                             // level 0's omega / diag (the Jacobi weights)
     std::vector<std::array<DBuf<uint16_t>, 4>> Dc;  // dense levels: couplings x, y, z, diagonal
 };
 

@@ -450,6 +450,7 @@ template <class C> struct CompT {
     const uint8_t* actc;
     const uint8_t* cmask;   // k_covermask
     const uint8_t* tc;      // the coarse cell types (level 0's matrix-free couplings)
+    const C* rwf = nullptr; // reduced precision, level 0: omega / diag per fine cell
 };
 
 // Flexible CG's fused step (mode 4 of the level-0 rows, sflip.jl:1884-1886 with
@@ -467,7 +468,11 @@ struct SearchArgs {
 
 // sflip.jl:1218-1269 on the fine cells of a level: mode 0 Jacobi (out = x + w (b - A x) /
 // diag), 1 residual (out = b - A x), 2 apply (out = A x), 3 Jacobi from x = 0 (x not
-// read); inactive cells and the cells of free slots get 0.
+// read); inactive cells and the cells of free slots get 0. Mode 5 writes w / diag (0 where
+// diag is not positive): in reduced precision the level-0 Jacobi sweep takes that weight
+// r per cell, out = (1 - w) x + r (b + off), and needs neither the diagonal nor the
+// neighbours' couplings, only their products with x (the weights round to the storage
+// type once per solve; the FP32 sweep keeps the reference's division).
 // Level 0, matrix-free and staged in LDS. A workgroup takes a quarter of a slot (4 z
 // planes of its 16^3 block), a thread 4 consecutive cells along x of one row. Level 0's
 // coupling of a LIQUID fine cell to a neighbour is 1 when the neighbour is a LIQUID fine
@@ -487,8 +492,10 @@ constexpr int RS = 24, PS = 18 * RS, HV = 6 * PS;   // LDS rows of 24: cell li (
 template <int MODE, class T, class C>
 __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
                                                    const T* __restrict__ xc, CompT<C> L, STopo t, SearchArgs S) {
-    constexpr bool Z = MODE == 3, SEARCH = MODE == 4;
-    __shared__ float4 sw4[HV / 4];             // the couplings
+    constexpr bool Z = MODE == 3 || MODE == 5, SEARCH = MODE == 4;
+    constexpr bool RW = MODE == 0 && !std::is_same<T, float>::value;   // the weighted sweep
+    constexpr bool NW = !RW;                                               // the couplings staged
+    __shared__ float4 sw4[NW ? HV / 4 : 1];    // the couplings
     __shared__ float4 sp4[Z ? 1 : HV / 4];     // coupling times x
     __shared__ double lds[4];
     float* sw = reinterpret_cast<float*>(sw4);
@@ -521,9 +528,9 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
     }
     // the thread's own cells: types, diagonal, right-hand side, x
     const uint32_t tq = *reinterpret_cast<const uint32_t*>(tf + n);
-    const Four<C> dq = load4(L.dDf, n);
+    const Four<C> dq = load4(RW ? L.rwf : L.dDf, n);   // the diagonal's Dirichlet part, or the weight
     Four<T> bq;
-    if constexpr (MODE == 2 || MODE == 4) {
+    if constexpr (MODE == 2 || MODE == 4 || MODE == 5) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) bq.v[e] = sf<T>(0.0f);
     } else {
@@ -552,7 +559,7 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
         p0[e] = w0[e] > 0.0f ? w0[e] * x0[e] : 0.0f;
     }
     const int hq = 4 + 4 * i4 + RS * (j + 1) + PS * (kk + 1);   // LDS place of the first cell
-    sw4[hq >> 2] = make_float4(w0[0], w0[1], w0[2], w0[3]);
+    if constexpr (NW) sw4[hq >> 2] = make_float4(w0[0], w0[1], w0[2], w0[3]);
     if constexpr (!Z) sp4[hq >> 2] = make_float4(p0[0], p0[1], p0[2], p0[3]);
 
     int32_t nb[6];
@@ -601,7 +608,7 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
 #pragma unroll
             for (int e = 0; e < 4; ++e) { wv[e] = wc; pv[e] = pc; }
         }
-        sw4[h >> 2] = make_float4(wv[0], wv[1], wv[2], wv[3]);
+        if constexpr (NW) sw4[h >> 2] = make_float4(wv[0], wv[1], wv[2], wv[3]);
         if constexpr (!Z) sp4[h >> 2] = make_float4(pv[0], pv[1], pv[2], pv[3]);
     };
     if (tid < 128) {   // the z layers below and above the quarter: 2 x 16 rows x 4 quads
@@ -622,7 +629,7 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
         } else {
             coarse(li, lj, lk, wv, pv);
         }
-        sw[h] = wv;
+        if constexpr (NW) sw[h] = wv;
         if constexpr (!Z) sp[h] = pv;
     }
     if (tid < 32) {    // the y sides: 2 x 4 planes x 4 quads
@@ -634,8 +641,12 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
 
     // the neighbours' couplings and products: x within the quad from registers, the
     // quad's x ends, y and z from LDS; order -x, +x, -y, +y, -z, +z
-    const float4 wym = sw4[(hq - RS) >> 2], wyp = sw4[(hq + RS) >> 2], wzm = sw4[(hq - PS) >> 2], wzp = sw4[(hq + PS) >> 2];
-    const float wl = sw[hq - 1], wr = sw[hq + 4];
+    float4 wym = make_float4(0, 0, 0, 0), wyp = wym, wzm = wym, wzp = wym;
+    float wl = 0.0f, wr = 0.0f;
+    if constexpr (NW) {
+        wym = sw4[(hq - RS) >> 2]; wyp = sw4[(hq + RS) >> 2]; wzm = sw4[(hq - PS) >> 2]; wzp = sw4[(hq + PS) >> 2];
+        wl = sw[hq - 1]; wr = sw[hq + 4];
+    }
     float4 pym = make_float4(0, 0, 0, 0), pyp = pym, pzm = pym, pzp = pym;
     float pl = 0.0f, pr = 0.0f;
     if constexpr (!Z) {
@@ -664,10 +675,17 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
         const bool act = (MODE == 2 || SEARCH) ? t0 == LIQUID : t0 != 0;
         float res = 0.0f;
         if (act) {
-            float ax = diag * x0[e] - off;
-            if (MODE == 0 || MODE == 3) res = diag > 0.0f ? x0[e] + __fdiv_rn(OMEGA * (bb - ax), diag) : 0.0f;
-            else if (MODE == 1) res = bb - ax;
-            else res = ax;
+            if constexpr (RW) {
+                const float rw = dd;   // omega / diag
+                res = rw > 0.0f ? (1.0f - OMEGA) * x0[e] + rw * (bb + off) : 0.0f;
+            } else if constexpr (MODE == 5) {
+                res = diag > 0.0f ? __fdiv_rn(OMEGA, diag) : 0.0f;
+            } else {
+                float ax = diag * x0[e] - off;
+                if (MODE == 0 || MODE == 3) res = diag > 0.0f ? x0[e] + __fdiv_rn(OMEGA * (bb - ax), diag) : 0.0f;
+                else if (MODE == 1) res = bb - ax;
+                else res = ax;
+            }
             if constexpr (SEARCH) pq += (double)x0[e] * (double)res;
         }
         oq.v[e] = sf<T>(res);
@@ -677,6 +695,29 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
         double sum = block_sum_red(pq, lds);
         if (threadIdx.x == 0) S.partials[tile] = sum;
     }
+}
+
+// The reduced-precision level-0 Jacobi sweep from x = 0: out = r b with r = omega / diag
+// per cell (mode 5 of k_rowfine16), on non-solid cells; four cells a thread.
+template <class T>
+__global__ __launch_bounds__(256) void k_jacobi0w(T* __restrict__ out, const T* __restrict__ b, const T* __restrict__ rw,
+                                                  const uint8_t* __restrict__ tf, const int4* __restrict__ where, int32_t nf) {
+    const uint32_t n = 4u * (uint32_t)(xcd_tile(blockIdx.x, gridDim.x) * 256u + threadIdx.x);
+    if (n >= (uint32_t)nf) return;
+    Four<T> o;
+    if (where[n >> 12].x < 0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) o.v[e] = sf<T>(0.0f);
+    } else {
+        const uint32_t ty = *reinterpret_cast<const uint32_t*>(tf + n);
+        const Four<T> bq = load4(b, n), rq = load4(rw, n);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float r = lf(rq.v[e]);
+            o.v[e] = sf<T>(((ty >> (8 * e)) & 0xffu) != 0 && r > 0.0f ? r * lf(bq.v[e]) : 0.0f);
+        }
+    }
+    *reinterpret_cast<Four<T>*>(out + n) = o;
 }
 
 // Level 1 (8^3 a slot), staged in LDS: a workgroup per slot stages x of the block and its
@@ -1423,7 +1464,8 @@ struct SparseSolver {
         }
         // level 0's rows take their couplings from the cell types: no fine coupling copies
         H.L0 = CompT<T>{{{nullptr, nullptr, nullptr}}, (T*)m.dDf.p, s.types.fine.p,
-                        {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p, s.cmask.p, s.types.coarse.p};
+                        {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p, s.cmask.p, s.types.coarse.p,
+                        (T*)m.rwf.p};
         H.L1 = CompT<T>{{{(T*)m.c1[0].p, (T*)m.c1[1].p, (T*)m.c1[2].p}}, (T*)m.dD1.p, s.act1.p,
                         {{(T*)m.cc[0].p, (T*)m.cc[1].p, (T*)m.cc[2].p}}, (T*)m.dDc.p, s.D[0].act.p, s.cmask.p, s.types.coarse.p};
         H.rscale = 0.25f;
@@ -1449,6 +1491,11 @@ struct SparseSolver {
             cast(m.Dc[l][3], s.D[l].dD.p, s.D[l].cells(), sc);
             sc *= 0.25f;
         }
+        if (s.nslots) {   // level 0's Jacobi weights, omega / diag (mode 5 of the level-0 rows)
+            Hier<T, T> H = hier16<T>();
+            hipLaunchKernelGGL((k_rowfine16<5, T, T>), dim3(4 * s.nslots), dim3(256), 0, st, (T*)m.rwf.p, (const T*)nullptr,
+                               (const T*)nullptr, (const T*)nullptr, H.L0, t, SearchArgs{});
+        }
         MAEL_LAUNCH_CHECK();
     }
 
@@ -1467,7 +1514,9 @@ struct SparseSolver {
         int32_t nfl = s.nslots * N * N * N;
         if (nfl) {
             if (N == 16 && MODE == 0) s.kt.begin(K_RBGS, st);
-            if constexpr (N == 16)
+            if constexpr (N == 16 && MODE == 3 && !std::is_same<T, float>::value)
+                hipLaunchKernelGGL((k_jacobi0w<T>), dim3(s.nslots * 4), dim3(256), 0, st, out.f, b.f, L.rwf, L.actf, t.where, nfl);
+            else if constexpr (N == 16)
                 hipLaunchKernelGGL((k_rowfine16<MODE, T, C>), dim3(4 * s.nslots), dim3(256), 0, st, out.f, x.f, b.f, x.c, L, t, S);
             else
                 hipLaunchKernelGGL((k_rowfine8<MODE, T, C>), dim3(s.nslots), dim3(256), 0, st, out.f, x.f, b.f, x.c, L, t);

@@ -101,7 +101,7 @@ void SparseBasin::mp_alloc() {
     for (int a = 0; a < 3; ++a) {
         mp.cc[a].alloc(nfaces(a)); mp.c1[a].alloc((size_t)blocks * SBFACES1);
     }
-    mp.dDf.alloc(nf); mp.dDc.alloc(Cc); mp.dD1.alloc(n1);
+    mp.dDf.alloc(nf); mp.dDc.alloc(Cc); mp.dD1.alloc(n1); mp.rwf.alloc(nf);
     // every vector starts at zero: free slots are never written and must read as zero
     for (auto* v : {mp.z, mp.r, mp.q, mp.tmp}) { fill_async(v[0].p, nf, (uint16_t)0, st.s); fill_async(v[1].p, Cc, (uint16_t)0, st.s); }
     for (auto& v : mp.X1) { fill_async(v[0].p, n1, (uint16_t)0, st.s); fill_async(v[1].p, Cc, (uint16_t)0, st.s); }

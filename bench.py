@@ -47,7 +47,8 @@ def sparse_bytes(kernel, live_cells, n_particles, precision=0):
     couplings."""
     e = 4 if precision == 0 else 2
     return {
-        # x, b, the Dirichlet diagonal, the result (e B each) and the type (1 B) per fine
+        # x, b, the Dirichlet diagonal (FP32) or the Jacobi weight omega/diag (FP16/BF16),
+        # the result (e B each) and the type (1 B) per fine
         # cell: level 0's couplings follow from the types (the row is matrix-free)
         "jacobi": (4 * e + 1) * live_cells,
         "g2p": 60 * n_particles,

@@ -1100,12 +1100,23 @@ __global__ __launch_bounds__(256) void k_sdot(double* __restrict__ partials, con
 }
 
 enum { SF_SET = 0, SF_NORM, SF_RHO0, SF_ALPHA, SF_BETA };
-__global__ __launch_bounds__(256) void k_sfinish(const double* __restrict__ partials, int np, double* __restrict__ scal, int op,
-                                                 int slot) {
-    __shared__ double lds[4];
-    double v = 0;
-    for (int l = threadIdx.x; l < np; l += blockDim.x) v += partials[l];
-    double s = block_sum_red(v, lds);
+// The sum of np partials by one workgroup of FINISH_THREADS: each thread adds its strided
+// share with four accumulators (loads in flight together), then the fixed tree of
+// block_sum_red; the order depends on np only.
+constexpr int FINISH_THREADS = 1024;
+__device__ inline double finish_sum(const double* __restrict__ p, int np, double* lds) {
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int l = threadIdx.x;
+    for (; l + 3 * FINISH_THREADS < np; l += 4 * FINISH_THREADS) {
+        a0 += p[l]; a1 += p[l + FINISH_THREADS]; a2 += p[l + 2 * FINISH_THREADS]; a3 += p[l + 3 * FINISH_THREADS];
+    }
+    for (; l < np; l += FINISH_THREADS) a0 += p[l];
+    return block_sum_red((a0 + a1) + (a2 + a3), lds);
+}
+__global__ __launch_bounds__(FINISH_THREADS) void k_sfinish(const double* __restrict__ partials, int np,
+                                                            double* __restrict__ scal, int op, int slot) {
+    __shared__ double lds[FINISH_THREADS / 64];
+    double s = finish_sum(partials, np, lds);
     if (threadIdx.x != 0) return;
     long long flags = (long long)scal[SS_FLAG];
     switch (op) {
@@ -1185,6 +1196,39 @@ __global__ __launch_bounds__(256) void k_fcg_update(double* __restrict__ partial
     double s = block_sum_red(v, lds);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
+// The reduced-precision V-cycle's correction back to FP32 (z = zh scal[slot], as k_convert)
+// fused with k_sdot2 on it: z.r and z.rold summed in k_sdot2's grouping (the same partials
+// for the same z).
+template <class T>
+__global__ __launch_bounds__(256) void k_convert_dot2(double* __restrict__ partials, float* __restrict__ zf,
+                                                      float* __restrict__ zc, const T* __restrict__ hf, const T* __restrict__ hc,
+                                                      const float* __restrict__ rf, const float* __restrict__ rc,
+                                                      const float* __restrict__ of, const float* __restrict__ oc,
+                                                      const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc,
+                                                      const uint8_t* __restrict__ cm, STopo t, int32_t nf,
+                                                      const double* __restrict__ scal, int slot) {
+    __shared__ double lds[4];
+    float c = (float)scal[slot];
+    if (!(c == c) || c == 0.0f || !(fabsf(c) <= 3.0e38f)) c = 1.0f;
+    size_t nf4 = (size_t)nf / 4, ng = nf4 + (size_t)t.g.nc(0) * t.g.nc(1) * t.g.nc(2) / 4;
+    double v1 = 0, v2 = 0;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += (size_t)gridDim.x * blockDim.x) {
+        const Four<T> h = g < nf4 ? reinterpret_cast<const Four<T>*>(hf)[g] : reinterpret_cast<const Four<T>*>(hc)[g - nf4];
+        const float4 z = make_float4(lf(h.v[0]) * c, lf(h.v[1]) * c, lf(h.v[2]) * c, lf(h.v[3]) * c);
+        grp(zf, zc, g, nf4) = z;
+        unsigned m = unknown4(g, nf4, tf, tc, cm, t);
+        if (!m) continue;
+        v1 += dot4(m, z, grp(rf, rc, g, nf4));
+        v2 += dot4(m, z, grp(of, oc, g, nf4));
+    }
+    double s1 = block_sum_red(v1, lds);
+    __syncthreads();
+    double s2 = block_sum_red(v2, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s1;
+        partials[gridDim.x + blockIdx.x] = s2;
+    }
+}
 // z.r and z.rold in one pass (sflip.jl:1894-1897): partials[b] and partials[B + b].
 __global__ __launch_bounds__(256) void k_sdot2(double* __restrict__ partials, const float* __restrict__ zf,
                                                const float* __restrict__ zc, const float* __restrict__ rf,
@@ -1211,13 +1255,12 @@ __global__ __launch_bounds__(256) void k_sdot2(double* __restrict__ partials, co
     }
 }
 // beta from the two sums (sflip.jl:1895-1900), rho = rho_new.
-__global__ __launch_bounds__(256) void k_sfinish_beta(const double* __restrict__ partials, int np, double* __restrict__ scal) {
-    __shared__ double lds[4];
-    double v1 = 0, v2 = 0;
-    for (int l = threadIdx.x; l < np; l += blockDim.x) { v1 += partials[l]; v2 += partials[np + l]; }
-    double rn = block_sum_red(v1, lds);
+__global__ __launch_bounds__(FINISH_THREADS) void k_sfinish_beta(const double* __restrict__ partials, int np,
+                                                                 double* __restrict__ scal) {
+    __shared__ double lds[FINISH_THREADS / 64];
+    double rn = finish_sum(partials, np, lds);
     __syncthreads();
-    double zo = block_sum_red(v2, lds);
+    double zo = finish_sum(partials + np, np, lds);
     if (threadIdx.x != 0) return;
     long long flags = (long long)scal[SS_FLAG];
     if (!(rn > 0)) flags |= EF_RHONEW_NONPOS;
@@ -1533,7 +1576,7 @@ struct SparseSolver {
         int32_t fine_groups = s.nslots ? 4 * s.nslots : 1;
         SearchArgs S{d.f, d.c, dn.f, dn.c, s.partials.p, fine_groups, s.scal.p};
         rows<16, 4>(SVecT<float>{q.f, q.c}, SVecT<float>{z.f, z.c}, SVecT<float>{z.f, z.c}, H.L0, S);
-        hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(256), 0, st, s.partials.p, fine_groups + (int)cdiv(Cc, 256), s.scal.p,
+        hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(FINISH_THREADS), 0, st, s.partials.p, fine_groups + (int)cdiv(Cc, 256), s.scal.p,
                            (int)SF_ALPHA, 0);
         MAEL_LAUNCH_CHECK();
     }
@@ -1611,16 +1654,21 @@ struct SparseSolver {
     int precision = 0;   // 0 FP32, 1 FP16 storage, 2 BF16 storage
     // z = M^-1 r. In reduced precision r is scaled by 1 / scal[nslot] on the way in (its
     // norm, so its entries are at most 1) and z back on the way out.
-    void vcycle(SVec z, SVec r, int sweeps, int coarse, float boost, int nslot = -1) {
+    // z = M^-1 r. With `rold` (flexible CG), z.r and z.rold are also summed into the
+    // partials as k_sdot2 does (fused with the conversion back in reduced precision).
+    void vcycle(SVec z, SVec r, int sweeps, int coarse, float boost, int nslot = -1, const SVec* rold = nullptr) {
         if (precision == 0) {
             Hier<float, float> H = hier32();
             vcycle_t<float, float, float>(H, SVecT<float>{z.f, z.c}, SVecT<float>{r.f, r.c}, sweeps, coarse, boost);
+            if (rold)
+                hipLaunchKernelGGL(k_sdot2, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, z.f, z.c, r.f, r.c, rold->f,
+                                   rold->c, s.types.fine.p, s.types.coarse.p, s.cmask.p, t, nf);
             return;
         }
-        if (precision == 1) vcycle_mp<__half>(z, r, sweeps, coarse, boost, nslot);
-        else vcycle_mp<bf16>(z, r, sweeps, coarse, boost, nslot);
+        if (precision == 1) vcycle_mp<__half>(z, r, sweeps, coarse, boost, nslot, rold);
+        else vcycle_mp<bf16>(z, r, sweeps, coarse, boost, nslot, rold);
     }
-    template <class T> void vcycle_mp(SVec z, SVec r, int sweeps, int coarse, float boost, int nslot) {
+    template <class T> void vcycle_mp(SVec z, SVec r, int sweeps, int coarse, float boost, int nslot, const SVec* rold) {
         Hier<T, T> H = hier16<T>();
         MPStore& m = s.mp;
         SVecT<T> zh{(T*)m.z[0].p, (T*)m.z[1].p}, rh{(T*)m.r[0].p, (T*)m.r[1].p};
@@ -1628,8 +1676,16 @@ struct SparseSolver {
         hipLaunchKernelGGL((k_convert<T, float>), dim3(g), dim3(256), 0, st, rh.f, r.f, (size_t)nf, rh.c, r.c, Cc, s.scal.p, nslot,
                            1);
         vcycle_t<T, T, double>(H, zh, rh, sweeps, coarse, boost);
-        hipLaunchKernelGGL((k_convert<float, T>), dim3(g), dim3(256), 0, st, z.f, zh.f, (size_t)nf, z.c, zh.c, Cc, s.scal.p, nslot,
-                           0);
+        if (rold && nslot >= 0)
+            hipLaunchKernelGGL((k_convert_dot2<T>), dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, z.f, z.c, zh.f, zh.c, r.f,
+                               r.c, rold->f, rold->c, s.types.fine.p, s.types.coarse.p, s.cmask.p, t, nf, s.scal.p, nslot);
+        else {
+            hipLaunchKernelGGL((k_convert<float, T>), dim3(g), dim3(256), 0, st, z.f, zh.f, (size_t)nf, z.c, zh.c, Cc, s.scal.p,
+                               nslot, 0);
+            if (rold)
+                hipLaunchKernelGGL(k_sdot2, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, z.f, z.c, r.f, r.c, rold->f,
+                                   rold->c, s.types.fine.p, s.types.coarse.p, s.cmask.p, t, nf);
+        }
     }
 
     void apply(SVec q, SVec x) {
@@ -1639,7 +1695,7 @@ struct SparseSolver {
     void dot(SVec a, SVec b, int op, int slot) {
         hipLaunchKernelGGL(k_sdot, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, a.f, b.f, a.c, b.c, s.types.fine.p,
                            s.types.coarse.p, s.cmask.p, t, nf);
-        hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(256), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p, op, slot);
+        hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(FINISH_THREADS), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p, op, slot);
         MAEL_LAUNCH_CHECK();
     }
     // y += c x (c = mult * scal[slot], or mult when slot < 0), over the used fine cells and the coarse part.
@@ -1758,7 +1814,7 @@ struct SparseSolver {
             hipLaunchKernelGGL(k_fcg_update, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, P.f, P.c, r.f, r.c, rold.f,
                                rold.c, d.f, d.c, q.f, q.c, s.types.fine.p, s.types.coarse.p, s.cmask.p, t, nf, s.scal.p,
                                (int)SS_ALPHA);
-            hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(256), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p, (int)SF_NORM,
+            hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(FINISH_THREADS), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p, (int)SF_NORM,
                                (int)SS_RNORM);
             double flags;
             read2(SS_RNORM, rnorm, flags);
@@ -1766,10 +1822,8 @@ struct SparseSolver {
             if (refine > 0 && (rnorm <= target || it % refine == 0)) rnorm = resid64(r);
             relres = rnorm / bnorm;
             if (rnorm <= target) { converged = true; return it; }
-            vcycle(z, r, sweeps, coarse, boost, SS_RNORM);
-            hipLaunchKernelGGL(k_sdot2, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, z.f, z.c, r.f, r.c, rold.f, rold.c,
-                               s.types.fine.p, s.types.coarse.p, s.cmask.p, t, nf);
-            hipLaunchKernelGGL(k_sfinish_beta, dim3(1), dim3(256), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p);
+            vcycle(z, r, sweeps, coarse, boost, SS_RNORM, &rold);   // and z.r, z.rold
+            hipLaunchKernelGGL(k_sfinish_beta, dim3(1), dim3(FINISH_THREADS), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p);
         }
         double flags;
         read2(SS_RNORM, rnorm, flags);

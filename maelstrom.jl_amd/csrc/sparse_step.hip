@@ -460,9 +460,11 @@ void SparseBasin::set_stir(const StirParams& p, const float* waves, int nwaves, 
 // One layer: an unflagged face takes the mean of its flagged neighbours (fine faces;
 // neighbours through finefaceat's rule, the blocks read from a table of the 27 around
 // the workgroup's block). A workgroup's 256 faces lie in one slot (4352 = 17 x 256). The
-// output arrays hold the layer before the input's (or, with `fresh`, nothing yet): a
-// flagged face keeps its value from layer to layer, so a face flagged in both is already
-// this layer's there and is skipped. A face's neighbours are loaded before any is summed.
+// output arrays hold the layer before the input's (or, with `fresh`, nothing yet). A
+// flagged face keeps its value from layer to layer, and so does an unflagged one until it
+// is flagged: a face whose flag is the same in both and that takes no neighbour's mean is
+// already this layer's there and is skipped. Neighbours' values are read only where
+// flagged.
 template <int A>
 __global__ __launch_bounds__(256) void k_sextrapfine(float* __restrict__ out, uint8_t* __restrict__ fout,
                                                      const float* __restrict__ src, const uint8_t* __restrict__ fin,
@@ -477,8 +479,9 @@ __global__ __launch_bounds__(256) void k_sextrapfine(float* __restrict__ out, ui
     const bool clear = blockclear(t.g, w);
     __syncthreads();
     const uint8_t f0 = fin[n];
+    const uint8_t fo = fresh ? 2 : fout[n];   // 2: not known
     if (f0) {
-        if (!fresh && fout[n]) return;
+        if (fo == 1) return;
         out[n] = src[n];
         if (out2) out2[n] = src2[n];
         fout[n] = 1;
@@ -502,38 +505,40 @@ __global__ __launch_bounds__(256) void k_sextrapfine(float* __restrict__ out, ui
         if (s2 == 0) return -1;
         return (s2 - 1) * SBFACES + ffn(16, A, A == 0 ? 16 : (GI & 15), A == 1 ? 16 : (GJ & 15), A == 2 ? 16 : (GK & 15));
     };
-    int32_t nidx[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) nidx[q] = face(I + NBX[q][0], J + NBX[q][1], K + NBX[q][2]);
-    uint8_t nf[6];
-    float nv[6], nv2[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        const int32_t m = nidx[q] >= 0 ? nidx[q] : n;
-        nf[q] = nidx[q] >= 0 ? fin[m] : 0;
-        nv[q] = src[m];
-        nv2[q] = out2 ? src2[m] : 0.0f;
-    }
     bool walled = false;
     if (!clear) {
         float p[3];
         facepos(t.g, A, I, J, K, t.g.dx, p);
         walled = walledface_dev(t.g, p, A, t.g.dx);
     }
+    int32_t nidx[6];
+    uint8_t nf[6];
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        nidx[q] = walled ? -1 : face(I + NBX[q][0], J + NBX[q][1], K + NBX[q][2]);
+        nf[q] = nidx[q] >= 0 ? fin[nidx[q]] : 0;
+        any = any || nf[q] != 0;
+    }
+    if (!any) {   // stays as it is
+        if (fo == 0) return;
+        out[n] = src[n];
+        if (out2) out2[n] = src2[n];
+        fout[n] = 0;
+        return;
+    }
     float acc = 0.0f, acc2 = 0.0f;
     int cnt = 0;
-    if (!walled) {
 #pragma unroll
-        for (int q = 0; q < 6; ++q)
-            if (nf[q]) {
-                acc += nv[q];
-                acc2 += nv2[q];
-                ++cnt;
-            }
-    }
-    out[n] = cnt > 0 ? __fdiv_rn(acc, (float)cnt) : src[n];
-    if (out2) out2[n] = cnt > 0 ? __fdiv_rn(acc2, (float)cnt) : src2[n];
-    fout[n] = cnt > 0 ? 1 : 0;
+    for (int q = 0; q < 6; ++q)
+        if (nf[q]) {
+            acc += src[nidx[q]];
+            if (out2) acc2 += src2[nidx[q]];
+            ++cnt;
+        }
+    out[n] = __fdiv_rn(acc, (float)cnt);
+    if (out2) out2[n] = __fdiv_rn(acc2, (float)cnt);
+    fout[n] = 1;
 }
 __global__ void k_sextrapcoarse(float* __restrict__ out, uint8_t* __restrict__ fout, const float* __restrict__ src,
                                 const uint8_t* __restrict__ fin, float* __restrict__ out2, const float* __restrict__ src2,

@@ -371,11 +371,15 @@ template <class V> __global__ void k_restrictcells(V* __restrict__ coarse, const
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
     int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
-    if (!covered(t, I, J, K)) return;
+    // the 64 fine cells lie in one block, the one covering the coarse cell
+    const int bi = I >> 2, bj = J >> 2, bk = K >> 2;
+    const int32_t sb = inblocks(t.g, bi, bj, bk) ? t.slot[blockid(t.g, bi, bj, bk)] : 0;
+    if (sb == 0) return;
+    const int32_t base = (sb - 1) * SBCELLS + ((4 * I) & 15) + 16 * (((4 * J) & 15) + 16 * ((4 * K) & 15));
     float acc = 0.0f;
     for (int c = 0; c < 4; ++c)
         for (int b = 0; b < 4; ++b)
-            for (int a = 0; a < 4; ++a) acc += (float)fine[fineat(t, 4 * I + a, 4 * J + b, 4 * K + c)];
+            for (int a = 0; a < 4; ++a) acc += (float)fine[base + a + 16 * b + 256 * c];
     coarse[l] = meanof<V>(acc, 64.0f);
 }
 template <class V> __global__ void k_restrictfaces(V* __restrict__ coarse, const V* __restrict__ fine, STopo t, int a) {
@@ -383,17 +387,31 @@ template <class V> __global__ void k_restrictfaces(V* __restrict__ coarse, const
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
     int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    // The 16 fine faces lie in one block's plane: finefaceat gives them all from the block
+    // holding fine face (4I, 4J, 4K), or all from the top plane of the block below along a,
+    // or none.
+    const int bi = I >> 2, bj = J >> 2, bk = K >> 2;
+    int32_t sb = inblocks(t.g, bi, bj, bk) ? t.slot[blockid(t.g, bi, bj, bk)] : 0;
+    bool top = false;
+    if (sb == 0) {
+        const int FA = 4 * (a == 0 ? I : (a == 1 ? J : K));
+        if ((FA & 15) != 0) return;
+        const int li = bi - (a == 0), lj = bj - (a == 1), lk = bk - (a == 2);
+        sb = inblocks(t.g, li, lj, lk) ? t.slot[blockid(t.g, li, lj, lk)] : 0;
+        if (sb == 0) return;
+        top = true;
+    }
+    const int32_t base = (sb - 1) * SBFACES;
     float acc = 0.0f;
-    int found = 0;
     for (int q = 0; q < 4; ++q)
         for (int p = 0; p < 4; ++p) {
-            int FI = 4 * I + (a == 0 ? 0 : p);
-            int FJ = 4 * J + (a == 0 ? p : (a == 1 ? 0 : q));
-            int FK = 4 * K + (a == 2 ? 0 : q);
-            int32_t idx = finefaceat(t, a, FI, FJ, FK);
-            if (idx >= 0) { acc += (float)fine[idx]; ++found; }
+            int fi = (4 * I + (a == 0 ? 0 : p)) & 15;
+            int fj = (4 * J + (a == 0 ? p : (a == 1 ? 0 : q))) & 15;
+            int fk = (4 * K + (a == 2 ? 0 : q)) & 15;
+            if (top) { if (a == 0) fi = 16; else if (a == 1) fj = 16; else fk = 16; }
+            acc += (float)fine[base + ffn(16, a, fi, fj, fk)];
         }
-    if (found == 16) coarse[l] = meanof<V>(acc, 16.0f);
+    coarse[l] = meanof<V>(acc, 16.0f);
 }
 
 void launch_restrictcells(const SparseBasin& s, float* coarse, const float* fine) {

@@ -100,14 +100,13 @@ template <class NodeMap, class Source>
 __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiled(PView ps, int64_t n, NodeMap map, Source src, P2GArgs A) {
     __shared__ unsigned long long smom[P2G_BOX], swt[P2G_BOX];
     __shared__ int red[P2G_THREADS / 64];
-    int64_t p = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * P2G_THREADS + threadIdx.x;   // neighbouring tiles on one XCD
-    bool live = p < n;
-    float xp = 0, yp = 0, zp = 0, wT = 0;
-    if (live) {
-        xp = ps.f[0][p]; yp = ps.f[1][p]; zp = ps.f[2][p];
-        wT = src.weight(p);
-    }
+    const int64_t p0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * P2G_THREADS;   // neighbouring tiles on one XCD
     for (int a = 0; a < 3; ++a) {
+        // the box from the particles in order (a wavefront's own box needs its 64 in order)
+        int64_t p = p0 + threadIdx.x;
+        bool live = p < n;
+        float xp = 0, yp = 0, zp = 0;
+        if (live) { xp = ps.f[0][p]; yp = ps.f[1][p]; zp = ps.f[2][p]; }
         float q0 = ispace(xp, A.lo[0], A.dx, a == 0 ? 0.0f : 0.5f);
         float q1 = ispace(yp, A.lo[1], A.dx, a == 1 ? 0.0f : 0.5f);
         float q2 = ispace(zp, A.lo[2], A.dx, a == 2 ? 0.0f : 0.5f);
@@ -118,8 +117,27 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiled(PView ps, int64_t n, 
         bool wgbox;
         TileBox T = tile_box(ok, bx.b, by.b, bz.b, 3, P2G_BOX, red, wgbox);
         int first = tile_first(wgbox), stride = tile_stride(wgbox);
-        float vp = 0, c0 = 0, c1 = 0, c2 = 0;
-        if (live) src.comp(p, a, vp, c0, c1, c2);
+        if (wgbox) {
+            // Depositing into the workgroup's box, a wavefront takes every eighth particle:
+            // particles of one cell (consecutive in cell order) would otherwise add to the
+            // same LDS nodes in the same instruction, which serialises the atomics. The box
+            // holds every particle's nodes, and integer sums do not depend on the order.
+            p = p0 + (int64_t)((threadIdx.x & 31) * 8 + (threadIdx.x >> 5));
+            live = p < n;
+            xp = yp = zp = 0;
+            if (live) { xp = ps.f[0][p]; yp = ps.f[1][p]; zp = ps.f[2][p]; }
+            q0 = ispace(xp, A.lo[0], A.dx, a == 0 ? 0.0f : 0.5f);
+            q1 = ispace(yp, A.lo[1], A.dx, a == 1 ? 0.0f : 0.5f);
+            q2 = ispace(zp, A.lo[2], A.dx, a == 2 ? 0.0f : 0.5f);
+            bx = bspline(q0); by = bspline(q1); bz = bspline(q2);
+            ok = live && bx.b > -(1 << 24) && bx.b < (1 << 24) && by.b > -(1 << 24) && by.b < (1 << 24) &&
+                 bz.b > -(1 << 24) && bz.b < (1 << 24) && isfinite(q0) && isfinite(q1) && isfinite(q2);
+        }
+        float vp = 0, c0 = 0, c1 = 0, c2 = 0, wT = 0;
+        if (live) {
+            src.comp(p, a, vp, c0, c1, c2);
+            wT = src.weight(p);
+        }
         if (T.use)
             for (int k = first; k < T.size(); k += stride) { smom[T.base + k] = 0ull; swt[T.base + k] = 0ull; }
         __syncthreads();

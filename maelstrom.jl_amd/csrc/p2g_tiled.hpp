@@ -31,6 +31,34 @@ __device__ inline int wg_min(int v, int* red) {
     return r;
 }
 __device__ inline int wg_max(int v, int* red) { return -wg_min(-v, red); }
+// Six workgroup minima at once (one pair of barriers); red holds 6 x (P2G_THREADS / 64).
+__device__ inline void wg_min6(int v[6], int* red) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[k] = min(v[k], __shfl_xor(v[k], o, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) red[k * (P2G_THREADS / 64) + (threadIdx.x >> 6)] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        int r = red[k * (P2G_THREADS / 64)];
+#pragma unroll
+        for (int w = 1; w < P2G_THREADS / 64; ++w) r = min(r, red[k * (P2G_THREADS / 64) + w]);
+        v[k] = r;
+    }
+}
+// The workgroup's bounds of (b0, b1, b2) over the `ok` lanes: x0 y0 z0 (BIG if none), x1 y1 z1.
+__device__ inline void wg_bounds(bool ok, int b0, int b1, int b2, int* red, int& x0, int& y0, int& z0, int& x1, int& y1,
+                                 int& z1) {
+    const int BIG = 1 << 30;
+    int v[6] = {ok ? b0 : BIG, ok ? b1 : BIG, ok ? b2 : BIG, ok ? -b0 : BIG, ok ? -b1 : BIG, ok ? -b2 : BIG};
+    wg_min6(v, red);
+    x0 = v[0]; y0 = v[1]; z0 = v[2];
+    x1 = v[3] == BIG ? -BIG : -v[3]; y1 = v[4] == BIG ? -BIG : -v[4]; z1 = v[5] == BIG ? -BIG : -v[5];
+}
 __device__ inline int wave_min(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
@@ -52,8 +80,8 @@ struct TileBox {
 __device__ inline TileBox tile_box(bool ok, int b0, int b1, int b2, int span, int cap, int* red, bool& wgbox) {
     const int BIG = 1 << 30;
     TileBox t;
-    int x0 = wg_min(ok ? b0 : BIG, red), y0 = wg_min(ok ? b1 : BIG, red), z0 = wg_min(ok ? b2 : BIG, red);
-    int x1 = wg_max(ok ? b0 : -BIG, red), y1 = wg_max(ok ? b1 : -BIG, red), z1 = wg_max(ok ? b2 : -BIG, red);
+    int x0, y0, z0, x1, y1, z1;
+    wg_bounds(ok, b0, b1, b2, red, x0, y0, z0, x1, y1, z1);
     int ex = x1 - x0 + span, ey = y1 - y0 + span, ez = z1 - z0 + span;
     wgbox = x0 != BIG && ex <= cap && ey <= cap && ez <= cap && ex * ey * ez <= cap;
     if (wgbox) {
@@ -99,7 +127,7 @@ struct P2GSource {   // flip.jl:513-537: v_a and the APIC row c_a, weighted by t
 template <class NodeMap, class Source>
 __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiled(PView ps, int64_t n, NodeMap map, Source src, P2GArgs A) {
     __shared__ unsigned long long smom[P2G_BOX], swt[P2G_BOX];
-    __shared__ int red[P2G_THREADS / 64];
+    __shared__ int red[6 * (P2G_THREADS / 64)];
     const int64_t p0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * P2G_THREADS;   // neighbouring tiles on one XCD
     for (int a = 0; a < 3; ++a) {
         // the box from the particles in order (a wavefront's own box needs its 64 in order)
@@ -197,13 +225,16 @@ template <class CellMap>
 __global__ __launch_bounds__(P2G_THREADS) void k_splat_tiled(PView ps, int64_t n, CellMap map, float lo0, float lo1, float lo2,
                                                             float dx, float dt, int32_t* __restrict__ phip) {
     __shared__ int32_t box[SPLAT_BOX];
-    __shared__ int red[P2G_THREADS / 64];
-    int64_t p = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * P2G_THREADS + threadIdx.x;   // neighbouring tiles on one XCD
-    bool live = p < n;
+    __shared__ int red[6 * (P2G_THREADS / 64)];
+    const int64_t p0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * P2G_THREADS;   // neighbouring tiles on one XCD
+    int64_t p = p0 + threadIdx.x;
+    bool live = false, ok = false;
     float xp = 0, yp = 0, zp = 0;
     int ci = 0, cj = 0, ck = 0;
-    bool ok = false;
-    if (live) {
+    auto prep = [&]() {   // particle p's advanced position and cell
+        live = p < n;
+        xp = yp = zp = 0.0f; ci = cj = ck = 0; ok = false;
+        if (!live) return;
         float vx = ps.f[3][p], vy = ps.f[4][p], vz = ps.f[5][p];
         float sp = sqrtf(vx * vx + vy * vy + vz * vz);
         float s = ps.f[15][p] + ahead(strength(sp, dt, dx)) * dt;
@@ -212,10 +243,11 @@ __global__ __launch_bounds__(P2G_THREADS) void k_splat_tiled(PView ps, int64_t n
         ci = (int)f0; cj = (int)f1; ck = (int)f2;
         // finite, in-range cells only shape the box; others take the direct path
         ok = fabsf(f0) < 16777216.0f && fabsf(f1) < 16777216.0f && fabsf(f2) < 16777216.0f;
-    }
+    };
+    prep();
     const int BIG = 1 << 30;
-    int x0 = wg_min(ok ? ci : BIG, red), y0 = wg_min(ok ? cj : BIG, red), z0 = wg_min(ok ? ck : BIG, red);
-    int x1 = wg_max(ok ? ci : -BIG, red), y1 = wg_max(ok ? cj : -BIG, red), z1 = wg_max(ok ? ck : -BIG, red);
+    int x0, y0, z0, x1, y1, z1;
+    wg_bounds(ok, ci, cj, ck, red, x0, y0, z0, x1, y1, z1);
     int ex = x1 - x0 + 3, ey = y1 - y0 + 3, ez = z1 - z0 + 3;
     bool boxed = x0 != BIG && ex <= SPLAT_BOX && ey <= SPLAT_BOX && ez <= SPLAT_BOX && ex * ey * ez <= SPLAT_BOX;
     x0 -= 1; y0 -= 1; z0 -= 1;
@@ -223,6 +255,10 @@ __global__ __launch_bounds__(P2G_THREADS) void k_splat_tiled(PView ps, int64_t n
         int V = ex * ey * ez;
         for (int k = threadIdx.x; k < V; k += P2G_THREADS) box[k] = 0x7fffffff;
         __syncthreads();
+        // every eighth particle per lane group, as in k_p2g_tiled (the minimum does not
+        // depend on the order)
+        p = p0 + (int64_t)((threadIdx.x & 31) * 8 + (threadIdx.x >> 5));
+        prep();
     }
     if (live) {
         float radius = 0.5f * dx;

@@ -330,17 +330,19 @@ __device__ inline void gather_component(const SGrid& g, int a, float px, float p
 
 // flip.jl:1226-1244 g2p_kernel! over the topology, tiled: a workgroup's 256 particles
 // (cell-sorted, so close together) gather each component from a box of faces staged in
-// LDS (the value, and the flag with a presence bit), loaded once per box node instead of
-// once per particle and node; a workgroup spread over more nodes than the box holds
-// reads the faces directly. Both paths sum the same terms in the same order.
-constexpr int G2P_BOX = 1024;
+// LDS (the value, and the flag or -1 where no fine face is there), loaded once per box
+// node instead of once per particle and node; a workgroup spread over more nodes than the
+// box holds reads the faces directly. Both paths sum the same terms in the same order. The
+// box's faces are found through a table of the slots of the blocks under it (and one
+// layer below along the component's axis, for a lower block's top faces).
+constexpr int G2P_BOX = 1024, G2P_TAB = 64;
 __global__ __launch_bounds__(256) void k_sparse_g2p(PView ps, int64_t n, STopo t, const float* __restrict__ u,
                                                     const float* __restrict__ v, const float* __restrict__ w,
                                                     const uint8_t* __restrict__ fu, const uint8_t* __restrict__ fv,
                                                     const uint8_t* __restrict__ fw, float dt, float gravity) {
-    __shared__ float sval[G2P_BOX];
-    __shared__ uint32_t sflag[G2P_BOX];   // bit 8: the face exists; low byte: its flag
-    __shared__ int red[4];
+    __shared__ float2 sbox[G2P_BOX];      // value, flag (-1: no face)
+    __shared__ int32_t stab[G2P_TAB];
+    __shared__ int red[6 * 4];
     const SGrid& g = t.g;
     int64_t p = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;   // neighbouring tiles on one XCD
     bool live = p < n;
@@ -349,6 +351,7 @@ __global__ __launch_bounds__(256) void k_sparse_g2p(PView ps, int64_t n, STopo t
     const float* U[3] = {u, v, w};
     const uint8_t* F[3] = {fu, fv, fw};
     float fl[3], val[3], c0[3], c1[3], c2[3];
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
         float q0 = ispace(xp, g.lo[0], g.dx, a == 0 ? 0.0f : 0.5f);
         float q1 = ispace(yp, g.lo[1], g.dx, a == 1 ? 0.0f : 0.5f);
@@ -357,29 +360,52 @@ __global__ __launch_bounds__(256) void k_sparse_g2p(PView ps, int64_t n, STopo t
         bool ok = live && bx.b > -(1 << 24) && bx.b < (1 << 24) && by.b > -(1 << 24) && by.b < (1 << 24) &&
                   bz.b > -(1 << 24) && bz.b < (1 << 24) && isfinite(q0) && isfinite(q1) && isfinite(q2);
         const int BIG = 1 << 30;
-        int x0 = wg_min(ok ? bx.b : BIG, red), y0 = wg_min(ok ? by.b : BIG, red), z0 = wg_min(ok ? bz.b : BIG, red);
-        int x1 = wg_max(ok ? bx.b : -BIG, red), y1 = wg_max(ok ? by.b : -BIG, red), z1 = wg_max(ok ? bz.b : -BIG, red);
+        int x0, y0, z0, x1, y1, z1;
+        wg_bounds(ok, bx.b, by.b, bz.b, red, x0, y0, z0, x1, y1, z1);
         int ex = x1 - x0 + 3, ey = y1 - y0 + 3, ez = z1 - z0 + 3;
         bool boxed = x0 != BIG && ex <= G2P_BOX && ey <= G2P_BOX && ez <= G2P_BOX && ex * ey * ez <= G2P_BOX;
         const float* Ua = U[a];
         const uint8_t* Fa = F[a];
         if (boxed) {
+            const int tx = (x0 >> 4) - (a == 0), ty = (y0 >> 4) - (a == 1), tz = (z0 >> 4) - (a == 2);
+            const int nx = ((x0 + ex - 1) >> 4) - tx + 1, ny = ((y0 + ey - 1) >> 4) - ty + 1, nz = ((z0 + ez - 1) >> 4) - tz + 1;
+            const bool tab = nx * ny * nz <= G2P_TAB;
+            if (tab && (int)threadIdx.x < nx * ny * nz) {
+                int bi = tx + (int)threadIdx.x % nx, bj = ty + ((int)threadIdx.x / nx) % ny, bk = tz + (int)threadIdx.x / (nx * ny);
+                stab[threadIdx.x] = inblocks(g, bi, bj, bk) ? t.slot[blockid(g, bi, bj, bk)] : 0;
+            }
+            __syncthreads();
             int V = ex * ey * ez;
             for (int k = threadIdx.x; k < V; k += 256) {
-                int32_t idx = finefaceat(t, a, x0 + k % ex, y0 + (k / ex) % ey, z0 + k / (ex * ey));
-                sval[k] = idx >= 0 ? Ua[idx] : 0.0f;
-                sflag[k] = idx >= 0 ? (256u | (uint32_t)Fa[idx]) : 0u;
+                const int I = x0 + k % ex, J = y0 + (k / ex) % ey, K = z0 + k / (ex * ey);
+                int32_t idx;
+                if (tab) {   // finefaceat through the table
+                    const int di = (I >> 4) - tx, dj = (J >> 4) - ty, dk = (K >> 4) - tz;
+                    const int32_t s1 = stab[di + nx * (dj + ny * dk)];
+                    if (s1 != 0) {
+                        idx = (s1 - 1) * SBFACES + ffn(16, a, I & 15, J & 15, K & 15);
+                    } else {
+                        const int GA = a == 0 ? I : (a == 1 ? J : K);
+                        const int32_t s2 =
+                            (GA & 15) == 0 ? stab[(di - (a == 0)) + nx * ((dj - (a == 1)) + ny * (dk - (a == 2)))] : 0;
+                        idx = s2 != 0 ? (s2 - 1) * SBFACES + ffn(16, a, a == 0 ? 16 : (I & 15), a == 1 ? 16 : (J & 15),
+                                                                 a == 2 ? 16 : (K & 15))
+                                      : -1;
+                    }
+                } else {
+                    idx = finefaceat(t, a, I, J, K);
+                }
+                sbox[k] = idx >= 0 ? make_float2(Ua[idx], (float)Fa[idx]) : make_float2(0.0f, -1.0f);
             }
             __syncthreads();
         }
         if (live) {
             if (boxed && ok) {
                 auto fetch = [&](int I, int J, int K, bool& pres, float& uv, float& fv_) {
-                    int l = (I - x0) + ex * ((J - y0) + ey * (K - z0));
-                    uint32_t f = sflag[l];
-                    pres = (f & 256u) != 0;
-                    uv = sval[l];
-                    fv_ = (float)(f & 255u);
+                    const float2 e = sbox[(I - x0) + ex * ((J - y0) + ey * (K - z0))];
+                    pres = e.y >= 0.0f;
+                    uv = e.x;
+                    fv_ = e.y;
                 };
                 gather_component(g, a, xp, yp, zp, fetch, fl[a], val[a], c0[a], c1[a], c2[a]);
             } else {

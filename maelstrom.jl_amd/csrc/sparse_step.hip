@@ -218,6 +218,85 @@ __global__ void k_sweepair(uint8_t* __restrict__ reach, int* __restrict__ change
     }
     if (count > 0) atomicAdd(changed, count);
 }
+// The same sweep with a wavefront per line: in the serial passes, a reached AIR cell
+// reaches every AIR cell after it in its run (forward) and then before it (backward). Each
+// lane takes up to SWEEP_CHUNK consecutive cells; a lane's effect on the carry is "pass it
+// on" (all its cells AIR) or a fixed value, and a scan over the wavefront gives every lane
+// the carry the serial pass brings into its cells, so the reached cells and the count of
+// newly reached ones are the serial sweep's.
+constexpr int SWEEP_CHUNK = 8;   // lines of at most 64 x 8 cells
+__device__ inline void carry_scan(int& pass, int& val, bool reverse) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int p2 = reverse ? __shfl_down(pass, o, 64) : __shfl_up(pass, o, 64);
+        int v2 = reverse ? __shfl_down(val, o, 64) : __shfl_up(val, o, 64);
+        bool has = reverse ? lane + o < 64 : lane >= o;
+        if (has) {   // the earlier lanes' effect (p2, v2), then this lane's (pass, val)
+            val = pass ? (v2 | val) : val;
+            pass = pass & p2;
+        }
+    }
+}
+__global__ __launch_bounds__(256) void k_sweepairw(uint8_t* __restrict__ reach, int* __restrict__ changed,
+                                                   const uint8_t* __restrict__ types, int3 n, int axis) {
+    const int lane = threadIdx.x & 63;
+    const int na = axis == 0 ? n.y : n.x, nb = axis == 2 ? n.y : n.z;
+    const int line = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (line >= na * nb) return;   // the whole wavefront
+    const int a = line % na, b = line / na;
+    const int len = axis == 0 ? n.x : (axis == 1 ? n.y : n.z), ch = (len + 63) / 64;
+    const size_t stride = axis == 0 ? 1 : (axis == 1 ? (size_t)n.x : (size_t)n.x * n.y);
+    const size_t c0 = axis == 0 ? cl(0, a, b, n.x, n.y) : (axis == 1 ? cl(a, 0, b, n.x, n.y) : cl(a, b, 0, n.x, n.y));
+    bool air[SWEEP_CHUNK], r0[SWEEP_CHUNK], r[SWEEP_CHUNK];
+    int allair = 1;
+#pragma unroll
+    for (int e = 0; e < SWEEP_CHUNK; ++e) {
+        const int m = lane * ch + e;
+        const bool in = e < ch && m < len;
+        air[e] = in && types[c0 + (size_t)m * stride] == AIR;
+        r0[e] = in && reach[c0 + (size_t)m * stride] != 0;
+        r[e] = r0[e];
+        if (e < ch && !air[e]) allair = 0;
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool back = pass == 1;
+        // this lane's effect from no carry
+        int val = 0;
+#pragma unroll
+        for (int e0 = 0; e0 < SWEEP_CHUNK; ++e0) {
+            const int e = back ? SWEEP_CHUNK - 1 - e0 : e0;
+            if (e >= ch) continue;
+            val = air[e] ? (val | (int)r[e]) : 0;
+        }
+        int pz = allair, vz = val;
+        carry_scan(pz, vz, back);   // inclusive over this lane and the earlier ones
+        int cin = back ? __shfl_down(vz, 1, 64) : __shfl_up(vz, 1, 64);
+        if (back ? lane == 63 : lane == 0) cin = 0;
+        int c = cin;
+#pragma unroll
+        for (int e0 = 0; e0 < SWEEP_CHUNK; ++e0) {
+            const int e = back ? SWEEP_CHUNK - 1 - e0 : e0;
+            if (e >= ch) continue;
+            if (air[e]) {
+                if (r[e]) c = 1;
+                else if (c) r[e] = true;
+            } else {
+                c = 0;
+            }
+        }
+    }
+    int count = 0;
+#pragma unroll
+    for (int e = 0; e < SWEEP_CHUNK; ++e)
+        if (r[e] && !r0[e]) {
+            reach[c0 + (size_t)(lane * ch + e) * stride] = 1;
+            ++count;
+        }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) count += __shfl_xor(count, o, 64);
+    if (lane == 0 && count > 0) atomicAdd(changed, count);
+}
 __global__ void k_fillfine(uint8_t* __restrict__ tf, float* __restrict__ phi, const uint8_t* __restrict__ reach,
                            const uint8_t* __restrict__ tc, STopo t, float depth, int32_t nf) {
     int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -249,12 +328,21 @@ void SparseBasin::filltrapped(int maxrounds) {
     hipLaunchKernelGGL(k_seedair, dim3(cdiv(ncells(), 256)), dim3(256), 0, s, reach.p, types.coarse.p, nn);
     for (int r = 0; r < maxrounds; ++r) {
         fill_async(changed.p, (size_t)1, 0, s);
-        hipLaunchKernelGGL(k_sweepair, dim3(cdiv(nn.x, 16), cdiv(nn.y, 16)), dim3(16, 16), 0, s, reach.p, changed.p,
-                           types.coarse.p, nn, 2);
-        hipLaunchKernelGGL(k_sweepair, dim3(cdiv(nn.y, 16), cdiv(nn.z, 16)), dim3(16, 16), 0, s, reach.p, changed.p,
-                           types.coarse.p, nn, 0);
-        hipLaunchKernelGGL(k_sweepair, dim3(cdiv(nn.x, 16), cdiv(nn.z, 16)), dim3(16, 16), 0, s, reach.p, changed.p,
-                           types.coarse.p, nn, 1);
+        if (std::max(nn.x, std::max(nn.y, nn.z)) <= 64 * SWEEP_CHUNK) {   // a wavefront per line
+            hipLaunchKernelGGL(k_sweepairw, dim3(cdiv((size_t)nn.x * nn.y, 4)), dim3(256), 0, s, reach.p, changed.p,
+                               types.coarse.p, nn, 2);
+            hipLaunchKernelGGL(k_sweepairw, dim3(cdiv((size_t)nn.y * nn.z, 4)), dim3(256), 0, s, reach.p, changed.p,
+                               types.coarse.p, nn, 0);
+            hipLaunchKernelGGL(k_sweepairw, dim3(cdiv((size_t)nn.x * nn.z, 4)), dim3(256), 0, s, reach.p, changed.p,
+                               types.coarse.p, nn, 1);
+        } else {
+            hipLaunchKernelGGL(k_sweepair, dim3(cdiv(nn.x, 16), cdiv(nn.y, 16)), dim3(16, 16), 0, s, reach.p, changed.p,
+                               types.coarse.p, nn, 2);
+            hipLaunchKernelGGL(k_sweepair, dim3(cdiv(nn.y, 16), cdiv(nn.z, 16)), dim3(16, 16), 0, s, reach.p, changed.p,
+                               types.coarse.p, nn, 0);
+            hipLaunchKernelGGL(k_sweepair, dim3(cdiv(nn.x, 16), cdiv(nn.z, 16)), dim3(16, 16), 0, s, reach.p, changed.p,
+                               types.coarse.p, nn, 1);
+        }
         MAEL_LAUNCH_CHECK();
         MAEL_HIP(hipMemcpyAsync(st.pinned, changed.p, sizeof(int), hipMemcpyDeviceToHost, s));
         st.sync();

@@ -332,3 +332,31 @@ def test_steps_keep_volume(M, surface):
     v = M.sparse.volume(s)
     assert 0.95 < v / v0 < 1.05
     assert np.isfinite(s.get("U_F")).all() and s.n > 0
+
+
+def test_filltrapped_matches_flood_fill(M):
+    """`filltrapped!` (sflip.jl:950-990): the line sweeps, repeated until nothing changes,
+    reach exactly the AIR cells joined to the top layer's AIR cells through face-adjacent
+    AIR cells; every other uncovered AIR coarse cell becomes LIQUID. Checked against a
+    connected-component labelling on random coarse types, with no fine blocks."""
+    import scipy.ndimage as ndi
+    from maelstrom_amd import _lib as L
+    print("case: sparse filltrapped vs flood fill", flush=True)
+    SOLID, LIQUID, AIR = 0, 1, 2
+    s = M.SparseBasin(_basin(M), dx=DX, blocks=64, capacity=10000)   # no fine blocks yet
+    shape = s.shape("TYPES_C")
+    rng = np.random.default_rng(11)
+    for trial in range(3):
+        # AIR pockets in liquid, with solid bits and open shafts to the top
+        t = np.where(rng.random(shape) < 0.45 + 0.1 * trial, AIR, LIQUID).astype(np.uint8)
+        t[rng.random(shape) < 0.05] = SOLID
+        s.set("TYPES_C", t)
+        s.set("PHI_C", np.zeros(shape, np.float32))
+        s.ctx.check(L.lib().mael_sparse_filltrapped(s.h, 256))
+        got = s.get("TYPES_C").reshape(shape)
+        lab, _ = ndi.label(t == AIR, structure=ndi.generate_binary_structure(3, 1))
+        top = np.unique(lab[-1][lab[-1] > 0])
+        reached = np.isin(lab, top) & (t == AIR)
+        want = t.copy()
+        want[(t == AIR) & ~reached] = LIQUID
+        assert np.array_equal(got, want), (trial, int((got != want).sum()))

@@ -394,16 +394,27 @@ __device__ inline void coarserow_acc(const STopo& t, unsigned msk, FX fx, const 
             int In = I + sg * (a == 0), Jn = J + sg * (a == 1), Kn = K + sg * (a == 2);
             int32_t sn = t.slot[blockid(t.g, In >> 2, Jn >> 2, Kn >> 2)];
             uint32_t cbase = (uint32_t)(sn - 1) * V, fbase = (uint32_t)(sn - 1) * per;
+            // the R x R patch of the neighbour block's cells on the shared side: along a its
+            // near plane (local 0 above, N - 1 below), across it R (coordinate mod 4) + p, q
+            const int la = sg > 0 ? 0 : m;
+            const int bi = a == 0 ? la : R * (I & 3), bj = a == 1 ? la : R * (J & 3), bk = a == 2 ? la : R * (K & 3);
+            const uint32_t c0 = cbase + (uint32_t)(bi + N * (bj + N * bk));
+            constexpr uint32_t sp0 = N, sq2 = N;   // cell strides of p and q (a = 0: j, k; 1: i, k; 2: i, j)
+            const uint32_t sp = a == 0 ? sp0 : 1u, sq = a == 2 ? sq2 : (uint32_t)(N * N);
+            // the faces between: the neighbour's own at its plane 0 above, the lower block's
+            // top plane (coordinate N) below
+            const int fa = sg > 0 ? 0 : N;
+            const uint32_t f0 = fbase + (uint32_t)ffn(N, a, a == 0 ? fa : bi, a == 1 ? fa : bj, a == 2 ? fa : bk);
+            const uint32_t fp = a == 0 ? (uint32_t)(N + 1) : 1u,
+                           fq = a == 0 ? (uint32_t)((N + 1) * N) : (a == 1 ? (uint32_t)(N * (N + 1)) : (uint32_t)N);
+#pragma unroll
             for (int q = 0; q < R; ++q)
+#pragma unroll
                 for (int p = 0; p < R; ++p) {
-                    int FI = a == 0 ? (sg > 0 ? R * In : R * I - 1) : R * I + p;
-                    int FJ = a == 1 ? (sg > 0 ? R * Jn : R * J - 1) : R * J + (a == 0 ? p : q);
-                    int FK = a == 2 ? (sg > 0 ? R * Kn : R * K - 1) : R * K + q;
-                    int li = FI & m, lj = FJ & m, lk = FK & m;
-                    uint32_t cidx = cbase + (uint32_t)(li + N * (lj + N * lk));
-                    uint32_t fidx = fbase + (uint32_t)(sg > 0 ? ffn(N, a, li, lj, lk)
-                                                              : ffn(N, a, a == 0 ? N : li, a == 1 ? N : lj, a == 2 ? N : lk));
-                    A c = ty ? (A)(ty->tf[cidx] == LIQUID ? FINE_COARSE : 0.0f) : (A)lf(cf.c[a][fidx]);
+                    const uint32_t cidx = c0 + (uint32_t)p * sp + (uint32_t)q * sq;
+                    A c;
+                    if (ty) c = (A)(ty->tf[cidx] == LIQUID ? FINE_COARSE : 0.0f);
+                    else c = (A)lf(cf.c[a][f0 + (uint32_t)p * fp + (uint32_t)q * fq]);
                     diag += c;
                     off += c * fx(cidx);
                 }

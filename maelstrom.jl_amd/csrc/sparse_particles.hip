@@ -293,7 +293,10 @@ void SparseBasin::p2g(float dt) {
 // One component's gather at a particle: the flag sum (finegather_value) and the APIC
 // value and affine row (finegather, sflip.jl:193-215) in one pass over the 27 nodes, in
 // the reference's order; `fetch(I, J, K, present, u, flag)` reads a node.
-template <class Fetch>
+// With ZERO the fetch returns 0 for the value and a negative flag where no face is, and a
+// missing node adds zero products instead of being skipped: the sums start at +0 and are
+// never -0, so adding a zero leaves them bit for bit as skipping does, without a branch.
+template <bool ZERO = false, class Fetch>
 __device__ inline void gather_component(const SGrid& g, int a, float px, float py, float pz, Fetch fetch, float& flag,
                                         float& vel, float& a0, float& a1, float& a2) {
     float q0 = ispace(px, g.lo[0], g.dx, a == 0 ? 0.0f : 0.5f);
@@ -312,8 +315,9 @@ __device__ inline void gather_component(const SGrid& g, int a, float px, float p
                 bool pres;
                 float uv, fv;
                 fetch(bx.b + mx, by.b + my, bz.b + mz, pres, uv, fv);
-                if (pres) {
+                if (ZERO || pres) {
                     float wgt = pick(bx, mx) * wy * wz;
+                    if (ZERO) wgt = pres ? wgt : 0.0f;
                     flag += wgt * fv;
                     float tt = wgt * uv;
                     vel += tt;
@@ -407,7 +411,7 @@ __global__ __launch_bounds__(256) void k_sparse_g2p(PView ps, int64_t n, STopo t
                     uv = e.x;
                     fv_ = e.y;
                 };
-                gather_component(g, a, xp, yp, zp, fetch, fl[a], val[a], c0[a], c1[a], c2[a]);
+                gather_component<true>(g, a, xp, yp, zp, fetch, fl[a], val[a], c0[a], c1[a], c2[a]);
             } else {
                 auto fetch = [&](int I, int J, int K, bool& pres, float& uv, float& fv_) {
                     int32_t idx = finefaceat(t, a, I, J, K);

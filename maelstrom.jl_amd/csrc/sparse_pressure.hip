@@ -1664,9 +1664,9 @@ struct SparseSolver {
 
     int precision = 0;   // 0 FP32, 1 FP16 storage, 2 BF16 storage
     // z = M^-1 r. In reduced precision r is scaled by 1 / scal[nslot] on the way in (its
-    // norm, so its entries are at most 1) and z back on the way out.
-    // z = M^-1 r. With `rold` (flexible CG), z.r and z.rold are also summed into the
-    // partials as k_sdot2 does (fused with the conversion back in reduced precision).
+    // norm, so its entries are at most 1) and z back on the way out. With `rold`
+    // (flexible CG), z.r and z.rold are also summed into the partials as k_sdot2 does
+    // (fused with the conversion back in reduced precision).
     void vcycle(SVec z, SVec r, int sweeps, int coarse, float boost, int nslot = -1, const SVec* rold = nullptr) {
         if (precision == 0) {
             Hier<float, float> H = hier32();

@@ -499,20 +499,24 @@ template <class T> struct alignas(4 * sizeof(T)) Four { T v[4]; };
 template <class T> __device__ inline Four<T> load4(const T* __restrict__ p, uint32_t i) {
     return *reinterpret_cast<const Four<T>*>(p + i);
 }
-constexpr int RS = 24, PS = 18 * RS, HV = 6 * PS;   // LDS rows of 24: cell li (-1..16) at 4 + li
-template <int MODE, class T, class C>
+constexpr int RS = 24, PS = 18 * RS;   // LDS rows of 24: cell li (-1..16) at 4 + li; planes of 18 rows
+// NQ: quads a thread (a workgroup takes 4 NQ planes of its slot, its 16 / (4 NQ) tiles
+// cover the slot); more cells a workgroup keep more loads in flight.
+template <int MODE, class T, class C, int NQ = 1>
 __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
                                                    const T* __restrict__ xc, CompT<C> L, STopo t, SearchArgs S) {
     constexpr bool Z = MODE == 3 || MODE == 5, SEARCH = MODE == 4;
     constexpr bool RW = MODE == 0 && !std::is_same<T, float>::value;   // the weighted sweep
     constexpr bool NW = !RW;                                               // the couplings staged
+    constexpr int PL = 4 * NQ, HV = (PL + 2) * PS, TPS = 4 / NQ;          // planes, LDS, tiles a slot
+    static_assert(NQ == 1 || NQ == 2, "a tile is 4 or 8 planes");
     __shared__ float4 sw4[NW ? HV / 4 : 1];    // the couplings
     __shared__ float4 sp4[Z ? 1 : HV / 4];     // coupling times x
     __shared__ double lds[4];
     float* sw = reinterpret_cast<float*>(sw4);
     float* sp = reinterpret_cast<float*>(sp4);
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int s = (int)(tile >> 2) + 1, k0 = (int)(tile & 3) * 4;
+    const int s = (int)(tile / TPS) + 1, k0 = (int)(tile % TPS) * PL;
     const uint32_t cb = (uint32_t)(s - 1) * 4096u;
     const uint8_t* __restrict__ tf = L.actf;   // level 0: the fine cell types
     const float beta = SEARCH ? (float)S.scal[SS_BETA] : 0.0f;
@@ -522,56 +526,68 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
         else return lf(xs[i]);
     };
     const int tid = threadIdx.x, i4 = tid & 3, j = (tid >> 2) & 15, kk = tid >> 6;
-    const uint32_t n = cb + 256u * (uint32_t)(k0 + kk) + 16u * (uint32_t)j + 4u * (uint32_t)i4;
+    // quad q of the thread: plane kk + 4 q of the tile
+    auto cell = [&](int q) { return cb + 256u * (uint32_t)(k0 + kk + 4 * q) + 16u * (uint32_t)j + 4u * (uint32_t)i4; };
     int4 w = t.where[s - 1];
     if (w.x < 0) {
-        Four<T> z4;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) z4.v[e] = sf<T>(0.0f);
-        *reinterpret_cast<Four<T>*>(out + n) = z4;
-        if constexpr (SEARCH) {
-            float4 d4;
-            d4.x = xat(x, S.dof, n); d4.y = xat(x, S.dof, n + 1); d4.z = xat(x, S.dof, n + 2); d4.w = xat(x, S.dof, n + 3);
-            *reinterpret_cast<float4*>(S.dnf + n) = d4;
-            if (threadIdx.x == 0) S.partials[tile] = 0.0;
+        for (int q = 0; q < NQ; ++q) {
+            const uint32_t n = cell(q);
+            Four<T> z4;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) z4.v[e] = sf<T>(0.0f);
+            *reinterpret_cast<Four<T>*>(out + n) = z4;
+            if constexpr (SEARCH) {
+                float4 d4;
+                d4.x = xat(x, S.dof, n); d4.y = xat(x, S.dof, n + 1); d4.z = xat(x, S.dof, n + 2); d4.w = xat(x, S.dof, n + 3);
+                *reinterpret_cast<float4*>(S.dnf + n) = d4;
+            }
         }
+        if constexpr (SEARCH)
+            if (threadIdx.x == 0) S.partials[tile] = 0.0;
         return;
     }
-    // the thread's own cells: types, diagonal, right-hand side, x
-    const uint32_t tq = *reinterpret_cast<const uint32_t*>(tf + n);
-    const Four<C> dq = load4(RW ? L.rwf : L.dDf, n);   // the diagonal's Dirichlet part, or the weight
-    Four<T> bq;
-    if constexpr (MODE == 2 || MODE == 4 || MODE == 5) {
+    // the thread's own cells: types, diagonal (or weight), right-hand side, x
+    uint32_t tq[NQ];
+    Four<C> dq[NQ];
+    Four<T> bq[NQ];
+    float x0[NQ][4], w0[NQ][4], p0[NQ][4];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) bq.v[e] = sf<T>(0.0f);
-    } else {
-        bq = load4(b, n);
-    }
-    float x0[4], w0[4], p0[4];
-    if constexpr (!Z) {
-        const Four<T> xq = load4(x, n);
-        if constexpr (SEARCH) {
-            const float4 dold = *reinterpret_cast<const float4*>(S.dof + n);
-            const float dv[4] = {dold.x, dold.y, dold.z, dold.w};
+    for (int q = 0; q < NQ; ++q) {
+        const uint32_t n = cell(q);
+        tq[q] = *reinterpret_cast<const uint32_t*>(tf + n);
+        dq[q] = load4(RW ? L.rwf : L.dDf, n);   // the diagonal's Dirichlet part, or the weight
+        if constexpr (MODE == 2 || MODE == 4 || MODE == 5) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) x0[e] = lf(sf<T>(lf(xq.v[e]) + beta * dv[e]));
-            *reinterpret_cast<float4*>(S.dnf + n) = make_float4(x0[0], x0[1], x0[2], x0[3]);
+            for (int e = 0; e < 4; ++e) bq[q].v[e] = sf<T>(0.0f);
+        } else {
+            bq[q] = load4(b, n);
+        }
+        if constexpr (!Z) {
+            const Four<T> xq = load4(x, n);
+            if constexpr (SEARCH) {
+                const float4 dold = *reinterpret_cast<const float4*>(S.dof + n);
+                const float dv[4] = {dold.x, dold.y, dold.z, dold.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) x0[q][e] = lf(sf<T>(lf(xq.v[e]) + beta * dv[e]));
+                *reinterpret_cast<float4*>(S.dnf + n) = make_float4(x0[q][0], x0[q][1], x0[q][2], x0[q][3]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) x0[q][e] = lf(xq.v[e]);
+            }
         } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) x0[e] = lf(xq.v[e]);
+            for (int e = 0; e < 4; ++e) x0[q][e] = 0.0f;
         }
-    } else {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) x0[e] = 0.0f;
+        for (int e = 0; e < 4; ++e) {
+            w0[q][e] = ((tq[q] >> (8 * e)) & 0xffu) == LIQUID ? FINE_FINE : 0.0f;
+            p0[q][e] = w0[q][e] > 0.0f ? w0[q][e] * x0[q][e] : 0.0f;
+        }
+        const int hq = 4 + 4 * i4 + RS * (j + 1) + PS * (kk + 4 * q + 1);   // LDS place of its first cell
+        if constexpr (NW) sw4[hq >> 2] = make_float4(w0[q][0], w0[q][1], w0[q][2], w0[q][3]);
+        if constexpr (!Z) sp4[hq >> 2] = make_float4(p0[q][0], p0[q][1], p0[q][2], p0[q][3]);
     }
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        w0[e] = ((tq >> (8 * e)) & 0xffu) == LIQUID ? FINE_FINE : 0.0f;
-        p0[e] = w0[e] > 0.0f ? w0[e] * x0[e] : 0.0f;
-    }
-    const int hq = 4 + 4 * i4 + RS * (j + 1) + PS * (kk + 1);   // LDS place of the first cell
-    if constexpr (NW) sw4[hq >> 2] = make_float4(w0[0], w0[1], w0[2], w0[3]);
-    if constexpr (!Z) sp4[hq >> 2] = make_float4(p0[0], p0[1], p0[2], p0[3]);
 
     int32_t nb[6];
 #pragma unroll
@@ -622,86 +638,93 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
         if constexpr (NW) sw4[h >> 2] = make_float4(wv[0], wv[1], wv[2], wv[3]);
         if constexpr (!Z) sp4[h >> 2] = make_float4(pv[0], pv[1], pv[2], pv[3]);
     };
-    if (tid < 128) {   // the z layers below and above the quarter: 2 x 16 rows x 4 quads
+    if (tid < 128) {   // the z layers below and above the tile: 2 x 16 rows x 4 quads
         int top = tid >> 6, lj = (tid >> 2) & 15, q = tid & 3;
-        int lk = top ? k0 + 4 : k0 - 1;
-        stage4(4 * q, lj, lk, top ? nb[5] : nb[4], 4 + 4 * q + RS * (lj + 1) + PS * (top ? 5 : 0));
-    } else {           // the x sides: 2 x 16 rows x 4 planes, one cell each
-        int e = tid - 128, side = e >> 6, lj = (e >> 2) & 15, p = e & 3;
-        int li = side ? 16 : -1, lk = k0 + p;
-        int h = 4 + li + RS * (lj + 1) + PS * (p + 1);
-        int32_t sn = side ? nb[1] : nb[0];
-        float wv, pv;
-        if (sn != 0) {
-            uint32_t idx = (uint32_t)(sn - 1) * 4096u + (uint32_t)((li & 15) + 16 * (lj + 16 * lk));
-            wv = tf[idx] == LIQUID ? FINE_FINE : 0.0f;
-            float xv = Z ? 0.0f : xat(x, S.dof, idx);
-            pv = wv > 0.0f ? wv * xv : 0.0f;
-        } else {
-            coarse(li, lj, lk, wv, pv);
+        int lk = top ? k0 + PL : k0 - 1;
+        stage4(4 * q, lj, lk, top ? nb[5] : nb[4], 4 + 4 * q + RS * (lj + 1) + PS * (top ? PL + 1 : 0));
+    } else {           // the x sides: 2 x 16 rows x PL planes, one cell each
+#pragma unroll
+        for (int r = 0; r < NQ; ++r) {
+            int e = tid - 128 + 128 * r, side = e & 1, lj = (e >> 1) & 15, p = e >> 5;
+            int li = side ? 16 : -1, lk = k0 + p;
+            int h = 4 + li + RS * (lj + 1) + PS * (p + 1);
+            int32_t sn = side ? nb[1] : nb[0];
+            float wv, pv;
+            if (sn != 0) {
+                uint32_t idx = (uint32_t)(sn - 1) * 4096u + (uint32_t)((li & 15) + 16 * (lj + 16 * lk));
+                wv = tf[idx] == LIQUID ? FINE_FINE : 0.0f;
+                float xv = Z ? 0.0f : xat(x, S.dof, idx);
+                pv = wv > 0.0f ? wv * xv : 0.0f;
+            } else {
+                coarse(li, lj, lk, wv, pv);
+            }
+            if constexpr (NW) sw[h] = wv;
+            if constexpr (!Z) sp[h] = pv;
         }
-        if constexpr (NW) sw[h] = wv;
-        if constexpr (!Z) sp[h] = pv;
     }
-    if (tid < 32) {    // the y sides: 2 x 4 planes x 4 quads
-        int side = tid >> 4, p = (tid >> 2) & 3, q = tid & 3;
+    if (tid < 32 * NQ) {   // the y sides: 2 x PL planes x 4 quads
+        int side = tid & 1, q = (tid >> 1) & 3, p = tid >> 3;
         int lj = side ? 16 : -1;
         stage4(4 * q, lj, k0 + p, side ? nb[3] : nb[2], 4 + 4 * q + RS * (lj + 1) + PS * (p + 1));
     }
     __syncthreads();
 
-    // the neighbours' couplings and products: x within the quad from registers, the
-    // quad's x ends, y and z from LDS; order -x, +x, -y, +y, -z, +z
-    float4 wym = make_float4(0, 0, 0, 0), wyp = wym, wzm = wym, wzp = wym;
-    float wl = 0.0f, wr = 0.0f;
-    if constexpr (NW) {
-        wym = sw4[(hq - RS) >> 2]; wyp = sw4[(hq + RS) >> 2]; wzm = sw4[(hq - PS) >> 2]; wzp = sw4[(hq + PS) >> 2];
-        wl = sw[hq - 1]; wr = sw[hq + 4];
-    }
-    float4 pym = make_float4(0, 0, 0, 0), pyp = pym, pzm = pym, pzp = pym;
-    float pl = 0.0f, pr = 0.0f;
-    if constexpr (!Z) {
-        pym = sp4[(hq - RS) >> 2]; pyp = sp4[(hq + RS) >> 2]; pzm = sp4[(hq - PS) >> 2]; pzp = sp4[(hq + PS) >> 2];
-        pl = sp[hq - 1]; pr = sp[hq + 4];
-    }
-    const float wy0[4] = {wym.x, wym.y, wym.z, wym.w}, wy1[4] = {wyp.x, wyp.y, wyp.z, wyp.w};
-    const float wz0[4] = {wzm.x, wzm.y, wzm.z, wzm.w}, wz1[4] = {wzp.x, wzp.y, wzp.z, wzp.w};
-    const float py0[4] = {pym.x, pym.y, pym.z, pym.w}, py1[4] = {pyp.x, pyp.y, pyp.z, pyp.w};
-    const float pz0[4] = {pzm.x, pzm.y, pzm.z, pzm.w}, pz1[4] = {pzp.x, pzp.y, pzp.z, pzp.w};
     double pq = 0.0;
-    Four<T> oq;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const uint32_t t0 = (tq >> (8 * e)) & 0xffu;
-        const float dd = lf(dq.v[e]), bb = lf(bq.v[e]);
-        const float wn[6] = {e == 0 ? wl : w0[e - 1], e == 3 ? wr : w0[e + 1], wy0[e], wy1[e], wz0[e], wz1[e]};
-        const float pn[6] = {e == 0 ? pl : p0[e - 1], e == 3 ? pr : p0[e + 1], py0[e], py1[e], pz0[e], pz1[e]};
-        float diag = dd, off = 0.0f;
-#pragma unroll
-        for (int d = 0; d < 6; ++d) {
-            diag = diag + wn[d];
-            off = off + pn[d];
+    for (int q = 0; q < NQ; ++q) {
+        const int hq = 4 + 4 * i4 + RS * (j + 1) + PS * (kk + 4 * q + 1);
+        // the neighbours' couplings and products: x within the quad from registers, the
+        // quad's x ends, y and z from LDS; order -x, +x, -y, +y, -z, +z
+        float4 wym = make_float4(0, 0, 0, 0), wyp = wym, wzm = wym, wzp = wym;
+        float wl = 0.0f, wr = 0.0f;
+        if constexpr (NW) {
+            wym = sw4[(hq - RS) >> 2]; wyp = sw4[(hq + RS) >> 2]; wzm = sw4[(hq - PS) >> 2]; wzp = sw4[(hq + PS) >> 2];
+            wl = sw[hq - 1]; wr = sw[hq + 4];
         }
-        if (t0 != LIQUID) { diag = dd; off = 0.0f; }   // only a LIQUID cell couples
-        const bool act = (MODE == 2 || SEARCH) ? t0 == LIQUID : t0 != 0;
-        float res = 0.0f;
-        if (act) {
-            if constexpr (RW) {
-                const float rw = dd;   // omega / diag
-                res = rw > 0.0f ? (1.0f - OMEGA) * x0[e] + rw * (bb + off) : 0.0f;
-            } else if constexpr (MODE == 5) {
-                res = diag > 0.0f ? __fdiv_rn(OMEGA, diag) : 0.0f;
-            } else {
-                float ax = diag * x0[e] - off;
-                if (MODE == 0 || MODE == 3) res = diag > 0.0f ? x0[e] + __fdiv_rn(OMEGA * (bb - ax), diag) : 0.0f;
-                else if (MODE == 1) res = bb - ax;
-                else res = ax;
+        float4 pym = make_float4(0, 0, 0, 0), pyp = pym, pzm = pym, pzp = pym;
+        float pl = 0.0f, pr = 0.0f;
+        if constexpr (!Z) {
+            pym = sp4[(hq - RS) >> 2]; pyp = sp4[(hq + RS) >> 2]; pzm = sp4[(hq - PS) >> 2]; pzp = sp4[(hq + PS) >> 2];
+            pl = sp[hq - 1]; pr = sp[hq + 4];
+        }
+        const float wy0[4] = {wym.x, wym.y, wym.z, wym.w}, wy1[4] = {wyp.x, wyp.y, wyp.z, wyp.w};
+        const float wz0[4] = {wzm.x, wzm.y, wzm.z, wzm.w}, wz1[4] = {wzp.x, wzp.y, wzp.z, wzp.w};
+        const float py0[4] = {pym.x, pym.y, pym.z, pym.w}, py1[4] = {pyp.x, pyp.y, pyp.z, pyp.w};
+        const float pz0[4] = {pzm.x, pzm.y, pzm.z, pzm.w}, pz1[4] = {pzp.x, pzp.y, pzp.z, pzp.w};
+        Four<T> oq;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t t0 = (tq[q] >> (8 * e)) & 0xffu;
+            const float dd = lf(dq[q].v[e]), bb = lf(bq[q].v[e]);
+            const float wn[6] = {e == 0 ? wl : w0[q][e - 1], e == 3 ? wr : w0[q][e + 1], wy0[e], wy1[e], wz0[e], wz1[e]};
+            const float pn[6] = {e == 0 ? pl : p0[q][e - 1], e == 3 ? pr : p0[q][e + 1], py0[e], py1[e], pz0[e], pz1[e]};
+            float diag = dd, off = 0.0f;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+                diag = diag + wn[d];
+                off = off + pn[d];
             }
-            if constexpr (SEARCH) pq += (double)x0[e] * (double)res;
+            if (t0 != LIQUID) { diag = dd; off = 0.0f; }   // only a LIQUID cell couples
+            const bool act = (MODE == 2 || SEARCH) ? t0 == LIQUID : t0 != 0;
+            float res = 0.0f;
+            if (act) {
+                if constexpr (RW) {
+                    const float rw = dd;   // omega / diag
+                    res = rw > 0.0f ? (1.0f - OMEGA) * x0[q][e] + rw * (bb + off) : 0.0f;
+                } else if constexpr (MODE == 5) {
+                    res = diag > 0.0f ? __fdiv_rn(OMEGA, diag) : 0.0f;
+                } else {
+                    float ax = diag * x0[q][e] - off;
+                    if (MODE == 0 || MODE == 3) res = diag > 0.0f ? x0[q][e] + __fdiv_rn(OMEGA * (bb - ax), diag) : 0.0f;
+                    else if (MODE == 1) res = bb - ax;
+                    else res = ax;
+                }
+                if constexpr (SEARCH) pq += (double)x0[q][e] * (double)res;
+            }
+            oq.v[e] = sf<T>(res);
         }
-        oq.v[e] = sf<T>(res);
+        *reinterpret_cast<Four<T>*>(out + cell(q)) = oq;
     }
-    *reinterpret_cast<Four<T>*>(out + n) = oq;
     if constexpr (SEARCH) {
         double sum = block_sum_red(pq, lds);
         if (threadIdx.x == 0) S.partials[tile] = sum;
@@ -1570,6 +1593,9 @@ struct SparseSolver {
             if (N == 16 && MODE == 0) s.kt.begin(K_RBGS, st);
             if constexpr (N == 16 && MODE == 3 && !std::is_same<T, float>::value)
                 hipLaunchKernelGGL((k_jacobi0w<T>), dim3(s.nslots * 4), dim3(256), 0, st, out.f, b.f, L.rwf, L.actf, t.where, nfl);
+            else if constexpr (N == 16 && MODE == 0 && !std::is_same<T, float>::value)   // 8-plane tiles
+                hipLaunchKernelGGL((k_rowfine16<MODE, T, C, 2>), dim3(2 * s.nslots), dim3(256), 0, st, out.f, x.f, b.f, x.c, L, t,
+                                   S);
             else if constexpr (N == 16)
                 hipLaunchKernelGGL((k_rowfine16<MODE, T, C>), dim3(4 * s.nslots), dim3(256), 0, st, out.f, x.f, b.f, x.c, L, t, S);
             else

@@ -483,7 +483,8 @@ struct SearchArgs {
 // diag is not positive): in reduced precision the level-0 Jacobi sweep takes that weight
 // r per cell, out = (1 - w) x + r (b + off), and needs neither the diagonal nor the
 // neighbours' couplings, only their products with x (the weights round to the storage
-// type once per solve; the FP32 sweep keeps the reference's division).
+// type once per solve; the FP32 sweep keeps the reference's division). That sweep relaxes
+// the operator with diagonal w / r, and the reduced-precision residual uses the same.
 // Level 0, matrix-free and staged in LDS. A workgroup takes a quarter of a slot (4 z
 // planes of its 16^3 block), a thread 4 consecutive cells along x of one row. Level 0's
 // coupling of a LIQUID fine cell to a neighbour is 1 when the neighbour is a LIQUID fine
@@ -506,7 +507,8 @@ template <int MODE, class T, class C, int NQ = 1>
 __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
                                                    const T* __restrict__ xc, CompT<C> L, STopo t, SearchArgs S) {
     constexpr bool Z = MODE == 3 || MODE == 5, SEARCH = MODE == 4;
-    constexpr bool RW = MODE == 0 && !std::is_same<T, float>::value;   // the weighted sweep
+    // reduced precision: the sweep and the residual take the weight r = omega / diag
+    constexpr bool RW = (MODE == 0 || MODE == 1) && !std::is_same<T, float>::value;
     constexpr bool NW = !RW;                                               // the couplings staged
     constexpr int PL = 4 * NQ, HV = (PL + 2) * PS, TPS = 4 / NQ;          // planes, LDS, tiles a slot
     static_assert(NQ == 1 || NQ == 2, "a tile is 4 or 8 planes");
@@ -708,9 +710,12 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
             const bool act = (MODE == 2 || SEARCH) ? t0 == LIQUID : t0 != 0;
             float res = 0.0f;
             if (act) {
-                if constexpr (RW) {
+                if constexpr (RW && MODE == 0) {
                     const float rw = dd;   // omega / diag
                     res = rw > 0.0f ? (1.0f - OMEGA) * x0[q][e] + rw * (bb + off) : 0.0f;
+                } else if constexpr (RW) {   // the residual of the operator the weighted sweep relaxes
+                    const float rw = dd, dg = rw > 0.0f ? OMEGA / rw : 0.0f;
+                    res = bb - (dg * x0[q][e] - off);
                 } else if constexpr (MODE == 5) {
                     res = diag > 0.0f ? __fdiv_rn(OMEGA, diag) : 0.0f;
                 } else {
@@ -1593,7 +1598,7 @@ struct SparseSolver {
             if (N == 16 && MODE == 0) s.kt.begin(K_RBGS, st);
             if constexpr (N == 16 && MODE == 3 && !std::is_same<T, float>::value)
                 hipLaunchKernelGGL((k_jacobi0w<T>), dim3(s.nslots * 4), dim3(256), 0, st, out.f, b.f, L.rwf, L.actf, t.where, nfl);
-            else if constexpr (N == 16 && MODE == 0 && !std::is_same<T, float>::value)   // 8-plane tiles
+            else if constexpr (N == 16 && (MODE == 0 || MODE == 1) && !std::is_same<T, float>::value)   // 8-plane tiles
                 hipLaunchKernelGGL((k_rowfine16<MODE, T, C, 2>), dim3(2 * s.nslots), dim3(256), 0, st, out.f, x.f, b.f, x.c, L, t,
                                    S);
             else if constexpr (N == 16)

@@ -410,6 +410,7 @@ struct SparseBasin {
 
     // pressure (sflip.jl:1059-1904)
     SField<float> b, r, z, d, q, rold, tmp, dD;
+    DBuf<float> dgf;                          // level 0's full fine diagonal (dD and couplings), per solve
     SField<float> d2;                         // flexible CG's second direction buffer, on first use
     DBuf<float> cf[3], cc[3];                 // level-0 face couplings, fine pools and coarse faces
     DBuf<float> c1[3], dD1;                   // level 1 (8^3 a block)

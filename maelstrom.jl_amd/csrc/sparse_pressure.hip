@@ -462,6 +462,7 @@ template <class C> struct CompT {
     const uint8_t* cmask;   // k_covermask
     const uint8_t* tc;      // the coarse cell types (level 0's matrix-free couplings)
     const C* rwf = nullptr; // reduced precision, level 0: omega / diag per fine cell
+    const C* dgf = nullptr; // FP32, level 0: the full diagonal per fine cell (mode 6)
 };
 
 // Flexible CG's fused step (mode 4 of the level-0 rows, sflip.jl:1884-1886 with
@@ -506,10 +507,12 @@ constexpr int RS = 24, PS = 18 * RS;   // LDS rows of 24: cell li (-1..16) at 4 
 template <int MODE, class T, class C, int NQ = 1>
 __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
                                                    const T* __restrict__ xc, CompT<C> L, STopo t, SearchArgs S) {
-    constexpr bool Z = MODE == 3 || MODE == 5, SEARCH = MODE == 4;
+    constexpr bool Z = MODE == 3 || MODE == 5 || MODE == 6, SEARCH = MODE == 4;
     // reduced precision: the sweep and the residual take the weight r = omega / diag
     constexpr bool RW = (MODE == 0 || MODE == 1) && !std::is_same<T, float>::value;
-    constexpr bool NW = !RW;                                               // the couplings staged
+    // FP32: the rows take the full diagonal computed once a solve (mode 6), the same sum
+    constexpr bool FD = MODE <= 4 && std::is_same<T, float>::value;
+    constexpr bool NW = !RW && !FD;                                        // the couplings staged
     constexpr int PL = 4 * NQ, HV = (PL + 2) * PS, TPS = 4 / NQ;          // planes, LDS, tiles a slot
     static_assert(NQ == 1 || NQ == 2, "a tile is 4 or 8 planes");
     __shared__ float4 sw4[NW ? HV / 4 : 1];    // the couplings
@@ -558,8 +561,8 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
     for (int q = 0; q < NQ; ++q) {
         const uint32_t n = cell(q);
         tq[q] = *reinterpret_cast<const uint32_t*>(tf + n);
-        dq[q] = load4(RW ? L.rwf : L.dDf, n);   // the diagonal's Dirichlet part, or the weight
-        if constexpr (MODE == 2 || MODE == 4 || MODE == 5) {
+        dq[q] = load4(RW ? L.rwf : (FD ? L.dgf : L.dDf), n);   // the Dirichlet part, the weight or the diagonal
+        if constexpr (MODE == 2 || MODE == 4 || MODE == 5 || MODE == 6) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) bq[q].v[e] = sf<T>(0.0f);
         } else {
@@ -703,11 +706,11 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
             float diag = dd, off = 0.0f;
 #pragma unroll
             for (int d = 0; d < 6; ++d) {
-                diag = diag + wn[d];
+                if constexpr (NW) diag = diag + wn[d];   // (with FD, dd is the whole diagonal)
                 off = off + pn[d];
             }
             if (t0 != LIQUID) { diag = dd; off = 0.0f; }   // only a LIQUID cell couples
-            const bool act = (MODE == 2 || SEARCH) ? t0 == LIQUID : t0 != 0;
+            const bool act = MODE == 6 ? true : ((MODE == 2 || SEARCH) ? t0 == LIQUID : t0 != 0);
             float res = 0.0f;
             if (act) {
                 if constexpr (RW && MODE == 0) {
@@ -718,6 +721,8 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
                     res = bb - (dg * x0[q][e] - off);
                 } else if constexpr (MODE == 5) {
                     res = diag > 0.0f ? __fdiv_rn(OMEGA, diag) : 0.0f;
+                } else if constexpr (MODE == 6) {
+                    res = diag;
                 } else {
                     float ax = diag * x0[q][e] - off;
                     if (MODE == 0 || MODE == 3) res = diag > 0.0f ? x0[q][e] + __fdiv_rn(OMEGA * (bb - ax), diag) : 0.0f;
@@ -1478,6 +1483,12 @@ struct SparseSolver {
         hipLaunchKernelGGL(k_activecoarse, dim3(cdiv(Cc, 256)), dim3(256), 0, st, s.D[0].act.p, s.types.coarse.p, t);
         if (!s.cmask.p) s.cmask.alloc(Cc);
         hipLaunchKernelGGL(k_covermask, dim3(cdiv(Cc, 256)), dim3(256), 0, st, s.cmask.p, t);
+        if (!s.dgf.p) s.dgf.alloc((size_t)s.blocks * SBCELLS);
+        if (s.nslots) {   // level 0's full diagonal (mode 6 of the level-0 rows)
+            Hier<float, float> H = hier32();
+            hipLaunchKernelGGL((k_rowfine16<6, float, float>), dim3(4 * s.nslots), dim3(256), 0, st, s.dgf.p, (const float*)nullptr,
+                               (const float*)nullptr, (const float*)nullptr, H.L0, t, SearchArgs{});
+        }
         MAEL_LAUNCH_CHECK();
     }
     void hierarchy() {
@@ -1524,7 +1535,8 @@ struct SparseSolver {
             H.D.push_back(DViewT<float>{i3(L.n), {L.c[0].p, L.c[1].p, L.c[2].p}, L.dD.p, L.act.p});
         }
         H.L0 = CompT<float>{{{s.cf[0].p, s.cf[1].p, s.cf[2].p}}, s.dD.fine.p, s.types.fine.p,
-                            {{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p, s.cmask.p, s.types.coarse.p};
+                            {{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p, s.cmask.p, s.types.coarse.p,
+                            nullptr, s.dgf.p};
         H.L1 = CompT<float>{{{s.c1[0].p, s.c1[1].p, s.c1[2].p}}, s.dD1.p, s.act1.p,
                             {{s.cc[0].p, s.cc[1].p, s.cc[2].p}}, s.dD.coarse.p, s.D[0].act.p, s.cmask.p, s.types.coarse.p};
         return H;
@@ -1598,7 +1610,7 @@ struct SparseSolver {
             if (N == 16 && MODE == 0) s.kt.begin(K_RBGS, st);
             if constexpr (N == 16 && MODE == 3 && !std::is_same<T, float>::value)
                 hipLaunchKernelGGL((k_jacobi0w<T>), dim3(s.nslots * 4), dim3(256), 0, st, out.f, b.f, L.rwf, L.actf, t.where, nfl);
-            else if constexpr (N == 16 && (MODE == 0 || MODE == 1) && !std::is_same<T, float>::value)   // 8-plane tiles
+            else if constexpr (N == 16 && (std::is_same<T, float>::value || MODE == 0 || MODE == 1))   // 8-plane tiles
                 hipLaunchKernelGGL((k_rowfine16<MODE, T, C, 2>), dim3(2 * s.nslots), dim3(256), 0, st, out.f, x.f, b.f, x.c, L, t,
                                    S);
             else if constexpr (N == 16)
@@ -1615,7 +1627,7 @@ struct SparseSolver {
     // (with the reference's checks) into alpha
     void search(SVec q, SVec dn, SVec z, SVec d) {
         Hier<float, float> H = hier32();
-        int32_t fine_groups = s.nslots ? 4 * s.nslots : 1;
+        int32_t fine_groups = s.nslots ? 2 * s.nslots : 1;   // the level-0 rows' 8-plane tiles
         SearchArgs S{d.f, d.c, dn.f, dn.c, s.partials.p, fine_groups, s.scal.p};
         rows<16, 4>(SVecT<float>{q.f, q.c}, SVecT<float>{z.f, z.c}, SVecT<float>{z.f, z.c}, H.L0, S);
         hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(FINISH_THREADS), 0, st, s.partials.p, fine_groups + (int)cdiv(Cc, 256), s.scal.p,

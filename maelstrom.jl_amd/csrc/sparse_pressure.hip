@@ -53,7 +53,8 @@ __global__ void k_coarsecoef(float* __restrict__ c, const uint8_t* __restrict__ 
     int n0 = nc0 + (a == 0), n1 = nc1 + (a == 1), n2 = nc2 + (a == 2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     int Im = I - (a == 0), Jm = J - (a == 1), Km = K - (a == 2);
     bool ok = Im >= 0 && Jm >= 0 && Km >= 0 && I < nc0 && J < nc1 && K < nc2;
     c[l] = (ok && !covered(t, I, J, K) && !covered(t, Im, Jm, Km) && tc[cl(I, J, K, nc0, nc1)] == LIQUID &&
@@ -87,7 +88,8 @@ __global__ void k_coarsedirichlet(float* __restrict__ dD, const uint8_t* __restr
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     float d = 0.0f;
     if (tc[l] == LIQUID && !covered(t, I, J, K)) {
         float pc = phic[l];
@@ -115,7 +117,8 @@ __global__ void k_activecoarse(uint8_t* __restrict__ act, const uint8_t* __restr
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     act[l] = (tc[l] == LIQUID && !covered(t, I, J, K)) ? 1 : 0;
 }
 
@@ -145,7 +148,8 @@ __global__ void k_coarserhs(float* __restrict__ b, float* __restrict__ P, const 
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     if (tc[l] == LIQUID && !covered(t, I, J, K)) {
         float h = t.g.dx;
         float div = u[cl(I + 1, J, K, n0 + 1, n1)] - u[cl(I, J, K, n0 + 1, n1)] + v[cl(I, J + 1, K, n0, n1 + 1)] -
@@ -199,7 +203,8 @@ __global__ void k_coef2(float* __restrict__ c2, const float* __restrict__ c0c, c
     int n0 = t.g.nc(0) + (a == 0), n1 = t.g.nc(1) + (a == 1), n2 = t.g.nc(2) + (a == 2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     if (covered(t, I, J, K) || covered(t, I - (a == 0), J - (a == 1), K - (a == 2))) {
         float acc = 0.0f;
         for (int q = 0; q < 2; ++q)
@@ -220,7 +225,8 @@ __global__ void k_cells2(float* __restrict__ dD2, uint8_t* __restrict__ act, con
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     if (covered(t, I, J, K)) {
         float d = 0.0f;
         uint8_t ac = 0;
@@ -242,7 +248,8 @@ __global__ void k_coefdense(float* __restrict__ cc, const float* __restrict__ cf
     int m0 = nf.x + (a == 0), m1 = nf.y + (a == 1);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     float acc = 0.0f;
     for (int q = 0; q < 2; ++q)
         for (int p = 0; p < 2; ++p) {
@@ -257,7 +264,8 @@ __global__ void k_cellsdense(float* __restrict__ dDc, uint8_t* __restrict__ actc
                              const uint8_t* __restrict__ actf, int3 nc, int3 nf) {
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)nc.x * nc.y * nc.z) return;
-    int I = (int)(l % nc.x), J = (int)((l / nc.x) % nc.y), K = (int)(l / ((size_t)nc.x * nc.y));
+    int I, J, K;
+    unlin(l, nc.x, nc.y, I, J, K);
     float d = 0.0f;
     uint8_t ac = 0;
     for (int c = 0; c < 2; ++c)
@@ -342,7 +350,8 @@ __global__ void k_covermask(uint8_t* __restrict__ m, STopo t) {
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     unsigned v = covered(t, I, J, K) ? 1u : 0u;
     for (int d = 0; d < 6; ++d) {
         int ax = d >> 1, sg = (d & 1) ? 1 : -1;
@@ -843,7 +852,8 @@ __global__ __launch_bounds__(256) void k_rowcoarse(T* __restrict__ out, const T*
     const float beta = SEARCH ? (float)S.scal[SS_BETA] : 0.0f;
     double pq = 0.0;
     if (live) {
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     float res = 0.0f;
     unsigned msk = L.cmask[l];
     if constexpr (SEARCH) S.dnc[l] = lf(x[l]) + beta * S.doc[l];
@@ -912,7 +922,8 @@ __global__ __launch_bounds__(256) void k_rowdense(T* __restrict__ out, const T* 
                                                   DViewT<C> L) {
     size_t l = (size_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;   // neighbouring rows on one XCD
     if (l >= (size_t)L.n.x * L.n.y * L.n.z) return;
-    int i = (int)(l % L.n.x), j = (int)((l / L.n.x) % L.n.y), k = (int)(l / ((size_t)L.n.x * L.n.y));
+    int i, j, k;
+    unlin(l, L.n.x, L.n.y, i, j, k);
     float res = 0.0f;
     if (L.act[l] != 0) {
         float diag, off;
@@ -930,7 +941,8 @@ template <class T>
 __global__ void k_restrictdense(T* __restrict__ bc, const T* __restrict__ rf, int3 nc, int3 nf, float scale) {
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)nc.x * nc.y * nc.z) return;
-    int I = (int)(l % nc.x), J = (int)((l / nc.x) % nc.y), K = (int)(l / ((size_t)nc.x * nc.y));
+    int I, J, K;
+    unlin(l, nc.x, nc.y, I, J, K);
     float acc = 0.0f;
     for (int c = 0; c < 2; ++c)
         for (int b = 0; b < 2; ++b)
@@ -942,7 +954,8 @@ __global__ void k_prolongdense(T* __restrict__ xf, const T* __restrict__ xc, con
                                float boost) {
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)nf.x * nf.y * nf.z) return;
-    int i = (int)(l % nf.x), j = (int)((l / nf.x) % nf.y), k = (int)(l / ((size_t)nf.x * nf.y));
+    int i, j, k;
+    unlin(l, nf.x, nf.y, i, j, k);
     if (actf[l] != 0) xf[l] = sf<T>(lf(xf[l]) + boost * lf(xc[cl(i >> 1, j >> 1, k >> 1, nc.x, nc.y)]));
 }
 // sflip.jl:1623-1657 coarsest_kernel!: red-black Gauss-Seidel from zero in one workgroup,
@@ -1005,7 +1018,8 @@ __global__ void k_restrict12(T* __restrict__ b2, const T* __restrict__ r1f, cons
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     if (covered(t, I, J, K)) {
         float acc = 0.0f;
         for (int c = 0; c < 2; ++c)
@@ -1032,7 +1046,8 @@ __global__ void k_prolongcoarse(T* __restrict__ xc, const T* __restrict__ from, 
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     if (act[l] != 0 && !covered(t, I, J, K)) xc[l] = sf<T>(lf(xc[l]) + boost * lf(from[l]));
 }
 // Four cells of an x row a thread (two level-1 parents); a workgroup's 1024 cells lie in
@@ -1361,7 +1376,8 @@ __global__ __launch_bounds__(256) void k_resid64coarse(float* __restrict__ r, co
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     double res = 0.0;
     unsigned msk = L.cmask[l];
     if (L.actc[l] != 0 && !(msk & 1u)) {
@@ -1425,7 +1441,8 @@ __global__ void k_projectcoarse(float* __restrict__ u, uint8_t* __restrict__ val
     int n0 = nc0 + (a == 0), n1 = nc1 + (a == 1), n2 = nc2 + (a == 2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     int Im = I - (a == 0), Jm = J - (a == 1), Km = K - (a == 2);
     bool inside = Im >= 0 && Jm >= 0 && Km >= 0 && I < nc0 && J < nc1 && K < nc2;
     if (inside && !covered(t, I, J, K) && !covered(t, Im, Jm, Km)) {

@@ -129,7 +129,8 @@ __global__ void k_sadvect_coarsecells(float* __restrict__ out, const float* __re
     int n0 = f.t.g.nc(0), n1 = f.t.g.nc(1), n2 = f.t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     if (covered(f.t, I, J, K)) return;
     float p[3];
     coarsecentre(f.t.g, I, J, K, p);
@@ -141,7 +142,8 @@ __global__ void k_sadvect_coarsefaces(float* __restrict__ out, const float* __re
     int n0 = f.t.g.nc(0) + (a == 0), n1 = f.t.g.nc(1) + (a == 1), n2 = f.t.g.nc(2) + (a == 2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     float p[3];
     facepos(f.t.g, a, I, J, K, f.t.g.cdx(), p);
     backtrace(f, p[0], p[1], p[2], dt, f.t.g.cdx());
@@ -315,7 +317,8 @@ __global__ void k_fillcoarse(uint8_t* __restrict__ tc, float* __restrict__ phic,
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     if (!covered(t, I, J, K) && tc[l] == AIR && reach[l] == 0) {
         tc[l] = LIQUID;
         phic[l] = jmin(phic[l], -depth);
@@ -490,7 +493,8 @@ __global__ void k_sforcescoarse(float* __restrict__ u, SFlow old, StirParams P, 
     int n0 = g.nc(0) + (A == 0), n1 = g.nc(1) + (A == 1), n2 = g.nc(2) + (A == 2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     float p[3];
     facepos(g, A, I, J, K, g.cdx(), p);
     if (sissolid(g, p[0], p[1], p[2])) return;
@@ -640,7 +644,8 @@ __global__ void k_sextrapcoarse(float* __restrict__ out, uint8_t* __restrict__ f
         fout[l] = 1;
         return;
     }
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     float h = g.cdx(), p[3];
     facepos(g, a, I, J, K, h, p);
     float acc = 0.0f, acc2 = 0.0f;

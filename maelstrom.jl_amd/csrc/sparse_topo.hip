@@ -119,7 +119,8 @@ __global__ void k_marksurface(uint8_t* __restrict__ marks, const float* __restri
     int n0 = g.nc(0), n1 = g.nc(1), n2 = g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     float p[3];
     coarsecentre(g, I, J, K, p);
     if (sissolid(g, p[0], p[1], p[2])) return;
@@ -147,7 +148,8 @@ __global__ void k_markflow(uint8_t* __restrict__ marks, const float* __restrict_
     int n0 = g.nc(0), n1 = g.nc(1), n2 = g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     float p[3];
     coarsecentre(g, I, J, K, p);
     if (!(phic[l] < 0.0f) || sissolid(g, p[0], p[1], p[2])) return;
@@ -370,7 +372,8 @@ template <class V> __global__ void k_restrictcells(V* __restrict__ coarse, const
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     // the 64 fine cells lie in one block, the one covering the coarse cell
     const int bi = I >> 2, bj = J >> 2, bk = K >> 2;
     const int32_t sb = inblocks(t.g, bi, bj, bk) ? t.slot[blockid(t.g, bi, bj, bk)] : 0;
@@ -386,7 +389,8 @@ template <class V> __global__ void k_restrictfaces(V* __restrict__ coarse, const
     int n0 = t.g.nc(0) + (a == 0), n1 = t.g.nc(1) + (a == 1), n2 = t.g.nc(2) + (a == 2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     // The 16 fine faces lie in one block's plane: finefaceat gives them all from the block
     // holding fine face (4I, 4J, 4K), or all from the top plane of the block below along a,
     // or none.
@@ -520,7 +524,8 @@ __global__ void k_classifycoarse(uint8_t* __restrict__ types, const float* __res
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     if (covered(t, I, J, K)) {
         bool air = false, liquid = false;
         for (int c = 0; c < 4; ++c)
@@ -562,7 +567,8 @@ __global__ void k_solidcoarse(float* __restrict__ u, SGrid g, int a) {
     int n0 = nc0 + (a == 0), n1 = nc1 + (a == 1), n2 = nc2 + (a == 2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     bool outside = (a == 0 && (I == 0 || I == nc0)) || (a == 1 && (J == 0 || J == nc1)) || (a == 2 && (K == 0 || K == nc2));
     float p[3];
     float h = g.cdx();
@@ -594,7 +600,8 @@ __global__ void k_gravitycoarse(float* __restrict__ w, SGrid g, float dv) {
     int n0 = g.nc(0), n1 = g.nc(1), n2 = g.nc(2) + 1;
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     float p[3];
     facepos(g, 2, I, J, K, g.cdx(), p);
     if (!sissolid(g, p[0], p[1], p[2])) w[l] += dv;
@@ -691,7 +698,8 @@ __global__ void k_coarsewall(float* __restrict__ out, const float* __restrict__ 
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
-    int I = (int)(l % n0), J = (int)((l / n0) % n1), K = (int)(l / ((size_t)n0 * n1));
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
     float c = phic[l], p[3];
     coarsecentre(t.g, I, J, K, p);
     if (!covered(t, I, J, K) && sissolid(t.g, p[0], p[1], p[2])) {

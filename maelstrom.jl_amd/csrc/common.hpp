@@ -220,6 +220,15 @@ __device__ inline uint32_t xcd_tile(uint32_t b, uint32_t G) {
     return x * q + min(x, r) + i;
 }
 
+// (I, J, K) of linear index l of an n0 x n1 x n2 array (x fastest), in 32-bit arithmetic
+// (every grid here has fewer than 2^31 entries).
+__device__ inline void unlin(size_t l, int n0, int n1, int& I, int& J, int& K) {
+    const uint32_t l32 = (uint32_t)l, q = l32 / (uint32_t)n0;
+    I = (int)(l32 - q * (uint32_t)n0);
+    K = (int)(q / (uint32_t)n1);
+    J = (int)(q - (uint32_t)K * (uint32_t)n1);
+}
+
 // ─── fixed-point accumulation (deterministic P2G) ────────────────────────────
 // A contribution c is accumulated as round(c * 2^32) in a 64-bit integer: integer
 // addition is associative, so the sum does not depend on the order the particles

@@ -32,7 +32,8 @@ struct DenseCells {   // what the scan needs to know about dense cell l
     int seeding;
     __device__ void get(size_t l, int32_t& count, int32_t& total) const {
         int n1 = g.n[0], n2 = g.n[1];
-        int i = (int)(l % (size_t)n1), j = (int)((l / (size_t)n1) % (size_t)n2), k = (int)(l / ((size_t)n1 * n2));
+        int i, j, k;
+        unlin(l, n1, n2, i, j, k);
         count = counts[l];
         int32_t kept, add;
         cellcounts(count, phi[l], issolidcell(g, i, j, k), nmin, deep, g.dx, seeding != 0, kept, add);
@@ -118,7 +119,8 @@ __global__ void k_dense_spawn(PView dst, const int32_t* __restrict__ counts, con
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= C) return;
     int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
-    int i = (int)(l % (size_t)n1), j = (int)((l / (size_t)n1) % (size_t)n2), k = (int)(l / ((size_t)n1 * n2));
+    int i, j, k;
+    unlin(l, n1, n2, i, j, k);
     int32_t kept, add;
     cellcounts(counts[l], phi[l], issolidcell(g, i, j, k), nmin, deep, g.dx, seeding != 0, kept, add);
     if (add <= 0) return;

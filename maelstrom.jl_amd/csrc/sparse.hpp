@@ -44,15 +44,6 @@ __host__ __device__ inline size_t cl(int i, int j, int k, int n0, int n1) {
     return (size_t)i + (size_t)n0 * ((size_t)j + (size_t)n1 * k);
 }
 
-// (I, J, K) of linear index l of an n0 x n1 x n2 array (x fastest), in 32-bit arithmetic
-// (every grid here has fewer than 2^31 entries).
-__device__ inline void unlin(size_t l, int n0, int n1, int& I, int& J, int& K) {
-    const uint32_t l32 = (uint32_t)l, q = l32 / (uint32_t)n0;
-    I = (int)(l32 - q * (uint32_t)n0);
-    K = (int)(q / (uint32_t)n1);
-    J = (int)(q - (uint32_t)K * (uint32_t)n1);
-}
-
 __device__ inline bool inblocks(const SGrid& g, int bi, int bj, int bk) {
     return bi >= 0 && bi < g.nb[0] && bj >= 0 && bj < g.nb[1] && bk >= 0 && bk < g.nb[2];
 }

@@ -455,7 +455,8 @@ __global__ void k_initcoarse(float* __restrict__ phic, float* __restrict__ uc, f
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t F = (size_t)(n0 + 1) * (n1 + 1) * (n2 + 1);
     if (l >= F) return;
-    int I = (int)(l % (n0 + 1)), J = (int)((l / (n0 + 1)) % (n1 + 1)), K = (int)(l / ((size_t)(n0 + 1) * (n1 + 1)));
+    int I, J, K;
+    unlin(l, n0 + 1, n1 + 1, I, J, K);
     float p[3];
     if (I < n0 && J < n1 && K < n2) {
         coarsecentre(g, I, J, K, p);
@@ -778,7 +779,8 @@ __global__ __launch_bounds__(256) void k_svolume(double* __restrict__ partials, 
         } else {
             size_t m = l - nf;
             int n0 = t.g.nc(0), n1 = t.g.nc(1);
-            int I = (int)(m % n0), J = (int)((m / n0) % n1), K = (int)(m / ((size_t)n0 * n1));
+            int I, J, K;
+            unlin(m, n0, n1, I, J, K);
             coarsecentre(t.g, I, J, K, p);
             if (covered(t, I, J, K) || sissolid(t.g, p[0], p[1], p[2])) continue;
             v += 64.0 * (double)jclamp(0.5f - __fdiv_rn(phic[m], t.g.cdx()), 0.0f, 1.0f);

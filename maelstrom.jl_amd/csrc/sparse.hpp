@@ -132,14 +132,22 @@ __device__ inline bool tapslots(const STopo& t, int I, int J, int K, int32_t s[8
     const bool inz[2] = {bz >= 0 && bz < g.nb[2], bz + sz >= 0 && bz + sz < g.nb[2]};
     const int nx = g.nb[0], nxy = g.nb[0] * g.nb[1];
     const int id0 = bx + nx * by + nxy * bz;
+    // one read per distinct block: a tap past no block side shares the slot of the tap
+    // with its offsets along the unstraddled axes dropped
+    auto look = [&](int dx, int dy, int dz) -> int32_t {
+        return inx[dx] && iny[dy] && inz[dz] ? t.slot[id0 + dx + nx * dy + nxy * dz] : 0;
+    };
+    s[0] = look(0, 0, 0);
+    s[1] = sx ? look(1, 0, 0) : s[0];
+    s[2] = sy ? look(0, 1, 0) : s[0];
+    s[3] = sx && sy ? look(1, 1, 0) : (sx ? s[1] : s[2]);
+    s[4] = sz ? look(0, 0, 1) : s[0];
+    s[5] = sz ? (sx ? look(1, 0, 1) : s[4]) : s[1];
+    s[6] = sz ? (sy ? look(0, 1, 1) : s[4]) : s[2];
+    s[7] = sz ? (sx && sy ? look(1, 1, 1) : (sx ? s[5] : s[6])) : s[3];
     bool all = true;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        const int dx = m & 1, dy = (m >> 1) & 1, dz = m >> 2;
-        const bool in = inx[dx] && iny[dy] && inz[dz];
-        s[m] = in ? t.slot[id0 + dx * sx + nx * (dy * sy) + nxy * (dz * sz)] : 0;
-        all = all && s[m] != 0;
-    }
+    for (int m = 0; m < 8; ++m) all = all && s[m] != 0;
     return all;
 }
 

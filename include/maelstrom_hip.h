@@ -348,8 +348,10 @@ enum {
     MAEL_K_SPAWN, MAEL_K_SEARCH, MAEL_K_ADVANCE, MAEL_K_RBGS, MAEL_K_RESIDUAL, MAEL_K_RESTRICT, MAEL_K_PROLONG,
     MAEL_K_COARSEST, MAEL_K_RESID64, MAEL_K_COUNT
 };
-/* Record HIP events around each of the next (up to `launches`) launches of `kernel`
- * (fine-level launches only for the multigrid kernels), on the basin's stream. */
+/* Record HIP events around every 8th of the next launches of `kernel` (fine-level
+ * launches only for the multigrid kernels; up to `launches` timed ones), on the basin's
+ * stream: an event pair leaves the stream idle for microseconds, so timing every launch
+ * would slow the run it measures. `*_ktime` returns their total ms and count. */
 int mael_dense_ktimer(mael_dense *d, int32_t kernel, int32_t launches);
 /* Total milliseconds and number of the recorded launches; stops recording. */
 int mael_dense_ktime(mael_dense *d, double *ms, int32_t *launches);

@@ -45,16 +45,28 @@ struct SolveStats {
 
 struct Stream {
     hipStream_t s = nullptr;
+    hipEvent_t rd = nullptr;
     void* pinned = nullptr;   // small pinned host scratch for scalars read back
     Stream() {
         MAEL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        MAEL_HIP(hipEventCreateWithFlags(&rd, hipEventDisableTiming));
         MAEL_HIP(hipHostMalloc(&pinned, 4096, hipHostMallocDefault));
     }
     ~Stream() {
+        if (rd) (void)hipEventDestroy(rd);
         if (s) (void)hipStreamDestroy(s);
         if (pinned) (void)hipHostFree(pinned);
     }
     void sync() { MAEL_HIP(hipStreamSynchronize(s)); }
+    // The same wait by polling an event: the Krylov loops read a scalar back once an
+    // iteration, on the critical path, and a blocking wait wakes the host later.
+    void spin() {
+        MAEL_HIP(hipEventRecord(rd, s));
+        hipError_t e;
+        while ((e = hipEventQuery(rd)) == hipErrorNotReady) {
+        }
+        MAEL_HIP(e);
+    }
 };
 
 // Kernel ids for the launch timer (mirrored as MAEL_K_* in include/maelstrom_hip.h).
@@ -65,10 +77,15 @@ enum KernelId {
 
 // HIP events around every launch of one chosen kernel, on the stream it runs on, so a
 // benchmark can read that kernel's launch durations over a timed region.
+// Times every STRIDE-th launch of the target kernel: each recorded event leaves the
+// stream idle for a few microseconds, so an event pair around every launch would slow
+// the run it measures.
 struct KTimer {
+    static constexpr size_t STRIDE = 8;
     int target = -1;
     std::vector<hipEvent_t> ev;
-    size_t used = 0;
+    size_t used = 0, seen = 0;
+    bool open = false;
     ~KTimer() { for (auto e : ev) (void)hipEventDestroy(e); }
     void enable(int id, size_t launches) {
         while (ev.size() < 2 * launches) {
@@ -78,13 +95,23 @@ struct KTimer {
         }
         target = id;
         used = 0;
+        seen = 0;
+        open = false;
     }
     bool on(int id) const { return id == target && used + 2 <= ev.size(); }
-    void begin(int id, hipStream_t s) { if (on(id)) MAEL_HIP(hipEventRecord(ev[used], s)); }
-    void end(int id, hipStream_t s) {
-        if (on(id)) { MAEL_HIP(hipEventRecord(ev[used + 1], s)); used += 2; }
+    void begin(int id, hipStream_t s) {
+        if (!on(id)) return;
+        open = seen++ % STRIDE == 0;
+        if (open) MAEL_HIP(hipEventRecord(ev[used], s));
     }
-    double read(int* launches) {   // total ms of the recorded launches; disables the timer
+    void end(int id, hipStream_t s) {
+        if (on(id) && open) {
+            MAEL_HIP(hipEventRecord(ev[used + 1], s));
+            used += 2;
+            open = false;
+        }
+    }
+    double read(int* launches) {   // total ms of the timed launches; disables the timer
         double t = 0;
         for (size_t i = 0; i < used; i += 2) {
             MAEL_HIP(hipEventSynchronize(ev[i + 1]));

@@ -459,7 +459,7 @@ struct DenseSolver {
     }
     double read(int slot) {
         MAEL_HIP(hipMemcpyAsync(s.st.pinned, s.scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, st));
-        MAEL_HIP(hipStreamSynchronize(st));
+        s.st.spin();
         return *(double*)s.st.pinned;
     }
     void dot(const float* a, const float* b, int op, int slot = 0) {

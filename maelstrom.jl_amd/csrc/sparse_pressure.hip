@@ -510,12 +510,28 @@ template <class T> struct alignas(4 * sizeof(T)) Four { T v[4]; };
 template <class T> __device__ inline Four<T> load4(const T* __restrict__ p, uint32_t i) {
     return *reinterpret_cast<const Four<T>*>(p + i);
 }
+// A 4-cell store marked non-temporal: a sweep's output is read by the next launch, not by
+// this one, so it need not stay in L2 (where the end of the launch writes back what is
+// dirty before the next can start).
+template <class T> __device__ inline void store4_nt(T* __restrict__ p, uint32_t i, const Four<T>& v) {
+    if constexpr (sizeof(T) == 2) {
+        uint64_t w;
+        __builtin_memcpy(&w, &v, 8);
+        __builtin_nontemporal_store(w, reinterpret_cast<uint64_t*>(p + i));
+    } else {
+        typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+        U4 w;
+        __builtin_memcpy(&w, &v, 16);
+        __builtin_nontemporal_store(w, reinterpret_cast<U4*>(p + i));
+    }
+}
 constexpr int RS = 24, PS = 18 * RS;   // LDS rows of 24: cell li (-1..16) at 4 + li; planes of 18 rows
 // NQ: quads a thread (a workgroup takes 4 NQ planes of its slot, its 16 / (4 NQ) tiles
 // cover the slot); more cells a workgroup keep more loads in flight.
 template <int MODE, class T, class C, int NQ = 1>
-__global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
-                                                   const T* __restrict__ xc, CompT<C> L, STopo t, SearchArgs S) {
+__device__ __forceinline__ void rowfine16_wg(const uint32_t tile, T* __restrict__ out, const T* __restrict__ x,
+                                             const T* __restrict__ b, const T* __restrict__ xc, const CompT<C>& L,
+                                             const STopo& t, const SearchArgs& S) {
     constexpr bool Z = MODE == 3 || MODE == 5 || MODE == 6, SEARCH = MODE == 4;
     // reduced precision: the sweep and the residual take the weight r = omega / diag
     constexpr bool RW = (MODE == 0 || MODE == 1) && !std::is_same<T, float>::value;
@@ -529,7 +545,6 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
     __shared__ double lds[4];
     float* sw = reinterpret_cast<float*>(sw4);
     float* sp = reinterpret_cast<float*>(sp4);
-    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int s = (int)(tile / TPS) + 1, k0 = (int)(tile % TPS) * PL;
     const uint32_t cb = (uint32_t)(s - 1) * 4096u;
     const uint8_t* __restrict__ tf = L.actf;   // level 0: the fine cell types
@@ -742,12 +757,17 @@ __global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T*
             }
             oq.v[e] = sf<T>(res);
         }
-        *reinterpret_cast<Four<T>*>(out + cell(q)) = oq;
+        store4_nt(out, cell(q), oq);
     }
     if constexpr (SEARCH) {
         double sum = block_sum_red(pq, lds);
         if (threadIdx.x == 0) S.partials[tile] = sum;
     }
+}
+template <int MODE, class T, class C, int NQ = 1>
+__global__ __launch_bounds__(256) void k_rowfine16(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
+                                                   const T* __restrict__ xc, CompT<C> L, STopo t, SearchArgs S) {
+    rowfine16_wg<MODE, T, C, NQ>(xcd_tile(blockIdx.x, gridDim.x), out, x, b, xc, L, t, S);
 }
 
 // The reduced-precision level-0 Jacobi sweep from x = 0: out = r b with r = omega / diag
@@ -777,12 +797,13 @@ __global__ __launch_bounds__(256) void k_jacobi0w(T* __restrict__ out, const T* 
 // face layers (face neighbours' slots, else the coarse cells, clamped), then each thread
 // computes 2 cells with finerow's couplings and order.
 template <int MODE, class T, class C>
-__global__ __launch_bounds__(256) void k_rowfine8(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
-                                                  const T* __restrict__ xc, CompT<C> L, STopo t) {
+__device__ __forceinline__ void rowfine8_wg(const uint32_t tile, T* __restrict__ out, const T* __restrict__ x,
+                                            const T* __restrict__ b, const T* __restrict__ xc, const CompT<C>& L,
+                                            const STopo& t) {
     constexpr bool Z = MODE == 3;
     constexpr int H8 = 10;
     __shared__ T sx[Z ? 1 : H8 * H8 * H8];
-    const int s = (int)xcd_tile(blockIdx.x, gridDim.x) + 1;
+    const int s = (int)tile + 1;
     const uint32_t cb = (uint32_t)(s - 1) * 512u, fb = (uint32_t)(s - 1) * 576u;
     int4 w = t.where[s - 1];
     if (w.x < 0) {
@@ -839,14 +860,21 @@ __global__ __launch_bounds__(256) void k_rowfine8(T* __restrict__ out, const T* 
         out[n] = sf<T>(res);
     }
 }
+template <int MODE, class T, class C>
+__global__ __launch_bounds__(256) void k_rowfine8(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
+                                                  const T* __restrict__ xc, CompT<C> L, STopo t) {
+    rowfine8_wg<MODE, T, C>(xcd_tile(blockIdx.x, gridDim.x), out, x, b, xc, L, t);
+}
 
+// Coarse cells 256 cb .. 256 cb + 255; the search's partial sum goes to slot `ps`.
 template <int N, int MODE, class T, class C>
-__global__ __launch_bounds__(256) void k_rowcoarse(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
-                                                   const T* __restrict__ xf, CompT<C> L, STopo t, SearchArgs S) {
+__device__ __forceinline__ void rowcoarse_wg(const uint32_t cb, const uint32_t ps, T* __restrict__ out,
+                                             const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ xf,
+                                             const CompT<C>& L, const STopo& t, const SearchArgs& S) {
     constexpr bool SEARCH = MODE == 4;
     __shared__ double lds[4];
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
-    size_t l = (size_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;   // neighbouring rows on one XCD
+    size_t l = (size_t)cb * 256u + threadIdx.x;
     bool live = l < (size_t)n0 * n1 * n2;
     if (!SEARCH && !live) return;
     const float beta = SEARCH ? (float)S.scal[SS_BETA] : 0.0f;
@@ -879,8 +907,31 @@ __global__ __launch_bounds__(256) void k_rowcoarse(T* __restrict__ out, const T*
     }
     if constexpr (SEARCH) {
         double sum = block_sum_red(pq, lds);
-        if (threadIdx.x == 0) S.partials[S.coarse0 + blockIdx.x] = sum;
+        if (threadIdx.x == 0) S.partials[S.coarse0 + ps] = sum;
     }
+}
+template <int N, int MODE, class T, class C>
+__global__ __launch_bounds__(256) void k_rowcoarse(T* __restrict__ out, const T* __restrict__ x, const T* __restrict__ b,
+                                                   const T* __restrict__ xf, CompT<C> L, STopo t, SearchArgs S) {
+    // neighbouring rows on one XCD
+    rowcoarse_wg<N, MODE, T, C>(xcd_tile(blockIdx.x, gridDim.x), blockIdx.x, out, x, b, xf, L, t, S);
+}
+
+// One launch for a whole composite-level pass: the fine part's workgroups (tiles of
+// level 0, slots of level 1) and the coarse part's, interleaved in proportion along the
+// XCD-ordered workgroup index, so the coarse rows (instruction-bound next to blocks)
+// overlap the fine tiles (bandwidth-bound) and the pass drains once. Workgroup g takes
+// coarse block floor(g C / G) when that count steps at g, else fine workgroup g minus
+// the coarse blocks before it: each index of either part is taken exactly once, in order.
+template <int N, int MODE, class T, class C, int NQ>
+__global__ __launch_bounds__(256) void k_rowcomp(T* __restrict__ outf, T* __restrict__ outc, const T* __restrict__ xf,
+                                                 const T* __restrict__ xc, const T* __restrict__ bf,
+                                                 const T* __restrict__ bc, CompT<C> L, STopo t, SearchArgs S, uint32_t ncb) {
+    const uint32_t G = gridDim.x, g = xcd_tile(blockIdx.x, G);
+    const uint32_t c0 = (uint32_t)(((uint64_t)g * ncb) / G), c1 = (uint32_t)(((uint64_t)(g + 1) * ncb) / G);
+    if (c1 > c0) rowcoarse_wg<N, MODE, T, C>(c0, c0, outc, xc, bc, xf, L, t, S);
+    else if constexpr (N == 16) rowfine16_wg<MODE, T, C, NQ>(g - c0, outf, xf, bf, xc, L, t, S);
+    else rowfine8_wg<MODE, T, C>(g - c0, outf, xf, bf, xc, L, t);
 }
 
 // ─── dense levels (sflip.jl:1574-1657) ───────────────────────────────────────
@@ -1623,6 +1674,21 @@ struct SparseSolver {
     template <int N, int MODE, class T, class C>
     void rows(SVecT<T> out, SVecT<T> x, SVecT<T> b, const CompT<C>& L, SearchArgs S = SearchArgs{}) {
         int32_t nfl = s.nslots * N * N * N;
+        // one launch for both parts, except the search (its coarse rows need far more
+        // registers than the tiles) and the reduced-precision sweep from zero (pointwise)
+        constexpr bool FUSE = MODE != 4 && !(N == 16 && MODE == 3 && !std::is_same<T, float>::value);
+        if constexpr (FUSE) {
+            if (nfl) {
+                constexpr int NQ = (N == 16 && (std::is_same<T, float>::value || MODE == 0 || MODE == 1)) ? 2 : 1;
+                const uint32_t nfw = N == 16 ? (uint32_t)s.nslots * (uint32_t)(4 / NQ) : (uint32_t)s.nslots;
+                const uint32_t ncb = (uint32_t)cdiv(Cc, 256);
+                if (N == 16 && MODE == 0) s.kt.begin(K_RBGS, st);
+                hipLaunchKernelGGL((k_rowcomp<N, MODE, T, C, NQ>), dim3(nfw + ncb), dim3(256), 0, st, out.f, out.c, x.f, x.c,
+                                   b.f, b.c, L, t, S, ncb);
+                if (N == 16 && MODE == 0) s.kt.end(K_RBGS, st);
+                return;
+            }
+        }
         if (nfl) {
             if (N == 16 && MODE == 0) s.kt.begin(K_RBGS, st);
             if constexpr (N == 16 && MODE == 3 && !std::is_same<T, float>::value)
@@ -1785,12 +1851,12 @@ struct SparseSolver {
     }
     double read(int slot) {
         MAEL_HIP(hipMemcpyAsync(s.st.pinned, s.scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, st));
-        MAEL_HIP(hipStreamSynchronize(st));
+        s.st.spin();
         return *(double*)s.st.pinned;
     }
     void read2(int slot, double& a, double& b) {   // scal[slot] and the error flags, one copy
         MAEL_HIP(hipMemcpyAsync(s.st.pinned, s.scal.p, 16 * sizeof(double), hipMemcpyDeviceToHost, st));
-        MAEL_HIP(hipStreamSynchronize(st));
+        s.st.spin();
         a = ((double*)s.st.pinned)[slot];
         b = ((double*)s.st.pinned)[SS_FLAG];
     }

@@ -61,7 +61,11 @@ struct Stream {
     // The same wait by polling an event: the Krylov loops read a scalar back once an
     // iteration, on the critical path, and a blocking wait wakes the host later.
     void spin() {
-        MAEL_HIP(hipEventRecord(rd, s));
+        mark();
+        poll();
+    }
+    void mark() { MAEL_HIP(hipEventRecord(rd, s)); }   // what poll() waits for
+    void poll() {
         hipError_t e;
         while ((e = hipEventQuery(rd)) == hipErrorNotReady) {
         }

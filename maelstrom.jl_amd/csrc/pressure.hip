@@ -425,8 +425,21 @@ struct DenseSolver {
         hipLaunchKernelGGL(k_rbgs, grid3(n.x / 2, n.y, n.z), dim3(BX, BY), 0, st, x, b, types(l), diag(l), n, colour);
         if (l == 0) s.kt.end(K_RBGS, st);
     }
+    // The V-cycle's first launches (level 0's start and first sweep) queued ahead of a
+    // host read that decides whether the V-cycle runs, so the device works through the
+    // read's round trip; the next vcycle() skips them. Whoever leaves the solve or changes
+    // r first clears `head`.
+    bool head = false;
+    void queue_head(float* z, const float* r, int sweeps) {
+        int3 n = dims(0);
+        hipLaunchKernelGGL(k_start, grid3(n.x, n.y, n.z), dim3(BX, BY), 0, st, z, r, types(0), diag(0), n);
+        if (2 * sweeps - 1 >= 1) smooth(0, z, r, 1);
+        head = true;
+    }
     // flip.jl:986-1014 vcycle!.
     void vcycle(float* z, const float* r, int sweeps, int coarse) {
+        const bool queued = head;
+        head = false;
         std::vector<float*> xs(L), bs(L), rs(L);
         for (int l = 0; l < L; ++l) {
             xs[l] = l == 0 ? z : s.lv[l].x.p;
@@ -435,8 +448,10 @@ struct DenseSolver {
         }
         for (int l = 0; l < L - 1; ++l) {
             int3 n = dims(l), nc = dims(l + 1);
-            hipLaunchKernelGGL(k_start, grid3(n.x, n.y, n.z), dim3(BX, BY), 0, st, xs[l], bs[l], types(l), diag(l), n);
-            for (int c = 0; c < 2 * sweeps - 1; ++c) smooth(l, xs[l], bs[l], (c & 1) ? 0 : 1);   // 1,0,1,...
+            const bool skip = queued && l == 0;   // start and first sweep already queued
+            if (!skip)
+                hipLaunchKernelGGL(k_start, grid3(n.x, n.y, n.z), dim3(BX, BY), 0, st, xs[l], bs[l], types(l), diag(l), n);
+            for (int c = skip ? 1 : 0; c < 2 * sweeps - 1; ++c) smooth(l, xs[l], bs[l], (c & 1) ? 0 : 1);   // 1,0,1,...
             residual(l, rs[l], xs[l], bs[l]);
             if (l == 0) s.kt.begin(K_RESTRICT, st);
             hipLaunchKernelGGL(k_restrict, grid3(nc.x, nc.y, nc.z), dim3(BX, BY), 0, st, bs[l + 1], rs[l], types(l),
@@ -458,8 +473,16 @@ struct DenseSolver {
         MAEL_LAUNCH_CHECK();
     }
     double read(int slot) {
+        post_read(slot);
+        return wait_read();
+    }
+    // the same in two halves, with other work queued between them
+    void post_read(int slot) {
         MAEL_HIP(hipMemcpyAsync(s.st.pinned, s.scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, st));
-        s.st.spin();
+        s.st.mark();
+    }
+    double wait_read() {
+        s.st.poll();
         return *(double*)s.st.pinned;
     }
     void dot(const float* a, const float* b, int op, int slot = 0) {
@@ -592,9 +615,14 @@ void DenseBasin::solve_pressure(const SolveOpts& o, SolveStats& stats) {
             kt.end(K_ADVANCE, s);
             hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, partials.p, RED_BLOCKS, scal.p, F_RR, 0);
             MAEL_LAUNCH_CHECK();
-            rrec = S.read(S_RNORM);
+            S.post_read(S_RNORM);
+            if (it < o.maxiter) S.queue_head(z.p, r.p, o.sweeps);   // the next V-cycle's head, during the read
+            rrec = S.wait_read();
             if (!std::isfinite(rrec)) throw Error(E_NONFINITE, "Nonfinite pressure residual");
-            if (rrec <= target) break;
+            if (rrec <= target) {
+                S.head = false;
+                break;
+            }
             S.vcycle(z.p, r.p, o.sweeps, o.coarse);
             S.dot(z.p, r.p, F_BETA);
         }

@@ -1726,18 +1726,18 @@ struct SparseSolver {
     // The same from x = 0: the first sweep reads no x, and x needs no clearing (every cell
     // of the level's slots and coarse part is written).
     template <int N, class T, class C>
-    void smooth0(SVecT<T> x, SVecT<T> b, SVecT<T> tmp, const CompT<C>& L, int sweeps, size_t nfine) {
+    void smooth0(SVecT<T> x, SVecT<T> b, SVecT<T> tmp, const CompT<C>& L, int sweeps, size_t nfine, bool queued = false) {
         if (sweeps < 1) { zero_t(x, nfine); return; }
-        rows<N, 3>(tmp, x, b, L);
+        if (!queued) rows<N, 3>(tmp, x, b, L);   // else already queued (queue_head)
         rows<N, 0>(x, tmp, b, L);
         smooth<N>(x, b, tmp, L, sweeps - 1);
     }
 
     // sflip.jl:1722-1762 vcycle!: z = M^-1 r in the hierarchy H (coarsest in arithmetic A).
     template <class T, class C, class A>
-    void vcycle_t(Hier<T, C>& H, SVecT<T> z, SVecT<T> r, int sweeps, int coarse, float boost) {
+    void vcycle_t(Hier<T, C>& H, SVecT<T> z, SVecT<T> r, int sweeps, int coarse, float boost, bool queued = false) {
         int ns = s.nslots;
-        smooth0<16>(z, r, H.tmp, H.L0, sweeps, (size_t)nf);
+        smooth0<16>(z, r, H.tmp, H.L0, sweeps, (size_t)nf, queued);
         rows<16, 1>(H.q, z, r, H.L0);
         if (ns) hipLaunchKernelGGL(k_restrict01<T>, dim3(cdiv(ns * SBCELLS1, 256)), dim3(256), 0, st, H.X1b.f, H.q.f, t, ns);
         MAEL_HIP(hipMemcpyAsync(H.X1b.c, H.q.c, Cc * sizeof(T), hipMemcpyDeviceToDevice, st));
@@ -1793,26 +1793,58 @@ struct SparseSolver {
     // norm, so its entries are at most 1) and z back on the way out. With `rold`
     // (flexible CG), z.r and z.rold are also summed into the partials as k_sdot2 does
     // (fused with the conversion back in reduced precision).
-    void vcycle(SVec z, SVec r, int sweeps, int coarse, float boost, int nslot = -1, const SVec* rold = nullptr) {
+    // The V-cycle's first launches (the reduced-precision copy of r and the level-0 sweep
+    // from zero) queued ahead of a host read that decides whether the V-cycle runs at
+    // all, so the device works through the read's round trip; the next vcycle() on the
+    // same z and r skips them. Whoever changes r (or leaves the solve) before that clears
+    // `head`, and the V-cycle then runs whole (overwriting what the queued launches wrote,
+    // all of it V-cycle scratch).
+    bool head = false;
+    void queue_head(SVec z, SVec r, int sweeps, int nslot) {
+        if (sweeps < 1) return;
         if (precision == 0) {
             Hier<float, float> H = hier32();
-            vcycle_t<float, float, float>(H, SVecT<float>{z.f, z.c}, SVecT<float>{r.f, r.c}, sweeps, coarse, boost);
-            if (rold)
-                hipLaunchKernelGGL(k_sdot2, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, z.f, z.c, r.f, r.c, rold->f,
-                                   rold->c, s.types.fine.p, s.types.coarse.p, s.cmask.p, t, nf);
-            return;
+            rows<16, 3>(H.tmp, SVecT<float>{z.f, z.c}, SVecT<float>{r.f, r.c}, H.L0);
+        } else if (precision == 1) {
+            queue_head_mp<__half>(r, nslot);
+        } else {
+            queue_head_mp<bf16>(r, nslot);
         }
-        if (precision == 1) vcycle_mp<__half>(z, r, sweeps, coarse, boost, nslot, rold);
-        else vcycle_mp<bf16>(z, r, sweeps, coarse, boost, nslot, rold);
+        head = true;
     }
-    template <class T> void vcycle_mp(SVec z, SVec r, int sweeps, int coarse, float boost, int nslot, const SVec* rold) {
+    template <class T> void queue_head_mp(SVec r, int nslot) {
         Hier<T, T> H = hier16<T>();
         MPStore& m = s.mp;
         SVecT<T> zh{(T*)m.z[0].p, (T*)m.z[1].p}, rh{(T*)m.r[0].p, (T*)m.r[1].p};
         unsigned g = std::min<unsigned>(cdiv((size_t)nf + Cc, 256), 8192);
         hipLaunchKernelGGL((k_convert<T, float>), dim3(g), dim3(256), 0, st, rh.f, r.f, (size_t)nf, rh.c, r.c, Cc, s.scal.p, nslot,
                            1);
-        vcycle_t<T, T, double>(H, zh, rh, sweeps, coarse, boost);
+        rows<16, 3>(H.tmp, zh, rh, H.L0);
+    }
+    void vcycle(SVec z, SVec r, int sweeps, int coarse, float boost, int nslot = -1, const SVec* rold = nullptr) {
+        const bool queued = head;
+        head = false;
+        if (precision == 0) {
+            Hier<float, float> H = hier32();
+            vcycle_t<float, float, float>(H, SVecT<float>{z.f, z.c}, SVecT<float>{r.f, r.c}, sweeps, coarse, boost, queued);
+            if (rold)
+                hipLaunchKernelGGL(k_sdot2, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, z.f, z.c, r.f, r.c, rold->f,
+                                   rold->c, s.types.fine.p, s.types.coarse.p, s.cmask.p, t, nf);
+            return;
+        }
+        if (precision == 1) vcycle_mp<__half>(z, r, sweeps, coarse, boost, nslot, rold, queued);
+        else vcycle_mp<bf16>(z, r, sweeps, coarse, boost, nslot, rold, queued);
+    }
+    template <class T>
+    void vcycle_mp(SVec z, SVec r, int sweeps, int coarse, float boost, int nslot, const SVec* rold, bool queued) {
+        Hier<T, T> H = hier16<T>();
+        MPStore& m = s.mp;
+        SVecT<T> zh{(T*)m.z[0].p, (T*)m.z[1].p}, rh{(T*)m.r[0].p, (T*)m.r[1].p};
+        unsigned g = std::min<unsigned>(cdiv((size_t)nf + Cc, 256), 8192);
+        if (!queued)
+            hipLaunchKernelGGL((k_convert<T, float>), dim3(g), dim3(256), 0, st, rh.f, r.f, (size_t)nf, rh.c, r.c, Cc, s.scal.p,
+                               nslot, 1);
+        vcycle_t<T, T, double>(H, zh, rh, sweeps, coarse, boost, queued);
         if (rold && nslot >= 0)
             hipLaunchKernelGGL((k_convert_dot2<T>), dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, z.f, z.c, zh.f, zh.c, r.f,
                                r.c, rold->f, rold->c, s.types.fine.p, s.types.coarse.p, s.cmask.p, t, nf, s.scal.p, nslot);
@@ -1855,10 +1887,20 @@ struct SparseSolver {
         return *(double*)s.st.pinned;
     }
     void read2(int slot, double& a, double& b) {   // scal[slot] and the error flags, one copy
-        MAEL_HIP(hipMemcpyAsync(s.st.pinned, s.scal.p, 16 * sizeof(double), hipMemcpyDeviceToHost, st));
-        s.st.spin();
-        a = ((double*)s.st.pinned)[slot];
-        b = ((double*)s.st.pinned)[SS_FLAG];
+        post_read2();
+        wait_read2(slot, a, b);
+    }
+    // the same in two halves, with other work queued between them (other reads too: this
+    // copy lands past the words read() uses)
+    static constexpr int READ2_AT = 32;
+    void post_read2() {
+        MAEL_HIP(hipMemcpyAsync((double*)s.st.pinned + READ2_AT, s.scal.p, 16 * sizeof(double), hipMemcpyDeviceToHost, st));
+        s.st.mark();
+    }
+    void wait_read2(int slot, double& a, double& b) {
+        s.st.poll();
+        a = ((double*)s.st.pinned)[READ2_AT + slot];
+        b = ((double*)s.st.pinned)[READ2_AT + SS_FLAG];
     }
     void set(int slot, double v) { fill_async(s.scal.p + slot, (size_t)1, v, st); }
 
@@ -1914,6 +1956,7 @@ struct SparseSolver {
     // iterations and before convergence is accepted.
     int fcg(double tol, int maxiter, int sweeps, int coarse, float boost, bool& converged, double& relres) {
         SVec P = vec(s.P), b = vec(s.b), r = vec(s.r), z = vec(s.z), d = vec(s.d), q = vec(s.q), rold = vec(s.rold);
+        head = false;
         converged = false;
         set(SS_FLAG, 0.0);
         dot(b, b, SF_NORM, SS_BB);
@@ -1954,11 +1997,29 @@ struct SparseSolver {
             hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(FINISH_THREADS), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p, (int)SF_NORM,
                                (int)SS_RNORM);
             double flags;
-            read2(SS_RNORM, rnorm, flags);
-            raise_flags((long long)flags);
-            if (refine > 0 && (rnorm <= target || it % refine == 0)) rnorm = resid64(r);
+            post_read2();
+            if (refine > 0 && it % refine == 0) {   // r is replaced whatever the read says
+                rnorm = resid64(r);
+                double recursive;
+                wait_read2(SS_RNORM, recursive, flags);
+                raise_flags((long long)flags);
+            } else {
+                // the next V-cycle's head runs during the read
+                if (it < maxiter) queue_head(z, r, sweeps, SS_RNORM);
+                wait_read2(SS_RNORM, rnorm, flags);
+                if (flags != 0.0) head = false;
+                raise_flags((long long)flags);
+                if (refine > 0 && rnorm <= target) {
+                    head = false;   // r is replaced: the V-cycle starts over
+                    rnorm = resid64(r);
+                }
+            }
             relres = rnorm / bnorm;
-            if (rnorm <= target) { converged = true; return it; }
+            if (rnorm <= target) {
+                head = false;
+                converged = true;
+                return it;
+            }
             vcycle(z, r, sweeps, coarse, boost, SS_RNORM, &rold);   // and z.r, z.rold
             hipLaunchKernelGGL(k_sfinish_beta, dim3(1), dim3(FINISH_THREADS), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p);
         }
@@ -1984,6 +2045,7 @@ struct SparseSolver {
     // fgmres.jl:26-92 pressure_fgmres!: right-preconditioned restarted FGMRES (Saad 1993),
     // Z_j = M V_j + w D^-1 V_j, two passes of Gram-Schmidt, the small system in FP64.
     int fgmres(double tol, int maxiter, int restart, int sweeps, int coarse, float boost, double& relres) {
+        head = false;
         workspace(restart);
         SVec P = vec(s.P), b = vec(s.b), r = vec(s.r), q = vec(s.q), z = vec(s.z), tmp = vec(s.tmp);
         dot(b, b, SF_NORM, SS_BB);

@@ -793,6 +793,40 @@ __global__ __launch_bounds__(256) void k_jacobi0w(T* __restrict__ out, const T* 
     *reinterpret_cast<Four<T>*>(out + n) = o;
 }
 
+// The reduced-precision V-cycle's entry on level 0's fine part in one pass: b = r c in T
+// with k_convert's scale (c = 1 / scal[slot], 1 when slot < 0) and rounding, stored for
+// the rest of the V-cycle, and k_jacobi0w's sweep from zero on it.
+template <class T>
+__global__ __launch_bounds__(256) void k_jacobi0w_cvt(T* __restrict__ out, T* __restrict__ bh, const float* __restrict__ r,
+                                                      const T* __restrict__ rw, const uint8_t* __restrict__ tf,
+                                                      const int4* __restrict__ where, int32_t nf,
+                                                      const double* __restrict__ scal, int slot) {
+    float c = 1.0f;
+    if (slot >= 0) c = (float)(1.0 / scal[slot]);
+    if (!(c == c) || c == 0.0f || !(fabsf(c) <= 3.0e38f)) c = 1.0f;
+    const uint32_t n = 4u * (uint32_t)(xcd_tile(blockIdx.x, gridDim.x) * 256u + threadIdx.x);
+    if (n >= (uint32_t)nf) return;
+    const float4 rv = *reinterpret_cast<const float4*>(r + n);
+    const float ra[4] = {rv.x, rv.y, rv.z, rv.w};
+    Four<T> bq, o;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) bq.v[e] = sf<T>(ra[e] * c);
+    *reinterpret_cast<Four<T>*>(bh + n) = bq;
+    if (where[n >> 12].x < 0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) o.v[e] = sf<T>(0.0f);
+    } else {
+        const uint32_t ty = *reinterpret_cast<const uint32_t*>(tf + n);
+        const Four<T> rq = load4(rw, n);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float w = lf(rq.v[e]);
+            o.v[e] = sf<T>(((ty >> (8 * e)) & 0xffu) != 0 && w > 0.0f ? w * lf(bq.v[e]) : 0.0f);
+        }
+    }
+    store4_nt(out, n, o);
+}
+
 // Level 1 (8^3 a slot), staged in LDS: a workgroup per slot stages x of the block and its
 // face layers (face neighbours' slots, else the coarse cells, clamped), then each thread
 // computes 2 cells with finerow's couplings and order.
@@ -1812,14 +1846,20 @@ struct SparseSolver {
         }
         head = true;
     }
+    // r into the hierarchy's precision and level 0's sweep from zero on it: the fine part
+    // in one pass (k_jacobi0w_cvt), the coarse part converted, then its rows
     template <class T> void queue_head_mp(SVec r, int nslot) {
         Hier<T, T> H = hier16<T>();
         MPStore& m = s.mp;
         SVecT<T> zh{(T*)m.z[0].p, (T*)m.z[1].p}, rh{(T*)m.r[0].p, (T*)m.r[1].p};
-        unsigned g = std::min<unsigned>(cdiv((size_t)nf + Cc, 256), 8192);
-        hipLaunchKernelGGL((k_convert<T, float>), dim3(g), dim3(256), 0, st, rh.f, r.f, (size_t)nf, rh.c, r.c, Cc, s.scal.p, nslot,
+        unsigned g = std::min<unsigned>(cdiv(Cc, 256), 8192);
+        hipLaunchKernelGGL((k_convert<T, float>), dim3(g), dim3(256), 0, st, rh.f, r.f, (size_t)0, rh.c, r.c, Cc, s.scal.p, nslot,
                            1);
-        rows<16, 3>(H.tmp, zh, rh, H.L0);
+        if (nf)
+            hipLaunchKernelGGL((k_jacobi0w_cvt<T>), dim3(s.nslots * 4), dim3(256), 0, st, H.tmp.f, rh.f, r.f, H.L0.rwf, H.L0.actf,
+                               t.where, nf, s.scal.p, nslot);
+        hipLaunchKernelGGL((k_rowcoarse<16, 3, T, T>), dim3(cdiv(Cc, 256)), dim3(256), 0, st, H.tmp.c, zh.c, rh.c, zh.f, H.L0, t,
+                           SearchArgs{});
     }
     void vcycle(SVec z, SVec r, int sweeps, int coarse, float boost, int nslot = -1, const SVec* rold = nullptr) {
         const bool queued = head;
@@ -1841,9 +1881,13 @@ struct SparseSolver {
         MPStore& m = s.mp;
         SVecT<T> zh{(T*)m.z[0].p, (T*)m.z[1].p}, rh{(T*)m.r[0].p, (T*)m.r[1].p};
         unsigned g = std::min<unsigned>(cdiv((size_t)nf + Cc, 256), 8192);
-        if (!queued)
+        if (!queued && sweeps >= 1) {
+            queue_head_mp<T>(r, nslot);
+            queued = true;
+        } else if (!queued) {
             hipLaunchKernelGGL((k_convert<T, float>), dim3(g), dim3(256), 0, st, rh.f, r.f, (size_t)nf, rh.c, r.c, Cc, s.scal.p,
                                nslot, 1);
+        }
         vcycle_t<T, T, double>(H, zh, rh, sweeps, coarse, boost, queued);
         if (rold && nslot >= 0)
             hipLaunchKernelGGL((k_convert_dot2<T>), dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, z.f, z.c, zh.f, zh.c, r.f,

@@ -1708,9 +1708,8 @@ struct SparseSolver {
     template <int N, int MODE, class T, class C>
     void rows(SVecT<T> out, SVecT<T> x, SVecT<T> b, const CompT<C>& L, SearchArgs S = SearchArgs{}) {
         int32_t nfl = s.nslots * N * N * N;
-        // one launch for both parts, except the search (its coarse rows need far more
-        // registers than the tiles) and the reduced-precision sweep from zero (pointwise)
-        constexpr bool FUSE = MODE != 4 && !(N == 16 && MODE == 3 && !std::is_same<T, float>::value);
+        // one launch for both parts, except the reduced-precision sweep from zero (pointwise)
+        constexpr bool FUSE = !(N == 16 && MODE == 3 && !std::is_same<T, float>::value);
         if constexpr (FUSE) {
             if (nfl) {
                 constexpr int NQ = (N == 16 && (std::is_same<T, float>::value || MODE == 0 || MODE == 1)) ? 2 : 1;

@@ -797,14 +797,14 @@ __global__ __launch_bounds__(256) void k_jacobi0w(T* __restrict__ out, const T* 
 // with k_convert's scale (c = 1 / scal[slot], 1 when slot < 0) and rounding, stored for
 // the rest of the V-cycle, and k_jacobi0w's sweep from zero on it.
 template <class T>
-__global__ __launch_bounds__(256) void k_jacobi0w_cvt(T* __restrict__ out, T* __restrict__ bh, const float* __restrict__ r,
-                                                      const T* __restrict__ rw, const uint8_t* __restrict__ tf,
-                                                      const int4* __restrict__ where, int32_t nf,
-                                                      const double* __restrict__ scal, int slot) {
+__device__ __forceinline__ void jacobi0w_cvt_wg(const uint32_t tile, T* __restrict__ out, T* __restrict__ bh,
+                                                const float* __restrict__ r, const T* __restrict__ rw,
+                                                const uint8_t* __restrict__ tf, const int4* __restrict__ where,
+                                                int32_t nf, const double* __restrict__ scal, int slot) {
     float c = 1.0f;
     if (slot >= 0) c = (float)(1.0 / scal[slot]);
     if (!(c == c) || c == 0.0f || !(fabsf(c) <= 3.0e38f)) c = 1.0f;
-    const uint32_t n = 4u * (uint32_t)(xcd_tile(blockIdx.x, gridDim.x) * 256u + threadIdx.x);
+    const uint32_t n = 4u * (tile * 256u + threadIdx.x);
     if (n >= (uint32_t)nf) return;
     const float4 rv = *reinterpret_cast<const float4*>(r + n);
     const float ra[4] = {rv.x, rv.y, rv.z, rv.w};
@@ -966,6 +966,19 @@ __global__ __launch_bounds__(256) void k_rowcomp(T* __restrict__ outf, T* __rest
     if (c1 > c0) rowcoarse_wg<N, MODE, T, C>(c0, c0, outc, xc, bc, xf, L, t, S);
     else if constexpr (N == 16) rowfine16_wg<MODE, T, C, NQ>(g - c0, outf, xf, bf, xc, L, t, S);
     else rowfine8_wg<MODE, T, C>(g - c0, outf, xf, bf, xc, L, t);
+}
+// The reduced-precision V-cycle's entry as one launch in the same interleave: level 0's
+// fine part through jacobi0w_cvt_wg (r converted and stored, the sweep from zero) and the
+// coarse rows of that sweep (their r already converted).
+template <class T>
+__global__ __launch_bounds__(256) void k_head0(T* __restrict__ outf, T* __restrict__ outc, T* __restrict__ bh,
+                                               const float* __restrict__ r, const T* __restrict__ xc, const T* __restrict__ bc,
+                                               const T* __restrict__ xf, CompT<T> L, STopo t, int32_t nf,
+                                               const double* __restrict__ scal, int slot, uint32_t ncb) {
+    const uint32_t G = gridDim.x, g = xcd_tile(blockIdx.x, G);
+    const uint32_t c0 = (uint32_t)(((uint64_t)g * ncb) / G), c1 = (uint32_t)(((uint64_t)(g + 1) * ncb) / G);
+    if (c1 > c0) rowcoarse_wg<16, 3, T, T>(c0, c0, outc, xc, bc, xf, L, t, SearchArgs{});
+    else jacobi0w_cvt_wg<T>(g - c0, outf, bh, r, L.rwf, L.actf, t.where, nf, scal, slot);
 }
 
 // ─── dense levels (sflip.jl:1574-1657) ───────────────────────────────────────
@@ -1845,8 +1858,9 @@ struct SparseSolver {
         }
         head = true;
     }
-    // r into the hierarchy's precision and level 0's sweep from zero on it: the fine part
-    // in one pass (k_jacobi0w_cvt), the coarse part converted, then its rows
+    // r into the hierarchy's precision and level 0's sweep from zero on it: the coarse part
+    // of r converted, then one launch (k_head0) for the fine part's conversion and sweep
+    // and the coarse rows
     template <class T> void queue_head_mp(SVec r, int nslot) {
         Hier<T, T> H = hier16<T>();
         MPStore& m = s.mp;
@@ -1854,11 +1868,9 @@ struct SparseSolver {
         unsigned g = std::min<unsigned>(cdiv(Cc, 256), 8192);
         hipLaunchKernelGGL((k_convert<T, float>), dim3(g), dim3(256), 0, st, rh.f, r.f, (size_t)0, rh.c, r.c, Cc, s.scal.p, nslot,
                            1);
-        if (nf)
-            hipLaunchKernelGGL((k_jacobi0w_cvt<T>), dim3(s.nslots * 4), dim3(256), 0, st, H.tmp.f, rh.f, r.f, H.L0.rwf, H.L0.actf,
-                               t.where, nf, s.scal.p, nslot);
-        hipLaunchKernelGGL((k_rowcoarse<16, 3, T, T>), dim3(cdiv(Cc, 256)), dim3(256), 0, st, H.tmp.c, zh.c, rh.c, zh.f, H.L0, t,
-                           SearchArgs{});
+        const uint32_t ncb = (uint32_t)cdiv(Cc, 256), nfw = (uint32_t)s.nslots * 4u;   // 1024 fine cells a workgroup
+        hipLaunchKernelGGL((k_head0<T>), dim3(nfw + ncb), dim3(256), 0, st, H.tmp.f, H.tmp.c, rh.f, r.f, zh.c, rh.c, zh.f, H.L0, t,
+                           nf, s.scal.p, nslot, ncb);
     }
     void vcycle(SVec z, SVec r, int sweeps, int coarse, float boost, int nslot = -1, const SVec* rold = nullptr) {
         const bool queued = head;

@@ -1129,20 +1129,33 @@ __global__ void k_restrict12(T* __restrict__ b2, const T* __restrict__ r1f, cons
     }
 }
 template <class T>
-__global__ void k_prolong21(T* __restrict__ x1f, const T* __restrict__ x2, const uint8_t* __restrict__ act1, STopo t,
-                            float boost, int nslots) {
-    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void prolong21_el(const int32_t n, T* __restrict__ x1f, const T* __restrict__ x2,
+                                             const uint8_t* __restrict__ act1, const STopo& t, float boost, int nslots) {
     if (n >= nslots * SBCELLS1) return;
     int s = (n >> 9) + 1, c = n & 511, i = c & 7, j = (c >> 3) & 7, k = c >> 6;
     int4 w = t.where[s - 1];
     if (w.x < 0 || act1[n] == 0) return;
     x1f[n] = sf<T>(lf(x1f[n]) + boost * lf(x2[cl((w.x * 8 + i) >> 1, (w.y * 8 + j) >> 1, (w.z * 8 + k) >> 1, t.g.nc(0), t.g.nc(1))]));
 }
+// A composite level's prolongation in one launch: its fine part (level 0 from level 1,
+// 1024 cells a workgroup, LVL 0; level 1 from the first dense level, LVL 1) in the first
+// `nfb` workgroups, its coarse part (from the coarser grid's coarse part or level) after.
+template <class T, int LVL>
+__global__ __launch_bounds__(256) void k_prolongcomp(T* __restrict__ xf, const T* __restrict__ fromf,
+                                                     const uint8_t* __restrict__ maskf, T* __restrict__ xc,
+                                                     const T* __restrict__ fromc, const uint8_t* __restrict__ actc, STopo t,
+                                                     float boost, int32_t nfine, uint32_t nfb) {
+    if (blockIdx.x < nfb) {
+        if constexpr (LVL == 0) prolong10_wg<T>(xcd_tile(blockIdx.x, nfb), xf, fromf, maskf, t, boost, nfine);
+        else prolong21_el<T>((int32_t)(blockIdx.x * 256u + threadIdx.x), xf, fromf, maskf, t, boost, nfine);
+    } else {
+        prolongcoarse_el<T>((size_t)(blockIdx.x - nfb) * 256u + threadIdx.x, xc, fromc, actc, t, boost);
+    }
+}
 template <class T>
-__global__ void k_prolongcoarse(T* __restrict__ xc, const T* __restrict__ from, const uint8_t* __restrict__ act, STopo t,
-                                float boost) {
+__device__ __forceinline__ void prolongcoarse_el(const size_t l, T* __restrict__ xc, const T* __restrict__ from,
+                                                 const uint8_t* __restrict__ act, const STopo& t, float boost) {
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
-    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= (size_t)n0 * n1 * n2) return;
     int I, J, K;
     unlin(l, n0, n1, I, J, K);
@@ -1151,9 +1164,8 @@ __global__ void k_prolongcoarse(T* __restrict__ xc, const T* __restrict__ from, 
 // Four cells of an x row a thread (two level-1 parents); a workgroup's 1024 cells lie in
 // one slot, so the slot's place is uniform.
 template <class T>
-__global__ __launch_bounds__(256) void k_prolong10(T* __restrict__ x0f, const T* __restrict__ x1f, const uint8_t* __restrict__ tf,
-                                                   STopo t, float boost, int32_t nf) {
-    uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void prolong10_wg(const uint32_t tile, T* __restrict__ x0f, const T* __restrict__ x1f,
+                                             const uint8_t* __restrict__ tf, const STopo& t, float boost, int32_t nf) {
     int32_t n = (int32_t)(tile * 1024u) + 4 * (int32_t)threadIdx.x;
     if ((int32_t)(tile * 1024u) >= nf) return;
     int s = (int)((tile * 1024u) >> 12) + 1, c = n & 4095;
@@ -1821,15 +1833,13 @@ struct SparseSolver {
             }
         }
         SLevel& D2 = s.D[0];
-        if (ns)
-            hipLaunchKernelGGL(k_prolong21<T>, dim3(cdiv(ns * SBCELLS1, 256)), dim3(256), 0, st, H.X1x.f, H.Dx[0], s.act1.p, t,
-                               boost, ns);
-        hipLaunchKernelGGL(k_prolongcoarse<T>, dim3(cdiv(Cc, 256)), dim3(256), 0, st, H.X1x.c, H.Dx[0], D2.act.p, t, boost);
+        const uint32_t ncb = (uint32_t)cdiv(Cc, 256), nfb1 = (uint32_t)cdiv((size_t)ns * SBCELLS1, 256);
+        hipLaunchKernelGGL((k_prolongcomp<T, 1>), dim3(nfb1 + ncb), dim3(256), 0, st, H.X1x.f, H.Dx[0], s.act1.p, H.X1x.c,
+                           H.Dx[0], D2.act.p, t, boost, ns, nfb1);
         smooth<8>(H.X1x, H.X1b, H.X1t, H.L1, sweeps);
-        if (nf)
-            hipLaunchKernelGGL(k_prolong10<T>, dim3(cdiv(nf, 1024)), dim3(256), 0, st, z.f, H.X1x.f, s.types.fine.p, t, boost,
-                               nf);
-        hipLaunchKernelGGL(k_prolongcoarse<T>, dim3(cdiv(Cc, 256)), dim3(256), 0, st, z.c, H.X1x.c, D2.act.p, t, boost);
+        const uint32_t nfb0 = (uint32_t)cdiv((size_t)nf, 1024);
+        hipLaunchKernelGGL((k_prolongcomp<T, 0>), dim3(nfb0 + ncb), dim3(256), 0, st, z.f, H.X1x.f, s.types.fine.p, z.c, H.X1x.c,
+                           D2.act.p, t, boost, nf, nfb0);
         smooth<16>(z, r, H.tmp, H.L0, sweeps);
         MAEL_LAUNCH_CHECK();
     }

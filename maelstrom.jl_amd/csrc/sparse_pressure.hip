@@ -1099,8 +1099,9 @@ __global__ __launch_bounds__(1024) void k_scoarsest(T* __restrict__ x, const T* 
 
 // ─── between the composite levels (sflip.jl:1661-1705) ───────────────────────
 
-template <class T> __global__ void k_restrict01(T* __restrict__ b1, const T* __restrict__ r0, STopo t, int nslots) {
-    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+template <class T>
+__device__ __forceinline__ void restrict01_el(const int32_t n, T* __restrict__ b1, const T* __restrict__ r0, const STopo& t,
+                                              int nslots) {
     if (n >= nslots * SBCELLS1) return;
     int s = (n >> 9) + 1, c = n & 511, i = c & 7, j = (c >> 3) & 7, k = c >> 6;
     if (t.where[s - 1].x < 0) return;
@@ -1110,6 +1111,18 @@ template <class T> __global__ void k_restrict01(T* __restrict__ b1, const T* __r
             for (int ax = 0; ax < 2; ++ax)
                 acc += lf(r0[(s - 1) * SBCELLS + (2 * i + ax) + 16 * ((2 * j + by) + 16 * (2 * k + cz))]);
     b1[n] = sf<T>(acc);
+}
+// Level 1's right-hand side in one launch: the fine part restricted from level 0's
+// residual in the first `nfb` workgroups, the coarse part (shared by both levels) copied.
+template <class T>
+__global__ __launch_bounds__(256) void k_restrictcomp(T* __restrict__ b1f, const T* __restrict__ r0f, T* __restrict__ b1c,
+                                                      const T* __restrict__ r0c, size_t nc, STopo t, int nslots, uint32_t nfb) {
+    if (blockIdx.x < nfb) {
+        restrict01_el<T>((int32_t)(blockIdx.x * 256u + threadIdx.x), b1f, r0f, t, nslots);
+    } else {
+        const size_t l = (size_t)(blockIdx.x - nfb) * 256u + threadIdx.x;
+        if (l < nc) b1c[l] = r0c[l];
+    }
 }
 template <class T>
 __global__ void k_restrict12(T* __restrict__ b2, const T* __restrict__ r1f, const T* __restrict__ r1c, STopo t) {
@@ -1797,8 +1810,9 @@ struct SparseSolver {
         int ns = s.nslots;
         smooth0<16>(z, r, H.tmp, H.L0, sweeps, (size_t)nf, queued);
         rows<16, 1>(H.q, z, r, H.L0);
-        if (ns) hipLaunchKernelGGL(k_restrict01<T>, dim3(cdiv(ns * SBCELLS1, 256)), dim3(256), 0, st, H.X1b.f, H.q.f, t, ns);
-        MAEL_HIP(hipMemcpyAsync(H.X1b.c, H.q.c, Cc * sizeof(T), hipMemcpyDeviceToDevice, st));
+        const uint32_t nfr = (uint32_t)cdiv((size_t)ns * SBCELLS1, 256);
+        hipLaunchKernelGGL(k_restrictcomp<T>, dim3(nfr + (uint32_t)cdiv(Cc, 256)), dim3(256), 0, st, H.X1b.f, H.q.f, H.X1b.c, H.q.c,
+                           Cc, t, ns, nfr);
         smooth0<8>(H.X1x, H.X1b, H.X1t, H.L1, sweeps, (size_t)ns * SBCELLS1);
         rows<8, 1>(H.X1r, H.X1x, H.X1b, H.L1);
         hipLaunchKernelGGL(k_restrict12<T>, dim3(cdiv(Cc, 256)), dim3(256), 0, st, H.Db[0], H.X1r.f, H.X1r.c, t);

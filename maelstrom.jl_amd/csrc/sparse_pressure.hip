@@ -1047,6 +1047,33 @@ __global__ void k_restrictdense(T* __restrict__ bc, const T* __restrict__ rf, in
             for (int a = 0; a < 2; ++a) acc += lf(rf[cl(2 * I + a, 2 * J + b, 2 * K + c, nf.x, nf.y)]);
     bc[l] = sf<T>(acc * scale);
 }
+// The residual and its restriction in one pass: each coarser cell forms its eight finer
+// cells' residuals (k_rowdense mode 1's arithmetic, rounded to T as that kernel stores
+// them) and sums them in k_restrictdense's order, so the result is the same bit for bit
+// without the residual array.
+template <class T, class C>
+__global__ void k_residrestrict(T* __restrict__ bc, const T* __restrict__ x, const T* __restrict__ b, DViewT<C> L, int3 nc,
+                                float scale) {
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)nc.x * nc.y * nc.z) return;
+    int I, J, K;
+    unlin(l, nc.x, nc.y, I, J, K);
+    float acc = 0.0f;
+    for (int c = 0; c < 2; ++c)
+        for (int bb = 0; bb < 2; ++bb)
+            for (int a = 0; a < 2; ++a) {
+                const int i = 2 * I + a, j = 2 * J + bb, k = 2 * K + c;
+                const size_t f = cl(i, j, k, L.n.x, L.n.y);
+                float res = 0.0f;
+                if (L.act[f] != 0) {
+                    float diag, off;
+                    denserow<T, C, false>(L, x, i, j, k, diag, off);
+                    res = lf(b[f]) - (diag * lf(x[f]) - off);
+                }
+                acc += lf(sf<T>(res));
+            }
+    bc[l] = sf<T>(acc * scale);
+}
 template <class T>
 __global__ void k_prolongdense(T* __restrict__ xf, const T* __restrict__ xc, const uint8_t* __restrict__ actf, int3 nf, int3 nc,
                                float boost) {
@@ -1830,10 +1857,9 @@ struct SparseSolver {
                                        H.D[l]);
                 hipLaunchKernelGGL((k_rowdense<0, T, C>), dim3(cdiv(c, 256)), dim3(256), 0, st, H.Dx[l], H.Dt[l], H.Db[l], H.D[l]);
             }
-            hipLaunchKernelGGL((k_rowdense<1, T, C>), dim3(cdiv(c, 256)), dim3(256), 0, st, H.Dr[l], H.Dx[l], H.Db[l], H.D[l]);
             SLevel& Cn = s.D[l + 1];
-            hipLaunchKernelGGL(k_restrictdense<T>, dim3(cdiv(Cn.cells(), 256)), dim3(256), 0, st, H.Db[l + 1], H.Dr[l], i3(Cn.n),
-                               i3(Dl.n), H.rscale);
+            hipLaunchKernelGGL((k_residrestrict<T, C>), dim3(cdiv(Cn.cells(), 256)), dim3(256), 0, st, H.Db[l + 1], H.Dx[l],
+                               H.Db[l], H.D[l], i3(Cn.n), H.rscale);
         }
         hipLaunchKernelGGL((k_scoarsest<T, C, A>), dim3(1), dim3(1024), 0, st, H.Dx[nl - 1], H.Db[nl - 1], H.D[nl - 1], coarse);
         for (int l = nl - 2; l >= 0; --l) {

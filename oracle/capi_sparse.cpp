@@ -132,6 +132,25 @@ void orc_sparse_init_fine(void* p, const float* eta, int32_t neta, float rmax, f
 void orc_sparse_classify(void* p) { ((OrcSparse*)p)->s.classify(); }
 void orc_sparse_solidify(void* p) { ((OrcSparse*)p)->s.solidify(); }
 void orc_sparse_restrict_velocity(void* p) { ((OrcSparse*)p)->s.restrict_velocity(); }
+// sflip.jl:583-594 / 596-612 on one named field: "phi", "P", "types" (cells); "u", "v", "w",
+// "fu", "fv", "fw" (faces along their axis). The coarse part under the blocks is rewritten.
+int orc_sparse_restrict_field(void* p, const char* name) {
+    auto& s = ((OrcSparse*)p)->s;
+    std::string n(name);
+    if (n == "phi") { restrictcells(s.t, s.phi.coarse.data(), (const float*)s.phi.fine.data()); return 0; }
+    if (n == "P") { restrictcells(s.t, s.P.coarse.data(), (const float*)s.P.fine.data()); return 0; }
+    if (n == "types") { restrictcells(s.t, s.types.coarse.data(), (const uint8_t*)s.types.fine.data()); return 0; }
+    static const char* ax[3] = {"u", "v", "w"};
+    for (int a = 0; a < 3; ++a) {
+        if (n == ax[a]) { restrictfaces(s.t, s.u[a].coarse.data(), (const float*)s.u[a].fine.data(), a); return 0; }
+        if (n == std::string("f") + ax[a]) {
+            restrictfaces(s.t, s.fl[a].coarse.data(), (const uint8_t*)s.fl[a].fine.data(), a);
+            return 0;
+        }
+    }
+    orc_set_error(("no field " + n).c_str());
+    return -1;
+}
 void orc_sparse_redistance(void* p, int32_t iterations) { ((OrcSparse*)p)->s.redistance(iterations); }
 double orc_sparse_volume(void* p) { return ((OrcSparse*)p)->s.volume(); }
 void orc_sparse_fill_flags(void* p, uint8_t v) {

@@ -381,6 +381,7 @@ def _slib():
             "orc_sparse_classify": (None, [p]),
             "orc_sparse_solidify": (None, [p]),
             "orc_sparse_restrict_velocity": (None, [p]),
+            "orc_sparse_restrict_field": (C.c_int, [p, C.c_char_p]),
             "orc_sparse_redistance": (None, [p, i32]),
             "orc_sparse_volume": (f64, [p]),
             "orc_sparse_fill_flags": (None, [p, u8]),
@@ -505,6 +506,11 @@ class SparseOracle:
 
     def restrict_velocity(self):
         _slib().orc_sparse_restrict_velocity(self.h)
+
+    def restrict_field(self, name):
+        """restrictcells (phi, P, types) or restrictfaces (u, v, w, fu, fv, fw) of one field."""
+        if _slib().orc_sparse_restrict_field(self.h, name.encode()) != 0:
+            raise KeyError(name)
 
     def redistance(self, iterations=None):
         _slib().orc_sparse_redistance(self.h, self.band + 3 if iterations is None else iterations)

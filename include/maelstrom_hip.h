@@ -41,6 +41,7 @@ extern "C" {
 #define MAEL_E_BREAKDOWN     -7  /* "Pressure Arnoldi breakdown", fgmres.jl:77 */
 #define MAEL_E_HIP           -8  /* a HIP runtime error */
 #define MAEL_E_GRID          -9  /* "the coarse grid ... does not halve ...", sflip.jl:326 */
+#define MAEL_E_TIMEOUT      -10  /* a device wait ran past the library's limit (no counterpart in the reference) */
 
 /* ── context ──────────────────────────────────────────────────────────────── */
 
@@ -159,12 +160,18 @@ typedef struct {
 } mael_solve_stats;
 
 /* Defaults of `pressure!(s::FlipBasin)` (flip.jl:1082): tol 1e-4, maxiter 60, sweeps 2, coarse 40,
- * warm start, FP64 refinement on. */
+ * warm start, no refinement (the reference's FP32 PCG). */
 void mael_solve_defaults(mael_solve_opts *o);
 
 /* `project!(s, dt)` (flip.jl:1144-1156) after the caller's γ (the volume control,
  * flip.jl:1146-1147): rhs, clear air, `pressure!`, project the faces and flag them. */
 int mael_dense_project(mael_dense *d, float dt, float gamma, const mael_solve_opts *opts, mael_solve_stats *stats);
+/* `vcycle!(s)` (flip.jl:986-1014) alone, as PCG applies it (flip.jl:1094, 1106): the operator
+ * from the basin's types and level set, then P = M RHS (the preconditioner applied once to
+ * the RHS array). `queued` nonzero runs it through the path the CG loop takes when the
+ * V-cycle's first launches were queued ahead of a host read; the result is the same. For
+ * tests of the preconditioner; `opts` NULL takes the defaults. */
+int mael_dense_vcycle(mael_dense *d, const mael_solve_opts *opts, int32_t queued);
 /* The transpose of `project!`'s linear part (see mael_dense_g2p_adjoint). */
 int mael_dense_project_adjoint(mael_dense *d, const mael_solve_opts *opts, mael_solve_stats *stats);
 
@@ -288,6 +295,12 @@ void mael_sparse_defaults(mael_sparse_opts *o);
 /* `project!(s, dt)` (sflip.jl:1998-2015) after the caller's γ: couplings, rhs, clear,
  * the solve by `opts->policy`, the faces and flags. Errors carry the reference's texts. */
 int mael_sparse_project(mael_sparse *s, float gamma, const mael_sparse_opts *opts, mael_sparse_stats *stats);
+
+/* The flexible CG's preconditioner alone (sflip.jl:1883, 1901 `vcycle!`, the V-cycle of
+ * sflip.jl:1218-1311 with opts->sweeps, coarse, boost and precision): the couplings and
+ * hierarchy from the basin's types and level set, then P = M RHS over the fine pool and the
+ * coarse grid. `queued` as in mael_dense_vcycle. For tests of the preconditioner. */
+int mael_sparse_vcycle(mael_sparse *s, const mael_sparse_opts *opts, int32_t queued);
 
 /* Support phases around the hot path (not part of it): */
 /* The start of `initialize!` (sflip.jl:681-694): φ from the surface table and the swirl,

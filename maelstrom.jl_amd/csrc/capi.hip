@@ -106,7 +106,7 @@ int mael_flipgrid(float radius, float depth, float top, float dx, int32_t pad, m
 
 void mael_solve_defaults(mael_solve_opts* o) {
     if (!o) return;
-    o->tol = 1e-4; o->maxiter = 60; o->sweeps = 2; o->coarse = 40; o->precision = 0; o->refine = 1; o->warm = 1;
+    o->tol = 1e-4; o->maxiter = 60; o->sweeps = 2; o->coarse = 40; o->precision = 0; o->refine = 0; o->warm = 1;
 }
 
 int mael_dense_create(mael_ctx* ctx, const mael_dense_grid* grid, int64_t capacity, int32_t band, int32_t nmin,
@@ -193,24 +193,6 @@ int mael_dense_g2p(mael_dense* d, float dt, float gravity) {
     if (!d) return MAEL_E_ARG;
     return guard(d->ctx, [&] { d->b->g2p(dt, gravity); });
 }
-int mael_dense_project(mael_dense* d, float dt, float gamma, const mael_solve_opts* opts, mael_solve_stats* stats) {
-    if (!d) return MAEL_E_ARG;
-    return guard(d->ctx, [&] {
-        mael::SolveOpts o;
-        if (opts) {
-            o.tol = opts->tol; o.maxiter = opts->maxiter; o.sweeps = opts->sweeps; o.coarse = opts->coarse;
-            o.precision = opts->precision; o.refine = opts->refine; o.warm = opts->warm;
-        }
-        if (o.sweeps < 1 || o.coarse < 1 || o.maxiter < 0 || !(o.tol >= 0.0))
-            throw mael::Error(mael::E_ARG, "bad solver options");
-        mael::SolveStats s;
-        d->b->project(dt, gamma, o, s);
-        if (stats) {
-            stats->iterations = s.iterations; stats->restarts = s.restarts; stats->rel_residual = s.rel_residual;
-            stats->rel_residual_rec = s.rel_residual_recursive; stats->ms = s.ms_total;
-        }
-    });
-}
 static mael::SolveOpts solve_opts(const mael_solve_opts* opts) {
     mael::SolveOpts o;
     if (opts) {
@@ -219,6 +201,22 @@ static mael::SolveOpts solve_opts(const mael_solve_opts* opts) {
     }
     if (o.sweeps < 1 || o.coarse < 1 || o.maxiter < 0 || !(o.tol >= 0.0)) throw mael::Error(mael::E_ARG, "bad solver options");
     return o;
+}
+int mael_dense_project(mael_dense* d, float dt, float gamma, const mael_solve_opts* opts, mael_solve_stats* stats) {
+    if (!d) return MAEL_E_ARG;
+    return guard(d->ctx, [&] {
+        mael::SolveOpts o = solve_opts(opts);
+        mael::SolveStats s;
+        d->b->project(dt, gamma, o, s);
+        if (stats) {
+            stats->iterations = s.iterations; stats->restarts = s.restarts; stats->rel_residual = s.rel_residual;
+            stats->rel_residual_rec = s.rel_residual_recursive; stats->ms = s.ms_total;
+        }
+    });
+}
+int mael_dense_vcycle(mael_dense* d, const mael_solve_opts* opts, int32_t queued) {
+    if (!d) return MAEL_E_ARG;
+    return guard(d->ctx, [&] { d->b->precondition(solve_opts(opts), queued != 0); });
 }
 int mael_dense_g2p_adjoint(mael_dense* d, float dt) {
     if (!d) return MAEL_E_ARG;

@@ -58,6 +58,19 @@ ArrRef sparse_array(mael::SparseBasin& b, int which) {
     throw mael::Error(mael::E_ARG, "unknown sparse array " + std::to_string(which));
 }
 
+mael::SparseOpts sparse_opts(const mael_sparse_opts* opts) {
+    mael::SparseOpts o;
+    if (opts) {
+        o.tol = opts->tol; o.maxiter = opts->maxiter; o.sweeps = opts->sweeps; o.coarse = opts->coarse;
+        o.restart = opts->restart; o.policy = opts->policy; o.precision = opts->precision; o.boost = opts->boost;
+        o.refine = opts->refine;
+    }
+    if (o.sweeps < 1 || o.coarse < 1 || o.maxiter < 0 || !(o.tol >= 0.0) || o.restart < 1 || o.restart > 40 ||
+        o.policy < 0 || o.policy > 2 || o.precision < 0 || o.precision > 2 || o.refine < 0)
+        throw mael::Error(mael::E_ARG, "bad solver options");
+    return o;
+}
+
 }  // namespace
 
 extern "C" {
@@ -236,15 +249,7 @@ void mael_sparse_defaults(mael_sparse_opts* o) {
 int mael_sparse_project(mael_sparse* s, float gamma, const mael_sparse_opts* opts, mael_sparse_stats* stats) {
     if (!s) return MAEL_E_ARG;
     return guard(s->ctx, [&] {
-        mael::SparseOpts o;
-        if (opts) {
-            o.tol = opts->tol; o.maxiter = opts->maxiter; o.sweeps = opts->sweeps; o.coarse = opts->coarse;
-            o.restart = opts->restart; o.policy = opts->policy; o.precision = opts->precision; o.boost = opts->boost;
-            o.refine = opts->refine;
-        }
-        if (o.sweeps < 1 || o.coarse < 1 || o.maxiter < 0 || !(o.tol >= 0.0) || o.restart < 1 || o.restart > 40 ||
-            o.policy < 0 || o.policy > 2 || o.precision < 0 || o.precision > 2 || o.refine < 0)
-            throw mael::Error(mael::E_ARG, "bad solver options");
+        mael::SparseOpts o = sparse_opts(opts);
         mael::SparseStats st;
         s->b->project(gamma, o, st);
         if (stats) {
@@ -253,6 +258,11 @@ int mael_sparse_project(mael_sparse* s, float gamma, const mael_sparse_opts* opt
             stats->rel_residual = st.rel_residual; stats->ms = st.ms;
         }
     });
+}
+
+int mael_sparse_vcycle(mael_sparse* s, const mael_sparse_opts* opts, int32_t queued) {
+    if (!s) return MAEL_E_ARG;
+    return guard(s->ctx, [&] { s->b->precondition(sparse_opts(opts), queued != 0); });
 }
 
 int mael_sparse_init_coarse(mael_sparse* s, const float* eta, int32_t neta, float rmax, float gamma_circ, float core) {

@@ -18,7 +18,7 @@ constexpr uint8_t SOLID = 0, LIQUID = 1, AIR = 2;
 // Status codes of the C ABI (mirrored in include/maelstrom_hip.h).
 enum Status : int {
     OK = 0, E_ARG = -1, E_CAPACITY = -2, E_BLOCKS = -3, E_NONFINITE = -4, E_CG_INDEFINITE = -5,
-    E_NOCONVERGE = -6, E_BREAKDOWN = -7, E_HIP = -8, E_GRID = -9,
+    E_NOCONVERGE = -6, E_BREAKDOWN = -7, E_HIP = -8, E_GRID = -9, E_TIMEOUT = -10,
 };
 
 struct Error : std::runtime_error {

@@ -3,6 +3,9 @@
 // where a host value is returned (the resample total, solver statistics).
 #pragma once
 #include "common.hpp"
+#include <chrono>
+#include <string>
+#include <thread>
 #include <vector>
 
 namespace mael {
@@ -34,7 +37,7 @@ struct SolveOpts {
     double tol = 1e-4;
     int maxiter = 60, sweeps = 2, coarse = 40;
     int precision = 0;        // 0 FP32 V-cycle, 1 FP16 storage, 2 BF16 storage
-    int refine = 1;           // FP64 true-residual recomputation with restarts
+    int refine = 0;           // 1: FP64 true-residual recomputation with restarts (0: the reference's FP32 PCG)
     int warm = 1;             // start from the last pressure (else from zero)
 };
 struct SolveStats {
@@ -65,9 +68,22 @@ struct Stream {
         poll();
     }
     void mark() { MAEL_HIP(hipEventRecord(rd, s)); }   // what poll() waits for
+    // Spins for the first POLL_SPIN queries (a Krylov read-back lands in tens of
+    // microseconds), then yields the core between queries; a wait longer than
+    // POLL_LIMIT_S seconds is reported as E_TIMEOUT instead of spinning on forever.
+    static constexpr int POLL_SPIN = 20000;
+    static constexpr double POLL_LIMIT_S = 300.0;
     void poll() {
         hipError_t e;
+        long long polls = 0;
+        std::chrono::steady_clock::time_point t0;
         while ((e = hipEventQuery(rd)) == hipErrorNotReady) {
+            if (++polls < POLL_SPIN) continue;
+            if (polls == POLL_SPIN) t0 = std::chrono::steady_clock::now();
+            std::this_thread::yield();
+            if ((polls & 1023) == 0 &&
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > POLL_LIMIT_S)
+                throw Error(E_TIMEOUT, "a device wait exceeded " + std::to_string((int)POLL_LIMIT_S) + " s");
         }
         MAEL_HIP(e);
     }
@@ -185,6 +201,10 @@ struct DenseBasin {
     void project(float dt, float gamma, const SolveOpts& o, SolveStats& s);   // flip.jl:1144-1156
     void operator_setup();                                   // coarse types and the fine diagonal
     void solve_pressure(const SolveOpts& o, SolveStats& s);  // A P = b from P
+    // z = M b: PCG's V-cycle preconditioner applied once to the right-hand side, into P
+    // (with `queued`, through the path the CG loop takes when the V-cycle's first launches
+    // were queued ahead of a host read)
+    void precondition(const SolveOpts& o, bool queued);
     // adjoints of the hot path's linear maps (no counterpart in the reference): cotangents
     // of the face velocities in ub (u0's in ub0) and of the particles' v, c in pb
     DBuf<float> ub[3], ub0[3], pb[12];

@@ -636,6 +636,16 @@ void DenseBasin::solve_pressure(const SolveOpts& o, SolveStats& stats) {
     stats.rel_residual = bnorm == 0.0 ? 0.0 : rnorm / bnorm;
 }
 
+// flip.jl:1094 / 1106 `vcycle!(s)` alone: the operator from the current types and level
+// set, then z = M b into P.
+void DenseBasin::precondition(const SolveOpts& o, bool queued) {
+    DenseSolver S(*this);
+    operator_setup();
+    if (queued) S.queue_head(P.p, b.p, o.sweeps);
+    S.vcycle(P.p, b.p, o.sweeps, o.coarse);
+    st.sync();
+}
+
 // The transpose of project!'s linear part, in place on ub; P is left holding the adjoint
 // pressure lambda = A^-1 (-G^T ub) (solved from zero).
 void DenseBasin::project_adjoint(const SolveOpts& o, SolveStats& stats) {

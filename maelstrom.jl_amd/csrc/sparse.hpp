@@ -467,6 +467,10 @@ struct SparseBasin {
     void p2g(float dt);                                               // sflip.jl:808-831
     void g2p(float dt, float gravity);                                // sflip.jl:2141-2144
     void project(float gamma, const SparseOpts& o, SparseStats& s);   // sflip.jl:1998-2015
+    // z = M b: the flexible CG's V-cycle preconditioner applied once to the right-hand
+    // side array, into P (with `queued`, through the path the Krylov loop takes when the
+    // V-cycle's first launches were queued ahead of a host read)
+    void precondition(const SparseOpts& o, bool queued);
     // support phases
     void init_coarse(const float* eta, int neta, float rmax, float G, float core);
     void init_fine(const float* eta, int neta, float rmax, float G, float core);

@@ -2230,6 +2230,28 @@ struct SparseSolver {
     }
 };
 
+// sflip.jl:1883 / 1901 `vcycle!(s, z, r)` alone: the couplings and hierarchy from the
+// current types and level set, then z = M b into P. The residual the V-cycle reads is a
+// copy of b; in reduced precision it is scaled by its norm as in the solve.
+void SparseBasin::precondition(const SparseOpts& o, bool queued) {
+    SparseSolver S(*this);
+    S.coefficients();
+    S.hierarchy();
+    S.precision = o.precision;
+    if (o.precision != 0) {
+        mp_alloc();
+        if (o.precision == 1) S.cast_couplings<__half>();
+        else S.cast_couplings<bf16>();
+    }
+    SVec bv = S.vec(b), rv = S.vec(r), z = S.vec(P);
+    S.copy(rv, bv);
+    S.dot(rv, rv, SF_NORM, SS_RNORM);
+    if (queued) S.queue_head(z, rv, o.sweeps, SS_RNORM);
+    S.vcycle(z, rv, o.sweeps, o.coarse, o.boost, SS_RNORM);
+    MAEL_LAUNCH_CHECK();
+    st.sync();
+}
+
 void SparseBasin::project(float gamma, const SparseOpts& o, SparseStats& stats) {
     auto t0 = std::chrono::steady_clock::now();
     hipStream_t s = st.s;

@@ -10,8 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(os.path.dirname(HERE), "lib", "libmaelstrom_hip.so")
 
 # status codes (include/maelstrom_hip.h)
-OK, E_ARG, E_CAPACITY, E_BLOCKS, E_NONFINITE, E_CG_INDEFINITE, E_NOCONVERGE, E_BREAKDOWN, E_HIP, E_GRID = (
-    0, -1, -2, -3, -4, -5, -6, -7, -8, -9)
+OK, E_ARG, E_CAPACITY, E_BLOCKS, E_NONFINITE, E_CG_INDEFINITE, E_NOCONVERGE, E_BREAKDOWN, E_HIP, E_GRID, E_TIMEOUT = (
+    0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10)
 
 # array ids (include/maelstrom_hip.h)
 ARRAYS = ["U", "V", "W", "U0", "V0", "W0", "PHI", "PHI0", "TYPES", "P", "FU", "FV", "FW", "RHS",
@@ -118,6 +118,7 @@ def lib():
         "mael_dense_p2g": (C.c_int, [p, f32]),
         "mael_dense_g2p": (C.c_int, [p, f32, f32]),
         "mael_dense_project": (C.c_int, [p, f32, f32, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+        "mael_dense_vcycle": (C.c_int, [p, C.POINTER(SolveOpts), i32]),
         "mael_dense_g2p_adjoint": (C.c_int, [p, f32]),
         "mael_dense_project_adjoint": (C.c_int, [p, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
         "mael_dense_p2g_adjoint": (C.c_int, [p, f32]),
@@ -149,6 +150,7 @@ def lib():
         "mael_sparse_g2p": (C.c_int, [p, f32, f32]),
         "mael_sparse_defaults": (None, [C.POINTER(SparseOpts)]),
         "mael_sparse_project": (C.c_int, [p, f32, C.POINTER(SparseOpts), C.POINTER(SparseStats)]),
+        "mael_sparse_vcycle": (C.c_int, [p, C.POINTER(SparseOpts), i32]),
         "mael_sparse_init_coarse": (C.c_int, [p, p, i32, f32, f32, f32]),
         "mael_sparse_init_fine": (C.c_int, [p, p, i32, f32, f32, f32]),
         "mael_sparse_classify": (C.c_int, [p]),

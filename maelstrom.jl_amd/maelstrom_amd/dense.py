@@ -154,8 +154,9 @@ def grid2particles(s, dt):
     return s
 
 
-def solve_options(tol=1e-4, maxiter=60, sweeps=2, coarse=40, precision=0, refine=1, warm=1):
-    """`pressure!`'s options (flip.jl:1082 defaults) plus precision, FP64 refinement and warm start."""
+def solve_options(tol=1e-4, maxiter=60, sweeps=2, coarse=40, precision=0, refine=0, warm=1):
+    """`pressure!`'s options (flip.jl:1082 defaults) plus precision, FP64 refinement (off: the
+    reference's FP32 PCG) and warm start."""
     return SolveOpts(float(tol), int(maxiter), int(sweeps), int(coarse), int(precision), int(refine), int(warm))
 
 

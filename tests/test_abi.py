@@ -35,6 +35,7 @@ def test_host_only_entry_points_work_without_gpu(mael):
     o = _l.SolveOpts()
     L.mael_solve_defaults(ctypes.byref(o))
     assert (o.tol, o.maxiter, o.sweeps, o.coarse) == (1e-4, 60, 2, 40)   # flip.jl:1082
+    assert (o.precision, o.refine, o.warm) == (0, 0, 1)   # the reference's FP32 PCG, warm-started
     assert L.mael_version().startswith(b"maelstrom-hip")
     g = _l.DenseGrid()
     assert L.mael_flipgrid(18.0, 12.0, 3.0, 0.0, 2, ctypes.byref(g)) == _l.E_ARG

@@ -22,12 +22,33 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-def make_basin(mael, orc, capacity=1_000_000, dx=0.2):
-    """The reference's dense test basin (test/runtests.jl:49): narutobasin(radius 4, depth 2, Γ 15, core 1)."""
-    b = mael.narutobasin(radius=4.0, depth=2.0, Γ=15.0, core=1.0, band=1.0, headroom=1.5)
-    s = mael.FlipBasin(b, dx=dx, capacity=capacity)
-    g, phi, u, v, w, types = orc.dense_scene(4.0, 2.0, 1.5, dx, 15.0, 1.0)
+# The scenes the dense parity tests run on: the reference's dense test basin
+# (test/runtests.jl:49, narutobasin(radius 4, depth 2, Γ 15, core 1) at 0.2 m, 44x44x20)
+# and the one-step workload (`Basin(radius 18, depth 12, Γ 225, core 2.5)` at 0.6 m, default
+# rim wall 3 m: 64x64x32 cells, 4 multigrid levels).
+SCENES = {
+    "testbasin": dict(basin=dict(radius=4.0, depth=2.0, Γ=15.0, core=1.0, band=1.0, headroom=1.5), naruto=True,
+                      top=1.5, dx=0.2),
+    "basin64": dict(basin=dict(radius=18.0, depth=12.0, Γ=225.0, core=2.5), naruto=False, top=3.0, dx=0.6),
+}
+
+
+@pytest.fixture(params=sorted(SCENES))
+def scene(request):
+    return request.param
+
+
+def make_basin(mael, orc, capacity=1_000_000, scene="testbasin"):
+    """A GPU FlipBasin holding the oracle's balanced-vortex state of `scene`."""
+    sc = SCENES[scene]
+    kw = sc["basin"]
+    b = mael.narutobasin(**kw) if sc["naruto"] else mael.Basin(**kw)
+    s = mael.FlipBasin(b, dx=sc["dx"], capacity=capacity)
+    g, phi, u, v, w, types = orc.dense_scene(kw["radius"], kw["depth"], sc["top"], sc["dx"], kw["Γ"], kw["core"])
     assert tuple(s.grid.n) == g.dims and tuple(s.grid.lo) == tuple(g.lo)
+    if scene == "basin64":
+        assert g.dims == (64, 64, 32) and s.grid.levels == 4
+    print("case: dense scene %s %s" % (scene, g.dims), flush=True)
     for name, arr in (("PHI", phi), ("U", u), ("V", v), ("W", w), ("TYPES", types)):
         s.set(name, arr)
     return s, dict(g=g, phi=phi, u=u, v=v, w=w, types=types)
@@ -41,8 +62,8 @@ def test_context_opens(mael, ctx):
     assert ctx.h
 
 
-def test_resample_bit_exact(mael, orc, ctx):
-    s, sc = make_basin(mael, orc)
+def test_resample_bit_exact(mael, orc, ctx, scene):
+    s, sc = make_basin(mael, orc, scene=scene)
     g = sc["g"]
     args = (sc["phi"], sc["u"], sc["v"], sc["w"])
     # seeding from empty, then the second pass that drops the NaN seeds (flip.jl:336-339)
@@ -68,9 +89,8 @@ def test_resample_bit_exact(mael, orc, ctx):
     ps = {f: ops2[f][perm].copy() for f in PF}
     ps["PX"] += rng.normal(0, 0.1, n2).astype(np.float32)
     ps["PZ"] += rng.normal(0, 0.1, n2).astype(np.float32)
-    ps["PX"][:40] = ps["PX"][40]
-    ps["PY"][:40] = ps["PY"][40]
-    ps["PZ"][:40] = ps["PZ"][40]
+    for f in ("PX", "PY", "PZ"):   # 40 on one kept particle's unjittered position
+        ps[f][:40] = ops2[f][0]
     ps["PY"][40:60] = np.nan
     s.set_particles(ps, n2)
     s.steps = 6
@@ -113,16 +133,16 @@ def test_capacity_error(mael, orc, ctx):
     assert s.n == 0
 
 
-def seeded(mael, orc):
-    s, sc = make_basin(mael, orc)
+def seeded(mael, orc, scene="testbasin"):
+    s, sc = make_basin(mael, orc, scene=scene)
     s.n = 0
     mael.resample(s, seeding=True)
     mael.resample(s)
     return s, sc
 
 
-def test_p2g_parity_and_determinism(mael, orc, ctx):
-    s, sc = seeded(mael, orc)
+def test_p2g_parity_and_determinism(mael, orc, ctx, scene):
+    s, sc = seeded(mael, orc, scene)
     n = s.n
     rng = np.random.default_rng(5)
     ps = gpu_particles(s)
@@ -152,8 +172,8 @@ def test_p2g_parity_and_determinism(mael, orc, ctx):
         assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
 
 
-def test_g2p_bit_exact(mael, orc, ctx):
-    s, sc = seeded(mael, orc)
+def test_g2p_bit_exact(mael, orc, ctx, scene):
+    s, sc = seeded(mael, orc, scene)
     n = s.n
     rng = np.random.default_rng(8)
     fields, flags = [], []
@@ -172,30 +192,77 @@ def test_g2p_bit_exact(mael, orc, ctx):
         assert np.array_equal(gp[f].view(np.uint32), ref[f].view(np.uint32)), f
 
 
-@pytest.mark.parametrize("tol", [1e-4, 1e-6])
-def test_projection_parity(mael, orc, ctx, tol):
-    s, sc = make_basin(mael, orc)
-    g = sc["g"]
-    dt = 0.05
+def _projection_case(mael, orc, scene, dt=0.05, gamma=0.001):
+    s, sc = make_basin(mael, orc, scene=scene)
     # gravity, then the walls (flip.jl:775-777): the projection's input velocities
     s.ctx.check(mael.lib().mael_dense_gravity(s.h, dt, 9.81))
     u, v, w = s.get("U"), s.get("V"), s.get("W")
-    gamma = 0.001
+    b = orc.dense_rhs(sc["g"], u, v, w, sc["types"], gamma)
+    return s, sc, (u, v, w), b
+
+
+@pytest.mark.parametrize("tol,refine", [(1e-6, 1), (1e-4, 0)])
+def test_projection_parity(mael, orc, ctx, scene, tol, refine):
+    """The pressure and the projected faces against the FP64 oracle's converged solve of
+    the same system: within 1e-5 relative L2 at 1e-6 (FP64 refinement), within the
+    residual's reach at the reference's FP32 PCG and its 1e-4 default; flags exact."""
+    dt, gamma = 0.05, 0.001
+    s, sc, (u, v, w), b = _projection_case(mael, orc, scene, dt, gamma)
+    g = sc["g"]
     s.set("P", np.zeros(orc.cells_shape(g), np.float32))
-    st = mael.project(s, dt, gamma=gamma, tol=tol, maxiter=200)
+    st = mael.project(s, dt, gamma=gamma, tol=tol, maxiter=200, refine=refine)
     assert st.rel_residual <= tol and st.iterations > 0
-    b = orc.dense_rhs(g, u, v, w, sc["types"], gamma)
     assert rel_l2(s.get("RHS"), b) < 1e-6
     P, it, rr = orc.dense_pressure(g, sc["types"], sc["phi"], b, tol=1e-13, maxiter=500)
     # the solution error is bounded by the residual times the conditioning
-    Pg = s.get("P")
-    err = rel_l2(Pg, P)
+    err = rel_l2(s.get("P"), P)
     assert err < (1e-5 if tol <= 1e-6 else 5e-3), err
     U, F = orc.dense_project_faces(g, P, sc["types"], sc["phi"], u, v, w)
     for name, ref in (("U", U[0]), ("V", U[1]), ("W", U[2])):
         assert rel_l2(s.get(name), ref) < (1e-5 if tol <= 1e-6 else 1e-3), name
     for name, ref in (("FU", F[0]), ("FV", F[1]), ("FW", F[2])):
         assert np.array_equal(s.get(name), ref), name
+
+
+def test_fp32_pcg_tracks_oracle(mael, orc, ctx, scene):
+    """The reference's plain FP32 PCG (no refinement, flip.jl:1075-1111), warm-started as
+    in a step: the same iteration count as the oracle's FP32 PCG on the same system (a
+    wrong preconditioner still converges, only in more iterations), and its recursive
+    residual at the tolerance while the true one sits at FP32's floor."""
+    dt, gamma = 0.05, 0.001
+    s, sc, (u, v, w), b = _projection_case(mael, orc, scene, dt, gamma)
+    g = sc["g"]
+    rng = np.random.default_rng(4)
+    P0 = (rng.normal(0, 1e-3, orc.cells_shape(g))).astype(np.float32)
+    s.set("P", P0)
+    st = mael.project(s, dt, gamma=gamma, tol=1e-6, maxiter=200)   # refine 0, warm start
+    _, it, rr = orc.dense_pressure(g, sc["types"], sc["phi"], b, P0=P0, tol=1e-6, maxiter=200, fp32=True)
+    assert abs(st.iterations - it) <= 1, (st.iterations, it)
+    assert st.rel_residual_rec <= 1e-6
+    assert st.rel_residual < 1e-4, st.rel_residual
+    # the drop-in default is the reference's: no refinement, warm start
+    opts = mael.dense.solve_options()
+    assert opts.refine == 0 and opts.warm == 1 and opts.tol == 1e-4 and opts.sweeps == 2
+
+
+@pytest.mark.parametrize("queued", [0, 1])
+def test_vcycle_matches_oracle(mael, orc, ctx, scene, queued):
+    """z = M r: the V-cycle preconditioner alone (flip.jl:986-1014) against the oracle's
+    FP64 V-cycle on the same r; the path with the first launches queued ahead of a host
+    read (as the CG loop runs it) gives the plain path's bits."""
+    s, sc = make_basin(mael, orc, scene=scene)
+    g = sc["g"]
+    rng = np.random.default_rng(9)
+    r = rng.normal(0, 1, orc.cells_shape(g)).astype(np.float32)
+    r[sc["types"] != 1] = 0
+    s.set("RHS", r)
+    s.ctx.check(mael.lib().mael_dense_vcycle(s.h, None, queued))
+    z = s.get("P")
+    zo = orc.dense_vcycle(g, sc["types"], sc["phi"], r)
+    print("V-cycle rel L2 vs FP64 oracle %.3e" % rel_l2(z, zo))
+    assert rel_l2(z, zo) < 1e-5, rel_l2(z, zo)
+    s.ctx.check(mael.lib().mael_dense_vcycle(s.h, None, 1 - queued))
+    assert np.array_equal(s.get("P").view(np.uint32), z.view(np.uint32))
 
 
 def test_initialize_and_steps_keep_water(mael, ctx):

@@ -354,6 +354,101 @@ int mael_sparse_set_flow(mael_sparse *s, int32_t reflected);
 /* `maxspeed(s)` (sflip.jl:2264-2272). */
 int mael_sparse_maxspeed(mael_sparse *s, float *speed);
 
+/* The rest of a dense step (flip.jl:1396-1410), support phases around the hot path: */
+/* `advect!(s, dt)` (flip.jl:490-499): the particles along (u, v, w) by ST-FLIP's own times
+ * (lag, jitter from hash01 with seed `s.steps`; ballistic where the flow was not carried
+ * out to), then u, v, w and φ semi-Lagrangian into u0, v0, w0 and φ0. */
+int mael_dense_advect(mael_dense *d, float dt, float gravity, int32_t seed);
+/* `filltrapped!(s; maxrounds)` (flip.jl:754-760): AIR not joined to the top layer's AIR
+ * becomes LIQUID, at least 2 dx deep in φ. */
+int mael_dense_filltrapped(mael_dense *d, int32_t maxrounds);
+/* The stirring model `Stir` (basin.jl:92-181; `surface` must be 0: the dense basin takes
+ * `Stir`) and its waves, as mael_sparse_set_stir. Without one, forces is gravity alone. */
+int mael_dense_set_stir(mael_dense *d, const mael_stir *stir, const float *waves, int32_t nwaves);
+/* `forces!(s, dt)` (flip.jl:786-798) at time `time`: gravity and the stir, then the walls. */
+int mael_dense_forces(mael_dense *d, float dt, float time, float gravity);
+/* `extrapolate!(s; layers)` (flip.jl:1194-1206): the valid faces' velocity `layers` faces
+ * out, the flags of the faces reached, zero beyond, then the walls. */
+int mael_dense_extrapolate(mael_dense *d, int32_t layers);
+/* `maxspeed(s)` (flip.jl:1387): the largest |u|, |v|, |w| on the faces. */
+int mael_dense_maxspeed(mael_dense *d, float *speed);
+
+/* ── the bake writers' device side (flip.jl:1846-1951, sflip.jl:2419-2440) ─── */
+/* `packband_kernel!` (flip.jl:1880-1893) for `record!(::BandParticles)`: each particle at
+ * the frame's time (x + lag v) kept if above -keep or within `throat` of the axis; the
+ * kept ones' positions and velocities as Float16 (IEEE binary16 bits, round to nearest
+ * even), 3 a particle, into host `pos` and `vel` (room for `cap` >= n particles), in
+ * particle order; `*count` how many. */
+int mael_dense_pack_band(mael_dense *d, float keep, float throat, uint16_t *pos, uint16_t *vel, int64_t cap, int64_t *count);
+int mael_sparse_pack_band(mael_sparse *s, float keep, float throat, uint16_t *pos, uint16_t *vel, int64_t cap,
+                          int64_t *count);
+/* `record!(::LevelSets, s)`: the dense level set averaged over cells twice as wide
+ * (flip.jl:1859-1867, (n1/2) x (n2/2) x (n3/2) bytes), or the sparse one sampled at the
+ * centres of cells 2 dx wide (sflip.jl:2427-2432, (8 nb0) x (8 nb1) x (8 nb2) bytes),
+ * clamped to +-range and quantised to 0..255, into host `out` (x fastest). */
+int mael_dense_quantize_levelset(mael_dense *d, float range, uint8_t *out);
+int mael_sparse_quantize_levelset(mael_sparse *s, float range, uint8_t *out);
+
+/* ── whitewater (flip.jl:1411-1511, sflip.jl:2320-2417, whitewater.jl) ───── */
+/* `WhitewaterRules` (whitewater.jl:30-44) and the whitewater's floor, gravity and the
+ * stream its waves ride (`carried`, whitewater.jl:306-310: stream 1 = the basin's swirl of
+ * circulation gamma_circ and core `core`, 0 = none). */
+typedef struct {
+    float air[2], crest[2], energy[2];   /* the potentials' clamp ranges */
+    float kair, kcrest;                  /* diffuse particles per second at full potential */
+    int32_t spray, bubble;               /* neighbour counts (the SPH path's classification) */
+    float life[2];                       /* lifetime range, s */
+    float buoyancy, drag, kick, tossed, within;
+    float floor, gravity;
+    int32_t stream;
+    float gamma_circ, core;
+} mael_white_rules;
+/* `Whitewater(b, backend; capacity, rules)` (whitewater.jl:255-300): a ring buffer of
+ * `capacity` diffuse particles (x, y, z, vx, vy, vz, life, kind), all dead. */
+typedef struct mael_white mael_white;
+int mael_white_create(mael_ctx *ctx, int64_t capacity, const mael_white_rules *rules, mael_white **out);
+void mael_white_destroy(mael_white *w);
+enum { MAEL_W_X = 0, MAEL_W_Y, MAEL_W_Z, MAEL_W_VX, MAEL_W_VY, MAEL_W_VZ, MAEL_W_LIFE /* float */,
+       MAEL_W_KIND /* uint8: 1 spray, 2 foam, 3 bubble */ };
+int mael_white_array(mael_white *w, int32_t which, void **dptr, int64_t *count, int32_t *elsize);
+int mael_white_upload(mael_white *w, int32_t which, const void *host, int64_t count);
+int mael_white_download(mael_white *w, int32_t which, void *host, int64_t count);
+/* The ring buffer's head (diffuse particles made so far) and the frames made. */
+int mael_white_state(mael_white *w, int64_t *head, int64_t *frames);
+int mael_white_set_state(mael_white *w, int64_t head, int64_t frames);
+/* `whitewater!(w, s, u, v, dt; substeps, near, above, below)` (flip.jl:1496-1511,
+ * sflip.jl:2401-2417): the frame's emission from the basin's particles through the cell
+ * runs of its last resample (offsets - totals), normals from the level set within
+ * `near` cells of the surface, the new particles in particle order from the head (the
+ * reference takes slots by an atomic counter), then `substeps` steps of the diffuse
+ * particles, spray above `above` cells, bubbles below `below`. */
+int mael_dense_whitewater(mael_dense *d, mael_white *w, double dt, int32_t substeps, float near, float above, float below);
+int mael_sparse_whitewater(mael_sparse *s, mael_white *w, double dt, int32_t substeps, float near, float above, float below);
+/* `packwhite_kernel!` (whitewater.jl:366-379) for `record!(::WhitewaterFrames)`: the live
+ * diffuse particles in slot order, (x, y, z, kind) and (vx, vy, vz) as Float16, into host
+ * `out` (4 a particle) and `vel` (3), room for `cap` >= capacity; `*count` how many. */
+int mael_white_pack(mael_white *w, uint16_t *out, uint16_t *vel, int64_t cap, int64_t *count);
+
+/* ── checkpoint and restore (checkpoint.jl:7-66) ─────────────────────────── */
+/* A versioned raw file of a basin's physical state (the reference writes Julia's
+ * `Serialization`, which only the same Julia version reads): a header, the host's
+ * metadata (JSON text, `meta`: the basin, grid, stirring model and clocks the reference
+ * keeps on its host), and the arrays a step reads (φ, types, faces and flags, P, the live
+ * particles; for the sparse basin the slot table, the live slots' blocks and the free
+ * list, the fine pools cut to the live slots). Written to `path`.tmp and renamed over
+ * `path`. `save_sparse_checkpoint(s, w, path)` (checkpoint.jl:7-35) is the sparse one. */
+int mael_dense_checkpoint(mael_dense *d, const char *path, const char *meta);
+int mael_sparse_checkpoint(mael_sparse *s, const char *path, const char *meta);
+/* The kind (0 dense, 1 sparse) and metadata of a checkpoint, to build the basin it is
+ * restored into; `buf` receives at most cap - 1 bytes and a NUL, `*len` the full length.
+ * Needs no device. */
+int mael_checkpoint_meta(const char *path, int32_t *kind, char *buf, int64_t cap, int64_t *len);
+/* Restore into a basin of the checkpoint's grid (a sparse one may have a larger pool and
+ * a dense or sparse one a larger particle capacity): the arrays, then φ0 = φ and u0 = u as
+ * `load_sparse_checkpoint` (checkpoint.jl:37-66) does. */
+int mael_dense_restore(mael_dense *d, const char *path);
+int mael_sparse_restore(mael_sparse *s, const char *path);
+
 /* ── launch timing (no counterpart in the reference, which has no profiler hooks) ── */
 /* Kernel ids for the launch timer. */
 enum {

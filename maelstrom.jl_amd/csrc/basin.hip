@@ -32,6 +32,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(unsigned long long*
     if (threadIdx.x == 0) bsums[nb] = carry;
 }
 
+void launch_scan_bsums(unsigned long long* bsums, int nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, s, bsums, nb);
+    MAEL_LAUNCH_CHECK();
+}
+
 DenseBasin::DenseBasin(const DGrid& grid, int lev, int64_t cap, int32_t bnd, int32_t nm)
     : g(grid), levels(lev), band(bnd), nmin(nm), capacity(cap) {
     size_t C = cells();

@@ -10,10 +10,6 @@
 #include <string>
 
 
-struct mael_dense {
-    mael_ctx* ctx;
-    std::unique_ptr<mael::DenseBasin> b;
-};
 
 namespace {
 
@@ -261,6 +257,41 @@ int mael_dense_solidify(mael_dense* d) {
 int mael_dense_gravity(mael_dense* d, float dt, float gravity) {
     if (!d) return MAEL_E_ARG;
     return guard(d->ctx, [&] { d->b->gravity(dt, gravity); });
+}
+int mael_dense_advect(mael_dense* d, float dt, float gravity, int32_t seed) {
+    if (!d || !(dt >= 0.0f)) return MAEL_E_ARG;
+    return guard(d->ctx, [&] { d->b->advect(dt, gravity, seed); });
+}
+int mael_dense_filltrapped(mael_dense* d, int32_t maxrounds) {
+    if (!d || maxrounds < 1) return MAEL_E_ARG;
+    return guard(d->ctx, [&] { d->b->filltrapped(maxrounds); });
+}
+int mael_dense_set_stir(mael_dense* d, const mael_stir* p, const float* waves, int32_t nwaves) {
+    if (!d || !p || nwaves < 0 || (nwaves && !waves) || p->surface) return MAEL_E_ARG;
+    return guard(d->ctx, [&] { d->b->set_stir(mael::stir_params(*p), waves, nwaves); });
+}
+int mael_dense_forces(mael_dense* d, float dt, float time, float gravity) {
+    if (!d) return MAEL_E_ARG;
+    return guard(d->ctx, [&] { d->b->forces(dt, time, gravity); });
+}
+int mael_dense_extrapolate(mael_dense* d, int32_t layers) {
+    if (!d || layers < 0) return MAEL_E_ARG;
+    return guard(d->ctx, [&] { d->b->extrapolate(layers); });
+}
+int mael_dense_maxspeed(mael_dense* d, float* speed) {
+    if (!d || !speed) return MAEL_E_ARG;
+    return guard(d->ctx, [&] { *speed = d->b->maxspeed(); });
+}
+int mael_dense_pack_band(mael_dense* d, float keep, float throat, uint16_t* pos, uint16_t* vel, int64_t cap, int64_t* count) {
+    if (!d || !pos || !vel || !count) return MAEL_E_ARG;
+    return guard(d->ctx, [&] {
+        if (cap < d->b->n) throw mael::Error(mael::E_ARG, "the packing buffers hold fewer than the particles");
+        *count = d->b->pack_band(keep, throat, pos, vel);
+    });
+}
+int mael_dense_quantize_levelset(mael_dense* d, float range, uint8_t* out) {
+    if (!d || !out || !(range > 0.0f)) return MAEL_E_ARG;
+    return guard(d->ctx, [&] { d->b->quantize_levelset(range, out); });
 }
 int mael_dense_copy(mael_dense* d, int32_t dst, int32_t src) {
     if (!d) return MAEL_E_ARG;

@@ -5,10 +5,6 @@
 #include <cmath>
 #include <memory>
 
-struct mael_sparse {
-    mael_ctx* ctx;
-    std::unique_ptr<mael::SparseBasin> b;
-};
 
 namespace {
 
@@ -336,15 +332,20 @@ int mael_sparse_set_stir(mael_sparse* s, const mael_stir* p, const float* waves,
     if (!s || !p || nwaves < 0 || nheights < 0 || (nwaves && !waves) || (nheights && !heights)) return MAEL_E_ARG;
     if (p->surface && nheights < 2) return MAEL_E_ARG;
     return guard(s->ctx, [&] {
-        mael::StirParams q;
-        q.radius = p->radius; q.band = p->band; q.relax = p->relax; q.core = p->core; q.Gamma = p->gamma_circ;
-        q.current[0] = p->current[0]; q.current[1] = p->current[1];
-        q.eddy = p->eddy; q.eddyscale = p->eddyscale; q.eddytime = p->eddytime; q.guide = p->guide; q.free = p->free;
-        q.turbulence = p->turbulence; q.boil = p->boil; q.reach = p->reach;
-        q.origin[0] = p->origin[0]; q.origin[1] = p->origin[1]; q.offset = p->offset;
-        q.surface = p->surface; q.hradius = p->hradius; q.ramp = p->ramp;
-        s->b->set_stir(q, waves, nwaves, heights, nheights);
+        s->b->set_stir(mael::stir_params(*p), waves, nwaves, heights, nheights);
     });
+}
+int mael_sparse_pack_band(mael_sparse* s, float keep, float throat, uint16_t* pos, uint16_t* vel, int64_t cap,
+                          int64_t* count) {
+    if (!s || !pos || !vel || !count) return MAEL_E_ARG;
+    return guard(s->ctx, [&] {
+        if (cap < s->b->n) throw mael::Error(mael::E_ARG, "the packing buffers hold fewer than the particles");
+        *count = s->b->pack_band(keep, throat, pos, vel);
+    });
+}
+int mael_sparse_quantize_levelset(mael_sparse* s, float range, uint8_t* out) {
+    if (!s || !out || !(range > 0.0f)) return MAEL_E_ARG;
+    return guard(s->ctx, [&] { s->b->quantize_levelset(range, out); });
 }
 int mael_sparse_forces(mael_sparse* s, float dt, float time, float gravity) {
     if (!s) return MAEL_E_ARG;

@@ -3,6 +3,8 @@
 // where a host value is returned (the resample total, solver statistics).
 #pragma once
 #include "common.hpp"
+#include "stir.hpp"
+#include <hip/hip_fp16.h>
 #include <chrono>
 #include <string>
 #include <thread>
@@ -221,7 +223,31 @@ struct DenseBasin {
     double volume();                                         // flip.jl:293
     void init_vortex(const float* eta, int neta, float rmax, float Gamma, float core);   // flip.jl:298-314
     void gravity(float dt, float g);                         // the gravity part of forces!, flip.jl:765-775
+    // the rest of a step (dense_step.hip)
+    DBuf<uint8_t> reach, flag2[3];                           // trapped air's reach; extrapolation's flags
+    DBuf<int32_t> changed;                                   // a device counter (trapped air, maxspeed)
+    StirParams stirp;                                        // the stirring model (basin.jl:92-181)
+    DBuf<float> stir_waves;
+    bool has_stir = false;
+    void advect(float dt, float gravity, int32_t seed);      // flip.jl:490-499
+    void filltrapped(int maxrounds);                         // flip.jl:754-760
+    void set_stir(const StirParams& p, const float* waves, int nwaves);
+    void forces(float dt, float time, float gravity);        // flip.jl:786-798
+    void extrapolate(int layers);                            // flip.jl:1194-1206
+    float maxspeed();                                        // flip.jl:1387
+    // the bake writers' device side (bake.hip): the kept band particles as Float16 into
+    // host `pos`, `vel` (3 a particle), returning how many; the level set as bytes
+    DBuf<__half> bake_pos, bake_vel;
+    DBuf<unsigned long long> bake_counts;
+    DBuf<uint8_t> bake_q;
+    int64_t pack_band(float keep, float throat, uint16_t* pos, uint16_t* vel);   // flip.jl:1880-1904
+    void quantize_levelset(float range, uint8_t* out);                          // flip.jl:1859-1874
 };
+
+// filltrapped!'s reach (flip.jl:690-745) on a dense grid of types: seed the top layer's
+// AIR, then sweep the lines along z, x and y until a round reaches nothing new (at most
+// `maxrounds`); `reach` then marks the AIR joined to the atmosphere.
+void sweep_trapped_air(uint8_t* reach, int* changed, const uint8_t* types, int3 n, int maxrounds, Stream& st);
 
 // scan helper shared by the dense and sparse resampling
 void launch_scan_bsums(unsigned long long* bsums, int nblocks, hipStream_t s);

@@ -13,6 +13,7 @@
 //    dense halvings of the coarse grid.
 #pragma once
 #include "dense.hpp"
+#include "stir.hpp"
 #include <array>
 
 namespace mael {
@@ -324,24 +325,6 @@ __device__ inline bool walledface_dev(const SGrid& g, const float p[3], int a, f
     return sissolid(g, lo[0], lo[1], lo[2]) || sissolid(g, hi[0], hi[1], hi[2]);
 }
 
-// The stirring model's parameters (basin.jl:13-30 Basin, :69-80 Race, :92-103 Stir,
-// :191-205 SurfaceStir); the waves (7 floats each: direction x, y, wave number, angular
-// frequency, amplitude already scaled by the race, steepness, phase) and the surface
-// table live in device arrays.
-struct StirParams {
-    float radius = 18, band = 3, relax = 0.3f, core = 2.5f, Gamma = 225;
-    float current[2] = {0, 0};
-    float eddy = 0.6f, eddyscale = 3.5f, eddytime = 4, guide = 4, free = 0.6f, turbulence = 0, boil = 0.5f, reach = 2.5f;
-    float origin[2] = {0, 0};
-    float offset = 0;
-    int surface = 0;                 // 0 Stir, 1 SurfaceStir
-    float hradius = 0, ramp = 2;     // SurfaceStir's table radius and onset ramp
-    const float* waves = nullptr;
-    int nwaves = 0;
-    const float* heights = nullptr;
-    int nheights = 0;
-};
-
 struct SFlow;
 
 // The mixed-precision hierarchy's storage (16-bit: FP16 or BF16 bit patterns), allocated
@@ -467,6 +450,12 @@ struct SparseBasin {
     void p2g(float dt);                                               // sflip.jl:808-831
     void g2p(float dt, float gravity);                                // sflip.jl:2141-2144
     void project(float gamma, const SparseOpts& o, SparseStats& s);   // sflip.jl:1998-2015
+    // the bake writers' device side (bake.hip), as the dense basin's
+    DBuf<__half> bake_pos, bake_vel;
+    DBuf<unsigned long long> bake_counts;
+    DBuf<uint8_t> bake_q;
+    int64_t pack_band(float keep, float throat, uint16_t* pos, uint16_t* vel);   // flip.jl:1880-1904
+    void quantize_levelset(float range, uint8_t* out);                          // sflip.jl:2427-2440
     // z = M b: the flexible CG's V-cycle preconditioner applied once to the right-hand
     // side array, into P (with `queued`, through the path the Krylov loop takes when the
     // V-cycle's first launches were queued ahead of a host read)

@@ -70,6 +70,14 @@ class SparseStats(C.Structure):
                 ("used_fgmres", C.c_int32), ("rel_residual", C.c_double), ("ms", C.c_double)]
 
 
+class WhiteRulesC(C.Structure):
+    _fields_ = [("air", C.c_float * 2), ("crest", C.c_float * 2), ("energy", C.c_float * 2), ("kair", C.c_float),
+                ("kcrest", C.c_float), ("spray", C.c_int32), ("bubble", C.c_int32), ("life", C.c_float * 2),
+                ("buoyancy", C.c_float), ("drag", C.c_float), ("kick", C.c_float), ("tossed", C.c_float),
+                ("within", C.c_float), ("floor", C.c_float), ("gravity", C.c_float), ("stream", C.c_int32),
+                ("gamma_circ", C.c_float), ("core", C.c_float)]
+
+
 class StirC(C.Structure):
     _fields_ = [("radius", C.c_float), ("band", C.c_float), ("relax", C.c_float), ("core", C.c_float),
                 ("gamma_circ", C.c_float), ("current", C.c_float * 2), ("eddy", C.c_float), ("eddyscale", C.c_float),
@@ -129,6 +137,12 @@ def lib():
         "mael_dense_solidify": (C.c_int, [p]),
         "mael_dense_gravity": (C.c_int, [p, f32, f32]),
         "mael_dense_copy": (C.c_int, [p, i32, i32]),
+        "mael_dense_advect": (C.c_int, [p, f32, f32, i32]),
+        "mael_dense_filltrapped": (C.c_int, [p, i32]),
+        "mael_dense_set_stir": (C.c_int, [p, C.POINTER(StirC), p, i32]),
+        "mael_dense_forces": (C.c_int, [p, f32, f32, f32]),
+        "mael_dense_extrapolate": (C.c_int, [p, i32]),
+        "mael_dense_maxspeed": (C.c_int, [p, C.POINTER(f32)]),
         "mael_dense_ktimer": (C.c_int, [p, i32, i32]),
         "mael_dense_ktime": (C.c_int, [p, C.POINTER(f64), C.POINTER(i32)]),
         "mael_dense_sync": (C.c_int, [p]),
@@ -172,6 +186,25 @@ def lib():
         "mael_sparse_extrapolate": (C.c_int, [p, i32, i32, i32]),
         "mael_sparse_set_flow": (C.c_int, [p, i32]),
         "mael_sparse_maxspeed": (C.c_int, [p, C.POINTER(f32)]),
+        "mael_dense_pack_band": (C.c_int, [p, f32, f32, p, p, i64, C.POINTER(i64)]),
+        "mael_sparse_pack_band": (C.c_int, [p, f32, f32, p, p, i64, C.POINTER(i64)]),
+        "mael_dense_quantize_levelset": (C.c_int, [p, f32, p]),
+        "mael_sparse_quantize_levelset": (C.c_int, [p, f32, p]),
+        "mael_white_create": (C.c_int, [p, i64, C.POINTER(WhiteRulesC), C.POINTER(p)]),
+        "mael_white_destroy": (None, [p]),
+        "mael_white_array": (C.c_int, [p, i32, C.POINTER(p), C.POINTER(i64), C.POINTER(i32)]),
+        "mael_white_upload": (C.c_int, [p, i32, p, i64]),
+        "mael_white_download": (C.c_int, [p, i32, p, i64]),
+        "mael_white_state": (C.c_int, [p, C.POINTER(i64), C.POINTER(i64)]),
+        "mael_white_set_state": (C.c_int, [p, i64, i64]),
+        "mael_white_pack": (C.c_int, [p, p, p, i64, C.POINTER(i64)]),
+        "mael_dense_whitewater": (C.c_int, [p, p, f64, i32, f32, f32, f32]),
+        "mael_sparse_whitewater": (C.c_int, [p, p, f64, i32, f32, f32, f32]),
+        "mael_dense_checkpoint": (C.c_int, [p, C.c_char_p, C.c_char_p]),
+        "mael_sparse_checkpoint": (C.c_int, [p, C.c_char_p, C.c_char_p]),
+        "mael_checkpoint_meta": (C.c_int, [C.c_char_p, C.POINTER(i32), p, i64, C.POINTER(i64)]),
+        "mael_dense_restore": (C.c_int, [p, C.c_char_p]),
+        "mael_sparse_restore": (C.c_int, [p, C.c_char_p]),
         "mael_sparse_ktimer": (C.c_int, [p, i32, i32]),
         "mael_sparse_ktime": (C.c_int, [p, C.POINTER(f64), C.POINTER(i32)]),
     }

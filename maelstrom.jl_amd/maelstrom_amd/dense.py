@@ -13,7 +13,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import ARRAY_ID, PARTICLE_FIELDS, DenseGrid, MaelstromError, SolveOpts, SolveStats
-from .basin import surface
+from .basin import Stir, surface
 
 _DTYPES = {1: np.uint8, 4: np.float32, 8: np.int64}
 _INT_ARRAYS = {"KEY", "RANK", "COUNTS", "TOTALS", "OFFSETS"}
@@ -30,15 +30,17 @@ def flipgrid(basin, dx, top=None, pad=2):
 
 
 class FlipBasin:
-    """flip.jl:189 `FlipBasin(b, backend; dx, stir, band = 4, nmin = 8, cfl = 2, capacity = 12_000_000)`.
+    """flip.jl:189 `FlipBasin(b, backend; dx, stir = Stir(b), band = 4, nmin = 8, cfl = 2, capacity = 12_000_000)`.
 
-    The stirring model is not on this path (only its gravity term is applied by `step`).
+    `stir` is the stirring model `forces!` applies (basin.jl:92-181 `Stir`); `stir=False`
+    leaves gravity alone.
     """
 
-    def __init__(self, basin, dx, band=4, nmin=8, cfl=2.0, capacity=12_000_000, ctx=None, top=None, pad=2):
+    def __init__(self, basin, dx, band=4, nmin=8, cfl=2.0, capacity=12_000_000, ctx=None, top=None, pad=2, stir=None):
         self.basin = basin
         self.ctx = ctx or _lib.default_context()
         self.grid = flipgrid(basin, dx, top, pad)
+        self.top, self.pad = top, pad
         self.band, self.nmin, self.cfl, self.capacity = int(band), int(nmin), float(cfl), int(capacity)
         self.h = C.c_void_p()
         self.ctx.check(_lib.lib().mael_dense_create(self.ctx.h, C.byref(self.grid), self.capacity, self.band,
@@ -48,6 +50,9 @@ class FlipBasin:
         self.steps = 0
         self.iterations = []
         self.last_solve = None
+        self.stir = None
+        if stir is not False:
+            set_stir(self, Stir(basin) if stir is None else stir)
 
     # ── arrays ─────────────────────────────────────────────────────────────
     @property
@@ -257,24 +262,61 @@ def initialize(s):
     return s
 
 
+def set_stir(s, stir):
+    """The stirring model `forces!` applies (basin.jl:92-103 `Stir`)."""
+    from .sparse import stir_struct
+    if stir.surface:
+        raise ValueError("the dense basin takes Stir, not SurfaceStir")
+    c, waves, nw, _ = stir_struct(stir)
+    s.ctx.check(_lib.lib().mael_dense_set_stir(s.h, C.byref(c), waves.ctypes.data if waves.size else None, nw))
+    s.stir = stir
+
+
 def maxspeed(s):
-    """flip.jl:1387 `maxspeed(s)`."""
-    return float(max(np.abs(s.get(f)).max() for f in ("U", "V", "W")))
+    """flip.jl:1387 `maxspeed(s)`: the largest face speed."""
+    v = C.c_float()
+    s.ctx.check(_lib.lib().mael_dense_maxspeed(s.h, C.byref(v)))
+    return v.value
+
+
+def advect(s, dt):
+    """flip.jl:490-499 `advect!(s, dt)`: the particles (ST-FLIP, seed `s.steps`), then the
+    grid's velocity and level set into u0, v0, w0 and φ0."""
+    s.ctx.check(_lib.lib().mael_dense_advect(s.h, float(dt), float(s.basin.g), int(s.steps)))
+    return s
+
+
+def filltrapped(s, maxrounds=32):
+    """flip.jl:754-760 `filltrapped!(s; maxrounds = 32)`."""
+    s.ctx.check(_lib.lib().mael_dense_filltrapped(s.h, int(maxrounds)))
+    return s
+
+
+def forces(s, dt):
+    """flip.jl:786-798 `forces!(s, dt)`: gravity and the stir at time `s.t`, then the walls."""
+    s.ctx.check(_lib.lib().mael_dense_forces(s.h, float(dt), float(s.t), float(s.basin.g)))
+    return s
+
+
+def extrapolate(s, layers=4):
+    """flip.jl:1194-1206 `extrapolate!(s; layers = 4)`."""
+    s.ctx.check(_lib.lib().mael_dense_extrapolate(s.h, int(layers)))
+    return s
 
 
 def step(s, dt, **solve):
-    """One step of the hot path in the reference's order (flip.jl:1396-1410): the grid
-    holds still in place of advection (φ0, u0 take φ, u), then particles to grid,
-    redistance, classify, gravity, projection, grid to particles, resample. Trapped air,
-    the stir and extrapolation are not on this path and are left out."""
-    s.copy("PHI0", "PHI")
-    for a, b in (("U0", "U"), ("V0", "V"), ("W0", "W")):
-        s.copy(a, b)
+    """flip.jl:1396-1410 `step!(s, dt)`: advect the particles and the grid, particles to
+    grid, redistance, classify, trapped air, forces, project, extrapolate six layers, grid
+    to particles, resample. `solve` overrides `pressure!`'s options. Returns the solve's
+    stats."""
+    advect(s, dt)
     particles2grid(s, dt)
     redistance(s)
     classify(s)
-    s.ctx.check(_lib.lib().mael_dense_gravity(s.h, float(dt), float(s.basin.g)))
+    filltrapped(s)
+    forces(s, dt)
     st = project(s, dt, **solve)
+    extrapolate(s, layers=6)
     grid2particles(s, dt)
     s.steps += 1
     resample(s)

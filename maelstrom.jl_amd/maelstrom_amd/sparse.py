@@ -43,6 +43,7 @@ class SparseBasin:
         self.basin = basin
         self.ctx = ctx or _lib.default_context()
         self.grid = sparsegrid(basin, dx, top, pad)
+        self.top, self.pad = top, pad
         self.band, self.nmin, self.cfl, self.capacity = int(band), int(nmin), float(cfl), int(capacity)
         self.blocks, self.rings, self.vary, self.reflect = int(blocks), int(rings), float(vary), bool(reflect)
         self.h = C.c_void_p()
@@ -319,8 +320,10 @@ def initialize(s):
     return s
 
 
-def set_stir(s, stir):
-    """The stirring model `forces!` applies (basin.jl:92-103 `Stir`, 191-205 `SurfaceStir`)."""
+def stir_struct(stir):
+    """The C ABI's form of a stirring model: the parameters (basin.jl:13-30 Basin, :69-80
+    Race, :92-103 `Stir`, 191-205 `SurfaceStir`), the waves at least `shortest` long scaled
+    by the race (basin.jl:101), 7 floats each, and SurfaceStir's surface table."""
     b, race = stir.basin, stir.race
     c = StirC()
     c.radius, c.band, c.relax, c.core, c.gamma_circ = b.radius, b.band, b.relax, b.core, b.Γ
@@ -331,7 +334,6 @@ def set_stir(s, stir):
     c.offset = stir.offset
     c.surface = 1 if stir.surface else 0
     c.ramp = stir.ramp
-    # basin.jl:101: the waves at least `shortest` long, scaled by the race
     w = [(wv.d[0], wv.d[1], wv.k, wv.omega, race.waves * wv.a, wv.q, wv.phase) for wv in stir.waves
          if 2 * math.pi / wv.k >= b.shortest]
     waves = np.ascontiguousarray(np.array(w, np.float32).reshape(-1), np.float32)
@@ -340,7 +342,13 @@ def set_stir(s, stir):
         c.hradius = float(np.float32(math.sqrt(2.0) * (b.radius + 1.0)))
         r = np.linspace(np.float32(0), np.float32(c.hradius), 2048, dtype=np.float32)
         heights = np.ascontiguousarray(surface(b, r), dtype=np.float32)
-    s.ctx.check(_lib.lib().mael_sparse_set_stir(s.h, C.byref(c), waves.ctypes.data if waves.size else None, len(w),
+    return c, waves, len(w), heights
+
+
+def set_stir(s, stir):
+    """The stirring model `forces!` applies (basin.jl:92-103 `Stir`, 191-205 `SurfaceStir`)."""
+    c, waves, nw, heights = stir_struct(stir)
+    s.ctx.check(_lib.lib().mael_sparse_set_stir(s.h, C.byref(c), waves.ctypes.data if waves.size else None, nw,
                                                 heights.ctypes.data if heights.size else None, heights.size))
     s.stir = stir
 

@@ -1,6 +1,8 @@
 // TEST INFRASTRUCTURE ONLY: C entry points of the CPU oracle for the Python tests
 // (ctypes). Nothing in the product links this library.
 #include "dense.hpp"
+#include "dense_step.hpp"
+#include "whitewater.hpp"
 #include <cstdio>
 #include <string>
 
@@ -130,6 +132,87 @@ int orc_dense_g2p(const orc_grid* g, orc_particles* ps, const float* u, const fl
     dense_g2p(G, s, U, F, dt, gravity);
     store(s, ps);
     return 0;
+}
+
+// flip.jl:438-464 (the particles) and 356-369 (the fields) of advect!.
+int orc_dense_advect_particles(const orc_grid* g, orc_particles* ps, const float* u, const float* v, const float* w,
+                               const uint8_t* fu, const uint8_t* fv, const uint8_t* fw, float dt, float gravity, int32_t seed) {
+    DenseGrid G = to_grid(g);
+    Particles s;
+    load(s, ps);
+    const float* U[3] = {u, v, w};
+    const uint8_t* F[3] = {fu, fv, fw};
+    dense_advect_particles(DenseFlow(G, U, F), s, dt, gravity, seed);
+    store(s, ps);
+    return 0;
+}
+void orc_dense_advect_grid(const orc_grid* g, const float* u, const float* v, const float* w, const float* src, float* out,
+                           int32_t n1, int32_t n2, int32_t n3, const float* off, float dt) {
+    DenseGrid G = to_grid(g);
+    const float* U[3] = {u, v, w};
+    dense_advect_grid(DenseFlow(G, U, nullptr), out, src, n1, n2, n3, off, dt);
+}
+// flip.jl:1194-1206 extrapolate! of component `a`, in place on (u, f).
+void orc_dense_extrapolate(const orc_grid* g, int32_t a, float* u, uint8_t* f, int32_t layers) {
+    dense_extrapolate(to_grid(g), a, u, f, layers);
+}
+// flip.jl:786-798 forces! with the Stir model (basin.jl:153-181); params: radius, band, relax,
+// core, gamma, current x, y, eddy, eddyscale, eddytime, guide, free, turbulence, boil, reach,
+// origin x, y, offset.
+void orc_dense_forces(const orc_grid* g, const float* params, const float* waves, int32_t nwaves, float* u, float* v,
+                      float* w, float dt, float t, float gravity) {
+    StirModel S;
+    const float* q = params;
+    S.radius = q[0]; S.band = q[1]; S.relax = q[2]; S.core = q[3]; S.gamma = q[4];
+    S.current[0] = q[5]; S.current[1] = q[6]; S.eddy = q[7]; S.eddyscale = q[8]; S.eddytime = q[9]; S.guide = q[10];
+    S.free = q[11]; S.turbulence = q[12]; S.boil = q[13]; S.reach = q[14]; S.origin[0] = q[15]; S.origin[1] = q[16];
+    S.offset = q[17];
+    S.waves = waves;
+    S.nwaves = nwaves;
+    float* U[3] = {u, v, w};
+    dense_forces(to_grid(g), S, U, dt, t, gravity);
+}
+// basin.jl:153-181 at one point: acc[3] for velocity vel[3] (params as orc_dense_forces).
+void orc_stir(const float* params, const float* waves, int32_t nwaves, float x, float y, float z, const float* vel, float t,
+              float* acc) {
+    StirModel S;
+    const float* q = params;
+    S.radius = q[0]; S.band = q[1]; S.relax = q[2]; S.core = q[3]; S.gamma = q[4];
+    S.current[0] = q[5]; S.current[1] = q[6]; S.eddy = q[7]; S.eddyscale = q[8]; S.eddytime = q[9]; S.guide = q[10];
+    S.free = q[11]; S.turbulence = q[12]; S.boil = q[13]; S.reach = q[14]; S.origin[0] = q[15]; S.origin[1] = q[16];
+    S.offset = q[17];
+    S.waves = waves;
+    S.nwaves = nwaves;
+    stir(S, x, y, z, vel, t, acc);
+}
+
+// The whitewater's rules from a float array: air 2, crest 2, energy 2, kair, kcrest, life 2,
+// buoyancy, drag, kick, tossed, within, floor, gravity, stream, gamma, core.
+WhiteRules white_rules(const float* q) {
+    WhiteRules R;
+    R.air[0] = q[0]; R.air[1] = q[1]; R.crest[0] = q[2]; R.crest[1] = q[3]; R.energy[0] = q[4]; R.energy[1] = q[5];
+    R.kair = q[6]; R.kcrest = q[7]; R.life[0] = q[8]; R.life[1] = q[9]; R.buoyancy = q[10]; R.drag = q[11];
+    R.kick = q[12]; R.tossed = q[13]; R.within = q[14]; R.floor = q[15]; R.gravity = q[16]; R.stream = (int)q[17];
+    R.gamma = q[18]; R.core = q[19];
+    return R;
+}
+WhiteBuf white_buf(float* wx, float* wy, float* wz, float* wvx, float* wvy, float* wvz, float* life, uint8_t* kind, int64_t cap,
+                   int64_t* hf) {
+    WhiteBuf W{{wx, wy, wz}, {wvx, wvy, wvz}, life, kind, cap, hf[0], hf[1]};
+    return W;
+}
+// flip.jl:1496-1511 whitewater! on the dense basin; hf = (head, frames) in and out.
+void orc_dense_whitewater(const orc_grid* g, const orc_particles* ps, const int32_t* offsets, const int32_t* totals,
+                          const float* phi, const float* u, const float* v, const float* w, const float* rules, float* wx,
+                          float* wy, float* wz, float* wvx, float* wvy, float* wvz, float* life, uint8_t* kind, int64_t cap,
+                          int64_t* hf, double dt, int32_t substeps, float near, float above, float below) {
+    WhiteBuf W = white_buf(wx, wy, wz, wvx, wvy, wvz, life, kind, cap, hf);
+    const float* X[3] = {ps->x[0], ps->x[1], ps->x[2]};
+    const float* V[3] = {ps->v[0], ps->v[1], ps->v[2]};
+    const float* U[3] = {u, v, w};
+    dense_whitewater(W, to_grid(g), X, V, ps->n, offsets, totals, phi, U, white_rules(rules), dt, substeps, near, above, below);
+    hf[0] = W.head;
+    hf[1] = W.frames;
 }
 
 void orc_dense_classify(const orc_grid* g, const float* phi, uint8_t* types) { dense_classify(to_grid(g), phi, types); }

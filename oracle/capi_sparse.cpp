@@ -2,6 +2,8 @@
 // Python tests (ctypes). A handle holds one basin's state; arrays are read and written
 // through pointers into it. Nothing in the product links this library.
 #include "sparse_state.hpp"
+#include "whitewater.hpp"
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -129,6 +131,38 @@ void orc_sparse_init_coarse(void* p, const float* eta, int32_t neta, float rmax,
 void orc_sparse_init_fine(void* p, const float* eta, int32_t neta, float rmax, float G, float core) {
     ((OrcSparse*)p)->s.init_fine(eta, neta, rmax, G, core);
 }
+// sflip.jl:2427-2432 sparsequantize_kernel!: the level set at the centres of cells `cell`
+// wide, clamped to +-range, as bytes (round to nearest even); out is n1 x n2 x n3, x fastest.
+void orc_sparse_quantize(void* p, float cell, float range, int32_t n1, int32_t n2, int32_t n3, uint8_t* out) {
+    auto& s = ((OrcSparse*)p)->s;
+    for (int k = 0; k < n3; ++k)
+        for (int j = 0; j < n2; ++j)
+            for (int i = 0; i < n1; ++i) {
+                float px = s.t.g.lo[0] + ((float)(i + 1) - 0.5f) * cell, py = s.t.g.lo[1] + ((float)(j + 1) - 0.5f) * cell,
+                      pz = s.t.g.lo[2] + ((float)(k + 1) - 0.5f) * cell;
+                float f = jclamp(cellsample(s.t, s.phi.fine.data(), s.phi.coarse.data(), px, py, pz) / range, -1.0f, 1.0f);
+                out[(size_t)i + (size_t)n1 * ((size_t)j + (size_t)n2 * k)] = (uint8_t)(int)std::nearbyint(127.5f * (f + 1.0f));
+            }
+}
+
+WhiteRules white_rules(const float* q);
+WhiteBuf white_buf(float* wx, float* wy, float* wz, float* wvx, float* wvy, float* wvz, float* life, uint8_t* kind, int64_t cap,
+                   int64_t* hf);
+// sflip.jl:2401-2417 whitewater! on the sparse basin's current state (its particles, φ and
+// u, v, w); `offsets`/`totals` of the resample that ordered the particles.
+void orc_sparse_whitewater(void* p, const int32_t* offsets, const int32_t* totals, const float* rules, float* wx, float* wy,
+                           float* wz, float* wvx, float* wvy, float* wvz, float* life, uint8_t* kind, int64_t cap, int64_t* hf,
+                           double dt, int32_t substeps, float near, float above, float below) {
+    auto& s = ((OrcSparse*)p)->s;
+    WhiteBuf W = white_buf(wx, wy, wz, wvx, wvy, wvz, life, kind, cap, hf);
+    const Particles& ps = s.ps[s.cur];
+    const float* X[3] = {ps.x[0].data(), ps.x[1].data(), ps.x[2].data()};
+    const float* V[3] = {ps.v[0].data(), ps.v[1].data(), ps.v[2].data()};
+    sparse_whitewater(W, s.t, X, V, ps.n, offsets, totals, s.phi, s.u, white_rules(rules), dt, substeps, near, above, below);
+    hf[0] = W.head;
+    hf[1] = W.frames;
+}
+
 void orc_sparse_classify(void* p) { ((OrcSparse*)p)->s.classify(); }
 void orc_sparse_solidify(void* p) { ((OrcSparse*)p)->s.solidify(); }
 void orc_sparse_restrict_velocity(void* p) { ((OrcSparse*)p)->s.restrict_velocity(); }

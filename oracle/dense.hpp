@@ -423,6 +423,7 @@ inline void dense_p2g(const DenseGrid& g, const Particles& ps, float dt, int32_t
     size_t C = g.cells();
     std::vector<int32_t> phip(C, orderkey(1e30f));
     std::vector<T> mom[3], wt[3];
+#pragma omp parallel for schedule(static, 1)
     for (int a = 0; a < 3; ++a) {
         mom[a].assign(g.faces(a), T(0)); wt[a].assign(g.faces(a), T(0));
         dense_p2g_component(g, ps, a, dt, mom[a].data(), wt[a].data());
@@ -461,6 +462,7 @@ inline void dense_g2p(const DenseGrid& g, Particles& ps, const float* u[3], cons
                               {u[2], g.n[0], g.n[1], g.n[2] + 1}};
     Arr3<const uint8_t> F[3] = {{f[0], g.n[0] + 1, g.n[1], g.n[2]}, {f[1], g.n[0], g.n[1] + 1, g.n[2]},
                                 {f[2], g.n[0], g.n[1], g.n[2] + 1}};
+#pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < ps.n; ++p) {
         float xp = ps.x[0][p], yp = ps.x[1][p], zp = ps.x[2][p];
         float vel[3], aff[3][3], fl, faff[3];
@@ -563,6 +565,7 @@ template <class T> struct DensePressure {
     // A x at level l (q = diag x - liquidsum on LIQUID cells, else 0).
     void apply(int l, const T* x, T* q) const {
         const MGLevel& L = lv[l];
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < L.n[2]; ++k)
             for (int j = 0; j < L.n[1]; ++j)
                 for (int i = 0; i < L.n[0]; ++i) {
@@ -573,6 +576,7 @@ template <class T> struct DensePressure {
     // flip.jl:859-863 residual_kernel!.
     void residual(int l, T* r, const T* x, const T* b) const {
         const MGLevel& L = lv[l];
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < L.n[2]; ++k)
             for (int j = 0; j < L.n[1]; ++j)
                 for (int i = 0; i < L.n[0]; ++i) {
@@ -584,6 +588,7 @@ template <class T> struct DensePressure {
     // 0-based cells with (i + j + k + 1) % 2 == colour.
     void rbgs(int l, T* x, const T* b, int colour) const {
         const MGLevel& L = lv[l];
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < L.n[2]; ++k)
             for (int j = 0; j < L.n[1]; ++j)
                 for (int i = 0; i < L.n[0]; ++i) {
@@ -597,6 +602,7 @@ template <class T> struct DensePressure {
     // flip.jl:964-973 start_kernel!: colour 0 from zero.
     void start(int l, T* x, const T* b) const {
         const MGLevel& L = lv[l];
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < L.n[2]; ++k)
             for (int j = 0; j < L.n[1]; ++j)
                 for (int i = 0; i < L.n[0]; ++i) {
@@ -613,6 +619,7 @@ template <class T> struct DensePressure {
     void restrict_(int l, T* bc, const T* rf) const {
         const MGLevel &F = lv[l], &Cl = lv[l + 1];
         static const T wB[4] = {T(0.125f), T(0.375f), T(0.375f), T(0.125f)};
+#pragma omp parallel for schedule(static)
         for (int K = 0; K < Cl.n[2]; ++K)
             for (int J = 0; J < Cl.n[1]; ++J)
                 for (int I = 0; I < Cl.n[0]; ++I) {

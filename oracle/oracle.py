@@ -65,6 +65,12 @@ def lib():
             "orc_dense_p2g": (C.c_int, [G, PP, f32, i32, i32] + [p] * 14),
             "orc_dense_p2g_f32": (C.c_int, [G, PP, f32, i32, i32, p, p, p, p, p, p, p, p]),
             "orc_dense_g2p": (C.c_int, [G, PP, p, p, p, p, p, p, f32, f32]),
+            "orc_dense_advect_particles": (C.c_int, [G, PP, p, p, p, p, p, p, f32, f32, i32]),
+            "orc_dense_advect_grid": (None, [G, p, p, p, p, p, i32, i32, i32, p, f32]),
+            "orc_dense_extrapolate": (None, [G, i32, p, p, i32]),
+            "orc_dense_forces": (None, [G, p, p, i32, p, p, p, f32, f32, f32]),
+            "orc_stir": (None, [p, p, i32, f32, f32, f32, p, f32, p]),
+            "orc_dense_whitewater": (None, [G, PP, p, p, p, p, p, p, p] + [p] * 8 + [i64, p, f64, i32, f32, f32, f32]),
             "orc_dense_classify": (None, [G, p, p]),
             "orc_dense_volume": (f64, [G, p]),
             "orc_dense_rhs": (None, [G, p, p, p, p, f32, p]),
@@ -219,6 +225,133 @@ def dense_g2p(g, ps, n, u, v, w, fu, fv, fw, dt, gravity=9.81):
     lib().orc_dense_g2p(C.byref(g), C.byref(cp), u.ctypes.data, v.ctypes.data, w.ctypes.data, fu.ctypes.data,
                         fv.ctypes.data, fw.ctypes.data, dt, gravity)
     return {f: keep[f][:n].copy() for f in PFIELDS}
+
+
+def dense_advect_particles(g, ps, n, u, v, w, fu, fv, fw, dt, gravity=9.81, seed=0):
+    """flip.jl:438-464 advectparticles_kernel!; returns the updated particle arrays."""
+    cp, keep = _cparts(ps, n, max(n, 1))
+    u, v, w = (_c(x, np.float32) for x in (u, v, w))
+    fu, fv, fw = (_c(x, np.uint8) for x in (fu, fv, fw))
+    lib().orc_dense_advect_particles(C.byref(g), C.byref(cp), u.ctypes.data, v.ctypes.data, w.ctypes.data, fu.ctypes.data,
+                                     fv.ctypes.data, fw.ctypes.data, C.c_float(dt), C.c_float(gravity), C.c_int32(seed))
+    return {f: keep[f][:n].copy() for f in PFIELDS}
+
+
+def dense_advect_grid(g, u, v, w, src, off, dt):
+    """flip.jl:356-369 advect_kernel!: `src` (samples at `off` in cells) traced back along (u, v, w)."""
+    u, v, w, src = (_c(x, np.float32) for x in (u, v, w, src))
+    out = np.zeros_like(src)
+    n3, n2, n1 = src.shape
+    o = np.asarray(off, np.float32)
+    lib().orc_dense_advect_grid(C.byref(g), u.ctypes.data, v.ctypes.data, w.ctypes.data, src.ctypes.data, out.ctypes.data,
+                                n1, n2, n3, o.ctypes.data, C.c_float(dt))
+    return out
+
+
+def dense_extrapolate(g, a, u, f, layers):
+    """flip.jl:1194-1206 extrapolate! of component `a`: returns (u, flags)."""
+    u, f = _c(u, np.float32).copy(), _c(f, np.uint8).copy()
+    lib().orc_dense_extrapolate(C.byref(g), a, u.ctypes.data, f.ctypes.data, layers)
+    return u, f
+
+
+STIR_KEYS = ("radius", "band", "relax", "core", "gamma", "current_x", "current_y", "eddy", "eddyscale", "eddytime",
+             "guide", "free", "turbulence", "boil", "reach", "origin_x", "origin_y", "offset")
+
+
+def stir_params(**kw):
+    """The Stir model's parameters in the order the oracle takes them (basin.jl:13-30 Basin
+    and :69-80 Race defaults, for the Basin(radius 18, depth 12, Γ 225, core 2.5) scene)."""
+    d = dict(radius=18.0, band=3.0, relax=0.3, core=2.5, gamma=225.0, current_x=0.0, current_y=0.0, eddy=0.6,
+             eddyscale=3.5, eddytime=4.0, guide=4.0, free=0.6, turbulence=0.0, boil=0.5, reach=2.5, origin_x=0.0,
+             origin_y=0.0, offset=0.0)
+    d.update(kw)
+    return np.array([d[k] for k in STIR_KEYS], np.float32)
+
+
+def dense_forces(g, params, waves, u, v, w, dt, t, gravity=9.81):
+    """flip.jl:786-798 forces! with `Stir`; returns (u, v, w)."""
+    u, v, w = (_c(x, np.float32).copy() for x in (u, v, w))
+    wv = _c(np.asarray(waves, np.float32).reshape(-1), np.float32)
+    lib().orc_dense_forces(C.byref(g), _c(params, np.float32).ctypes.data, wv.ctypes.data if wv.size else None,
+                           wv.size // 7, u.ctypes.data, v.ctypes.data, w.ctypes.data, C.c_float(dt), C.c_float(t),
+                           C.c_float(gravity))
+    return u, v, w
+
+
+def stir(params, waves, x, y, z, vel, t):
+    """basin.jl:153-181 at one point: the acceleration for velocity `vel`."""
+    wv = _c(np.asarray(waves, np.float32).reshape(-1), np.float32)
+    vel = _c(np.asarray(vel, np.float32), np.float32)
+    acc = np.zeros(3, np.float32)
+    lib().orc_stir(_c(params, np.float32).ctypes.data, wv.ctypes.data if wv.size else None, wv.size // 7, C.c_float(x),
+                   C.c_float(y), C.c_float(z), vel.ctypes.data, C.c_float(t), acc.ctypes.data)
+    return acc
+
+
+def dense_quantize(g, phi, range_):
+    """flip.jl:1859-1867 quantize_kernel!: the level set averaged over cells twice as wide
+    (the 8 summed in the reference's order, x fastest), clamped to +-range, as bytes
+    (round to nearest even). `phi` is (z, y, x); returns (z/2, y/2, x/2)."""
+    phi = np.asarray(phi, np.float32)
+    n3, n2, n1 = phi.shape
+    s = np.zeros((n3 // 2, n2 // 2, n1 // 2), np.float32)
+    for c in range(2):
+        for b in range(2):
+            for a in range(2):
+                s = s + phi[c:2 * (n3 // 2):2, b:2 * (n2 // 2):2, a:2 * (n1 // 2):2]
+    f = np.clip(np.float32(0.125) * s / np.float32(range_), np.float32(-1), np.float32(1))
+    return np.rint(np.float32(127.5) * (f + np.float32(1))).astype(np.uint8)
+
+
+def pack_band(ps, n, keep, throat):
+    """flip.jl:1880-1893 packband_kernel!: each particle at the frame's time (x + lag v) kept
+    if above -keep or within `throat` of the axis; the kept ones' positions and velocities
+    as Float16 (round to nearest even), in particle order, each (m, 3)."""
+    f = {k: np.asarray(ps[k][:n], np.float32) for k in PFIELDS}
+    lag = f["LAG"]
+    xp = [f["PX"] + lag * f["VX"], f["PY"] + lag * f["VY"], f["PZ"] + lag * f["VZ"]]
+    kept = (xp[2] > -np.float32(keep)) | (xp[0] * xp[0] + xp[1] * xp[1] < np.float32(throat) * np.float32(throat))
+    pos = np.stack([x[kept] for x in xp], axis=1).astype(np.float16)
+    vel = np.stack([f[k][kept] for k in ("VX", "VY", "VZ")], axis=1).astype(np.float16)
+    return pos, vel
+
+
+WHITE_KEYS = ("X", "Y", "Z", "VX", "VY", "VZ", "LIFE", "KIND")
+
+
+def white_rules(air=(5.0, 20.0), crest=(2.0, 8.0), energy=(2.0, 20.0), kair=40.0, kcrest=80.0, life=(2.0, 8.0),
+                buoyancy=2.0, drag=0.5, kick=0.0, tossed=1.0, within=np.inf, floor=-12.0, gravity=9.81, stream=1,
+                gamma=225.0, core=2.5):
+    """whitewater.jl:30-44 WhitewaterRules with the whitewater's floor, gravity and stream,
+    as the float array the oracle takes."""
+    return np.array([*air, *crest, *energy, kair, kcrest, *life, buoyancy, drag, kick, tossed, within, floor, gravity,
+                     stream, gamma, core], np.float32)
+
+
+def empty_white(capacity):
+    w = {k: np.zeros(capacity, np.float32) for k in WHITE_KEYS[:-1]}
+    w["KIND"] = np.zeros(capacity, np.uint8)
+    return w
+
+
+def _white_args(w):
+    return [w[k].ctypes.data for k in WHITE_KEYS]
+
+
+def dense_whitewater(g, ps, n, offsets, totals, phi, u, v, w, rules, white, head, frames, dt, substeps=4, near=1.5,
+                     above=0.25, below=0.75):
+    """flip.jl:1496-1511 whitewater! on the dense basin; `white` (dict of arrays) is updated
+    in place; returns (head, frames)."""
+    cp, keep = _cparts(ps, n, max(n, 1))
+    offsets, totals = _c(offsets, np.int32), _c(totals, np.int32)
+    phi, u, v, w = (_c(x, np.float32) for x in (phi, u, v, w))
+    cap = len(white["LIFE"])
+    hf = np.array([head, frames], np.int64)
+    lib().orc_dense_whitewater(C.byref(g), C.byref(cp), offsets.ctypes.data, totals.ctypes.data, phi.ctypes.data,
+                               u.ctypes.data, v.ctypes.data, w.ctypes.data, _c(rules, np.float32).ctypes.data,
+                               *_white_args(white), cap, hf.ctypes.data, float(dt), substeps, near, above, below)
+    return int(hf[0]), int(hf[1])
 
 
 def dense_classify(g, phi):
@@ -378,6 +511,8 @@ def _slib():
             "orc_sparse_refine": (C.c_int, [p, p, p, P32, p, P32]),
             "orc_sparse_init_coarse": (None, [p, p, i32, f32, f32, f32]),
             "orc_sparse_init_fine": (None, [p, p, i32, f32, f32, f32]),
+            "orc_sparse_quantize": (None, [p, f32, f32, i32, i32, i32, p]),
+            "orc_sparse_whitewater": (None, [p, p, p, p] + [p] * 8 + [i64, p, f64, i32, f32, f32, f32]),
             "orc_sparse_classify": (None, [p]),
             "orc_sparse_solidify": (None, [p]),
             "orc_sparse_restrict_velocity": (None, [p]),
@@ -497,6 +632,23 @@ class SparseOracle:
     def init_fine(self, eta, rmax, G, core):
         eta = _c(eta, np.float32)
         _slib().orc_sparse_init_fine(self.h, _ptr(eta), eta.size, rmax, G, core)
+
+    def quantize(self, cell, range_):
+        """sflip.jl:2427-2432 sparsequantize_kernel!, (8 nb2, 8 nb1, 8 nb0) bytes."""
+        nb = self.grid.nb
+        out = np.zeros((8 * nb[2], 8 * nb[1], 8 * nb[0]), np.uint8)
+        _slib().orc_sparse_quantize(self.h, C.c_float(cell), C.c_float(range_), 8 * nb[0], 8 * nb[1], 8 * nb[2],
+                                    out.ctypes.data)
+        return out
+
+    def whitewater(self, offsets, totals, rules, white, head, frames, dt, substeps=4, near=1.5, above=0.25, below=0.75):
+        """sflip.jl:2401-2417 whitewater! on this basin's state; returns (head, frames)."""
+        offsets, totals = _c(offsets, np.int32), _c(totals, np.int32)
+        hf = np.array([head, frames], np.int64)
+        _slib().orc_sparse_whitewater(self.h, offsets.ctypes.data, totals.ctypes.data, _c(rules, np.float32).ctypes.data,
+                                      *_white_args(white), len(white["LIFE"]), hf.ctypes.data, float(dt), substeps, near,
+                                      above, below)
+        return int(hf[0]), int(hf[1])
 
     def classify(self):
         _slib().orc_sparse_classify(self.h)

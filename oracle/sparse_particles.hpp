@@ -119,6 +119,7 @@ inline void sparse_p2g(const Topo& t, const Particles& ps, float dt, int32_t ban
     const SparseGrid& g = t.g;
     size_t Fn = (size_t)t.nslots * BFACES, Cn = (size_t)t.nslots * BCELLS;
     std::vector<T> mom[3], wt[3];
+#pragma omp parallel for schedule(static, 1)
     for (int a = 0; a < 3; ++a) {
         mom[a].assign(Fn, T(0)); wt[a].assign(Fn, T(0));
         float off[3]; faceoff(a, off);
@@ -192,6 +193,7 @@ inline void sparse_p2g(const Topo& t, const Particles& ps, float dt, int32_t ban
 
 // sflip.jl:2141-2144 grid2particles! over Flow{Topology} (sflip.jl:226-230 gathered).
 inline void sparse_g2p(const Topo& t, Particles& ps, const float* u[3], const uint8_t* f[3], float dt, float gravity) {
+#pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < ps.n; ++p) {
         float xp = ps.x[0][p], yp = ps.x[1][p], zp = ps.x[2][p];
         float vel[3], aff[3][3], found, fl, faff[3];

@@ -390,6 +390,7 @@ template <class T> struct SparsePressure {
     void jacobi(int N, Field<T>& dst, const Field<T>& src, const Field<T>& b) const {
         int64_t per = (int64_t)N * N * N;
         const uint8_t* act = factive(N);
+#pragma omp parallel for schedule(dynamic, 16)
         for (int s = 1; s <= ns; ++s) {
             if (!live(s)) continue;
             for (int k = 0; k < N; ++k)
@@ -405,6 +406,7 @@ template <class T> struct SparsePressure {
                         }
                     }
         }
+#pragma omp parallel for schedule(static)
         for (int K = 0; K < nc[2]; ++K)
             for (int J = 0; J < nc[1]; ++J)
                 for (int I = 0; I < nc[0]; ++I) {
@@ -421,6 +423,7 @@ template <class T> struct SparsePressure {
     void residual(int N, Field<T>& r, const Field<T>& x, const Field<T>& b) const {
         int64_t per = (int64_t)N * N * N;
         const uint8_t* act = factive(N);
+#pragma omp parallel for schedule(dynamic, 16)
         for (int s = 1; s <= ns; ++s) {
             if (!live(s)) continue;
             for (int k = 0; k < N; ++k)
@@ -436,6 +439,7 @@ template <class T> struct SparsePressure {
                         }
                     }
         }
+#pragma omp parallel for schedule(static)
         for (int K = 0; K < nc[2]; ++K)
             for (int J = 0; J < nc[1]; ++J)
                 for (int I = 0; I < nc[0]; ++I) {
@@ -451,6 +455,7 @@ template <class T> struct SparsePressure {
     }
     // q = A x at level 0 (sflip.jl:1824-1856): LIQUID fine cells, active uncovered coarse cells.
     void apply(Field<T>& q, const Field<T>& x) const {
+#pragma omp parallel for schedule(dynamic, 16)
         for (int s = 1; s <= ns; ++s) {
             if (!live(s)) continue;
             for (int c = 0; c < BCELLS; ++c) {
@@ -464,6 +469,7 @@ template <class T> struct SparsePressure {
                 }
             }
         }
+#pragma omp parallel for schedule(static)
         for (int K = 0; K < nc[2]; ++K)
             for (int J = 0; J < nc[1]; ++J)
                 for (int I = 0; I < nc[0]; ++I) {
@@ -493,6 +499,7 @@ template <class T> struct SparsePressure {
         c = L.c[2][cidx(nw, i, j, k + 1)]; if (c > 0 && k < n[2] - 1) { diag += c; off += c * x[cidx(n, i, j, k + 1)]; }
     }
     void jacobidense(DL& L, std::vector<T>& dst, const std::vector<T>& src) const {
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < L.n[2]; ++k)
             for (int j = 0; j < L.n[1]; ++j)
                 for (int i = 0; i < L.n[0]; ++i) {
@@ -504,6 +511,7 @@ template <class T> struct SparsePressure {
                 }
     }
     void residualdense(DL& L) const {
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < L.n[2]; ++k)
             for (int j = 0; j < L.n[1]; ++j)
                 for (int i = 0; i < L.n[0]; ++i) {
@@ -660,10 +668,12 @@ template <class T> struct SparsePressure {
         return s;
     }
     void axpy(Field<T>& y, T a, const Field<T>& x) const {   // y += a x over the used fine cells and all coarse
+#pragma omp parallel for schedule(static)
         for (size_t n = 0; n < nf(); ++n) y.fine[n] += a * x.fine[n];
         for (size_t c = 0; c < y.coarse.size(); ++c) y.coarse[c] += a * x.coarse[c];
     }
     void xpby(Field<T>& y, const Field<T>& x, T b) const {   // y = x + b y
+#pragma omp parallel for schedule(static)
         for (size_t n = 0; n < nf(); ++n) y.fine[n] = x.fine[n] + b * y.fine[n];
         for (size_t c = 0; c < y.coarse.size(); ++c) y.coarse[c] = x.coarse[c] + b * y.coarse[c];
     }

@@ -120,6 +120,16 @@ int mael_dense_set_count(mael_dense *d, int64_t n);
  * particles than the capacity (then the particles are unchanged). */
 int mael_dense_resample(mael_dense *d, int32_t seeding, uint32_t seed, int64_t *total);
 
+/* mael_dense_resample for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). It has no
+ * counterpart in the reference. The same result as mael_dense_resample, bit for bit; the
+ * packed scan's tile sums live in a static device array of the library (128 KiB per device,
+ * held until the process ends) instead of the basin's buffer, so the call allocates
+ * nothing. Limit: grids of at most 16,384 x 1024 cells (16.7 M); larger ones get
+ * MAEL_E_ARG. All such calls on one device share that array: calls in flight at once on
+ * one device (basins on different streams) race. */
+int mael_dense_resample_noalloc(mael_dense *d, int32_t seeding, uint32_t seed, int64_t *total);
+
 /* `particles2grid!(s, dt)` (flip.jl:587-606): APIC deposit (deterministic), splat,
  * union with φ0 + dx, combine with u0 within (band - 1) dx of the surface. */
 int mael_dense_p2g(mael_dense *d, float dt);

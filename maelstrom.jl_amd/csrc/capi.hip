@@ -68,6 +68,7 @@ int mael_create(int device, mael_ctx** out) {
         if (device < 0 || device >= count)
             throw mael::Error(mael::E_ARG, "no HIP device " + std::to_string(device));
         MAEL_HIP(hipSetDevice(device));
+        mael::noalloc_setup(device);   // the static scratch of mael_dense_resample_noalloc
     });
     if (st != MAEL_OK) {
         delete c;
@@ -178,6 +179,20 @@ int mael_dense_resample(mael_dense* d, int32_t seeding, uint32_t seed, int64_t* 
     if (!d) return MAEL_E_ARG;
     return guard(d->ctx, [&] {
         int64_t t = d->b->resample(seeding != 0, seed);
+        if (total) *total = t;
+    });
+}
+int mael_dense_resample_noalloc(mael_dense* d, int32_t seeding, uint32_t seed, int64_t* total) {
+    if (!d) return MAEL_E_ARG;
+    return guard(d->ctx, [&] {
+        unsigned long long* tilesums = mael::noalloc_scratch(d->ctx->device);
+        if (!tilesums) throw mael::Error(mael::E_ARG, "the static scratch is not set up on this device");
+        size_t need = d->b->cells() / mael::SCAN_TILE + 2;
+        if (need > mael::NOALLOC_TILE_SUMS)
+            throw mael::Error(mael::E_ARG, "resampling without allocation needs " + std::to_string(need) +
+                                               " scan tile sums, more than the " +
+                                               std::to_string(mael::NOALLOC_TILE_SUMS) + " of its static scratch");
+        int64_t t = d->b->resample(seeding != 0, seed, tilesums);
         if (total) *total = t;
     });
 }

@@ -197,7 +197,9 @@ struct DenseBasin {
     DenseBasin(const DGrid& grid, int levels, int64_t capacity, int32_t band, int32_t nmin);
 
     // phases
-    int64_t resample(bool seeding, uint32_t seed);          // flip.jl:1364-1382
+    // flip.jl:1364-1382; `tilesums` (nullptr: the basin's bsums) receives the packed
+    // scan's tile sums, cells() / SCAN_TILE + 1 of them
+    int64_t resample(bool seeding, uint32_t seed, unsigned long long* tilesums = nullptr);
     void p2g(float dt);                                      // flip.jl:587-606
     void g2p(float dt, float gravity);                       // flip.jl:1246-1252
     void project(float dt, float gamma, const SolveOpts& o, SolveStats& s);   // flip.jl:1144-1156
@@ -243,6 +245,14 @@ struct DenseBasin {
     int64_t pack_band(float keep, float throat, uint16_t* pos, uint16_t* vel);   // flip.jl:1880-1904
     void quantize_levelset(float range, uint8_t* out);                          // flip.jl:1859-1874
 };
+
+// The resampling scan's tile sums for callers that cannot allocate: a file-scope device
+// array (resample.hip) of NOALLOC_TILE_SUMS entries, one copy per device.
+constexpr size_t NOALLOC_TILE_SUMS = 16386;
+// Resolve the array's address on `device` (current); once per device, at context set-up.
+void noalloc_setup(int device);
+// Its address on `device`, or nullptr if noalloc_setup has not run there.
+unsigned long long* noalloc_scratch(int device);
 
 // filltrapped!'s reach (flip.jl:690-745) on a dense grid of types: seed the top layer's
 // AIR, then sweep the lines along z, x and y until a round reaches nothing new (at most

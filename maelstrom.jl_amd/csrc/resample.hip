@@ -11,6 +11,7 @@
 // same on every run.
 #include "dense.hpp"
 #include "scan.hpp"
+#include <mutex>
 
 namespace mael {
 
@@ -163,7 +164,37 @@ __global__ void k_dense_spawn(PView dst, const int32_t* __restrict__ counts, con
     }
 }
 
-int64_t DenseBasin::resample(bool seeding, uint32_t seed) {
+// ─── the scan's tile sums without allocation ─────────────────────────────────
+// For callers that cannot call hipMalloc (code under HIP graph capture, tight serving
+// loops): mael_dense_resample_noalloc keeps the packed scan's tile sums in this array
+// instead of the basin's buffer. 16,386 entries (128 KiB) hold the 16,384 tiles of
+// 1024 cells of grids up to 16.7 M cells, plus the grand total. In host code `scratch`
+// names a host-side stand-in, not a device address: each device holds its own copy, whose
+// address hipGetSymbolAddress returns once per device (noalloc_setup, run by mael_create
+// with the device current), and only that address is passed to kernels. Every call on a
+// device shares its copy: the scan writes every entry it reads before reading it, so
+// nothing needs clearing, but two such calls in flight at once on one device race.
+static __device__ __align__(256) unsigned long long scratch[NOALLOC_TILE_SUMS];
+constexpr int NOALLOC_DEVICES = 64;
+static std::mutex noalloc_mu;
+static unsigned long long* noalloc_ptr[NOALLOC_DEVICES] = {};
+
+void noalloc_setup(int device) {
+    if (device < 0 || device >= NOALLOC_DEVICES) return;
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (noalloc_ptr[device]) return;
+    void* p = nullptr;
+    MAEL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    noalloc_ptr[device] = (unsigned long long*)p;
+}
+unsigned long long* noalloc_scratch(int device) {
+    if (device < 0 || device >= NOALLOC_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    return noalloc_ptr[device];
+}
+
+int64_t DenseBasin::resample(bool seeding, uint32_t seed, unsigned long long* tilesums) {
+    if (!tilesums) tilesums = bsums.p;
     hipStream_t s = st.s;
     size_t C = cells();
     float deep = (float)band * g.dx;
@@ -177,7 +208,7 @@ int64_t DenseBasin::resample(bool seeding, uint32_t seed) {
         MAEL_LAUNCH_CHECK();
     }
     DenseCells cellsinfo{counts.p, phi.p, g, nmin, deep, seeding ? 1 : 0};
-    int64_t total = scan_cells(cellsinfo, C, totals.p, offsets.p, cstart.p, bsums.p, (unsigned long long*)st.pinned, s);
+    int64_t total = scan_cells(cellsinfo, C, totals.p, offsets.p, cstart.p, tilesums, (unsigned long long*)st.pinned, s);
     if (total > capacity)
         throw Error(E_CAPACITY, "the narrow band needs " + std::to_string(total) + " particles, more than the " +
                                     std::to_string(capacity) + " allocated");

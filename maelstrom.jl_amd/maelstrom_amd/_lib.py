@@ -123,6 +123,7 @@ def lib():
         "mael_dense_count": (C.c_int, [p, C.POINTER(i64)]),
         "mael_dense_set_count": (C.c_int, [p, i64]),
         "mael_dense_resample": (C.c_int, [p, i32, u32, C.POINTER(i64)]),
+        "mael_dense_resample_noalloc": (C.c_int, [p, i32, u32, C.POINTER(i64)]),
         "mael_dense_p2g": (C.c_int, [p, f32]),
         "mael_dense_g2p": (C.c_int, [p, f32, f32]),
         "mael_dense_project": (C.c_int, [p, f32, f32, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),

@@ -18,6 +18,11 @@ import os
 import sys
 import time
 
+
+def log(msg):
+    """A progress line on stderr (stdout carries only the result line)."""
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "maelstrom.jl_amd"))
 
@@ -57,8 +62,17 @@ def sparse_bytes(kernel, live_cells, n_particles, precision=0, coarse_cells=0, c
         # the blocks: x, b, the Dirichlet diagonal and the result (e B each), the type, the
         # activity and the cover mask (1 B each); under a block: activity, mask, result.
         "jacobi": (4 * e + 1) * live_cells + (4 * e + 3) * (coarse_cells - covered) + (e + 2) * covered,
+        # the flexible CG's search (FP32): z, d read, d, q written, the full diagonal read
+        # (4 B each) and the type (1 B) per fine cell; per coarse cell outside the blocks the
+        # same and the activity and cover mask; under a block activity, mask and q
+        "search": 21 * live_cells + 23 * (coarse_cells - covered) + 6 * covered,
+        # the flexible CG's update: P, r read and written, r_old written, d, q read (4 B
+        # each) and the type (1 B) per unknown; the cover mask too on the coarse grid
+        "advance": 29 * live_cells + 30 * coarse_cells,
         "g2p": 60 * n_particles,
         "p2g": 64 * n_particles,
+        # x (12 B) read, key and rank (8 B) written per particle
+        "bin": 20 * n_particles,
         "scatter": 136 * n_particles,
     }[kernel]
 
@@ -132,16 +146,27 @@ def cpu_baseline(s, iters_per_step, dt):
     per_iter = (t2 - t1 - (t1 - t0)) / 3
     setup = (t1 - t0) - per_iter
     ms = 1e3 * (t["p2g"] + t["g2p"] + t["resample"] + setup + per_iter * iters_per_step)
-    sample = ("oracle FP32 port, one core: P2G, G2P and resample over all %d particles timed once; MG-PCG set-up "
-              "plus 1 and 4 iterations timed, per-iteration cost scaled to the GPU's %.1f iterations/step "
-              "(%.1f s of CPU work)" % (n, iters_per_step, (t2 - t0) + t["p2g"] + t["g2p"] + t["resample"]))
-    return {"value": round(ms, 1), "unit": "ms/step", "cores": 1, "kind": "port", "sample": sample}
+    cores = host_cores()
+    sample = ("oracle FP32 port on %d host threads (OpenMP stencils and particle loops, sequential resampling): "
+              "P2G, G2P and resample over all %d particles timed once; MG-PCG set-up plus 1 and 4 iterations timed, "
+              "per-iteration cost scaled to the GPU's %.1f iterations/step (%.1f s of CPU work)"
+              % (cores, n, iters_per_step, (t2 - t0) + t["p2g"] + t["g2p"] + t["resample"]))
+    return {"value": round(ms, 1), "unit": "ms/step", "cores": cores, "kind": "port", "sample": sample}
 
 
-def sparse_cpu_baseline(s, iters_per_step, dt, sample_particles=1_000_000):
-    """The oracle's FP32 CPU port on the same sparse state (one core): P2G, G2P and
-    resample on a sample of the particles (scaled to all), the pressure set-up and flexible
-    CG iterations on the whole grid (scaled to the GPU's iterations per step)."""
+def host_cores():
+    """The threads the oracle's OpenMP loops run on: OMP_NUM_THREADS if set, else every core."""
+    try:
+        return int(os.environ.get("OMP_NUM_THREADS", "")) or os.cpu_count()
+    except ValueError:
+        return os.cpu_count()
+
+
+def sparse_cpu_baseline(s, iters_per_step, dt, sample_particles=2_000_000):
+    """The oracle's FP32 CPU port on the same sparse state, on the host's cores (its
+    stencil, V-cycle and particle loops are OpenMP-parallel; the resampling is sequential):
+    P2G, G2P and resample on a sample of the particles (scaled to all), the pressure set-up
+    and flexible CG iterations on the whole grid (scaled to the GPU's iterations per step)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as orc
@@ -177,17 +202,19 @@ def sparse_cpu_baseline(s, iters_per_step, dt, sample_particles=1_000_000):
     t0 = time.perf_counter()
     o.project(gamma=0.0, tol=1e-30, maxiter=1, policy=0, fp32=True)
     t1 = time.perf_counter()
-    o.project(gamma=0.0, tol=1e-30, maxiter=3, policy=0, fp32=True)
+    o.project(gamma=0.0, tol=1e-30, maxiter=5, policy=0, fp32=True)
     t2 = time.perf_counter()
-    per_iter = (t2 - t1 - (t1 - t0)) / 2
+    per_iter = (t2 - t1 - (t1 - t0)) / 4
     setup = max((t1 - t0) - per_iter, 0.0)
     halves = 2 if s.reflect else 1
     ms = 1e3 * (halves * ((t["p2g"] + t["g2p"] + t["resample"]) * scale + setup) + per_iter * iters_per_step)
-    sample = ("oracle FP32 port, one core, the hot path only: P2G, G2P and resample on %d of the %d particles "
-              "(scaled by %.1f), the pressure set-up and 1 and 3 flexible-CG iterations on the whole grid, the "
+    cores = host_cores()
+    sample = ("oracle FP32 port on %d host threads (OpenMP stencils, V-cycle and particle loops; sequential "
+              "resampling and dot products), the hot path only: P2G, G2P and resample on %d of the %d particles "
+              "(scaled by %.1f), the pressure set-up and 1 and 5 flexible-CG iterations on the whole grid, the "
               "per-iteration cost scaled to the GPU's %.1f iterations a step, %d half-steps a step (%.1f s of CPU "
-              "work)" % (m, n, scale, iters_per_step, halves, (t2 - t0) + t["p2g"] + t["g2p"] + t["resample"]))
-    return {"value": round(ms, 1), "unit": "ms/step", "cores": 1, "kind": "port", "sample": sample}
+              "work)" % (cores, m, n, scale, iters_per_step, halves, (t2 - t0) + t["p2g"] + t["g2p"] + t["resample"]))
+    return {"value": round(ms, 1), "unit": "ms/step", "cores": cores, "kind": "port", "sample": sample}
 
 
 def sparse_mp_error(s, dt, tol, refine, oracle_max_cells=4_000_000):
@@ -279,22 +306,27 @@ def sparse_main(args, desc, kw, dx):
     # a 12000-block pool: the stirred surface's band outgrows the default 8000 within ~20 steps
     s = m.SparseBasin(b, dx=dx, stir=stir, band=4, nmin=8, cfl=4.0, capacity=30_000_000, blocks=12000)
     sp.initialize(s)
+    log("initialized: %d particles" % s.n)
     umax = max(float(np.abs(s.get(f)).max()) for f in ("U_F", "V_F", "W_F", "U_C", "V_C", "W_C"))
     left = 1.0 / 24
     dt = left / math.ceil(left / (s.cfl * dx / max(umax, 1e-3)))
     def solve_for(precision):
         return dict(tol=args.tol, maxiter=240, restart=20, policy=2, boost=1.5, precision=precision, refine=args.refine)
 
-    def timed(precision, steps, kernel_id):
-        """`steps` whole steps between two synchronisations; the kernel timer on `kernel_id`."""
+    def timed(s, precision, steps, kernel_id):
+        """`steps` whole steps between two synchronisations; the kernel timer on `kernel_id`.
+        Each step ends in the resample's host read of its total, so the per-step wall times
+        (for the median) need no extra synchronisation."""
         solve = solve_for(precision)
         s.sync()
         s.ctx.check(L.mael_sparse_ktimer(s.h, kernel_id, 64 * 1024))
-        iters, fg, pms, nparts = [], [], [], []
+        iters, fg, pms, nparts, each = [], [], [], [], []
         t0 = time.perf_counter()
         for _ in range(steps):
             k0 = len(s.iterations)
+            ts = time.perf_counter()
             st = sp.step(s, dt, **solve)
+            each.append(time.perf_counter() - ts)
             iters.append(sum(s.iterations[k0:]))   # both half-steps' solves
             fg.append(st.used_fgmres)
             pms.append(sum(s.solve_ms[k0:]))
@@ -303,15 +335,26 @@ def sparse_main(args, desc, kw, dx):
         t1 = time.perf_counter()
         kms, kl = C.c_double(), C.c_int32()
         s.ctx.check(L.mael_sparse_ktime(s.h, C.byref(kms), C.byref(kl)))
-        return dict(ms=1e3 * (t1 - t0) / steps, pressure_ms=sum(pms) / steps, iterations=sum(iters) / steps,
-                    fgmres=int(sum(fg)), particles=sum(nparts) / steps, kms=kms.value, launches=kl.value)
+        return dict(ms=1e3 * (t1 - t0) / steps, median_ms=1e3 * float(np.median(each)), pressure_ms=sum(pms) / steps,
+                    iterations=sum(iters) / steps, fgmres=int(sum(fg)), particles=sum(nparts) / steps, kms=kms.value,
+                    launches=kl.value)
 
     L = m.lib()
-    for _ in range(args.warmup):
+    for w in range(args.warmup):
         sp.step(s, dt, **solve_for(args.precision))
+        log("warm-up step %d done" % (w + 1))
+    # the state the timed steps start from, for the FP32 comparison from the same state
+    import shutil
+    import tempfile
+    shm = os.path.isdir("/dev/shm") and shutil.disk_usage("/dev/shm").free > 16 << 30
+    tmpdir = tempfile.mkdtemp(dir="/dev/shm" if shm else None)
+    start = os.path.join(tmpdir, "start.ckpt")
+    s.t = round(s.t * 24) / 24   # a whole number of frames (the steps are frames' sub-steps)
+    m.save_checkpoint(s, start)
     kernel = "jacobi" if args.kernel in ("rbgs", "jacobi") else args.kernel
     kid = KERNEL_ID["rbgs" if kernel == "jacobi" else kernel]
-    R = timed(args.precision, args.steps, kid)
+    R = timed(s, args.precision, args.steps, kid)
+    log("timed %d steps: %.1f ms/step" % (args.steps, R["ms"]))
     slot, where, ns, free = s.topology()
     live_slots = int((where[:ns, 0] >= 0).sum())
     ms_step = R["ms"]
@@ -343,18 +386,60 @@ def sparse_main(args, desc, kw, dx):
                      "launches": R["launches"], "timed_every": KTIMER_STRIDE, "avg_launch_us": round(avg_launch_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": int(abytes)},
     }
+    out["ms_per_step_median"] = round(R["median_ms"], 3)
+    # the other hot kernels, each timed over one more step (their launches' mean duration)
+    census = {}
+    for name in ("search", "advance", "p2g", "g2p", "bin"):
+        Q = timed(s, args.precision, 1, KERNEL_ID[name])
+        if Q["launches"]:
+            us = 1e3 * Q["kms"] / Q["launches"]
+            ab = sparse_bytes(name, live_slots * 4096, s.n, args.precision, nc[0] * nc[1] * nc[2], live_slots * 64)
+            gbs = ab / (us * 1e-6) / 1e9
+            census[name] = {"avg_launch_us": round(us, 2), "algorithmic_bytes": int(ab), "achieved": round(gbs, 1),
+                            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    out["kernels"] = census
+    log("kernel census done")
+    # the blocks the marks ask for at the state reached, by kind (sflip.jl:518-526): the
+    # surface's, then with the flow's variation, then within reach of a particle
+    reach = float(np.float32((math.ceil(s.cfl) + 3) * np.float32(s.grid.dx)))
+    out["config"]["marked_blocks"] = {
+        "surface": int(sp.marks(s, vary=1e30, particles=False, rings=0).sum()),
+        "surface_flow": int(sp.marks(s, particles=False, rings=0).sum()),
+        "surface_flow_particles": int(sp.marks(s, rings=0).sum()),
+        "reach_m": round(reach, 3)}
     if args.precision != 0 and not args.no_fp32:
-        # the same steps with the FP32 V-cycle (the same FP64 refinement), from the state the
-        # mixed-precision steps left: the comparison the metric names
-        F = timed(0, args.steps, kid)
-        out["fp32"] = {"ms_per_step": round(F["ms"], 3), "pressure_ms": round(F["pressure_ms"], 3),
-                       "iterations": round(F["iterations"], 2),
-                       "jacobi_avg_launch_us": round(1e3 * F["kms"] / max(F["launches"], 1), 3)}
+        # the same steps with the FP32 V-cycle (the same FP64 refinement) from the same state
+        # as the mixed-precision ones: the comparison the metric names
+        del s
+        f = m.load_checkpoint(start, blocks=12000)
+        F = timed(f, 0, args.steps, kid)
+        log("FP32 V-cycle steps: %.1f ms/step" % F["ms"])
+        out["fp32"] = {"ms_per_step": round(F["ms"], 3), "ms_per_step_median": round(F["median_ms"], 3),
+                       "pressure_ms": round(F["pressure_ms"], 3), "iterations": round(F["iterations"], 2),
+                       "jacobi_avg_launch_us": round(1e3 * F["kms"] / max(F["launches"], 1), 3),
+                       "from": "the same state as the mixed-precision steps (a checkpoint after the warm-up)"}
         out["speedup_vs_fp32"] = {"step": round(F["ms"] / ms_step, 3),
                                   "pressure": round(F["pressure_ms"] / max(R["pressure_ms"], 1e-9), 3)}
+        # the reference's plain FP32 solve (no refinement) on the next half-step's system
+        sp.before_projection(f, dt / 2, reflect=True)
+        note = ("the reference's FP32 flexible CG then FGMRES(20), no FP64 refinement, at most 240 iterations, "
+                "on the next half-step's system")
+        try:
+            st = sp.project(f, dt / 2, tol=args.tol, maxiter=240, restart=20, policy=2, boost=1.5, precision=0,
+                            refine=0)
+            out["fp32_plain"] = {"converged": True, "iterations": st.iterations, "fgmres": bool(st.used_fgmres),
+                                 "rel_residual": float("%.3e" % st.rel_residual), "note": note}
+        except m.MaelstromError as e:
+            # the reference raises here too (fgmres.jl:90): FP32 alone does not reach the tolerance
+            out["fp32_plain"] = {"converged": False, "error": str(e), "note": note}
+        log("plain FP32 solve: %s" % out["fp32_plain"])
+        s = f
+    os.remove(start)
+    os.rmdir(tmpdir)
     if args.mp_error:
         out["mp_error"] = sparse_mp_error(s, dt, args.tol, args.refine)
     if not args.no_cpu_baseline:
+        log("CPU baseline")
         out["cpu_baseline"] = sparse_cpu_baseline(s, out["iterations"], dt)
     print(json.dumps(out), flush=True)
 
@@ -394,7 +479,8 @@ def main():
     # run.jl:44: dt = left / ceil(left / (cfl dx / max(maxspeed, 1e-3))) within a 1/24 s frame
     left = 1.0 / 24
     dt = left / math.ceil(left / (s.cfl * dx / max(m.maxspeed(s), 1e-3)))
-    solve = dict(tol=args.tol, maxiter=400, precision=args.precision, warm=0)
+    # the reference's warm start; FP64 refinement to reach 1e-6 (its FP32 PCG stalls near 1e-5)
+    solve = dict(tol=args.tol, maxiter=400, precision=args.precision, warm=1, refine=1)
     for _ in range(args.warmup):
         m.step(s, dt, **solve)
     s.sync()
@@ -431,7 +517,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": desc, "cells": list(s.n_cells), "particles": int(n_avg), "dt_s": round(dt, 5),
-                   "tol": args.tol, "precision": ["fp32", "fp16", "bf16"][args.precision], "solve_start": "zero"},
+                   "tol": args.tol, "precision": ["fp32", "fp16", "bf16"][args.precision], "solve_start": "warm",
+                   "phases": "advect, p2g, redistance, classify, trapped air, stir forces, project, extrapolate, "
+                             "g2p, resample"},
         "pressure_ms": round(sum(pressure_ms) / len(pressure_ms), 3),
         "cg_iterations": round(sum(iters) / len(iters), 2),
         "mparticles_per_s": round(n_avg / (ms_step * 1e-3) / 1e6, 2),

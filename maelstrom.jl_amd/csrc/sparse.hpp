@@ -334,7 +334,7 @@ struct MPStore {
     bool ready = false;
     DBuf<uint16_t> z[2], r[2], q[2], tmp[2];        // fine pool, coarse
     DBuf<uint16_t> X1[4][2];                        // level-1 x, b, r, t
-    std::vector<std::array<DBuf<uint16_t>, 4>> Dv;  // dense levels: x, b, r, t
+    std::vector<std::array<DBuf<uint16_t>, 3>> Dv;  // dense levels: x, b, t
     DBuf<uint16_t> cc[3], c1[3], dDf, dDc, dD1;   // level 0's couplings follow from the types
     DBuf<uint16_t> rwf;                             // level 0's omega / diag (the Jacobi weights)
     std::vector<std::array<DBuf<uint16_t>, 4>> Dc;  // dense levels: couplings x, y, z, diagonal
@@ -363,7 +363,7 @@ struct SparseStats {
 // Dense levels of the sparse hierarchy (sflip.jl:234-240).
 struct SLevel {
     int n[3];
-    DBuf<float> x, b, r, t, c[3], dD;
+    DBuf<float> x, b, t, c[3], dD;
     DBuf<uint8_t> act;
     size_t cells() const { return (size_t)n[0] * n[1] * n[2]; }
     size_t faces(int a) const { return (size_t)(n[0] + (a == 0)) * (n[1] + (a == 1)) * (n[2] + (a == 2)); }

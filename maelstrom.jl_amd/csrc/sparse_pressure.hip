@@ -1032,25 +1032,12 @@ __global__ __launch_bounds__(256) void k_rowdense(T* __restrict__ out, const T* 
     }
     out[l] = sf<T>(res);
 }
-// The coarser level's right-hand side: the sum of the eight finer residuals, times
-// `scale` (1 in the reference's solve; the reduced-precision hierarchy carries each
-// dense level's equations divided by 4 per level, so its values stay in FP16's range).
-template <class T>
-__global__ void k_restrictdense(T* __restrict__ bc, const T* __restrict__ rf, int3 nc, int3 nf, float scale) {
-    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= (size_t)nc.x * nc.y * nc.z) return;
-    int I, J, K;
-    unlin(l, nc.x, nc.y, I, J, K);
-    float acc = 0.0f;
-    for (int c = 0; c < 2; ++c)
-        for (int b = 0; b < 2; ++b)
-            for (int a = 0; a < 2; ++a) acc += lf(rf[cl(2 * I + a, 2 * J + b, 2 * K + c, nf.x, nf.y)]);
-    bc[l] = sf<T>(acc * scale);
-}
-// The residual and its restriction in one pass: each coarser cell forms its eight finer
-// cells' residuals (k_rowdense mode 1's arithmetic, rounded to T as that kernel stores
-// them) and sums them in k_restrictdense's order, so the result is the same bit for bit
-// without the residual array.
+// The coarser level's right-hand side, the residual and its restriction in one pass:
+// each coarser cell forms its eight finer cells' residuals (k_rowdense mode 1's
+// arithmetic, rounded to T as that kernel would store them), sums them in x-fastest
+// order and multiplies by `scale` (1 in the reference's solve; the reduced-precision
+// hierarchy carries each dense level's equations divided by 4 per level, so its values
+// stay in FP16's range). No residual array is written.
 template <class T, class C>
 __global__ void k_residrestrict(T* __restrict__ bc, const T* __restrict__ x, const T* __restrict__ b, DViewT<C> L, int3 nc,
                                 float scale) {
@@ -1686,7 +1673,7 @@ struct SparseSolver {
     // The hierarchy in one storage precision: its vectors and couplings.
     template <class T, class C> struct Hier {
         SVecT<T> q, tmp, X1x, X1b, X1r, X1t;
-        std::vector<T*> Dx, Db, Dr, Dt;
+        std::vector<T*> Dx, Db, Dt;
         CompT<C> L0, L1;
         std::vector<DViewT<C>> D;
         float rscale = 1.0f;   // dense-level restriction scale
@@ -1698,7 +1685,7 @@ struct SparseSolver {
         H.X1x = {s.X1x.fine.p, s.X1x.coarse.p}; H.X1b = {s.X1b.fine.p, s.X1b.coarse.p};
         H.X1r = {s.X1r.fine.p, s.X1r.coarse.p}; H.X1t = {s.X1t.fine.p, s.X1t.coarse.p};
         for (auto& L : s.D) {
-            H.Dx.push_back(L.x.p); H.Db.push_back(L.b.p); H.Dr.push_back(L.r.p); H.Dt.push_back(L.t.p);
+            H.Dx.push_back(L.x.p); H.Db.push_back(L.b.p); H.Dt.push_back(L.t.p);
             H.D.push_back(DViewT<float>{i3(L.n), {L.c[0].p, L.c[1].p, L.c[2].p}, L.dD.p, L.act.p});
         }
         H.L0 = CompT<float>{{{s.cf[0].p, s.cf[1].p, s.cf[2].p}}, s.dD.fine.p, s.types.fine.p,
@@ -1719,7 +1706,7 @@ struct SparseSolver {
         H.X1r = V(m.X1[2][0], m.X1[2][1]); H.X1t = V(m.X1[3][0], m.X1[3][1]);
         for (size_t l = 0; l < s.D.size(); ++l) {
             H.Dx.push_back((T*)m.Dv[l][0].p); H.Db.push_back((T*)m.Dv[l][1].p);
-            H.Dr.push_back((T*)m.Dv[l][2].p); H.Dt.push_back((T*)m.Dv[l][3].p);
+            H.Dt.push_back((T*)m.Dv[l][2].p);
             H.D.push_back(DViewT<T>{i3(s.D[l].n), {(T*)m.Dc[l][0].p, (T*)m.Dc[l][1].p, (T*)m.Dc[l][2].p}, (T*)m.Dc[l][3].p,
                                     s.D[l].act.p});
         }
@@ -1810,7 +1797,9 @@ struct SparseSolver {
         Hier<float, float> H = hier32();
         int32_t fine_groups = s.nslots ? 2 * s.nslots : 1;   // the level-0 rows' 8-plane tiles
         SearchArgs S{d.f, d.c, dn.f, dn.c, s.partials.p, fine_groups, s.scal.p};
+        s.kt.begin(K_SEARCH, st);
         rows<16, 4>(SVecT<float>{q.f, q.c}, SVecT<float>{z.f, z.c}, SVecT<float>{z.f, z.c}, H.L0, S);
+        s.kt.end(K_SEARCH, st);
         hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(FINISH_THREADS), 0, st, s.partials.p, fine_groups + (int)cdiv(Cc, 256), s.scal.p,
                            (int)SF_ALPHA, 0);
         MAEL_LAUNCH_CHECK();
@@ -2096,9 +2085,11 @@ struct SparseSolver {
                 cur ^= 1;
                 d = dbuf[cur];
             }
+            s.kt.begin(K_ADVANCE, st);
             hipLaunchKernelGGL(k_fcg_update, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, P.f, P.c, r.f, r.c, rold.f,
                                rold.c, d.f, d.c, q.f, q.c, s.types.fine.p, s.types.coarse.p, s.cmask.p, t, nf, s.scal.p,
                                (int)SS_ALPHA);
+            s.kt.end(K_ADVANCE, st);
             hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(FINISH_THREADS), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p, (int)SF_NORM,
                                (int)SS_RNORM);
             double flags;

@@ -39,7 +39,7 @@ SparseBasin::SparseBasin(const SGrid& grid, int nblk, int64_t cap, int32_t bnd, 
         SLevel& L = D.back();
         for (int a = 0; a < 3; ++a) L.n[a] = m[a];
         size_t c = L.cells();
-        L.x.alloc(c); L.b.alloc(c); L.r.alloc(c); L.t.alloc(c); L.dD.alloc(c); L.act.alloc(c);
+        L.x.alloc(c); L.b.alloc(c); L.t.alloc(c); L.dD.alloc(c); L.act.alloc(c);
         for (int a = 0; a < 3; ++a) L.c[a].alloc(L.faces(a));
         if ((long)m[0] * m[1] * m[2] <= 1024 || (m[0] & 1) || (m[1] & 1) || (m[2] & 1)) break;
         for (int a = 0; a < 3; ++a) m[a] /= 2;

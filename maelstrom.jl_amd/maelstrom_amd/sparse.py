@@ -176,10 +176,13 @@ def reach(s):
     return float(np.float32((math.ceil(s.cfl) + 3) * s.grid.dx))
 
 
-def marks(s, particles=True):
-    """The marks of `refine!` (sflip.jl:518-526), one byte a block."""
+def marks(s, particles=True, vary=None, rings=None):
+    """The marks of `refine!` (sflip.jl:518-526), one byte a block (`vary` and `rings`
+    default to the basin's)."""
     out = np.zeros(s.nblocks, np.uint8)
-    s.ctx.check(_lib.lib().mael_sparse_marks(s.h, s.vary, reach(s), s.rings, 1 if particles else 0, out.ctypes.data))
+    vary = s.vary if vary is None else float(vary)
+    rings = s.rings if rings is None else int(rings)
+    s.ctx.check(_lib.lib().mael_sparse_marks(s.h, vary, reach(s), rings, 1 if particles else 0, out.ctypes.data))
     return out
 
 

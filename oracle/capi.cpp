@@ -153,6 +153,8 @@ void orc_dense_advect_grid(const orc_grid* g, const float* u, const float* v, co
     dense_advect_grid(DenseFlow(G, U, nullptr), out, src, n1, n2, n3, off, dt);
 }
 // flip.jl:1194-1206 extrapolate! of component `a`, in place on (u, f).
+// flip.jl:672-679 redistance!(s; iterations), phi in place.
+void orc_dense_redistance(const orc_grid* g, float* phi, int32_t iterations) { dense_redistance(to_grid(g), phi, iterations); }
 void orc_dense_extrapolate(const orc_grid* g, int32_t a, float* u, uint8_t* f, int32_t layers) {
     dense_extrapolate(to_grid(g), a, u, f, layers);
 }

@@ -151,6 +151,22 @@ inline DenseGrid flipgrid(float radius, float depth, float top, float dx, int pa
     }
 }
 
+// flip.jl:611-622 eikonal.
+inline float eikonal(float a, float b, float c, float h) {
+    float lo = std::min(a, b), hi = std::max(a, b); a = lo; b = hi;
+    lo = std::min(b, c); hi = std::max(b, c); b = lo; c = hi;
+    lo = std::min(a, b); hi = std::max(a, b); a = lo; b = hi;
+    float d = a + h;
+    if (d > b) {
+        d = 0.5f * (a + b + std::sqrt(std::max(2.0f * h * h - (a - b) * (a - b), 0.0f)));
+        if (d > c) {
+            float s = a + b + c;
+            d = (s + std::sqrt(std::max(s * s - 3.0f * (a * a + b * b + c * c - h * h), 0.0f))) / 3.0f;
+        }
+    }
+    return d;
+}
+
 // flip.jl:64-66 issolid / cellcentre.
 inline bool issolid(float radius, float fl, float px, float py, float pz) {
     return px * px + py * py > radius * radius || pz < fl;

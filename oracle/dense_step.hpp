@@ -127,6 +127,57 @@ inline void dense_advect_grid(const DenseFlow& fl, float* out, const float* src,
             }
 }
 
+// flip.jl:624-679 reinit_kernel! and redistance!: `iterations` Jacobi sweeps of the level
+// set in place. Cells with a neighbour of the other sign keep their value; the others take
+// the upwind (Godunov) distance from the smallest neighbour magnitude along each axis, at
+// most phimax = (iterations + 1) dx; a solid cell takes the lowest value of the non-solid
+// cells next to it (or its own, clamped to +-phimax, when there is none).
+inline void dense_redistance(const DenseGrid& g, float* phi, int iterations) {
+    const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
+    const size_t N = (size_t)n1 * n2 * n3;
+    const float h = g.dx, phimax = (float)(iterations + 1) * g.dx;
+    auto at = [&](int i, int j, int k) { return (size_t)i + (size_t)n1 * ((size_t)j + (size_t)n2 * k); };
+    auto open = [&](int i, int j, int k) {
+        return i >= 0 && j >= 0 && k >= 0 && i < n1 && j < n2 && k < n3 && !issolidcell(g, i, j, k);
+    };
+    std::vector<float> out(N);
+    for (int it = 0; it < iterations; ++it) {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = 0; k < n3; ++k)
+            for (int j = 0; j < n2; ++j)
+                for (int i = 0; i < n1; ++i) {
+                    float c = phi[at(i, j, k)];
+                    if (issolidcell(g, i, j, k)) {
+                        float lowest = c;
+                        bool found = false;
+                        for (int m = 0; m < 6; ++m) {
+                            int a = i + NB[m][0], b = j + NB[m][1], cc = k + NB[m][2];
+                            if (!open(a, b, cc)) continue;
+                            float f = phi[at(a, b, cc)];
+                            lowest = found ? std::min(lowest, f) : f;
+                            found = true;
+                        }
+                        out[at(i, j, k)] = found ? lowest : std::clamp(c, -phimax, phimax);
+                        continue;
+                    }
+                    bool inside = c < 0.0f, frozen = false;
+                    float ax = phimax, ay = phimax, az = phimax;
+                    for (int m = 0; m < 6; ++m) {
+                        int a = i + NB[m][0], b = j + NB[m][1], cc = k + NB[m][2];
+                        if (!open(a, b, cc)) continue;
+                        float f = phi[at(a, b, cc)];
+                        if ((f < 0.0f) != inside) frozen = true;
+                        float d = std::fabs(f);
+                        if (m < 2) ax = std::min(ax, d); else if (m < 4) ay = std::min(ay, d); else az = std::min(az, d);
+                    }
+                    if (frozen) { out[at(i, j, k)] = c; continue; }
+                    float d = std::min(eikonal(ax, ay, az, h), phimax);
+                    out[at(i, j, k)] = inside ? -d : d;
+                }
+        std::copy(out.begin(), out.end(), phi);
+    }
+}
+
 // flip.jl:779-784 walled: face (i, j, k) of axis a has a wall cell on a side or is the
 // grid's own edge.
 inline bool walled(const DenseGrid& g, int i, int j, int k, int a) {

@@ -68,6 +68,7 @@ def lib():
             "orc_dense_advect_particles": (C.c_int, [G, PP, p, p, p, p, p, p, f32, f32, i32]),
             "orc_dense_advect_grid": (None, [G, p, p, p, p, p, i32, i32, i32, p, f32]),
             "orc_dense_extrapolate": (None, [G, i32, p, p, i32]),
+            "orc_dense_redistance": (None, [G, p, i32]),
             "orc_dense_forces": (None, [G, p, p, i32, p, p, p, f32, f32, f32]),
             "orc_stir": (None, [p, p, i32, f32, f32, f32, p, f32, p]),
             "orc_dense_whitewater": (None, [G, PP, p, p, p, p, p, p, p] + [p] * 8 + [i64, p, f64, i32, f32, f32, f32]),
@@ -246,6 +247,14 @@ def dense_advect_grid(g, u, v, w, src, off, dt):
     lib().orc_dense_advect_grid(C.byref(g), u.ctypes.data, v.ctypes.data, w.ctypes.data, src.ctypes.data, out.ctypes.data,
                                 n1, n2, n3, o.ctypes.data, C.c_float(dt))
     return out
+
+
+def dense_redistance(g, phi, iterations):
+    """flip.jl:672-679 redistance!(s; iterations): returns the level set after `iterations`
+    Jacobi sweeps."""
+    phi = _c(phi, np.float32).copy()
+    lib().orc_dense_redistance(C.byref(g), phi.ctypes.data, iterations)
+    return phi
 
 
 def dense_extrapolate(g, a, u, f, layers):

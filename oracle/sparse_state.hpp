@@ -22,22 +22,6 @@ inline float swirl(float G, float core, const float p[3], int a) {
     return a == 0 ? -turn * p[1] : (a == 1 ? turn * p[0] : 0.0f);
 }
 
-// flip.jl:611-622 eikonal.
-inline float eikonal(float a, float b, float c, float h) {
-    float lo = std::min(a, b), hi = std::max(a, b); a = lo; b = hi;
-    lo = std::min(b, c); hi = std::max(b, c); b = lo; c = hi;
-    lo = std::min(a, b); hi = std::max(a, b); a = lo; b = hi;
-    float d = a + h;
-    if (d > b) {
-        d = 0.5f * (a + b + std::sqrt(std::max(2.0f * h * h - (a - b) * (a - b), 0.0f)));
-        if (d > c) {
-            float s = a + b + c;
-            d = (s + std::sqrt(std::max(s * s - 3.0f * (a * a + b * b + c * c - h * h), 0.0f))) / 3.0f;
-        }
-    }
-    return d;
-}
-
 inline void faceijk(int a, int f, int& i, int& j, int& k) {   // sflip.jl:364-377 fineface
     if (a == 0) { i = f % 17; int r = f / 17; j = r & 15; k = r >> 4; }
     else if (a == 1) { i = f & 15; int r = f >> 4; j = r % 17; k = r / 17; }

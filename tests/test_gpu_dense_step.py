@@ -1,6 +1,7 @@
 """GPU parity of the dense basin's support phases (the rest of `step!(::FlipBasin)`,
 flip.jl:1396-1410) with the CPU oracle, through the C ABI: ST-FLIP particle advection and
-the grid's semi-Lagrangian advection bit for bit, extrapolation bit for bit, trapped air
+the grid's semi-Lagrangian advection bit for bit, extrapolation and redistancing bit for
+bit, trapped air
 against a connected-component labelling, the stirring forces to FP32 rounding of the
 transcendental functions (the library's expf, sinf and cosf are not the host's), the
 fastest speed exactly."""
@@ -63,6 +64,21 @@ def test_extrapolate_matches_oracle(mael, orc, ctx, scene):
     for (name, fl), (u, f) in zip((("U", "FU"), ("V", "FV"), ("W", "FW")), want):
         assert np.array_equal(s.get(fl), f), fl
         assert np.array_equal(s.get(name).view(np.uint32), u.view(np.uint32)), name
+
+
+def test_redistance_matches_oracle(mael, orc, ctx, scene):
+    """`redistance!` (flip.jl:611-679) bit for bit: the scene's level set with noise (so
+    cells of both signs meet away from the surface too), `band + 3` sweeps."""
+    s, sc = make_basin(mael, orc, scene=scene)
+    g = sc["g"]
+    rng = np.random.default_rng(21)
+    phi = (sc["phi"] + rng.normal(0, 0.3 * g.dx, sc["phi"].shape)).astype(np.float32)
+    s.set("PHI", phi)
+    it = s.band + 3
+    print("case: dense redistance, %d sweeps, %s" % (it, scene), flush=True)
+    mael.redistance(s, it)
+    want = orc.dense_redistance(g, phi, it)
+    assert np.array_equal(s.get("PHI").view(np.uint32), want.view(np.uint32))
 
 
 def _stir_params(orc, b, race, stir):

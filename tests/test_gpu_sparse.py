@@ -437,3 +437,16 @@ def test_filltrapped_matches_flood_fill(M):
         want = t.copy()
         want[(t == AIR) & ~reached] = LIQUID
         assert np.array_equal(got, want), (trial, int((got != want).sum()))
+
+
+def test_redistance_bitexact(M):
+    """`redistance!` (sflip.jl:833-919) on the GPU gives the oracle's level set bit for bit,
+    from a tilted plane's distance (which the oracle keeps where the sweeps are exact)."""
+    from test_oracle_pins import _plane_phi
+    o = _scene()
+    _plane_phi(o, np.array([0.48, -0.36, 0.8]), -5.0)
+    s = _upload(M, o)
+    M.sparse.redistance(s, 3)
+    o.redistance(3)
+    np.testing.assert_array_equal(s.get("PHI_F").view(np.uint32), _fine(o, "phi_f", BCELLS).view(np.uint32))
+    np.testing.assert_array_equal(s.get("PHI_C").ravel().view(np.uint32), o.array("phi_c").view(np.uint32))

@@ -407,3 +407,61 @@ def test_sparse_combine_pinned():
         exp = N.combine(pf, u0[a][:F], momwt[a].astype(f32), momwt[3 + a].astype(f32), r, f32(8))
         got = o.array(nm + "_f")[:F]
         np.testing.assert_array_equal(got[live], exp[live], err_msg=nm)
+
+
+def _plane_phi(o, n, c):
+    """The signed distance to the plane n.x = c at every fine and coarse cell centre."""
+    g = o.grid
+    ns = o.nslots
+    where = o.array("where").reshape(-1, 3)[:ns]
+    k, j, i = np.meshgrid(np.arange(16), np.arange(16), np.arange(16), indexing="ij")
+    phif = o.array("phi_f")
+    for s in range(ns):
+        if where[s, 0] < 0:
+            continue
+        x = g.lo[0] + (where[s, 0] * 16 + i + 0.5) * g.dx
+        y = g.lo[1] + (where[s, 1] * 16 + j + 0.5) * g.dx
+        z = g.lo[2] + (where[s, 2] * 16 + k + 0.5) * g.dx
+        phif[s * 4096:(s + 1) * 4096] = (n[0] * x + n[1] * y + n[2] * z - c).astype(f32).ravel()
+    n1, n2, n3 = g.nc
+    K, J, I = np.meshgrid(np.arange(n3), np.arange(n2), np.arange(n1), indexing="ij")
+    h = 4 * g.dx
+    x, y, z = g.lo[0] + (I + 0.5) * h, g.lo[1] + (J + 0.5) * h, g.lo[2] + (K + 0.5) * h
+    o.array("phi_c")[:] = (n[0] * x + n[1] * y + n[2] * z - c).astype(f32).ravel()
+
+
+def test_redistance_keeps_a_planar_distance():
+    """The redistancing sweep (sflip.jl:910-919 with flip.jl:610-664's eikonal update) leaves
+    a signed distance to a tilted plane unchanged where it is not clamped: the three-axis
+    Godunov update reproduces a linear distance function exactly. A plane far from the
+    basin's walls and floor, cells away from the clamp at (iterations + 1) h."""
+    o = _scene()
+    nrm = np.array([0.48, -0.36, 0.8])   # unit length
+    _plane_phi(o, nrm, -5.0)
+    before_f, before_c = o.array("phi_f").copy(), o.array("phi_c").copy()
+    it = 3
+    o.redistance(it)
+    ns = o.nslots
+    after_f = o.array("phi_f")[:ns * 4096]
+    bf = before_f[:ns * 4096]
+    types = o.array("types_f")[:ns * 4096]
+    # inside the blocks: a cell on a block's side reads the coarse grid across it, and each
+    # Jacobi sweep carries that a cell further in
+    loc = np.arange(4096)
+    li, lj, lk = loc % 16, (loc // 16) % 16, loc // 256
+    m = it
+    interior = np.tile((li >= m) & (li < 16 - m) & (lj >= m) & (lj < 16 - m) & (lk >= m) & (lk < 16 - m), ns)
+    # and away from the walls and the floor (solid cells take their water neighbours' value)
+    # by more than the sweeps' reach
+    where = o.array("where").reshape(-1, 3)[:ns]
+    g = o.grid
+    cx = g.lo[0] + (np.repeat(where[:, 0], 4096) * 16 + np.tile(li, ns) + 0.5) * g.dx
+    cy = g.lo[1] + (np.repeat(where[:, 1], 4096) * 16 + np.tile(lj, ns) + 0.5) * g.dx
+    cz = g.lo[2] + (np.repeat(where[:, 2], 4096) * 16 + np.tile(lk, ns) + 0.5) * g.dx
+    clear = (np.hypot(cx, cy) < g.radius - (it + 2) * g.dx) & (cz > g.floor + (it + 2) * g.dx)
+    inner = (np.abs(bf) < (it - 1) * 0.6) & (np.abs(bf) > 0.6) & (types != 0) & interior & clear
+    assert inner.sum() > 300
+    np.testing.assert_allclose(after_f[inner], bf[inner], rtol=0, atol=2e-5)
+    # far from the plane the value is clamped at the sweep's reach, its sign kept
+    far = np.abs(bf) > (it + 2) * 0.6 * 4
+    assert np.all(np.sign(after_f[far]) == np.sign(bf[far]))

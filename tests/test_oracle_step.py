@@ -222,3 +222,61 @@ def test_stir_eddies_are_divergence_free():
         scale = max(scale, abs(ex(x, y)), abs(ey(x, y)))
     assert scale > 0.05
     assert worst < 1e-4 * scale / d
+
+
+def _cell_centres(g):
+    n1, n2, n3 = g.dims
+    k, j, i = np.meshgrid(np.arange(n3), np.arange(n2), np.arange(n1), indexing="ij")
+    x = (f32(g.lo[0]) + (i.astype(f32) + f32(0.5)) * f32(g.dx)).astype(f32)
+    y = (f32(g.lo[1]) + (j.astype(f32) + f32(0.5)) * f32(g.dx)).astype(f32)
+    z = (f32(g.lo[2]) + (k.astype(f32) + f32(0.5)) * f32(g.dx)).astype(f32)
+    return i, j, k, x, y, z
+
+
+def _solid(g, x, y, z):   # flip.jl:64-66
+    return (x * x + y * y > f32(g.radius) * f32(g.radius)) | (z < f32(g.floor))
+
+
+def test_dense_redistance_planar_and_rules(grid):
+    """The redistancing sweep (flip.jl:611-679): a signed distance to a tilted plane is a
+    fixed point of the Godunov update (its upwind neighbours are exact, and the three-axis
+    root reproduces a linear distance), so it is kept wherever it is below the reach
+    (iterations + 1) dx and the cell is clear of the walls by more than the sweeps carry;
+    above the reach it is clamped to exactly the reach, sign kept. One sweep's solid cells
+    take the lowest value of their non-solid neighbours (or their own, clamped), as a NumPy
+    restatement of the rule has them."""
+    g = grid
+    n1, n2, n3 = g.dims
+    i, j, k, x, y, z = _cell_centres(g)
+    nrm = np.array([0.48, -0.36, 0.8])
+    phi = (nrm[0] * x + nrm[1] * y + nrm[2] * z + 3.0).astype(f32)
+    it = 4
+    h = f32(g.dx)
+    reach = f32(it + 1) * h
+    out = O.dense_redistance(g, phi, it)
+    solid = _solid(g, x, y, z)
+    # distance to the walls, the floor and the grid's edges, in cells
+    r = np.hypot(x, y)
+    m = it + 1
+    clear = ((r < g.radius - m * g.dx) & (z > g.floor + m * g.dx) & (i >= m) & (i < n1 - m) & (j >= m) &
+             (j < n2 - m) & (k >= m) & (k < n3 - m))
+    near = clear & (np.abs(phi) < reach - h)
+    assert near.sum() > 2000
+    np.testing.assert_allclose(out[near], phi[near], rtol=0, atol=3e-5)
+    far = clear & (np.abs(phi) > reach + h)
+    assert far.sum() > 2000
+    assert np.all(np.abs(out[far]) == reach) and np.all(np.sign(out[far]) == np.sign(phi[far]))
+    # one sweep's solid cells, restated
+    rng = np.random.default_rng(3)
+    phi = rng.uniform(-4, 4, phi.shape).astype(f32)
+    one = O.dense_redistance(g, phi, 1)
+    reach1 = f32(2) * h
+    pad = np.pad(phi, 1, constant_values=np.nan)
+    opn = np.pad(~solid, 1, constant_values=False)
+    best = np.full(phi.shape, np.inf, f32)
+    for dk, dj, di in ((0, 0, 1), (0, 0, -1), (0, 1, 0), (0, -1, 0), (1, 0, 0), (-1, 0, 0)):
+        sl = (slice(1 + dk, 1 + dk + n3), slice(1 + dj, 1 + dj + n2), slice(1 + di, 1 + di + n1))
+        best = np.where(opn[sl], np.minimum(best, pad[sl]), best)
+    exp = np.where(np.isfinite(best), best, np.clip(phi, -reach1, reach1))
+    assert solid.sum() > 1000 and np.isfinite(best[solid]).sum() > 500
+    np.testing.assert_array_equal(one[solid], exp[solid])

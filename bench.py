@@ -69,8 +69,12 @@ def sparse_bytes(kernel, live_cells, n_particles, precision=0, coarse_cells=0, c
         # the flexible CG's update: P, r read and written, r_old written, d, q read (4 B
         # each) and the type (1 B) per unknown; the cover mask too on the coarse grid
         "advance": 29 * live_cells + 30 * coarse_cells,
-        "g2p": 60 * n_particles,
-        "p2g": 64 * n_particles,
+        # x (12 B) read, v and the APIC rows (48 B) written per particle; u, v, w and their
+        # valid flags (15 B) read per fine cell's face triple
+        "g2p": 60 * n_particles + 15 * live_cells,
+        # the deposit: x, v, the rows and the lag (64 B) read per particle; the momentum
+        # and weight sums (8 B each) added to each of a fine cell's three faces
+        "p2g": 64 * n_particles + 48 * live_cells,
         # x (12 B) read, key and rank (8 B) written per particle
         "bin": 20 * n_particles,
         "scatter": 136 * n_particles,
@@ -112,8 +116,8 @@ def algorithmic_bytes(kernel, s, n_particles):
 
 
 def cpu_baseline(s, iters_per_step, dt):
-    """The oracle's FP32 CPU port on the same state (one core): resample, P2G and G2P
-    over all particles, and a few MG-PCG iterations, scaled to the GPU's iterations."""
+    """The oracle's FP32 CPU port on the same state, on the host's cores: resample, P2G and
+    G2P over all particles, and a few MG-PCG iterations, scaled to the GPU's iterations."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as orc
@@ -217,16 +221,15 @@ def sparse_cpu_baseline(s, iters_per_step, dt, sample_particles=2_000_000):
     return {"value": round(ms, 1), "unit": "ms/step", "cores": cores, "kind": "port", "sample": sample}
 
 
-def sparse_mp_error(s, dt, tol, refine, oracle_max_cells=4_000_000):
+def sparse_mp_error(s, dt, tol, refine, chunk=8):
     """The error of the GPU solves (FP32, FP16 and BF16 V-cycles, all with the FP64
-    refinement) against a tight solve of the same system: the state of a first half-step's
-    projection, the same pressure to start from, gamma = 0. The tight solve is the FP64
-    oracle's (to 1e-10) up to `oracle_max_cells` live fine cells; above that, where the
-    oracle's single-threaded solve would take many minutes, the GPU's FP32 V-cycle with the
-    FP64 refinement every iteration, to 1e-10 (the oracle comparison at the smaller sizes
-    bounds that solve's error). Reports the relative L2 error of P over the unknowns and
-    the relative divergence left after each projection (||b(u)|| / ||b(u_before)|| over
-    the unknowns, b the oracle's FP64 right-hand side, evaluated on the host)."""
+    refinement) against the FP64 oracle's solve of the same system to 1e-10: the state of
+    a first half-step's projection, the same pressure to start from, gamma = 0. The oracle
+    (OpenMP on the host's cores) solves in chunks of `chunk` flexible-CG iterations, each
+    warm-started from the last chunk's pressure and the velocities before the projection,
+    with a progress line after each. Reports the relative L2 error of P over the unknowns
+    and the relative divergence left after each projection (||b(u)|| / ||b(u_before)||
+    over the unknowns, b the oracle's FP64 right-hand side, evaluated on the host)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as orc
@@ -264,22 +267,27 @@ def sparse_mp_error(s, dt, tol, refine, oracle_max_cells=4_000_000):
 
     b0 = bnorm()
     t0 = time.perf_counter()
-    if int((where[:ns, 0] >= 0).sum()) * 4096 <= oracle_max_cells:
-        ref = o.project(gamma=0.0, tol=1e-10, maxiter=300, policy=0)
-        P64 = np.concatenate([o.array("P_f")[:ns * 4096][uf].astype(np.float64), o.array("P_c")[uc].astype(np.float64)])
-        rep = {"reference": "oracle FP64 MGPCG to 1e-10",
-               "tight": {"iterations": ref["iterations"], "rel_residual": ref["rel_residual"]}}
-    else:
-        st, P64 = gpu_solve(0, 1e-10, 1, 400)
-        rep = {"reference": "GPU FP32 V-cycle, FP64 refinement every iteration, to 1e-10",
-               "tight": {"iterations": st.iterations, "rel_residual": st.rel_residual}}
+    before = {nm: (o.array(nm + "_f").copy(), o.array(nm + "_c").copy()) for nm in ("u", "v", "w")}
+    iters, ref = 0, None
+    while iters < 600:
+        for nm, (fv, cv) in before.items():
+            o.array(nm + "_f")[:] = fv
+            o.array(nm + "_c")[:] = cv
+        ref = o.project(gamma=0.0, tol=1e-10, maxiter=chunk, policy=0)
+        iters += ref["iterations"]
+        log("mp_error: oracle FP64 solve, %d iterations, relative residual %.3e (%.0f s)"
+            % (iters, ref["rel_residual"], time.perf_counter() - t0))
+        if ref["rel_residual"] <= 1e-10:
+            break
+    P64 = np.concatenate([o.array("P_f")[:ns * 4096][uf].astype(np.float64), o.array("P_c")[uc].astype(np.float64)])
+    rep = {"reference": "oracle FP64 composite MG flexible CG to 1e-10 (chunks of %d iterations, warm-started)" % chunk,
+           "tight": {"iterations": iters, "rel_residual": ref["rel_residual"]}}
     rep["tight"].update(seconds=round(time.perf_counter() - t0, 1), div_rel=bnorm() / b0)
-    print("mp_error: reference solve done", flush=True)
     for p, name in ((0, "fp32"), (1, "fp16"), (2, "bf16")):
         st, Pg = gpu_solve(p, tol, refine, 240)
         rep[name] = {"iterations": st.iterations, "rel_residual": st.rel_residual,
                      "P_rel_l2": float(np.linalg.norm(Pg - P64) / np.linalg.norm(P64)), "div_rel": bnorm() / b0}
-        print("mp_error: %s done" % name, flush=True)
+        log("mp_error: %s done" % name)
     for f, v in keep.items():
         s.set(f, v)
     return rep
@@ -323,13 +331,13 @@ def sparse_main(args, desc, kw, dx):
         iters, fg, pms, nparts, each = [], [], [], [], []
         t0 = time.perf_counter()
         for _ in range(steps):
-            k0 = len(s.iterations)
+            k0, j0 = len(s.iterations), len(s.solve_ms)
             ts = time.perf_counter()
             st = sp.step(s, dt, **solve)
             each.append(time.perf_counter() - ts)
             iters.append(sum(s.iterations[k0:]))   # both half-steps' solves
             fg.append(st.used_fgmres)
-            pms.append(sum(s.solve_ms[k0:]))
+            pms.append(sum(s.solve_ms[j0:]))
             nparts.append(s.n)
         s.sync()
         t1 = time.perf_counter()
@@ -420,18 +428,34 @@ def sparse_main(args, desc, kw, dx):
                        "from": "the same state as the mixed-precision steps (a checkpoint after the warm-up)"}
         out["speedup_vs_fp32"] = {"step": round(F["ms"] / ms_step, 3),
                                   "pressure": round(F["pressure_ms"] / max(R["pressure_ms"], 1e-9), 3)}
-        # the reference's plain FP32 solve (no refinement) on the next half-step's system
-        sp.before_projection(f, dt / 2, reflect=True)
+        # the reference's plain FP32 solve (no refinement) and the mixed-precision one on the
+        # same system: the first half-step's projection from the saved state
         note = ("the reference's FP32 flexible CG then FGMRES(20), no FP64 refinement, at most 240 iterations, "
-                "on the next half-step's system")
-        try:
-            st = sp.project(f, dt / 2, tol=args.tol, maxiter=240, restart=20, policy=2, boost=1.5, precision=0,
-                            refine=0)
-            out["fp32_plain"] = {"converged": True, "iterations": st.iterations, "fgmres": bool(st.used_fgmres),
-                                 "rel_residual": float("%.3e" % st.rel_residual), "note": note}
-        except m.MaelstromError as e:
-            # the reference raises here too (fgmres.jl:90): FP32 alone does not reach the tolerance
-            out["fp32_plain"] = {"converged": False, "error": str(e), "note": note}
+                "on the first half-step's system from the saved state")
+        for label, prec, ref in (("fp32_plain", 0, 0), ("mp_same_system", args.precision, args.refine)):
+            h = m.load_checkpoint(start, blocks=12000)
+            if prec:
+                # the reduced-precision hierarchy's storage is allocated by a basin's first use:
+                # one V-cycle does it, then the saved state again (the V-cycle overwrote P)
+                o = sp.solve_options(**dict(sp.pressure_defaults(h), precision=prec))
+                h.ctx.check(L.mael_sparse_vcycle(h.h, C.byref(o), 0))
+                h.ctx.check(L.mael_sparse_restore(h.h, start.encode()))
+            sp.before_projection(h, dt / 2, reflect=True)
+            h.sync()
+            t0 = time.perf_counter()
+            try:
+                st = sp.project(h, dt / 2, tol=args.tol, maxiter=240, restart=20, policy=2, boost=1.5, precision=prec,
+                                refine=ref)
+                h.sync()
+                out[label] = {"converged": True, "iterations": st.iterations, "fgmres": bool(st.used_fgmres),
+                              "rel_residual": float("%.3e" % st.rel_residual),
+                              "ms": round(1e3 * (time.perf_counter() - t0), 2)}
+            except m.MaelstromError as e:
+                # the reference raises here too (fgmres.jl:90): FP32 alone does not reach the tolerance
+                h.sync()
+                out[label] = {"converged": False, "error": str(e), "ms": round(1e3 * (time.perf_counter() - t0), 2)}
+            del h
+        out["fp32_plain"]["note"] = note
         log("plain FP32 solve: %s" % out["fp32_plain"])
         s = f
     os.remove(start)

@@ -229,6 +229,28 @@ __device__ inline void unlin(size_t l, int n0, int n1, int& I, int& J, int& K) {
     J = (int)(q - (uint32_t)K * (uint32_t)n1);
 }
 
+// atomicAdd(&counts[idx], 1) for every lane with idx >= 0, with one atomic per distinct
+// index among the wavefront's lanes (cell-sorted particles share a few cells a wavefront,
+// and same-address atomics serialise): returns the lane's slot, the old count plus its
+// place among the lanes of that index in lane order (-1 for idx < 0). Every active lane of
+// the wavefront must call it (the loop runs on wavefront-uniform values).
+__device__ inline int32_t wave_count_slot(int32_t* __restrict__ counts, int32_t idx) {
+    const int lane = (int)(threadIdx.x & 63);
+    unsigned long long todo = __ballot(idx >= 0);
+    int32_t slot = -1;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int32_t k = __shfl(idx, leader, 64);
+        const unsigned long long grp = __ballot(idx == k) & todo;
+        int32_t base = 0;
+        if (lane == leader) base = atomicAdd(&counts[k], (int32_t)__popcll(grp));
+        base = __shfl(base, leader, 64);
+        if ((grp >> lane) & 1ull) slot = base + (int32_t)__popcll(grp & ((1ull << lane) - 1ull));
+        todo &= ~grp;
+    }
+    return slot;
+}
+
 // ─── fixed-point accumulation (deterministic P2G) ────────────────────────────
 // A contribution c is accumulated as round(c * 2^32) in a 64-bit integer: integer
 // addition is associative, so the sum does not depend on the order the particles
@@ -237,6 +259,14 @@ constexpr double FIX_SCALE = 4294967296.0;
 constexpr double FIX_INV = 1.0 / 4294967296.0;
 __device__ inline unsigned long long tofix(float c) {
     return (unsigned long long)(long long)__double2ll_rn((double)c * FIX_SCALE);
+}
+// tofix for |c| < 2^19, in fewer instructions: c 2^32 is exact in FP64, and adding
+// 1.5 * 2^52 rounds it to an integer (to nearest even, as tofix does) whose two's
+// complement is the sum's low bits.
+__device__ inline unsigned long long tofix_near(float c) {
+    constexpr double MAGIC = 6755399441055744.0;   // 1.5 * 2^52
+    const double x = (double)c * FIX_SCALE + MAGIC;
+    return (unsigned long long)(__double_as_longlong(x) - __double_as_longlong(MAGIC));
 }
 __device__ inline float fromfix(unsigned long long v) { return (float)((double)(long long)v * FIX_INV); }
 

@@ -125,7 +125,7 @@ struct P2GSource {   // flip.jl:513-537: v_a and the APIC row c_a, weighted by t
 // grid (the dense grid's array bounds, or no fine block holding the face). `ps` gives the
 // positions (fields 0-2).
 template <class NodeMap, class Source>
-__global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiled(PView ps, int64_t n, NodeMap map, Source src, P2GArgs A) {
+__global__ __launch_bounds__(P2G_THREADS, 4) void k_p2g_tiled(PView ps, int64_t n, NodeMap map, Source src, P2GArgs A) {
     __shared__ unsigned long long smom[P2G_BOX], swt[P2G_BOX];
     __shared__ int red[6 * (P2G_THREADS / 64)];
     const int64_t p0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * P2G_THREADS;   // neighbouring tiles on one XCD
@@ -169,14 +169,42 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiled(PView ps, int64_t n, 
         if (T.use)
             for (int k = first; k < T.size(); k += stride) { smom[T.base + k] = 0ull; swt[T.base + k] = 0ull; }
         __syncthreads();
-        if (live) {
+        // The 27 contributions in the order of operations of flip.jl:524-533, with the
+        // factors shared along rows hoisted: wgt = ((wT w_x) w_y) w_z and
+        // v = vp + ((c0 d0 + c1 d1) + c2 d2) are the same operations on the same values, so
+        // the same bits. A particle whose contributions all lie within 2^18 in magnitude
+        // takes the short fixed-point rounding (tofix_near); others take tofix.
+        const float bound = wT * (fabsf(vp) + (fabsf(c0) + fabsf(c1) + fabsf(c2)) * (2.0f * A.dx)) * fabsf(A.scale);
+        if (live && T.use && ok && bound < 262144.0f) {
+            const float wx[3] = {bx.w0, bx.w1, bx.w2}, wy[3] = {by.w0, by.w1, by.w2}, wz[3] = {bz.w0, bz.w1, bz.w2};
+            float cd0[3], cd1[3], cd2[3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                cd0[m] = c0 * (((float)(bx.b + m) - q0) * A.dx);
+                cd1[m] = c1 * (((float)(by.b + m) - q1) * A.dx);
+                cd2[m] = c2 * (((float)(bz.b + m) - q2) * A.dx);
+            }
+            const int l0 = T.at(bx.b, by.b, bz.b);
+#pragma unroll
+            for (int my = 0; my < 3; ++my)
+#pragma unroll
+                for (int mx = 0; mx < 3; ++mx) {
+                    const float wxy = (wT * wx[mx]) * wy[my], cxy = cd0[mx] + cd1[my];
+#pragma unroll
+                    for (int mz = 0; mz < 3; ++mz) {
+                        const float wgt = wxy * wz[mz];
+                        const int l = l0 + mx + T.ex * (my + T.ey * mz);
+                        atomicAdd(&smom[l], tofix_near((wgt * (vp + (cxy + cd2[mz]))) * A.scale));
+                        atomicAdd(&swt[l], tofix_near(wgt));
+                    }
+                }
+        } else if (live) {   // the rare path: large values, or a particle outside the box
             for (int mz = 0; mz < 3; ++mz) {
                 int K = bz.b + mz;
                 float wz = pick(bz, mz), d2 = ((float)K - q2) * A.dx;
                 for (int my = 0; my < 3; ++my) {
                     int J = by.b + my;
                     float wy = pick(by, my), d1 = ((float)J - q1) * A.dx;
-#pragma unroll
                     for (int mx = 0; mx < 3; ++mx) {
                         int I = bx.b + mx;
                         float wgt = wT * pick(bx, mx) * wy * wz;

@@ -58,14 +58,9 @@ __global__ void k_dense_bin(PView ps, int64_t n, const float* __restrict__ phi, 
         float f = trilinear(phi, g.n[0], g.n[1], g.n[2], c0 - 0.5f, c1 - 0.5f, c2 - 0.5f);
         keep = -deep <= f && f < above;
     }
-    if (keep) {
-        int32_t l = (int32_t)floorf(c0) + g.n[0] * ((int32_t)floorf(c1) + g.n[1] * (int32_t)floorf(c2));
-        key[p] = l;
-        rank[p] = atomicAdd(&counts[l], 1);   // arrival: only sizes the segment
-    } else {
-        key[p] = -1;
-        rank[p] = -1;
-    }
+    const int32_t l = keep ? (int32_t)floorf(c0) + g.n[0] * ((int32_t)floorf(c1) + g.n[1] * (int32_t)floorf(c2)) : -1;
+    key[p] = l;
+    rank[p] = wave_count_slot(counts, l);   // arrival: only sizes the segment
 }
 
 // Each kept particle into its cell's segment, at its arrival slot.
@@ -77,16 +72,41 @@ __global__ void k_place(int64_t n, const int32_t* __restrict__ key, const int32_
     if (l >= 0) seg[cstart[l] + rank[p]] = (int32_t)p;
 }
 
-// Each cell's segment into index order (insertion sort: segments are short and
-// nearly sorted, the previous resample wrote particles cell by cell); the rank is the
-// position there.
+// Each cell's particles in index order; the rank is the position there. Up to 16 (nearly
+// every cell: nmin is 8, and 2 nmin survive) are sorted in registers by a bitonic network;
+// a more crowded cell is sorted in place by insertion (its segment is nearly sorted: the
+// previous resample wrote particles cell by cell, and the binning numbers a wavefront's
+// particles of one cell in lane order).
 __global__ void k_fixrank(size_t C, const int32_t* __restrict__ counts, const int32_t* __restrict__ cstart,
                           int32_t* __restrict__ seg, int32_t* __restrict__ rank) {
     size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= C) return;
-    int32_t c = counts[l];
+    const int32_t c = counts[l];
     if (c < 2) return;
     int32_t* s = seg + cstart[l];
+    if (c <= 16) {
+        int32_t v[16];
+#pragma unroll
+        for (int a = 0; a < 16; ++a) v[a] = a < c ? s[a] : INT32_MAX;
+#pragma unroll
+        for (int k = 2; k <= 16; k <<= 1)
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int m = i ^ j;
+                    if (m > i) {
+                        const int32_t x = v[i], y = v[m];
+                        const bool up = (i & k) == 0;
+                        v[i] = up ? min(x, y) : max(x, y);
+                        v[m] = up ? max(x, y) : min(x, y);
+                    }
+                }
+#pragma unroll
+        for (int a = 0; a < 16; ++a)
+            if (a < c) rank[v[a]] = a;
+        return;
+    }
     for (int32_t a = 1; a < c; ++a) {
         int32_t v = s[a];
         int32_t b = a - 1;

@@ -65,7 +65,7 @@ __global__ void k_sparse_bin(PView ps, int64_t n, const float* __restrict__ phi,
         }
     }
     key[p] = idx;
-    rank[p] = idx >= 0 ? atomicAdd(&counts[idx], 1) : -1;
+    rank[p] = wave_count_slot(counts, idx);   // arrival: only sizes the segment
 }
 
 // sflip.jl:2184-2218 sparsespawn_kernel!, in two passes so that the threads of a launch
@@ -405,13 +405,38 @@ __global__ __launch_bounds__(256) void k_sparse_g2p(PView ps, int64_t n, STopo t
         }
         if (live) {
             if (boxed && ok) {
-                auto fetch = [&](int I, int J, int K, bool& pres, float& uv, float& fv_) {
-                    const float2 e = sbox[(I - x0) + ex * ((J - y0) + ey * (K - z0))];
-                    pres = e.y >= 0.0f;
-                    uv = e.x;
-                    fv_ = e.y;
-                };
-                gather_component<true>(g, a, xp, yp, zp, fetch, fl[a], val[a], c0[a], c1[a], c2[a]);
+                // gather_component<true> over the box, the factors shared along rows hoisted:
+                // the same operations on the same values in the same order, so the same bits
+                const float wx[3] = {bx.w0, bx.w1, bx.w2}, wy[3] = {by.w0, by.w1, by.w2}, wz[3] = {bz.w0, bz.w1, bz.w2};
+                float d0[3], d1[3], d2[3];
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    d0[m] = (float)(bx.b + m) - q0;
+                    d1[m] = (float)(by.b + m) - q1;
+                    d2[m] = (float)(bz.b + m) - q2;
+                }
+                const float2* bp = sbox + ((bx.b - x0) + ex * ((by.b - y0) + ey * (bz.b - z0)));
+                float f_ = 0.0f, v_ = 0.0f, s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+                for (int mz = 0; mz < 3; ++mz)
+#pragma unroll
+                    for (int my = 0; my < 3; ++my) {
+                        const float2* row = bp + ex * (my + ey * mz);
+#pragma unroll
+                        for (int mx = 0; mx < 3; ++mx) {
+                            const float2 e = row[mx];
+                            float wgt = (wx[mx] * wy[my]) * wz[mz];
+                            wgt = e.y >= 0.0f ? wgt : 0.0f;
+                            f_ += wgt * e.y;
+                            const float tt = wgt * e.x;
+                            v_ += tt;
+                            s0 += (tt * d0[mx]) * g.dx;
+                            s1 += (tt * d1[my]) * g.dx;
+                            s2 += (tt * d2[mz]) * g.dx;
+                        }
+                    }
+                const float c = __fdiv_rn(4.0f, g.dx * g.dx);
+                fl[a] = f_; val[a] = v_; c0[a] = s0 * c; c1[a] = s1 * c; c2[a] = s2 * c;
             } else {
                 auto fetch = [&](int I, int J, int K, bool& pres, float& uv, float& fv_) {
                     int32_t idx = finefaceat(t, a, I, J, K);

@@ -134,25 +134,98 @@ __device__ inline float gather_value(const V* __restrict__ u, int n1, int n2, in
     return vel;
 }
 
-// flip.jl:1226-1244 g2p_kernel!.
+// flip.jl:1226-1244 g2p_kernel!, tiled: per component, a workgroup's 256 particles
+// (cell-sorted, so close together) gather the flags and the velocity from a box of
+// (value, flag) staged in LDS, one read of each face per workgroup instead of two per
+// particle and node; a workgroup spread over more nodes than the box holds, or a particle
+// outside it, reads the faces directly. Both paths take the flags' sum (gather_value) and
+// the APIC sums (gather) in their order of operations, with the indices clamped to the
+// array as there, so the result is theirs bit for bit.
+constexpr int DG2P_BOX = 1024;
 __global__ __launch_bounds__(256) void k_dense_g2p(PView ps, int64_t n, DGrid g, const float* __restrict__ u,
                                                    const float* __restrict__ v, const float* __restrict__ w,
                                                    const uint8_t* __restrict__ fu, const uint8_t* __restrict__ fv,
                                                    const uint8_t* __restrict__ fw, float dt, float gravity) {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    float xp = ps.f[0][p], yp = ps.f[1][p], zp = ps.f[2][p];
-    int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
-    bool k1 = gather_value(fu, n1 + 1, n2, n3, g, 0, xp, yp, zp) > 0.999f;
-    bool k2 = gather_value(fv, n1, n2 + 1, n3, g, 1, xp, yp, zp) > 0.999f;
-    bool k3 = gather_value(fw, n1, n2, n3 + 1, g, 2, xp, yp, zp) > 0.999f;
+    __shared__ float2 sbox[DG2P_BOX];   // value, flag
+    __shared__ int red[6 * 4];
+    const int64_t p = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;   // neighbouring tiles on one XCD
+    const bool live = p < n;
+    float xp = 0, yp = 0, zp = 0;
+    if (live) { xp = ps.f[0][p]; yp = ps.f[1][p]; zp = ps.f[2][p]; }
+    const float* U[3] = {u, v, w};
+    const uint8_t* F[3] = {fu, fv, fw};
+    float fl[3], val[3], c0[3], c1[3], c2[3];
+    for (int a = 0; a < 3; ++a) {
+        const int n1 = g.n[0] + (a == 0), n2 = g.n[1] + (a == 1), n3 = g.n[2] + (a == 2);
+        const float q0 = ispace(xp, g.lo[0], g.dx, a == 0 ? 0.0f : 0.5f);
+        const float q1 = ispace(yp, g.lo[1], g.dx, a == 1 ? 0.0f : 0.5f);
+        const float q2 = ispace(zp, g.lo[2], g.dx, a == 2 ? 0.0f : 0.5f);
+        const BSpline bx = bspline(q0), by = bspline(q1), bz = bspline(q2);
+        const bool ok = live && bx.b > -(1 << 24) && bx.b < (1 << 24) && by.b > -(1 << 24) && by.b < (1 << 24) &&
+                        bz.b > -(1 << 24) && bz.b < (1 << 24) && isfinite(q0) && isfinite(q1) && isfinite(q2);
+        const int BIG = 1 << 30;
+        int x0, y0, z0, x1, y1, z1;
+        wg_bounds(ok, bx.b, by.b, bz.b, red, x0, y0, z0, x1, y1, z1);
+        const int ex = x1 - x0 + 3, ey = y1 - y0 + 3, ez = z1 - z0 + 3;
+        const bool boxed = x0 != BIG && ex <= DG2P_BOX && ey <= DG2P_BOX && ez <= DG2P_BOX && ex * ey * ez <= DG2P_BOX;
+        const float* Ua = U[a];
+        const uint8_t* Fa = F[a];
+        if (boxed) {
+            const int V = ex * ey * ez;
+            for (int k = threadIdx.x; k < V; k += 256) {
+                const int i = min(max(x0 + k % ex, 0), n1 - 1), j = min(max(y0 + (k / ex) % ey, 0), n2 - 1),
+                          kk = min(max(z0 + k / (ex * ey), 0), n3 - 1);
+                const size_t idx = (size_t)i + (size_t)n1 * ((size_t)j + (size_t)n2 * kk);
+                sbox[k] = make_float2(Ua[idx], (float)Fa[idx]);
+            }
+            __syncthreads();
+        }
+        if (live) {
+            if (boxed && ok) {
+                const float wx[3] = {bx.w0, bx.w1, bx.w2}, wy[3] = {by.w0, by.w1, by.w2}, wz[3] = {bz.w0, bz.w1, bz.w2};
+                float d0[3], d1[3], d2[3];
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    d0[m] = (float)(bx.b + m) - q0;
+                    d1[m] = (float)(by.b + m) - q1;
+                    d2[m] = (float)(bz.b + m) - q2;
+                }
+                const float2* bp = sbox + ((bx.b - x0) + ex * ((by.b - y0) + ey * (bz.b - z0)));
+                float f_ = 0.0f, v_ = 0.0f, s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+                for (int mz = 0; mz < 3; ++mz)
+#pragma unroll
+                    for (int my = 0; my < 3; ++my) {
+                        const float2* row = bp + ex * (my + ey * mz);
+#pragma unroll
+                        for (int mx = 0; mx < 3; ++mx) {
+                            const float2 e = row[mx];
+                            const float wgt = (wx[mx] * wy[my]) * wz[mz];
+                            f_ += wgt * e.y;
+                            const float t = wgt * e.x;
+                            v_ += t;
+                            s0 += (t * d0[mx]) * g.dx;
+                            s1 += (t * d1[my]) * g.dx;
+                            s2 += (t * d2[mz]) * g.dx;
+                        }
+                    }
+                const float c = __fdiv_rn(4.0f, g.dx * g.dx);
+                fl[a] = f_; val[a] = v_; c0[a] = s0 * c; c1[a] = s1 * c; c2[a] = s2 * c;
+            } else {
+                fl[a] = gather_value(Fa, n1, n2, n3, g, a, xp, yp, zp);
+                gather(Ua, n1, n2, n3, g, a, xp, yp, zp, val[a], c0[a], c1[a], c2[a]);
+            }
+        }
+        if (boxed) __syncthreads();   // the box is reused by the next component
+    }
+    if (!live) return;
     // v and c (fields 3..14) built in registers and stored unconditionally, so no store's
     // target depends on the divergent branch (see k_dense_spawn).
     float rec[12];
-    if (k1 && k2 && k3) {
-        gather(u, n1 + 1, n2, n3, g, 0, xp, yp, zp, rec[0], rec[3], rec[4], rec[5]);
-        gather(v, n1, n2 + 1, n3, g, 1, xp, yp, zp, rec[1], rec[6], rec[7], rec[8]);
-        gather(w, n1, n2, n3 + 1, g, 2, xp, yp, zp, rec[2], rec[9], rec[10], rec[11]);
+    if (fl[0] > 0.999f && fl[1] > 0.999f && fl[2] > 0.999f) {
+        rec[0] = val[0]; rec[3] = c0[0]; rec[4] = c1[0]; rec[5] = c2[0];
+        rec[1] = val[1]; rec[6] = c0[1]; rec[7] = c1[1]; rec[8] = c2[1];
+        rec[2] = val[2]; rec[9] = c0[2]; rec[10] = c1[2]; rec[11] = c2[2];
     } else {
         rec[0] = ps.f[3][p];
         rec[1] = ps.f[4][p];

@@ -240,6 +240,14 @@ void orc_dense_vcycle(const orc_grid* g, const uint8_t* types, const float* phi,
     S.setup(types, phi);
     S.vcycle(z, r, sweeps, coarse);
 }
+// The same V-cycle in FP32 arithmetic, in the reference's order of operations.
+void orc_dense_vcycle_f32(const orc_grid* g, const uint8_t* types, const float* phi, const float* r, float* z,
+                          int32_t sweeps, int32_t coarse) {
+    DensePressure<float> S;
+    S.g = to_grid(g);
+    S.setup(types, phi);
+    S.vcycle(z, r, sweeps, coarse);
+}
 // The fine level's operator as COO triplets (rows, cols, vals); returns nnz. Pass null to count.
 int64_t orc_dense_matrix(const orc_grid* g, const uint8_t* types, const float* phi, int64_t* rows, int64_t* cols,
                          double* vals) {

@@ -19,6 +19,7 @@ void orc_set_error(const char* msg);
 
 struct OrcSparse {
     SparseState s;
+    Field<double> P64;   // orc_sparse_solve64's pressure, kept between calls
 };
 
 static void grid_out(const SparseGrid& g, orc_sgrid* o) {
@@ -380,6 +381,64 @@ extern "C" {
 
 // sflip.jl:1998-2015 project! with the given γ: coefficients, rhs, clear, the solve
 // (policy 0: flexible CG; 1: FGMRES; 2: CG then FGMRES, fgmres.jl:96-110), the faces.
+// The FP64 pressure solve alone (sflip.jl:1867-1904 pressure! with the rhs and clearing of
+// project!, sflip.jl:1998-2013), continuable: with `cont` it starts from the FP64 pressure
+// the previous call left (not the FP32 state's), so a long solve can run in chunks of
+// `maxiter` iterations with the host reporting between them. The velocities are not
+// projected; s.P receives the pressure rounded to FP32. Returns the true relative residual.
+int orc_sparse_solve64(void* p, float gamma, double tol, int32_t maxiter, int32_t sweeps, int32_t coarse, double boost,
+                       int32_t cont, int32_t* iters, double* relres) {
+    auto& h = *(OrcSparse*)p;
+    auto& s = h.s;
+    try {
+        SparsePressure<double> S(s.t, s.types.fine.data(), s.types.coarse.data());
+        S.coefficients(s.phi.fine.data(), s.phi.coarse.data());
+        SFc u[3] = {{s.u[0].fine.data(), s.u[0].coarse.data()}, {s.u[1].fine.data(), s.u[1].coarse.data()},
+                    {s.u[2].fine.data(), s.u[2].coarse.data()}};
+        Field<double> b, P = S.field();
+        sparse_rhs<double>(s.t, s.types.fine.data(), s.types.coarse.data(), u, gamma, b);
+        const bool warm = cont && h.P64.fine.size() == P.fine.size() && h.P64.coarse.size() == P.coarse.size();
+        for (size_t n = 0; n < S.nf(); ++n)
+            P.fine[n] = (s.live((int)(n >> 12) + 1) && s.types.fine[n] != LIQUID) ? 0.0
+                                                                                   : (warm ? h.P64.fine[n] : (double)s.P.fine[n]);
+        for (int K = 0; K < s.nc[2]; ++K)
+            for (int J = 0; J < s.nc[1]; ++J)
+                for (int I = 0; I < s.nc[0]; ++I) {
+                    size_t ci = cidx(s.nc, I, J, K);
+                    P.coarse[ci] = (s.types.coarse[ci] != LIQUID || s.t.covered(I, J, K))
+                                       ? 0.0
+                                       : (warm ? h.P64.coarse[ci] : (double)s.P.coarse[ci]);
+                }
+        S.hierarchy();
+        double rr = 0;
+        int it = S.fcg(P, b, tol, maxiter, sweeps, boost, coarse, &rr);
+        Field<double> q = S.field(), r = S.field();
+        S.apply(q, P);
+        r = b;
+        S.axpy(r, -1.0, q);
+        double bn = std::sqrt(S.inner(b, b)), rn = std::sqrt(S.inner(r, r));
+        h.P64 = P;
+        for (size_t n = 0; n < S.nf(); ++n) s.P.fine[n] = (float)P.fine[n];
+        for (size_t c = 0; c < P.coarse.size(); ++c) s.P.coarse[c] = (float)P.coarse[c];
+        if (iters) *iters = it;
+        if (relres) *relres = bn > 0 ? rn / bn : 0.0;
+        return 0;
+    } catch (const SolveError& e) {
+        orc_set_error(e.what());
+        return -5;
+    } catch (const std::exception& e) {
+        orc_set_error(e.what());
+        return -1;
+    }
+}
+// The FP64 pressure of orc_sparse_solve64 (fine pools, then the coarse grid), as doubles.
+int orc_sparse_p64(void* p, double* fine, double* coarse) {
+    auto& h = *(OrcSparse*)p;
+    if (h.P64.fine.empty()) return -1;
+    std::copy(h.P64.fine.begin(), h.P64.fine.end(), fine);
+    std::copy(h.P64.coarse.begin(), h.P64.coarse.end(), coarse);
+    return 0;
+}
 int orc_sparse_project(void* p, float gamma, double tol, int32_t maxiter, int32_t sweeps, int32_t coarse, double boost,
                        int32_t restart, int32_t policy, int32_t fp32, int32_t* iters, double* relres, int32_t* used_fgmres) {
     auto& s = ((OrcSparse*)p)->s;

@@ -76,6 +76,7 @@ def lib():
             "orc_dense_volume": (f64, [G, p]),
             "orc_dense_rhs": (None, [G, p, p, p, p, f32, p]),
             "orc_dense_apply": (None, [G, p, p, p, p]),
+            "orc_dense_vcycle_f32": (None, [G, p, p, p, p, i32, i32]),
             "orc_dense_vcycle": (None, [G, p, p, p, p, i32, i32]),
             "orc_dense_matrix": (i64, [G, p, p, p, p, p]),
             "orc_dense_pressure": (i32, [G, p, p, p, p, f64, i32, i32, i32, C.POINTER(f64)]),
@@ -391,11 +392,13 @@ def dense_apply(g, types, phi, x):
     return q
 
 
-def dense_vcycle(g, types, phi, r, sweeps=2, coarse=40):
-    types, phi, r = _c(types, np.uint8), _c(phi, np.float32), _c(r, np.float64)
-    z = np.zeros(cells_shape(g), np.float64)
-    lib().orc_dense_vcycle(C.byref(g), types.ctypes.data, phi.ctypes.data, r.ctypes.data, z.ctypes.data, sweeps,
-                           coarse)
+def dense_vcycle(g, types, phi, r, sweeps=2, coarse=40, fp32=False):
+    """flip.jl:986-1014 vcycle!: z = M r, in FP64 (or FP32 with `fp32`)."""
+    dt = np.float32 if fp32 else np.float64
+    types, phi, r = _c(types, np.uint8), _c(phi, np.float32), _c(r, dt)
+    z = np.zeros(cells_shape(g), dt)
+    fn = lib().orc_dense_vcycle_f32 if fp32 else lib().orc_dense_vcycle
+    fn(C.byref(g), types.ctypes.data, phi.ctypes.data, r.ctypes.data, z.ctypes.data, sweeps, coarse)
     return z
 
 
@@ -539,6 +542,8 @@ def _slib():
             "orc_sparse_vcycle": (None, [p, p, p, p, p, i32, i32, f64]),
             "orc_sparse_levels": (i32, [p, p]),
             "orc_sparse_project": (C.c_int, [p, f32, f64, i32, i32, i32, f64, i32, i32, i32, P32, PF64, P32]),
+            "orc_sparse_solve64": (C.c_int, [p, f32, f64, i32, i32, i32, f64, i32, P32, PF64]),
+            "orc_sparse_p64": (C.c_int, [p, p, p]),
         }
         for name, (res, args) in _SPARSE_SIG.items():
             fn = getattr(L, name)
@@ -748,6 +753,25 @@ class SparseOracle:
         if st != 0:
             raise OracleError(st, lib().orc_last_error().decode())
         return dict(iterations=it.value, rel_residual=rr.value, fgmres=bool(fg.value))
+
+    def solve64(self, gamma=0.0, tol=1e-10, maxiter=8, sweeps=1, coarse=40, boost=1.0, cont=False):
+        """The FP64 flexible-CG pressure solve alone (no face projection), continuable: with
+        `cont` it starts from the FP64 pressure the previous call left. Returns the
+        iterations and the true relative residual."""
+        it, rr = C.c_int32(), C.c_double()
+        st = _slib().orc_sparse_solve64(self.h, gamma, tol, maxiter, sweeps, coarse, boost, 1 if cont else 0,
+                                        C.byref(it), C.byref(rr))
+        if st != 0:
+            raise OracleError(st, lib().orc_last_error().decode())
+        return dict(iterations=it.value, rel_residual=rr.value)
+
+    def p64(self):
+        """The FP64 pressure of the last solve64: (fine pools, coarse grid)."""
+        nf, nc = self.array("P_f").size, self.array("P_c").size
+        fine, coarse = np.zeros(nf, np.float64), np.zeros(nc, np.float64)
+        if _slib().orc_sparse_p64(self.h, _ptr(fine), _ptr(coarse)) != 0:
+            raise OracleError(-1, "no FP64 pressure yet")
+        return fine, coarse
 
 
 def sparse_scene(radius=18.0, depth=12.0, top=3.0, dx=0.6, G=225.0, core=2.5, gravity=9.81, blocks=64,

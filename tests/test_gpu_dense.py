@@ -261,6 +261,11 @@ def test_vcycle_matches_oracle(mael, orc, ctx, scene, queued):
     zo = orc.dense_vcycle(g, sc["types"], sc["phi"], r)
     print("V-cycle rel L2 vs FP64 oracle %.3e" % rel_l2(z, zo))
     assert rel_l2(z, zo) < 1e-5, rel_l2(z, zo)
+    # the oracle's FP32 V-cycle takes every sum and division in the same order: the same
+    # bits (the fused two-colour smoothing passes included)
+    z32 = orc.dense_vcycle(g, sc["types"], sc["phi"], r, fp32=True)
+    print("V-cycle vs FP32 oracle: %d of %d differ" % ((z.view(np.uint32) != z32.view(np.uint32)).sum(), z.size))
+    assert np.array_equal(z.view(np.uint32), z32.view(np.uint32))
     s.ctx.check(mael.lib().mael_dense_vcycle(s.h, None, 1 - queued))
     assert np.array_equal(s.get("P").view(np.uint32), z.view(np.uint32))
 

@@ -392,6 +392,10 @@ def test_vcycle_is_symmetric_linear(orc, scene):
     Mxy = orc.dense_vcycle(g, scene["types"], scene["phi"], 2 * x + 3 * y)
     assert np.allclose(Mxy, 2 * Mx + 3 * My, atol=1e-10)
     assert (Mx * x).sum() > 0
+    # the FP32 V-cycle is the same algorithm in single precision
+    Mx32 = orc.dense_vcycle(g, scene["types"], scene["phi"], x.astype(np.float32), fp32=True)
+    assert Mx32.dtype == np.float32
+    assert np.linalg.norm(Mx32 - Mx) <= 1e-5 * np.linalg.norm(Mx)
 
 
 def test_pcg_matches_scipy_direct_solve(orc, scene):

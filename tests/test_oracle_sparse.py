@@ -399,6 +399,39 @@ def test_solvers_match_direct(policy):
     assert abs(b2[act]).max() < 2e-5 * abs(b[act]).max() + 1e-6
 
 
+def test_chunked_fp64_solve_matches_direct():
+    """The continuable FP64 solve (solve64) in chunks of 4 iterations, each warm-started from
+    the last chunk's FP64 pressure, converges to spsolve's solution to 1e-9 relative; a
+    chunk that restarted from the FP32-rounded pressure instead could not go below the
+    rounding of P."""
+    o = O.sparse_scene(dx=0.6, blocks=64, capacity=300000)
+    rng = np.random.default_rng(12)
+    for nm in ("u_f", "v_f", "w_f", "u_c", "v_c", "w_c"):
+        arr = o.array(nm)
+        arr[:] += rng.normal(0, 0.3, arr.size).astype(np.float32)
+    o.solidify()
+    o.restrict_velocity()
+    o.array("P_f")[:] = 0
+    o.array("P_c")[:] = 0
+    A = o.matrix(16)
+    bf, bc = o.rhs(0.0)
+    b = np.concatenate([bf, bc])
+    act = np.flatnonzero(np.diff(A.indptr))
+    x = np.zeros_like(b)
+    x[act] = spla.spsolve(A[act][:, act].tocsc(), b[act])
+    total, st = 0, None
+    for k in range(60):
+        st = o.solve64(gamma=0.0, tol=1e-12, maxiter=4, cont=k > 0)
+        total += st["iterations"]
+        if st["rel_residual"] <= 1e-12:
+            break
+    assert st["rel_residual"] <= 1e-12 and total > 8   # several chunks were needed
+    pf, pc = o.p64()
+    ns = o.nslots
+    P = np.concatenate([pf[:ns * 4096], pc])
+    assert np.linalg.norm(P[act] - x[act]) <= 1e-9 * np.linalg.norm(x[act])
+
+
 def test_policy_error_texts():
     """A zero right-hand side returns at once; a nonfinite one raises the reference's text."""
     o = O.sparse_scene(dx=0.6, blocks=64, capacity=300000)

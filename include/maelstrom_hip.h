@@ -247,6 +247,11 @@ enum {
     MAEL_S_MOM_U, MAEL_S_MOM_V, MAEL_S_MOM_W, MAEL_S_WT_U, MAEL_S_WT_V, MAEL_S_WT_W,   /* int64 fixed point 2^-32 */
     MAEL_S_SLOT,                                                      /* per block: 0 or the 1-based slot (int32) */
     MAEL_S_WHERE,                                                     /* per slot: block x, y, z, 0 (int32 x4; -1 free) */
+    MAEL_S_UB_U_F, MAEL_S_UB_U_C, MAEL_S_UB_V_F, MAEL_S_UB_V_C, MAEL_S_UB_W_F, MAEL_S_UB_W_C,   /* adjoints: face cotangents */
+    MAEL_S_UB0_U_F, MAEL_S_UB0_U_C, MAEL_S_UB0_V_F, MAEL_S_UB0_V_C, MAEL_S_UB0_W_F, MAEL_S_UB0_W_C, /* ... of u0 */
+    MAEL_S_PB_VX, MAEL_S_PB_VY, MAEL_S_PB_VZ,                         /* ... of the particles' v (capacity) */
+    MAEL_S_PB_C0, MAEL_S_PB_C1, MAEL_S_PB_C2, MAEL_S_PB_C3, MAEL_S_PB_C4, MAEL_S_PB_C5, MAEL_S_PB_C6, MAEL_S_PB_C7,
+    MAEL_S_PB_C8,                                                     /* ... and of their APIC rows */
     MAEL_S_NARRAYS
 };
 int mael_sparse_array(mael_sparse *s, int32_t which, void **dptr, int64_t *count, int32_t *elsize);
@@ -305,6 +310,22 @@ void mael_sparse_defaults(mael_sparse_opts *o);
 /* `project!(s, dt)` (sflip.jl:1998-2015) after the caller's γ: couplings, rhs, clear,
  * the solve by `opts->policy`, the faces and flags. Errors carry the reference's texts. */
 int mael_sparse_project(mael_sparse *s, float gamma, const mael_sparse_opts *opts, mael_sparse_stats *stats);
+
+/* Adjoints of the sparse hot path's linear maps, as mael_dense_g2p_adjoint's (no
+ * counterpart in the reference, which has no AD), at the state the forward phase left
+ * (positions, lags, level set, topology, types, flags and deposit weights held fixed).
+ * Cotangents live in MAEL_S_UB_*, MAEL_S_UB0_* (every stored face copy: a face on the
+ * plane between two fine blocks has one in each, as the forward phases read and write
+ * each) and MAEL_S_PB_*.
+ *   g2p_adjoint:     UB = G2P^T (PB) (coarse parts 0: G2P reads fine faces; with
+ *                    mael_sparse_set_flow(1) the faces are the reflected flow's); PB
+ *                    becomes the cotangent of G2P's input particles.
+ *   project_adjoint: UB <- R^T UB, then UB += D^T A^-1 (-G^T UB) (the transposed solve with
+ *                    `opts`; A is symmetric); P is left holding the adjoint pressure.
+ *   p2g_adjoint:     PB += P2G^T (UB) for the particles' v and rows, UB0 = the part for u0. */
+int mael_sparse_g2p_adjoint(mael_sparse *s, float dt);
+int mael_sparse_p2g_adjoint(mael_sparse *s, float dt);
+int mael_sparse_project_adjoint(mael_sparse *s, const mael_sparse_opts *opts, mael_sparse_stats *stats);
 
 /* The flexible CG's preconditioner alone (sflip.jl:1883, 1901 `vcycle!`, the V-cycle of
  * sflip.jl:1218-1311 with opts->sweeps, coarse, boost and precision): the couplings and

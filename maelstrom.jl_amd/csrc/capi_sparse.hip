@@ -51,6 +51,18 @@ ArrRef sparse_array(mael::SparseBasin& b, int which) {
     if (which >= MAEL_S_PX && which <= MAEL_S_LAG) return {b.pbuf[c][which - MAEL_S_PX].p, b.capacity, 4};
     if (which >= MAEL_S_MOM_U && which <= MAEL_S_MOM_W) return {b.mom[which - MAEL_S_MOM_U].p, nfa, 8};
     if (which >= MAEL_S_WT_U && which <= MAEL_S_WT_W) return {b.wt[which - MAEL_S_WT_U].p, nfa, 8};
+    if (which >= MAEL_S_UB_U_F && which <= MAEL_S_UB0_W_C) {   // allocated on first use
+        b.adj_alloc();
+        bool zero = which >= MAEL_S_UB0_U_F;
+        int k = which - (zero ? MAEL_S_UB0_U_F : MAEL_S_UB_U_F), a = k / 2;
+        bool fine = (k & 1) == 0;
+        SField<float>& f = zero ? b.ub0[a] : b.ub[a];
+        return {fine ? (void*)f.fine.p : (void*)f.coarse.p, face(a, fine), 4};
+    }
+    if (which >= MAEL_S_PB_VX && which <= MAEL_S_PB_C8) {
+        b.adj_alloc();
+        return {b.pb[which - MAEL_S_PB_VX].p, b.capacity, 4};
+    }
     throw mael::Error(mael::E_ARG, "unknown sparse array " + std::to_string(which));
 }
 
@@ -248,6 +260,27 @@ int mael_sparse_project(mael_sparse* s, float gamma, const mael_sparse_opts* opt
         mael::SparseOpts o = sparse_opts(opts);
         mael::SparseStats st;
         s->b->project(gamma, o, st);
+        if (stats) {
+            stats->iterations = st.iterations; stats->fcg_iterations = st.fcg_iterations;
+            stats->fgmres_iterations = st.fgmres_iterations; stats->used_fgmres = st.used_fgmres;
+            stats->rel_residual = st.rel_residual; stats->ms = st.ms;
+        }
+    });
+}
+
+int mael_sparse_g2p_adjoint(mael_sparse* s, float dt) {
+    if (!s) return MAEL_E_ARG;
+    return guard(s->ctx, [&] { s->b->g2p_adjoint(dt); });
+}
+int mael_sparse_p2g_adjoint(mael_sparse* s, float dt) {
+    if (!s) return MAEL_E_ARG;
+    return guard(s->ctx, [&] { s->b->p2g_adjoint(dt); });
+}
+int mael_sparse_project_adjoint(mael_sparse* s, const mael_sparse_opts* opts, mael_sparse_stats* stats) {
+    if (!s) return MAEL_E_ARG;
+    return guard(s->ctx, [&] {
+        mael::SparseStats st;
+        s->b->project_adjoint(sparse_opts(opts), st);
         if (stats) {
             stats->iterations = st.iterations; stats->fcg_iterations = st.fcg_iterations;
             stats->fgmres_iterations = st.fgmres_iterations; stats->used_fgmres = st.used_fgmres;

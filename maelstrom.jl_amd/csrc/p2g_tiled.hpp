@@ -318,4 +318,65 @@ __global__ __launch_bounds__(P2G_THREADS) void k_splat_tiled(PView ps, int64_t n
     }
 }
 
+// ─── adjoints of the transfers (no counterpart in the reference) ─────────────
+// With positions, lags, level set and weights fixed, P2G is linear in the particles' v
+// and APIC rows and G2P in the face velocities. The G2P adjoint deposits through
+// k_p2g_tiled with this source; the P2G adjoint gathers with k_p2g_adj_gather.
+
+// The G2P adjoint's particle side: the output cotangents, the rows times the gather's
+// 4 / dx^2, for the particles that took the grid's velocity (the others deposit 0).
+struct G2PAdjSource {
+    const float* pb[12];
+    const uint8_t* reached;
+    float cs;
+    __device__ float weight(int64_t p) const { return reached[p] ? 1.0f : 0.0f; }
+    __device__ void comp(int64_t p, int a, float& v, float& c0, float& c1, float& c2) const {
+        if (!reached[p]) { v = c0 = c1 = c2 = 0.0f; return; }
+        v = pb[a][p];
+        c0 = pb[3 + 3 * a][p] * cs; c1 = pb[4 + 3 * a][p] * cs; c2 = pb[5 + 3 * a][p] * cs;
+    }
+};
+struct PB12 { float* f[12]; };
+
+// The P2G adjoint's particle part: each particle gathers the momentum cotangents s of the
+// nodes it deposited to (NodeMap as k_p2g_tiled's), with the deposit's weights and
+// offsets, and adds them to its v and row cotangents.
+template <class NodeMap>
+__global__ __launch_bounds__(256) void k_p2g_adj_gather(PView ps, int64_t n, NodeMap map, float lo0, float lo1, float lo2,
+                                                        float dx, float dt, const float* __restrict__ s0,
+                                                        const float* __restrict__ s1, const float* __restrict__ s2, PB12 pb) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float xp = ps.f[0][p], yp = ps.f[1][p], zp = ps.f[2][p];
+    float wT = temporal(ps.f[15][p], dt);
+    const float* S[3] = {s0, s1, s2};
+    for (int a = 0; a < 3; ++a) {
+        float q0 = ispace(xp, lo0, dx, a == 0 ? 0.0f : 0.5f);
+        float q1 = ispace(yp, lo1, dx, a == 1 ? 0.0f : 0.5f);
+        float q2 = ispace(zp, lo2, dx, a == 2 ? 0.0f : 0.5f);
+        BSpline bx = bspline(q0), by = bspline(q1), bz = bspline(q2);
+        float v = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+        for (int mz = 0; mz < 3; ++mz) {
+            int K = bz.b + mz;
+            float wz = pick(bz, mz), d2 = ((float)K - q2) * dx;
+            for (int my = 0; my < 3; ++my) {
+                int J = by.b + my;
+                float wy = pick(by, my), d1 = ((float)J - q1) * dx;
+                for (int mx = 0; mx < 3; ++mx) {
+                    int I = bx.b + mx;
+                    int64_t idx = (int64_t)map.index(a, I, J, K);
+                    if (idx < 0) continue;
+                    float t = wT * pick(bx, mx) * wy * wz * S[a][idx];
+                    v += t;
+                    c0 += t * (((float)I - q0) * dx);
+                    c1 += t * d1;
+                    c2 += t * d2;
+                }
+            }
+        }
+        pb.f[a][p] += v;
+        pb.f[3 + 3 * a][p] += c0; pb.f[4 + 3 * a][p] += c1; pb.f[5 + 3 * a][p] += c2;
+    }
+}
+
 }  // namespace mael

@@ -77,6 +77,12 @@ template <int N> __device__ inline int32_t faceat(const STopo& t, int a, int I, 
     return (s - 1) * per + ffn(N, a, a == 0 ? N : (Im & m), a == 1 ? N : (Jm & m), a == 2 ? N : (Km & m));
 }
 __device__ inline int32_t finefaceat(const STopo& t, int a, int I, int J, int K) { return faceat<16>(t, a, I, J, K); }
+// flip.jl:513-537 over the topology: the deposit's nodes are fine faces (the upper block's
+// copy on a plane between blocks), as the tiled fixed-point reduction indexes them.
+struct SparseNodes {
+    STopo t;
+    __device__ int32_t index(int a, int I, int J, int K) const { return finefaceat(t, a, I, J, K); }
+};
 // Whether coarse cell (I, J, K) lies in a fine block (sflip.jl:127-128).
 __device__ inline bool covered(const STopo& t, int I, int J, int K) {
     int bi = I >> 2, bj = J >> 2, bk = K >> 2;
@@ -483,11 +489,29 @@ struct SparseBasin {
     void reflected();
     float maxspeed();
     bool reflect_flow = false;                // G2P and resampling read ur (the reflected velocity)
+    // adjoints of the transfers (sparse_adjoint.hip) and the projection (sparse_pressure.hip),
+    // at the state the forward phase left: cotangents of the faces (ub: fine pools and
+    // coarse grid), of u0 (ub0), of the particles' v and APIC rows (pb)
+    SField<float> ub[3], ub0[3];
+    DBuf<float> pb[12];
+    DBuf<unsigned long long> adjacc[3];        // fixed-point deposits of the G2P adjoint
+    DBuf<unsigned long long> adjcell[2];       // fixed-point sums of the gradient's transpose (fine, coarse)
+    DBuf<uint8_t> reached;
+    void adj_alloc();
+    void g2p_adjoint(float dt);
+    void p2g_adjoint(float dt);
+    void project_adjoint(const SparseOpts& o, SparseStats& stats);
+    // the pressure solve on the right-hand side in b (P the start), shared by the projection
+    // and its adjoint; the hierarchy and couplings must be set
+    void solve_rhs(const SparseOpts& o, SparseStats& stats);
 };
 
 // restriction kernels shared by several phases (sflip.jl:583-612)
 void launch_restrictcells(const SparseBasin& s, float* coarse, const float* fine);
 void launch_restrictfaces_f(const SparseBasin& s, float* coarse, const float* fine, int a);
 void launch_restrictfaces_u8(const SparseBasin& s, uint8_t* coarse, const uint8_t* fine, int a);
+// the transpose of the face restriction on cotangents (sparse_adjoint.hip): fine += the
+// restricted coarse faces' shares, cout = the coarse faces' own (cout may be cin)
+void launch_restrictfaces_T(const SparseBasin& s, float* fine, float* cout, const float* cin, int a);
 
 }  // namespace mael

@@ -189,12 +189,6 @@ int64_t SparseBasin::resample(bool seeding, uint32_t seed) {
 
 // ─── particles to grid ───────────────────────────────────────────────────────
 
-// flip.jl:513-537 over the topology: nodes are fine faces (the upper block's copy on a
-// plane between blocks), deposited by the tiled fixed-point reduction.
-struct SparseNodes {
-    STopo t;
-    __device__ int32_t index(int a, int I, int J, int K) const { return finefaceat(t, a, I, J, K); }
-};
 struct FineCellMap {
     STopo t;
     __device__ int32_t index(int I, int J, int K) const { return fineat(t, I, J, K); }

@@ -1599,6 +1599,163 @@ __global__ void k_projectcoarse(float* __restrict__ u, uint8_t* __restrict__ val
     }
 }
 
+// ─── the projection's adjoint (no counterpart in the reference) ───────────────
+// project! maps the faces u to R (u - G A^-1 (D u + gamma terms)): D the scaled
+// divergences of k_finerhs / k_coarserhs, G the face differences of k_projectfine /
+// k_projectcoarse, R the restriction of the coarse faces under fine blocks. With the
+// types and the level set fixed its linear part's transpose is ub -> u'' + D^T A^-1
+// (-G^T u''), u'' = R^T ub, A being symmetric: the same solver on the adjoint
+// right-hand side.
+
+// The fine cell (I, J, K) as fineorcoarse reads it: its pool index, or -1 and the coarse
+// cell's (clamped) index.
+__device__ inline int32_t cellref(const STopo& t, int I, int J, int K, size_t& ci) {
+    int32_t idx = fineat(t, I, J, K);
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    ci = cl(min(max(I >> 2, 0), n0 - 1), min(max(J >> 2, 0), n1 - 1), min(max(K >> 2, 0), n2 - 1), n0, n1);
+    return idx;
+}
+// The largest |cotangent| over the allocated fine face copies and the coarse faces, as
+// float bits (+inf for a nonfinite one).
+__global__ __launch_bounds__(256) void k_sadj_absmax(const float* __restrict__ x, size_t n, const int4* __restrict__ where,
+                                                     int* __restrict__ bound) {
+    __shared__ int red[4];
+    size_t l = (size_t)blockIdx.x * 256 + threadIdx.x;
+    float m = 0.0f;
+    if (l < n && (!where || where[l / SBFACES].x >= 0)) {
+        float v = x[l];
+        m = isfinite(v) ? fabsf(v) : __int_as_float(0x7f800000);
+    }
+    int v = __float_as_int(m);
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(bound, max(max(red[0], red[1]), max(red[2], red[3])));
+}
+// -G^T ub, scaled into fixed-point sums: each valid fine face copy (k_projectfine's test)
+// adds -ub dDl/dP / d to the cells whose pressures it differences (a fine cell, or the
+// coarse cell under an unallocated block).
+__global__ void k_sadj_gradT_fine(unsigned long long* __restrict__ accf, unsigned long long* __restrict__ accc,
+                                  const float* __restrict__ ub, const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc,
+                                  const float* __restrict__ phi, const float* __restrict__ phic, STopo t, int a, int32_t nfa,
+                                  float scale) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nfa) return;
+    SCell s = fineface(t, a, n);
+    if (s.w.x < 0) return;
+    int I = s.w.x * 16 + s.i, J = s.w.y * 16 + s.j, K = s.w.z * 16 + s.k;
+    int Im = I - (a == 0), Jm = J - (a == 1), Km = K - (a == 2);
+    bool fp, fm;
+    uint8_t tp = celltype(t, tf, tc, I, J, K, fp), tm = celltype(t, tf, tc, Im, Jm, Km, fm);
+    if (!(tp != SOLID && tm != SOLID && (tp == LIQUID || tm == LIQUID))) return;
+    float d = (fp && fm ? 1.0f : 2.5f) * t.g.dx;
+    float x = ub[n] * scale / d;
+    size_t cp, cm;
+    int32_t ip = cellref(t, I, J, K, cp), im = cellref(t, Im, Jm, Km, cm);
+    float gp = 0.0f, gm = 0.0f;   // -ub dDl/dPp / d, -ub dDl/dPm / d
+    if (tp == LIQUID && tm == LIQUID) {
+        gp = -x;
+        gm = x;
+    } else if (tm == LIQUID) {
+        gm = x / theta(fineorcoarse(t, phi, phic, Im, Jm, Km), fineorcoarse(t, phi, phic, I, J, K));
+    } else {
+        gp = -x / theta(fineorcoarse(t, phi, phic, I, J, K), fineorcoarse(t, phi, phic, Im, Jm, Km));
+    }
+    if (gp != 0.0f) atomicAdd(ip >= 0 ? &accf[ip] : &accc[cp], tofix(gp));
+    if (gm != 0.0f) atomicAdd(im >= 0 ? &accf[im] : &accc[cm], tofix(gm));
+}
+// The same for the coarse faces k_projectcoarse moves (both cells uncovered).
+__global__ void k_sadj_gradT_coarse(unsigned long long* __restrict__ accc, const float* __restrict__ ub,
+                                    const uint8_t* __restrict__ tc, const float* __restrict__ phic, STopo t, int a, float scale) {
+    int nc0 = t.g.nc(0), nc1 = t.g.nc(1), nc2 = t.g.nc(2);
+    int n0 = nc0 + (a == 0), n1 = nc1 + (a == 1), n2 = nc2 + (a == 2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
+    int Im = I - (a == 0), Jm = J - (a == 1), Km = K - (a == 2);
+    bool inside = Im >= 0 && Jm >= 0 && Km >= 0 && I < nc0 && J < nc1 && K < nc2;
+    if (!inside || covered(t, I, J, K) || covered(t, Im, Jm, Km)) return;
+    size_t cp = cl(I, J, K, nc0, nc1), cm = cl(Im, Jm, Km, nc0, nc1);
+    uint8_t tp = tc[cp], tm = tc[cm];
+    if (!(tp != SOLID && tm != SOLID && (tp == LIQUID || tm == LIQUID))) return;
+    float x = ub[l] * scale / t.g.cdx();
+    float fp = phic[cp], fm = phic[cm];
+    float gp = 0.0f, gm = 0.0f;
+    if (tp == LIQUID && tm == LIQUID) {
+        gp = -x;
+        gm = x;
+    } else if (tm == LIQUID) {
+        gm = x / theta(fm, fp);
+    } else {
+        gp = -x / theta(fp, fm);
+    }
+    if (gp != 0.0f) atomicAdd(&accc[cp], tofix(gp));
+    if (gm != 0.0f) atomicAdd(&accc[cm], tofix(gm));
+}
+// The adjoint right-hand side from the sums on the unknowns (fine LIQUID cells, uncovered
+// coarse LIQUID cells), 0 elsewhere, and the solve's start P = 0.
+__global__ void k_sadj_rhs_fine(float* __restrict__ b, float* __restrict__ P, const unsigned long long* __restrict__ acc,
+                                const uint8_t* __restrict__ tf, const int4* __restrict__ where, int32_t nf, double inv) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nf) return;
+    bool u = where[n >> 12].x >= 0 && tf[n] == LIQUID;
+    b[n] = u ? (float)((double)(long long)acc[n] * FIX_INV * inv) : 0.0f;
+    P[n] = 0.0f;
+}
+__global__ void k_sadj_rhs_coarse(float* __restrict__ b, float* __restrict__ P, const unsigned long long* __restrict__ acc,
+                                  const uint8_t* __restrict__ tc, STopo t, double inv) {
+    int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
+    bool u = tc[l] == LIQUID && !covered(t, I, J, K);
+    b[l] = u ? (float)((double)(long long)acc[l] * FIX_INV * inv) : 0.0f;
+    P[l] = 0.0f;
+}
+// ub += D^T y per fine face copy: the copy is the low face of its block's cell at its
+// index (db/du = +h) and the high face of the cell below it in the same block (-h), for
+// LIQUID cells (k_finerhs reads each cell's own block's copies).
+template <class PT>
+__global__ void k_sadj_divT_fine(float* __restrict__ ub, const PT* __restrict__ y, const uint8_t* __restrict__ tf, STopo t,
+                                 int a, int32_t nfa, float h) {
+    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nfa) return;
+    SCell s = fineface(t, a, n);
+    if (s.w.x < 0) return;
+    int ia = a == 0 ? s.i : (a == 1 ? s.j : s.k);
+    int32_t base = (s.s - 1) * SBCELLS;
+    int32_t c = base + (s.i + 16 * (s.j + 16 * s.k)), st = a == 0 ? 1 : (a == 1 ? 16 : 256);
+    float add = 0.0f;
+    if (ia <= 15 && tf[c] == LIQUID) add += h * (float)y[c];
+    if (ia >= 1 && tf[c - st] == LIQUID) add -= h * (float)y[c - st];
+    ub[n] += add;
+}
+// ub += D^T y per coarse face: +16h from the cell above it, -16h from the cell below, for
+// uncovered LIQUID cells (k_coarserhs).
+template <class PT>
+__global__ void k_sadj_divT_coarse(float* __restrict__ ub, const PT* __restrict__ y, const uint8_t* __restrict__ tc, STopo t,
+                                   int a, float h16) {
+    int nc0 = t.g.nc(0), nc1 = t.g.nc(1), nc2 = t.g.nc(2);
+    int n0 = nc0 + (a == 0), n1 = nc1 + (a == 1), n2 = nc2 + (a == 2);
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (size_t)n0 * n1 * n2) return;
+    int I, J, K;
+    unlin(l, n0, n1, I, J, K);
+    int Im = I - (a == 0), Jm = J - (a == 1), Km = K - (a == 2);
+    float add = 0.0f;
+    if (I < nc0 && J < nc1 && K < nc2) {
+        size_t cp = cl(I, J, K, nc0, nc1);
+        if (tc[cp] == LIQUID && !covered(t, I, J, K)) add += h16 * (float)y[cp];
+    }
+    if (Im >= 0 && Jm >= 0 && Km >= 0) {
+        size_t cm = cl(Im, Jm, Km, nc0, nc1);
+        if (tc[cm] == LIQUID && !covered(t, Im, Jm, Km)) add -= h16 * (float)y[cm];
+    }
+    ub[l] += add;
+}
+
 // ─── the solver ──────────────────────────────────────────────────────────────
 
 template <class T> struct SVecT {   // a pressure-sized vector in storage T
@@ -2243,19 +2400,11 @@ void SparseBasin::precondition(const SparseOpts& o, bool queued) {
     st.sync();
 }
 
-void SparseBasin::project(float gamma, const SparseOpts& o, SparseStats& stats) {
-    auto t0 = std::chrono::steady_clock::now();
-    hipStream_t s = st.s;
+// The pressure solve (sflip.jl:1867-1904 with fgmres.jl:26-110) on the right-hand side in
+// b, from P (and with refinement the FP64 pressure from P): the projection's and its
+// adjoint's. The couplings (SparseSolver::coefficients) must be those of the state.
+void SparseBasin::solve_rhs(const SparseOpts& o, SparseStats& stats) {
     SparseSolver S(*this);
-    STopo t = topo();
-    stats = SparseStats{};
-    S.coefficients();
-    if (S.nf)
-        hipLaunchKernelGGL(k_finerhs, dim3(cdiv(S.nf, 256)), dim3(256), 0, s, b.fine.p, P.fine.p, types.fine.p, u[0].fine.p,
-                           u[1].fine.p, u[2].fine.p, t, gamma, S.nf);
-    hipLaunchKernelGGL(k_coarserhs, dim3(cdiv(S.Cc, 256)), dim3(256), 0, s, b.coarse.p, P.coarse.p, types.coarse.p, u[0].coarse.p,
-                       u[1].coarse.p, u[2].coarse.p, t, gamma);
-    MAEL_LAUNCH_CHECK();
     S.hierarchy();
     S.precision = o.precision;
     S.refine = o.refine;
@@ -2304,10 +2453,26 @@ void SparseBasin::project(float gamma, const SparseOpts& o, SparseStats& stats) 
     // with refinement the stored pressure is the FP64 one rounded, and the faces take
     // their differences from the FP64 one
     if (S.refine > 0) S.ir(2);
+}
+
+void SparseBasin::project(float gamma, const SparseOpts& o, SparseStats& stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = st.s;
+    SparseSolver S(*this);
+    STopo t = topo();
+    stats = SparseStats{};
+    S.coefficients();
+    if (S.nf)
+        hipLaunchKernelGGL(k_finerhs, dim3(cdiv(S.nf, 256)), dim3(256), 0, s, b.fine.p, P.fine.p, types.fine.p, u[0].fine.p,
+                           u[1].fine.p, u[2].fine.p, t, gamma, S.nf);
+    hipLaunchKernelGGL(k_coarserhs, dim3(cdiv(S.Cc, 256)), dim3(256), 0, s, b.coarse.p, P.coarse.p, types.coarse.p, u[0].coarse.p,
+                       u[1].coarse.p, u[2].coarse.p, t, gamma);
+    MAEL_LAUNCH_CHECK();
+    solve_rhs(o, stats);
     // sflip.jl:2008-2013 the faces, then the coarse faces under fine blocks from theirs
     for (int a = 0; a < 3; ++a) {
         if (S.nf) {
-            if (S.refine > 0)
+            if (o.refine > 0)
                 hipLaunchKernelGGL(k_projectfine<double>, dim3(cdiv(finefaces(), 256)), dim3(256), 0, s, u[a].fine.p,
                                    fl[a].fine.p, Pd.fine.p, Pd.coarse.p, types.fine.p, types.coarse.p, phi.fine.p, phi.coarse.p,
                                    t, a, (int32_t)finefaces());
@@ -2316,7 +2481,7 @@ void SparseBasin::project(float gamma, const SparseOpts& o, SparseStats& stats) 
                                    fl[a].fine.p, P.fine.p, P.coarse.p, types.fine.p, types.coarse.p, phi.fine.p, phi.coarse.p,
                                    t, a, (int32_t)finefaces());
         }
-        if (S.refine > 0)
+        if (o.refine > 0)
             hipLaunchKernelGGL(k_projectcoarse<double>, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, s, u[a].coarse.p,
                                fl[a].coarse.p, Pd.coarse.p, types.coarse.p, phi.coarse.p, t, a);
         else
@@ -2324,6 +2489,75 @@ void SparseBasin::project(float gamma, const SparseOpts& o, SparseStats& stats) 
                                fl[a].coarse.p, P.coarse.p, types.coarse.p, phi.coarse.p, t, a);
         launch_restrictfaces_f(*this, u[a].coarse.p, u[a].fine.p, a);
         launch_restrictfaces_u8(*this, fl[a].coarse.p, fl[a].fine.p, a);
+    }
+    MAEL_LAUNCH_CHECK();
+    st.sync();
+    stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// The transpose of project!'s linear part, in place on ub (fine pools and coarse grid); P
+// is left holding the adjoint pressure y = A^-1 (-G^T R^T ub), solved from zero with the
+// same options as the projection.
+void SparseBasin::project_adjoint(const SparseOpts& o, SparseStats& stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    adj_alloc();
+    hipStream_t s = st.s;
+    SparseSolver S(*this);
+    STopo t = topo();
+    stats = SparseStats{};
+    size_t F = finefaces(), C = finecells();
+    for (int a = 0; a < 3; ++a) launch_restrictfaces_T(*this, ub[a].fine.p, ub[a].coarse.p, ub[a].coarse.p, a);
+    // the fixed-point scale from the largest cotangent: a cell takes at most 6 coarse and
+    // 96 fine faces' terms, each at most |ub| / (0.01 dx) (theta >= 0.01, d >= dx)
+    int* bound = (int*)partials.p;
+    fill_async(bound, (size_t)1, 0, s);
+    for (int a = 0; a < 3; ++a) {
+        if (F) hipLaunchKernelGGL(k_sadj_absmax, dim3(cdiv(F, 256)), dim3(256), 0, s, ub[a].fine.p, F, where.p, bound);
+        hipLaunchKernelGGL(k_sadj_absmax, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, s, ub[a].coarse.p, nfaces(a),
+                           (const int4*)nullptr, bound);
+    }
+    MAEL_HIP(hipMemcpyAsync(st.pinned, bound, sizeof(int), hipMemcpyDeviceToHost, s));
+    st.sync();
+    float m;
+    std::memcpy(&m, st.pinned, sizeof(float));
+    if (!std::isfinite(m)) throw Error(E_NONFINITE, "Nonfinite face cotangent");
+    double scale = 1.0;
+    if (m > 0.0f) {
+        double big = (double)m * 100.0 / (double)g.dx;
+        scale = std::ldexp(1.0, std::min(60, std::max(-60, (int)std::floor(std::log2(std::ldexp(1.0, 23) / big)))));
+    }
+    if (C) fill_async(adjcell[0].p, C, 0ull, s);
+    fill_async(adjcell[1].p, ncells(), 0ull, s);
+    for (int a = 0; a < 3; ++a) {
+        if (F)
+            hipLaunchKernelGGL(k_sadj_gradT_fine, dim3(cdiv(F, 256)), dim3(256), 0, s, adjcell[0].p, adjcell[1].p, ub[a].fine.p,
+                               types.fine.p, types.coarse.p, phi.fine.p, phi.coarse.p, t, a, (int32_t)F, (float)scale);
+        hipLaunchKernelGGL(k_sadj_gradT_coarse, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, s, adjcell[1].p, ub[a].coarse.p,
+                           types.coarse.p, phi.coarse.p, t, a, (float)scale);
+    }
+    if (C)
+        hipLaunchKernelGGL(k_sadj_rhs_fine, dim3(cdiv(C, 256)), dim3(256), 0, s, b.fine.p, P.fine.p, adjcell[0].p, types.fine.p,
+                           where.p, (int32_t)C, 1.0 / scale);
+    hipLaunchKernelGGL(k_sadj_rhs_coarse, dim3(cdiv(ncells(), 256)), dim3(256), 0, s, b.coarse.p, P.coarse.p, adjcell[1].p,
+                       types.coarse.p, t, 1.0 / scale);
+    MAEL_LAUNCH_CHECK();
+    S.coefficients();
+    solve_rhs(o, stats);
+    float h = g.dx;
+    for (int a = 0; a < 3; ++a) {
+        if (o.refine > 0) {
+            if (F)
+                hipLaunchKernelGGL(k_sadj_divT_fine<double>, dim3(cdiv(F, 256)), dim3(256), 0, s, ub[a].fine.p, Pd.fine.p,
+                                   types.fine.p, t, a, (int32_t)F, h);
+            hipLaunchKernelGGL(k_sadj_divT_coarse<double>, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, s, ub[a].coarse.p,
+                               Pd.coarse.p, types.coarse.p, t, a, 16.0f * h);
+        } else {
+            if (F)
+                hipLaunchKernelGGL(k_sadj_divT_fine<float>, dim3(cdiv(F, 256)), dim3(256), 0, s, ub[a].fine.p, P.fine.p,
+                                   types.fine.p, t, a, (int32_t)F, h);
+            hipLaunchKernelGGL(k_sadj_divT_coarse<float>, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, s, ub[a].coarse.p,
+                               P.coarse.p, types.coarse.p, t, a, 16.0f * h);
+        }
     }
     MAEL_LAUNCH_CHECK();
     st.sync();

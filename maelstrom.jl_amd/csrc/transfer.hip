@@ -267,20 +267,6 @@ struct DenseClampNodes {
     }
 };
 
-// The G2P adjoint's particle side: the output cotangents, the rows times the gather's
-// 4 / dx^2, for the particles that took the grid's velocity (the others deposit 0).
-struct G2PAdjSource {
-    const float* pb[12];
-    const uint8_t* reached;
-    float cs;
-    __device__ float weight(int64_t p) const { return reached[p] ? 1.0f : 0.0f; }
-    __device__ void comp(int64_t p, int a, float& v, float& c0, float& c1, float& c2) const {
-        if (!reached[p]) { v = c0 = c1 = c2 = 0.0f; return; }
-        v = pb[a][p];
-        c0 = pb[3 + 3 * a][p] * cs; c1 = pb[4 + 3 * a][p] * cs; c2 = pb[5 + 3 * a][p] * cs;
-    }
-};
-struct PB12 { float* f[12]; };
 
 // Which particles took the grid's velocity in G2P (the flags' gathers, as k_dense_g2p),
 // and the largest contribution one of them makes to a node, as float bits (for the scale).
@@ -409,45 +395,6 @@ __global__ __launch_bounds__(256) void k_p2g_adj_faces(float* __restrict__ sface
         ub0[idx] = u;
     }
 }
-// The P2G adjoint's particle part: each particle gathers the momentum cotangents of the
-// nodes it deposited to, with the deposit's weights and offsets.
-__global__ __launch_bounds__(256) void k_p2g_adj_gather(PView ps, int64_t n, DGrid g, float dt, const float* __restrict__ s0,
-                                                        const float* __restrict__ s1, const float* __restrict__ s2, PB12 pb) {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    float xp = ps.f[0][p], yp = ps.f[1][p], zp = ps.f[2][p];
-    float wT = temporal(ps.f[15][p], dt);
-    const float* S[3] = {s0, s1, s2};
-    DenseNodes map{{g.n[0], g.n[1], g.n[2]}};
-    for (int a = 0; a < 3; ++a) {
-        float q0 = ispace(xp, g.lo[0], g.dx, a == 0 ? 0.0f : 0.5f);
-        float q1 = ispace(yp, g.lo[1], g.dx, a == 1 ? 0.0f : 0.5f);
-        float q2 = ispace(zp, g.lo[2], g.dx, a == 2 ? 0.0f : 0.5f);
-        BSpline bx = bspline(q0), by = bspline(q1), bz = bspline(q2);
-        float v = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
-        for (int mz = 0; mz < 3; ++mz) {
-            int K = bz.b + mz;
-            float wz = pick(bz, mz), d2 = ((float)K - q2) * g.dx;
-            for (int my = 0; my < 3; ++my) {
-                int J = by.b + my;
-                float wy = pick(by, my), d1 = ((float)J - q1) * g.dx;
-                for (int mx = 0; mx < 3; ++mx) {
-                    int I = bx.b + mx;
-                    int64_t idx = map.index(a, I, J, K);
-                    if (idx < 0) continue;
-                    float t = wT * pick(bx, mx) * wy * wz * S[a][idx];
-                    v += t;
-                    c0 += t * (((float)I - q0) * g.dx);
-                    c1 += t * d1;
-                    c2 += t * d2;
-                }
-            }
-        }
-        pb.f[a][p] += v;
-        pb.f[3 + 3 * a][p] += c0; pb.f[4 + 3 * a][p] += c1; pb.f[5 + 3 * a][p] += c2;
-    }
-}
-
 // The transpose of particles2grid! (flip.jl:587-606) in the particles' v and APIC rows and
 // in u0, at the positions, lags, level set and weights of the last deposit: pb += P2G^T
 // (ub) for the particles (added to what G2P's adjoint passed through), ub0 = its part for
@@ -469,7 +416,8 @@ void DenseBasin::p2g_adjoint(float dt) {
     if (n > 0) {
         PB12 P;
         for (int f = 0; f < 12; ++f) P.f[f] = pb[f].p;
-        hipLaunchKernelGGL(k_p2g_adj_gather, dim3(cdiv(n, 256)), dim3(256), 0, s, view(cur), n, g, dt, sf[0], sf[1], sf[2], P);
+        hipLaunchKernelGGL(k_p2g_adj_gather<DenseNodes>, dim3(cdiv(n, 256)), dim3(256), 0, s, view(cur), n,
+                           DenseNodes{{g.n[0], g.n[1], g.n[2]}}, g.lo[0], g.lo[1], g.lo[2], g.dx, dt, sf[0], sf[1], sf[2], P);
     }
     MAEL_LAUNCH_CHECK();
 }

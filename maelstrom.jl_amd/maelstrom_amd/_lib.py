@@ -33,7 +33,10 @@ SPARSE_ARRAYS = ["PHI_F", "PHI_C", "PHI0_F", "PHI0_C", "P_F", "P_C", "TYPES_F", 
                  "U_F", "U_C", "V_F", "V_C", "W_F", "W_C", "U0_F", "U0_C", "V0_F", "V0_C", "W0_F", "W0_C",
                  "FU_F", "FU_C", "FV_F", "FV_C", "FW_F", "FW_C", "RHS_F", "RHS_C"] + PARTICLE_FIELDS + [
                  "KEY", "RANK", "COUNTS", "TOTALS", "OFFSETS",
-                 "MOM_U", "MOM_V", "MOM_W", "WT_U", "WT_V", "WT_W", "SLOT", "WHERE"]
+                 "MOM_U", "MOM_V", "MOM_W", "WT_U", "WT_V", "WT_W", "SLOT", "WHERE",
+                 "UB_U_F", "UB_U_C", "UB_V_F", "UB_V_C", "UB_W_F", "UB_W_C",
+                 "UB0_U_F", "UB0_U_C", "UB0_V_F", "UB0_V_C", "UB0_W_F", "UB0_W_C"] + [
+                 "PB_" + f for f in ["VX", "VY", "VZ"] + ["C%d" % k for k in range(9)]]
 SPARSE_ID = {n: i for i, n in enumerate(SPARSE_ARRAYS)}
 
 
@@ -166,6 +169,9 @@ def lib():
         "mael_sparse_defaults": (None, [C.POINTER(SparseOpts)]),
         "mael_sparse_project": (C.c_int, [p, f32, C.POINTER(SparseOpts), C.POINTER(SparseStats)]),
         "mael_sparse_vcycle": (C.c_int, [p, C.POINTER(SparseOpts), i32]),
+        "mael_sparse_g2p_adjoint": (C.c_int, [p, f32]),
+        "mael_sparse_p2g_adjoint": (C.c_int, [p, f32]),
+        "mael_sparse_project_adjoint": (C.c_int, [p, C.POINTER(SparseOpts), C.POINTER(SparseStats)]),
         "mael_sparse_init_coarse": (C.c_int, [p, p, i32, f32, f32, f32]),
         "mael_sparse_init_fine": (C.c_int, [p, p, i32, f32, f32, f32]),
         "mael_sparse_classify": (C.c_int, [p]),

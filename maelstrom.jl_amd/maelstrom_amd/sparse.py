@@ -73,6 +73,8 @@ class SparseBasin:
                   "RHS_C": (n3, n2, n1), "U_C": (n3, n2, n1 + 1), "U0_C": (n3, n2, n1 + 1), "FU_C": (n3, n2, n1 + 1),
                   "V_C": (n3, n2 + 1, n1), "V0_C": (n3, n2 + 1, n1), "FV_C": (n3, n2 + 1, n1),
                   "W_C": (n3 + 1, n2, n1), "W0_C": (n3 + 1, n2, n1), "FW_C": (n3 + 1, n2, n1)}
+        for ax, sh in (("U", (n3, n2, n1 + 1)), ("V", (n3, n2 + 1, n1)), ("W", (n3 + 1, n2, n1))):
+            coarse["UB_%s_C" % ax] = coarse["UB0_%s_C" % ax] = sh
         if name in coarse:
             return coarse[name]
         return (self._info(name)[1],)
@@ -87,7 +89,7 @@ class SparseBasin:
         """Download array `name`; particle arrays are cut to `count` (default n), fine
         parts to the slots in use unless `count` is given."""
         _, cnt, dt = self._info(name)
-        if name in PARTICLE_FIELDS or name in ("KEY", "RANK"):
+        if name in PARTICLE_FIELDS or name in ("KEY", "RANK") or name.startswith("PB_"):
             cnt = self.n if count is None else count
         elif count is not None:
             cnt = count
@@ -215,6 +217,33 @@ def grid2particles(s, dt):
     """sflip.jl:2141-2144 `grid2particles!(s, dt)`."""
     s.ctx.check(_lib.lib().mael_sparse_g2p(s.h, float(dt), float(s.basin.g)))
     return s
+
+
+def grid2particles_adjoint(s, dt):
+    """The transpose of `grid2particles!` at the state it left (no counterpart in the
+    reference): UB_*_F = G2P^T (PB_*), the coarse parts 0; PB_* becomes the cotangent of
+    the input particles (v passes through where the grid was not reached)."""
+    s.ctx.check(_lib.lib().mael_sparse_g2p_adjoint(s.h, float(dt)))
+    return s
+
+
+def particles2grid_adjoint(s, dt):
+    """The transpose of `particles2grid!` at the state it left: PB_* += P2G^T (UB_*) for
+    the particles' v and APIC rows, UB0_* = the part for u0 (fine pools and coarse grid)."""
+    s.ctx.check(_lib.lib().mael_sparse_p2g_adjoint(s.h, float(dt)))
+    return s
+
+
+def project_adjoint(s, **solve):
+    """The transpose of `project!`'s linear part at the state it left, in place on UB_*
+    (the restriction's transpose, then UB += D^T A^-1 (-G^T UB)); `solve` as `project`'s.
+    P is left holding the adjoint pressure."""
+    o = pressure_defaults(s)
+    o.update(solve)
+    opts = solve_options(**o)
+    st = SparseStats()
+    s.ctx.check(_lib.lib().mael_sparse_project_adjoint(s.h, C.byref(opts), C.byref(st)))
+    return st
 
 
 def solve_options(tol=5e-4, maxiter=240, sweeps=1, coarse=40, restart=8, policy=2, precision=0, boost=1.5,

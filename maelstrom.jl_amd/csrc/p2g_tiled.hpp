@@ -16,8 +16,11 @@
 namespace mael {
 
 constexpr int P2G_THREADS = 256;
-constexpr int P2G_BOX = 2048;                             // nodes per component in LDS (mom + wt: 32 KB)
-constexpr int P2G_WBOX = P2G_BOX / (P2G_THREADS / 64);    // a wavefront's share
+// Nodes per component in LDS (mom + wt: 16 B a node). 2048 (32 KB, four workgroups a CU)
+// suits the sparse pools, whose workgroups stay inside a block or two; the dense grid
+// takes 4096 (64 KB, two a CU): its cell order runs along whole rows, where a row crossing
+// the funnel's ring of band cells splits a workgroup's particles into two runs far apart.
+constexpr int P2G_BOX = 2048, P2G_BOX_DENSE = 4096;
 
 __device__ inline int wg_min(int v, int* red) {
 #pragma unroll
@@ -124,9 +127,10 @@ struct P2GSource {   // flip.jl:513-537: v_a and the APIC row c_a, weighted by t
 // NodeMap: `int32_t/size_t index(int a, int I, int J, int K)` of a face node, < 0 off the
 // grid (the dense grid's array bounds, or no fine block holding the face). `ps` gives the
 // positions (fields 0-2).
-template <class NodeMap, class Source>
-__global__ __launch_bounds__(P2G_THREADS, 4) void k_p2g_tiled(PView ps, int64_t n, NodeMap map, Source src, P2GArgs A) {
-    __shared__ unsigned long long smom[P2G_BOX], swt[P2G_BOX];
+template <class NodeMap, class Source, int BOX = P2G_BOX>
+__global__ __launch_bounds__(P2G_THREADS, BOX > 2048 ? 2 : 4) void k_p2g_tiled(PView ps, int64_t n, NodeMap map, Source src,
+                                                                              P2GArgs A) {
+    __shared__ unsigned long long smom[BOX], swt[BOX];
     __shared__ int red[6 * (P2G_THREADS / 64)];
     const int64_t p0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * P2G_THREADS;   // neighbouring tiles on one XCD
     for (int a = 0; a < 3; ++a) {
@@ -143,7 +147,7 @@ __global__ __launch_bounds__(P2G_THREADS, 4) void k_p2g_tiled(PView ps, int64_t 
         bool ok = live && bx.b > -(1 << 24) && bx.b < (1 << 24) && by.b > -(1 << 24) && by.b < (1 << 24) &&
                   bz.b > -(1 << 24) && bz.b < (1 << 24) && isfinite(q0) && isfinite(q1) && isfinite(q2);
         bool wgbox;
-        TileBox T = tile_box(ok, bx.b, by.b, bz.b, 3, P2G_BOX, red, wgbox);
+        TileBox T = tile_box(ok, bx.b, by.b, bz.b, 3, BOX, red, wgbox);
         int first = tile_first(wgbox), stride = tile_stride(wgbox);
         if (wgbox) {
             // Depositing into the workgroup's box, a wavefront takes every eighth particle:
@@ -247,12 +251,14 @@ __global__ __launch_bounds__(P2G_THREADS, 4) void k_p2g_tiled(PView ps, int64_t 
 // touched cell of the box takes one global atomic min. Minimum is order-independent, so
 // the result is the direct splat's bit for bit. CellMap: `index(I, J, K)` of a cell (< 0
 // off the grid or outside the fine blocks).
-constexpr int SPLAT_BOX = 2048;   // cells in LDS (8 KB)
+// Cells in LDS (4 B a cell): 4096 for the sparse pools, 8192 for the dense grid's rows (as
+// the deposit's boxes); a workgroup whose cells do not fit splats directly.
+constexpr int SPLAT_BOX = 4096, SPLAT_BOX_DENSE = 8192;
 
-template <class CellMap>
+template <class CellMap, int BOX = SPLAT_BOX>
 __global__ __launch_bounds__(P2G_THREADS) void k_splat_tiled(PView ps, int64_t n, CellMap map, float lo0, float lo1, float lo2,
                                                             float dx, float dt, int32_t* __restrict__ phip) {
-    __shared__ int32_t box[SPLAT_BOX];
+    __shared__ int32_t box[BOX];
     __shared__ int red[6 * (P2G_THREADS / 64)];
     const int64_t p0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * P2G_THREADS;   // neighbouring tiles on one XCD
     int64_t p = p0 + threadIdx.x;
@@ -277,7 +283,7 @@ __global__ __launch_bounds__(P2G_THREADS) void k_splat_tiled(PView ps, int64_t n
     int x0, y0, z0, x1, y1, z1;
     wg_bounds(ok, ci, cj, ck, red, x0, y0, z0, x1, y1, z1);
     int ex = x1 - x0 + 3, ey = y1 - y0 + 3, ez = z1 - z0 + 3;
-    bool boxed = x0 != BIG && ex <= SPLAT_BOX && ey <= SPLAT_BOX && ez <= SPLAT_BOX && ex * ey * ez <= SPLAT_BOX;
+    bool boxed = x0 != BIG && ex <= BOX && ey <= BOX && ez <= BOX && ex * ey * ez <= BOX;
     x0 -= 1; y0 -= 1; z0 -= 1;
     if (boxed) {
         int V = ex * ey * ez;

@@ -333,7 +333,7 @@ __device__ inline void gather_component(const SGrid& g, int a, float px, float p
 // box holds reads the faces directly. Both paths sum the same terms in the same order. The
 // box's faces are found through a table of the slots of the blocks under it (and one
 // layer below along the component's axis, for a lower block's top faces).
-constexpr int G2P_BOX = 1024, G2P_TAB = 64;
+constexpr int G2P_BOX = 2048, G2P_TAB = 64;   // 16 KB: a band sheet through a block spans up to ~2000 nodes
 __global__ __launch_bounds__(256) void k_sparse_g2p(PView ps, int64_t n, STopo t, const float* __restrict__ u,
                                                     const float* __restrict__ v, const float* __restrict__ w,
                                                     const uint8_t* __restrict__ fu, const uint8_t* __restrict__ fv,

@@ -82,11 +82,12 @@ void DenseBasin::p2g(float dt) {
         kt.begin(K_P2G, s);
         DenseNodes map{{g.n[0], g.n[1], g.n[2]}};
         P2GArgs A{{g.lo[0], g.lo[1], g.lo[2]}, g.dx, {mom[0].p, mom[1].p, mom[2].p}, {wt[0].p, wt[1].p, wt[2].p}, 1.0f};
-        hipLaunchKernelGGL((k_p2g_tiled<DenseNodes, P2GSource>), dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s, ps, n,
+        hipLaunchKernelGGL((k_p2g_tiled<DenseNodes, P2GSource, P2G_BOX_DENSE>), dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0,
+                           s, ps, n,
                            map, P2GSource{ps, dt}, A);
         kt.end(K_P2G, s);
         kt.begin(K_SPLAT, s);
-        hipLaunchKernelGGL(k_splat_tiled<DenseCells>, dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s, ps, n,
+        hipLaunchKernelGGL((k_splat_tiled<DenseCells, SPLAT_BOX_DENSE>), dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s, ps, n,
                            DenseCells{{g.n[0], g.n[1], g.n[2]}}, g.lo[0], g.lo[1], g.lo[2], g.dx, dt, phip.p);
         kt.end(K_SPLAT, s);
         MAEL_LAUNCH_CHECK();
@@ -141,7 +142,7 @@ __device__ inline float gather_value(const V* __restrict__ u, int n1, int n2, in
 // outside it, reads the faces directly. Both paths take the flags' sum (gather_value) and
 // the APIC sums (gather) in their order of operations, with the indices clamped to the
 // array as there, so the result is theirs bit for bit.
-constexpr int DG2P_BOX = 1024;
+constexpr int DG2P_BOX = 4096;   // 32 KB: the cell order runs along whole rows, two runs of a row far apart
 __global__ __launch_bounds__(256) void k_dense_g2p(PView ps, int64_t n, DGrid g, const float* __restrict__ u,
                                                    const float* __restrict__ v, const float* __restrict__ w,
                                                    const uint8_t* __restrict__ fu, const uint8_t* __restrict__ fv,
@@ -359,7 +360,8 @@ void DenseBasin::g2p_adjoint(float dt) {
                      reached.p, 4.0f / (g.dx * g.dx)};
     P2GArgs A{{g.lo[0], g.lo[1], g.lo[2]}, g.dx, {adjacc[0].p, adjacc[1].p, adjacc[2].p}, {nullptr, nullptr, nullptr},
               (float)scale};
-    hipLaunchKernelGGL((k_p2g_tiled<DenseClampNodes, G2PAdjSource>), dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS), 0, s,
+    hipLaunchKernelGGL((k_p2g_tiled<DenseClampNodes, G2PAdjSource, P2G_BOX_DENSE>), dim3(cdiv(n, P2G_THREADS)), dim3(P2G_THREADS),
+                       0, s,
                        view(cur), n, DenseClampNodes{{g.n[0], g.n[1], g.n[2]}}, src, A);
     for (int a = 0; a < 3; ++a)
         hipLaunchKernelGGL(k_adj_fromfix, dim3(cdiv(faces(a), 256)), dim3(256), 0, s, ub[a].p, adjacc[a].p, faces(a),

@@ -26,6 +26,36 @@ __device__ inline uint8_t typeat(const uint8_t* __restrict__ t, int i, int j, in
     return (i < 0 || j < 0 || k < 0 || i >= n1 || j >= n2 || k >= n3) ? SOLID : t[lin(i, j, k, n1, n2)];
 }
 
+// The stencil kernels issue all of a cell's neighbour loads before using any: a load
+// under a branch that depends on an earlier load (a neighbour's value read only if its
+// type is LIQUID) waits for that load, which makes a one-thread-per-cell pass a chain of
+// memory round trips per thread (15 in a colour pass, 129 in the restriction). Each
+// neighbour is read unconditionally, at the cell's own index where it is off the grid,
+// and the sums take it only where the reference's test holds, in the reference's order:
+// the same operations on the same values.
+// The six neighbours of cell (i, j, k) in the order of NB6 below: their indices and
+// whether each is on the grid (typeat's SOLID otherwise).
+struct Nb6 {
+    size_t e[6];
+    bool in[6];
+};
+__device__ inline Nb6 nb6(int i, int j, int k, int3 n) {
+    Nb6 r;
+    const size_t c = lin(i, j, k, n.x, n.y), sy = (size_t)n.x, sz = (size_t)n.x * n.y;
+    r.in[0] = i + 1 < n.x; r.e[0] = r.in[0] ? c + 1 : c;
+    r.in[1] = i > 0;       r.e[1] = r.in[1] ? c - 1 : c;
+    r.in[2] = j + 1 < n.y; r.e[2] = r.in[2] ? c + sy : c;
+    r.in[3] = j > 0;       r.e[3] = r.in[3] ? c - sy : c;
+    r.in[4] = k + 1 < n.z; r.e[4] = r.in[4] ? c + sz : c;
+    r.in[5] = k > 0;       r.e[5] = r.in[5] ? c - sz : c;
+    return r;
+}
+// (i, j, k) clamped into the grid, and whether it was inside.
+__device__ inline size_t clampidx(int i, int j, int k, int3 n, bool& in) {
+    in = i >= 0 && j >= 0 && k >= 0 && i < n.x && j < n.y && k < n.z;
+    return lin(min(max(i, 0), n.x - 1), min(max(j, 0), n.y - 1), min(max(k, 0), n.z - 1), n.x, n.y);
+}
+
 __device__ inline double block_sum_d(double v, double* lds) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -48,13 +78,16 @@ __global__ void k_coarsen(uint8_t* __restrict__ tc, const uint8_t* __restrict__ 
     int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y * blockDim.y + threadIdx.y, K = blockIdx.z;
     if (I >= nc.x || J >= nc.y || K >= nc.z) return;
     bool air = false, liq = false;
-    for (int c = 0; c < 2; ++c)
-        for (int b = 0; b < 2; ++b)
-            for (int a = 0; a < 2; ++a) {
-                uint8_t t = typeat(tf, 2 * I + a, 2 * J + b, 2 * K + c, nf.x, nf.y, nf.z);
-                air |= t == AIR;
-                liq |= t == LIQUID;
-            }
+    uint8_t tv[8];
+    bool in[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) tv[m] = tf[clampidx(2 * I + (m & 1), 2 * J + ((m >> 1) & 1), 2 * K + (m >> 2), nf, in[m])];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        uint8_t t = in[m] ? tv[m] : SOLID;
+        air |= t == AIR;
+        liq |= t == LIQUID;
+    }
     tc[lin(I, J, K, nc.x, nc.y)] = air ? AIR : (liq ? LIQUID : SOLID);
 }
 
@@ -67,13 +100,17 @@ __global__ void k_diag(float* __restrict__ diag, const uint8_t* __restrict__ typ
     size_t c = lin(i, j, k, n.x, n.y);
     float d = 0.0f;
     if (types[c] == LIQUID) {
+        const Nb6 nb = nb6(i, j, k, n);
+        uint8_t tn[6];
+        float pn[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) { tn[m] = types[nb.e[m]]; pn[m] = phi[nb.e[m]]; }
         float pc = phi[c];
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
-            int a = i + NB6[m][0], b = j + NB6[m][1], cc = k + NB6[m][2];
-            uint8_t t = typeat(types, a, b, cc, n.x, n.y, n.z);
+            uint8_t t = nb.in[m] ? tn[m] : SOLID;
             if (t == LIQUID) d += 1.0f;
-            if (t == AIR) d += __fdiv_rn(1.0f, theta(pc, phi[lin(a, b, cc, n.x, n.y)]));
+            if (t == AIR) d += __fdiv_rn(1.0f, theta(pc, pn[m]));
         }
     }
     diag[c] = d;
@@ -84,16 +121,21 @@ struct Row { float d, s; };
 template <class X>
 __device__ inline Row row(const X* __restrict__ x, const uint8_t* __restrict__ t, const float* __restrict__ diag, int i, int j,
                           int k, int3 n) {
+    const Nb6 nb = nb6(i, j, k, n);
+    uint8_t tv[6];
+    float xv[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) { tv[m] = t[nb.e[m]]; xv[m] = (float)x[nb.e[m]]; }
+    const bool fine = diag != nullptr;
+    const float dg = fine ? diag[lin(i, j, k, n.x, n.y)] : 0.0f;
     Row r{0.0f, 0.0f};
-    bool fine = diag != nullptr;
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
-        int a = i + NB6[m][0], b = j + NB6[m][1], c = k + NB6[m][2];
-        uint8_t tn = typeat(t, a, b, c, n.x, n.y, n.z);
+        const uint8_t tn = nb.in[m] ? tv[m] : SOLID;
         if (!fine && tn != SOLID) r.d += 1.0f;
-        if (tn == LIQUID) r.s += (float)x[lin(a, b, c, n.x, n.y)];
+        if (tn == LIQUID) r.s += xv[m];
     }
-    if (fine) r.d = diag[lin(i, j, k, n.x, n.y)];
+    if (fine) r.d = dg;
     return r;
 }
 
@@ -152,9 +194,13 @@ __global__ void k_start(float* __restrict__ x, const float* __restrict__ b, cons
         if (diag) d = diag[c];
         else {
             d = 0.0f;
+            const Nb6 nb = nb6(i, j, k, n);
+            uint8_t tv[6];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) tv[m] = t[nb.e[m]];
 #pragma unroll
             for (int m = 0; m < 6; ++m)
-                if (typeat(t, i + NB6[m][0], j + NB6[m][1], k + NB6[m][2], n.x, n.y, n.z) != SOLID) d += 1.0f;
+                if ((nb.in[m] ? tv[m] : SOLID) != SOLID) d += 1.0f;
         }
         if (d > 0.0f) v = __fdiv_rn(b[c], d);
     }
@@ -170,13 +216,25 @@ __global__ void k_restrict(float* __restrict__ bc, const float* __restrict__ rf,
     float s = 0.0f;
     if (tc[C] == LIQUID) {
         const float wB[4] = {0.125f, 0.375f, 0.375f, 0.125f};
-        for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {   // a plane of 16 taps' loads at a time, then their sum
+            uint8_t tv[16];
+            float rv[16];
+            bool in[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const size_t e = clampidx(2 * I - 1 + (m & 3), 2 * J - 1 + (m >> 2), 2 * K - 1 + c, nf, in[m]);
+                tv[m] = tf[e];
+                rv[m] = rf[e];
+            }
+#pragma unroll
             for (int b = 0; b < 4; ++b)
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
-                    int i = 2 * I - 1 + a, j = 2 * J - 1 + b, k = 2 * K - 1 + c;
-                    if (typeat(tf, i, j, k, nf.x, nf.y, nf.z) == LIQUID) s += wB[a] * wB[b] * wB[c] * rf[lin(i, j, k, nf.x, nf.y)];
+                    const int m = a + 4 * b;
+                    if (in[m] && tv[m] == LIQUID) s += wB[a] * wB[b] * wB[c] * rv[m];
                 }
+        }
     }
     bc[C] = 4.0f * s;
 }
@@ -197,14 +255,21 @@ __global__ void k_prolong(float* __restrict__ xf, const float* __restrict__ xc, 
     pweights(i, A[0], WA[0], A[1], WA[1]);
     pweights(j, B[0], WB[0], B[1], WB[1]);
     pweights(k, Cc[0], WC[0], Cc[1], WC[1]);
-    float s = 0.0f;
+    uint8_t tv[8];
+    float xv[8];
+    bool in[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        int a = A[m & 1], b = B[(m >> 1) & 1], cc = Cc[(m >> 2) & 1];
-        if (typeat(tc, a, b, cc, nc.x, nc.y, nc.z) == LIQUID)
-            s += WA[m & 1] * WB[(m >> 1) & 1] * WC[(m >> 2) & 1] * xc[lin(a, b, cc, nc.x, nc.y)];
+        const size_t e = clampidx(A[m & 1], B[(m >> 1) & 1], Cc[(m >> 2) & 1], nc, in[m]);
+        tv[m] = tc[e];
+        xv[m] = xc[e];
     }
-    xf[c] += s;
+    const float x0 = xf[c];
+    float s = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+        if (in[m] && tv[m] == LIQUID) s += WA[m & 1] * WB[(m >> 1) & 1] * WC[(m >> 2) & 1] * xv[m];
+    xf[c] = x0 + s;
 }
 
 // flip.jl:931-962 coarse_kernel!: the coarsest level (<= 1024 cells) in one workgroup,
@@ -266,17 +331,18 @@ __global__ __launch_bounds__(256) void k_search(double* __restrict__ partials, f
         size_t c = lin(i, j, k, n.x, n.y);
         float qv = 0.0f;
         if (t[c] == LIQUID) {
+            const Nb6 nb = nb6(i, j, k, n);
+            uint8_t tv[6];
+            float zv[6], dv[6];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) { tv[m] = t[nb.e[m]]; zv[m] = z[nb.e[m]]; dv[m] = d[nb.e[m]]; }
+            const float dg = diag[c];
             float p = z[c] + beta * d[c];
             float acc = 0.0f;
 #pragma unroll
-            for (int m = 0; m < 6; ++m) {
-                int a = i + NB6[m][0], b = j + NB6[m][1], cc = k + NB6[m][2];
-                if (typeat(t, a, b, cc, n.x, n.y, n.z) == LIQUID) {
-                    size_t e = lin(a, b, cc, n.x, n.y);
-                    acc += z[e] + beta * d[e];
-                }
-            }
-            qv = diag[c] * p - acc;
+            for (int m = 0; m < 6; ++m)
+                if (nb.in[m] && tv[m] == LIQUID) acc += zv[m] + beta * dv[m];
+            qv = dg * p - acc;
             v = (double)p * (double)qv;
         }
         q[c] = qv;
@@ -344,14 +410,18 @@ __global__ __launch_bounds__(256) void k_resid64(double* __restrict__ partials, 
         size_t c = lin(i, j, k, n.x, n.y);
         double rr = 0;
         if (t[c] == LIQUID) {
+            const Nb6 nb = nb6(i, j, k, n);
+            uint8_t tv[6];
+            double pv[6];
+            float fv[6];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) { tv[m] = t[nb.e[m]]; pv[m] = P[nb.e[m]]; fv[m] = phi[nb.e[m]]; }
             double d = 0, s = 0, pc = phi[c];
 #pragma unroll
             for (int m = 0; m < 6; ++m) {
-                int a = i + NB6[m][0], bb = j + NB6[m][1], cc = k + NB6[m][2];
-                uint8_t tn = typeat(t, a, bb, cc, n.x, n.y, n.z);
-                size_t e = lin(a, bb, cc, n.x, n.y);
-                if (tn == LIQUID) { d += 1.0; s += P[e]; }
-                if (tn == AIR) d += 1.0 / theta(pc, (double)phi[e]);
+                const uint8_t tn = nb.in[m] ? tv[m] : SOLID;
+                if (tn == LIQUID) { d += 1.0; s += pv[m]; }
+                if (tn == AIR) d += 1.0 / theta(pc, (double)fv[m]);
             }
             rr = (double)b[c] - (d * P[c] - s);
         }

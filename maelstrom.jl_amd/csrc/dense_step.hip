@@ -234,13 +234,18 @@ __global__ void k_dextrapolate(float* __restrict__ out, uint8_t* __restrict__ fo
     const bool walled = !(a >= 0 && b >= 0 && c >= 0 && i < g.n[0] && j < g.n[1] && k < g.n[2]) || issolidcell(g, a, b, c) ||
                         issolidcell(g, i, j, k);
     if (!walled) {
-        // the six neighbours in the reference's order: +x, -x, +y, -y, +z, -z
-        if (i + 1 < n1 && fin[l + 1]) { s += src[l + 1]; ++m; }
-        if (i > 0 && fin[l - 1]) { s += src[l - 1]; ++m; }
-        if (j + 1 < n2 && fin[l + s1]) { s += src[l + s1]; ++m; }
-        if (j > 0 && fin[l - s1]) { s += src[l - s1]; ++m; }
-        if (k + 1 < n3 && fin[l + s2]) { s += src[l + s2]; ++m; }
-        if (k > 0 && fin[l - s2]) { s += src[l - s2]; ++m; }
+        // the six neighbours in the reference's order: +x, -x, +y, -y, +z, -z, their flags
+        // and values read together (one off the grid reads this face's own and is skipped)
+        const bool in[6] = {i + 1 < n1, i > 0, j + 1 < n2, j > 0, k + 1 < n3, k > 0};
+        const size_t e[6] = {in[0] ? l + 1 : l, in[1] ? l - 1 : l, in[2] ? l + s1 : l,
+                             in[3] ? l - s1 : l, in[4] ? l + s2 : l, in[5] ? l - s2 : l};
+        uint8_t fv[6];
+        float sv[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) { fv[q] = fin[e[q]]; sv[q] = src[e[q]]; }
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            if (in[q] && fv[q]) { s += sv[q]; ++m; }
     }
     out[l] = m > 0 ? __fdiv_rn(s, (float)m) : src[l];
     fout[l] = m > 0 ? 1 : 0;

@@ -382,12 +382,19 @@ __global__ __launch_bounds__(256) void k_dot(double* __restrict__ partials, cons
 
 // Sum the partials in a fixed order and update the Krylov scalars.
 enum { F_SET = 0, F_RHO0, F_ALPHA, F_RR, F_BETA };
-__global__ __launch_bounds__(256) void k_finish(const double* __restrict__ partials, int np, double* __restrict__ scal, int op,
-                                                int slot) {
-    __shared__ double lds[4];
-    double v = 0;
-    for (int l = threadIdx.x; l < np; l += blockDim.x) v += partials[l];
-    double s = block_sum_d(v, lds);
+// One workgroup of FIN_THREADS, four independent sums a thread (the partials of a fine
+// level's search are tens of thousands).
+constexpr int FIN_THREADS = 1024;
+__global__ __launch_bounds__(FIN_THREADS) void k_finish(const double* __restrict__ partials, int np, double* __restrict__ scal,
+                                                        int op, int slot) {
+    __shared__ double lds[FIN_THREADS / 64];
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int l = threadIdx.x;
+    for (; l + 3 * FIN_THREADS < np; l += 4 * FIN_THREADS) {
+        a0 += partials[l]; a1 += partials[l + FIN_THREADS]; a2 += partials[l + 2 * FIN_THREADS]; a3 += partials[l + 3 * FIN_THREADS];
+    }
+    for (; l < np; l += FIN_THREADS) a0 += partials[l];
+    double s = block_sum_d((a0 + a1) + (a2 + a3), lds);
     if (threadIdx.x != 0) return;
     switch (op) {
         case F_SET: scal[slot] = s; break;
@@ -558,7 +565,7 @@ struct DenseSolver {
     void dot(const float* a, const float* b, int op, int slot = 0) {
         size_t N = s.cells();
         hipLaunchKernelGGL(k_dot, dim3(RED_BLOCKS), dim3(256), 0, st, s.partials.p, a, b, N);
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, s.partials.p, RED_BLOCKS, s.scal.p, op, slot);
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(FIN_THREADS), 0, st, s.partials.p, RED_BLOCKS, s.scal.p, op, slot);
     }
     // FP64 true residual into r (FP32) and |r|^2 into scal[S_RR].
     double resid64(double* P64, float* r) {
@@ -567,7 +574,7 @@ struct DenseSolver {
         s.kt.begin(K_RESID64, st);
         hipLaunchKernelGGL(k_resid64, gr, dim3(BX, BY), 0, st, s.partials.p, r, P64, s.b.p, s.types.p, s.phi.p, n);
         s.kt.end(K_RESID64, st);
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, s.partials.p, (int)(gr.x * gr.y * gr.z), s.scal.p, F_RR, 0);
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(FIN_THREADS), 0, st, s.partials.p, (int)(gr.x * gr.y * gr.z), s.scal.p, F_RR, 0);
         MAEL_LAUNCH_CHECK();
         return std::sqrt(read(S_RR));
     }
@@ -679,11 +686,11 @@ void DenseBasin::solve_pressure(const SolveOpts& o, SolveStats& stats) {
             kt.begin(K_SEARCH, s);
             hipLaunchKernelGGL(k_search, gr, dim3(BX, BY), 0, s, partials.p, q.p, z.p, d.p, types.p, diag.p, n, scal.p);
             kt.end(K_SEARCH, s);
-            hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, partials.p, (int)(gr.x * gr.y * gr.z), scal.p, F_ALPHA, 0);
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(FIN_THREADS), 0, s, partials.p, (int)(gr.x * gr.y * gr.z), scal.p, F_ALPHA, 0);
             kt.begin(K_ADVANCE, s);
             hipLaunchKernelGGL(k_advance, dim3(RED_BLOCKS), dim3(256), 0, s, partials.p, e, r.p, d.p, z.p, q.p, N, scal.p);
             kt.end(K_ADVANCE, s);
-            hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, partials.p, RED_BLOCKS, scal.p, F_RR, 0);
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(FIN_THREADS), 0, s, partials.p, RED_BLOCKS, scal.p, F_RR, 0);
             MAEL_LAUNCH_CHECK();
             S.post_read(S_RNORM);
             if (it < o.maxiter) S.queue_head(z.p, r.p, o.sweeps);   // the next V-cycle's head, during the read

@@ -15,6 +15,7 @@
 #include "dense.hpp"
 #include "stir.hpp"
 #include <array>
+#include <type_traits>
 
 namespace mael {
 
@@ -125,12 +126,31 @@ __device__ inline void facepos(const SGrid& g, int a, int I, int J, int K, float
     p[2] = g.lo[2] + ((float)K + (a == 2 ? 0.0f : 0.5f)) * h;
 }
 
+// Block slot lookups for the samplers: the global table, or a workgroup's LDS table of the
+// 27 blocks around a home block w (as stage27 fills it) with the global table beyond it.
+// Both return the table's entry (0 off the grid), so samples are the same either way.
+struct SlotsGlobal {
+    __device__ int32_t operator()(const STopo& t, int bx, int by, int bz) const {
+        return inblocks(t.g, bx, by, bz) ? t.slot[blockid(t.g, bx, by, bz)] : 0;
+    }
+};
+struct Slots27 {
+    const int32_t* s27;
+    int4 w;
+    __device__ int32_t operator()(const STopo& t, int bx, int by, int bz) const {
+        const int di = bx - w.x + 1, dj = by - w.y + 1, dk = bz - w.z + 1;
+        if ((unsigned)di < 3u && (unsigned)dj < 3u && (unsigned)dk < 3u) return s27[di + 3 * dj + 9 * dk];
+        return inblocks(t.g, bx, by, bz) ? t.slot[blockid(t.g, bx, by, bz)] : 0;
+    }
+};
+
 // The slots (1-based, 0 for none) of the fine blocks that hold the eight trilinear taps
 // I..I+1, J..J+1, K..K+1 (tap m at +(m & 1, m >> 1 & 1, m >> 2)): the lower corner's block,
 // and across a block side for the taps past it. When all eight are allocated, fineat and
 // finefaceat return each tap's entry in its own block (the upper block's copy of a face
 // comes first), so the taps index from these slots with no further table reads.
-__device__ inline bool tapslots(const STopo& t, int I, int J, int K, int32_t s[8]) {
+template <class L = SlotsGlobal>
+__device__ inline bool tapslots(const STopo& t, int I, int J, int K, int32_t s[8], const L& lk = L()) {
     const SGrid& g = t.g;
     const int bx = I >> 4, by = J >> 4, bz = K >> 4;
     const int sx = (I & 15) == 15, sy = (J & 15) == 15, sz = (K & 15) == 15;
@@ -142,7 +162,10 @@ __device__ inline bool tapslots(const STopo& t, int I, int J, int K, int32_t s[8
     // one read per distinct block: a tap past no block side shares the slot of the tap
     // with its offsets along the unstraddled axes dropped
     auto look = [&](int dx, int dy, int dz) -> int32_t {
-        return inx[dx] && iny[dy] && inz[dz] ? t.slot[id0 + dx + nx * dy + nxy * dz] : 0;
+        if constexpr (std::is_same<L, SlotsGlobal>::value)
+            return inx[dx] && iny[dy] && inz[dz] ? t.slot[id0 + dx + nx * dy + nxy * dz] : 0;
+        else
+            return lk(t, bx + dx, by + dy, bz + dz);
     };
     s[0] = look(0, 0, 0);
     s[1] = sx ? look(1, 0, 0) : s[0];
@@ -166,16 +189,16 @@ __device__ inline float lerp8(const float c[8], float fx, float fy, float fz) {
 }
 
 // sflip.jl:136-155 cellsample: fine trilinear where all 8 taps are fine, else the coarse grid's.
-template <class V>
+template <class V, class L = SlotsGlobal>
 __device__ inline float cellsample(const STopo& t, const V* __restrict__ fine, const V* __restrict__ coarse, float px, float py,
-                                   float pz) {
+                                   float pz, const L& lk = L()) {
     const SGrid& g = t.g;
     float q0 = __fdiv_rn(px - g.lo[0], g.dx) - 0.5f, q1 = __fdiv_rn(py - g.lo[1], g.dx) - 0.5f,
           q2 = __fdiv_rn(pz - g.lo[2], g.dx) - 0.5f;
     float x = floorf(q0), y = floorf(q1), z = floorf(q2);
     int I = (int)x, J = (int)y, K = (int)z;
     int32_t s8[8];
-    if (tapslots(t, I, J, K, s8)) {   // the 8 taps in allocated blocks: fineat's indices
+    if (tapslots(t, I, J, K, s8, lk)) {   // the 8 taps in allocated blocks: fineat's indices
         float c[8];
 #pragma unroll
         for (int m = 0; m < 8; ++m)
@@ -192,16 +215,16 @@ __device__ inline float cellsample(const STopo& t, const V* __restrict__ fine, c
 // can still be fine: the top face of the block below along A, held there at coordinate 16
 // (finefaceat's second choice). That block is the one of the tap's neighbour below along
 // A when the taps straddle the plane, else one table read.
-template <int A, class V>
+template <int A, class V, class L = SlotsGlobal>
 __device__ inline float facesample_a(const STopo& t, const V* __restrict__ fine, const V* __restrict__ coarse, float px,
-                                     float py, float pz) {
+                                     float py, float pz, const L& lk = L()) {
     const SGrid& g = t.g;
     constexpr float o0 = A == 0 ? 0.0f : 0.5f, o1 = A == 1 ? 0.0f : 0.5f, o2 = A == 2 ? 0.0f : 0.5f;
     float q0 = __fdiv_rn(px - g.lo[0], g.dx) - o0, q1 = __fdiv_rn(py - g.lo[1], g.dx) - o1, q2 = __fdiv_rn(pz - g.lo[2], g.dx) - o2;
     float x = floorf(q0), y = floorf(q1), z = floorf(q2);
     int I = (int)x, J = (int)y, K = (int)z;
     int32_t s8[8];
-    bool all = tapslots(t, I, J, K, s8);
+    bool all = tapslots(t, I, J, K, s8, lk);
     int32_t ix[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m)
@@ -220,7 +243,7 @@ __device__ inline float facesample_a(const STopo& t, const V* __restrict__ fine,
                 } else {
                     int bx = (I + (m & 1)) >> 4, by = (J + ((m >> 1) & 1)) >> 4, bz = (K + (m >> 2)) >> 4;
                     bx -= A == 0; by -= A == 1; bz -= A == 2;
-                    lower = inblocks(g, bx, by, bz) ? t.slot[blockid(g, bx, by, bz)] : 0;
+                    lower = lk(t, bx, by, bz);
                 }
             }
             const int li = (I + (m & 1)) & 15, lj = (J + ((m >> 1) & 1)) & 15, lk = (K + (m >> 2)) & 15;
@@ -238,11 +261,11 @@ __device__ inline float facesample_a(const STopo& t, const V* __restrict__ fine,
     return trilinear(coarse, g.nc(0) + (A == 0), g.nc(1) + (A == 1), g.nc(2) + (A == 2), __fdiv_rn(px - g.lo[0], cd) - o0,
                      __fdiv_rn(py - g.lo[1], cd) - o1, __fdiv_rn(pz - g.lo[2], cd) - o2);
 }
-template <class V>
+template <class V, class L = SlotsGlobal>
 __device__ inline float facesample(const STopo& t, const V* __restrict__ fine, const V* __restrict__ coarse, int a, float px,
-                                   float py, float pz) {
-    return a == 0 ? facesample_a<0>(t, fine, coarse, px, py, pz)
-                  : (a == 1 ? facesample_a<1>(t, fine, coarse, px, py, pz) : facesample_a<2>(t, fine, coarse, px, py, pz));
+                                   float py, float pz, const L& lk = L()) {
+    return a == 0 ? facesample_a<0>(t, fine, coarse, px, py, pz, lk)
+                  : (a == 1 ? facesample_a<1>(t, fine, coarse, px, py, pz, lk) : facesample_a<2>(t, fine, coarse, px, py, pz, lk));
 }
 
 // sflip.jl:193-215 finegather: APIC gather over the fine faces only, with the weight found.

@@ -220,16 +220,22 @@ __global__ void k_sparse_union(float* __restrict__ phi, const float* __restrict_
 }
 
 // sflip.jl:796-806 sparsecombine_kernel!.
-__global__ void k_sparse_combine(float* __restrict__ u, const float* __restrict__ u0, const unsigned long long* __restrict__ mom,
-                                 const unsigned long long* __restrict__ wt, const float* __restrict__ phi,
-                                 const float* __restrict__ phic, STopo t, float r, float full, int a, int32_t nf) {
+// A workgroup's 256 faces lie in one slot (4352 = 17 x 256): the level set's taps find
+// their blocks in a table of the 27 blocks around it.
+__global__ __launch_bounds__(256) void k_sparse_combine(float* __restrict__ u, const float* __restrict__ u0,
+                                                        const unsigned long long* __restrict__ mom,
+                                                        const unsigned long long* __restrict__ wt, const float* __restrict__ phi,
+                                                        const float* __restrict__ phic, STopo t, float r, float full, int a,
+                                                        int32_t nf) {
+    __shared__ int32_t s27[27];
     int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nf) return;
     SCell c = fineface(t, a, n);
-    if (c.w.x < 0) return;
+    if (c.w.x < 0) return;   // the whole workgroup
+    stage27(t, c.w, s27);
+    __syncthreads();
     float p[3];
     facepos(t.g, a, c.w.x * 16 + c.i, c.w.y * 16 + c.j, c.w.z * 16 + c.k, t.g.dx, p);
-    float phif = cellsample(t, phi, phic, p[0], p[1], p[2]);
+    float phif = cellsample(t, phi, phic, p[0], p[1], p[2], Slots27{s27, c.w});
     float wgt = fromfix(wt[n]), m = fromfix(mom[n]), v0 = u0[n];
     u[n] = phif >= -r ? __fdiv_rn(m + jmax(full - wgt, 0.0f) * v0, jmax(full, wgt)) : v0;
 }

@@ -19,15 +19,17 @@ struct SFlow {
     const uint8_t* fc[3];
 };
 
-// sflip.jl:177-183 velocity and reached through the topology.
-__device__ inline void svelocity(const SFlow& f, float px, float py, float pz, float v[3]) {
+// sflip.jl:177-183 velocity and reached through the topology (block slots through `lk`).
+template <class L = SlotsGlobal>
+__device__ inline void svelocity(const SFlow& f, float px, float py, float pz, float v[3], const L& lk = L()) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) v[a] = facesample(f.t, f.uf[a], f.uc[a], a, px, py, pz);
+    for (int a = 0; a < 3; ++a) v[a] = facesample(f.t, f.uf[a], f.uc[a], a, px, py, pz, lk);
 }
-__device__ inline bool sreached(const SFlow& f, float px, float py, float pz) {
-    float m = facesample(f.t, f.ff[0], f.fc[0], 0, px, py, pz);
-    m = jmin(m, facesample(f.t, f.ff[1], f.fc[1], 1, px, py, pz));
-    m = jmin(m, facesample(f.t, f.ff[2], f.fc[2], 2, px, py, pz));
+template <class L = SlotsGlobal>
+__device__ inline bool sreached(const SFlow& f, float px, float py, float pz, const L& lk = L()) {
+    float m = facesample(f.t, f.ff[0], f.fc[0], 0, px, py, pz, lk);
+    m = jmin(m, facesample(f.t, f.ff[1], f.fc[1], 1, px, py, pz, lk));
+    m = jmin(m, facesample(f.t, f.ff[2], f.fc[2], 2, px, py, pz, lk));
     return m > 0.999f;
 }
 // flip.jl:367 substeps.
@@ -48,9 +50,24 @@ __device__ inline void sinwalls(const SGrid& g, float& x, float& y, float& z) {
 }
 
 // flip.jl:438-464 advectparticles_kernel! over the topology.
+// The workgroup's particles are cell-sorted, so their samples' blocks are looked up in a
+// table of the 27 blocks around the first particle's block (others in the global table).
 __global__ __launch_bounds__(256) void k_sadvect_particles(PView ps, int64_t n, SFlow flow, float dt, float gravity,
                                                            int32_t seed) {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int32_t s27[27];
+    const int64_t p0 = (int64_t)blockIdx.x * blockDim.x;
+    int4 home;
+    {
+        const SGrid& g = flow.t.g;
+        const float f0 = floorf(__fdiv_rn(ps.f[0][p0] - g.lo[0], g.dx)), f1 = floorf(__fdiv_rn(ps.f[1][p0] - g.lo[1], g.dx)),
+                    f2 = floorf(__fdiv_rn(ps.f[2][p0] - g.lo[2], g.dx));
+        const bool ok = fabsf(f0) < 1e8f && fabsf(f1) < 1e8f && fabsf(f2) < 1e8f;   // NaN fails too
+        home = ok ? make_int4((int)f0 >> 4, (int)f1 >> 4, (int)f2 >> 4, 0) : make_int4(-4, -4, -4, 0);
+    }
+    stage27(flow.t, home, s27);
+    __syncthreads();
+    const Slots27 lk{s27, home};
+    int64_t p = p0 + threadIdx.x;
     if (p >= n) return;
     float y0 = ps.f[0][p], y1 = ps.f[1][p], y2 = ps.f[2][p];
     float b0 = ps.f[3][p], b1 = ps.f[4][p], b2 = ps.f[5][p];
@@ -62,16 +79,16 @@ __global__ __launch_bounds__(256) void k_sadvect_particles(PView ps, int64_t n, 
     float act = jclamp(dt + tau + gm * xi * dt, 0.0f, 2.0f * dt);
     ps.f[15][p] = dt + tau - act;
     float vf[3];
-    svelocity(flow, y0, y1, y2, vf);
+    svelocity(flow, y0, y1, y2, vf, lk);
     int m = substeps(__fdiv_rn(jmax(speed, sqrtf(vf[0] * vf[0] + vf[1] * vf[1] + vf[2] * vf[2])) * act, h0));
     float h = __fdiv_rn(act, (float)m);
     const float vs0 = vf[0], vs1 = vf[1], vs2 = vf[2];   // the flow at the start point
     for (int s = 0; s < m; ++s) {
-        if (sreached(flow, y0, y1, y2)) {
+        if (sreached(flow, y0, y1, y2, lk)) {
             float v1[3] = {vs0, vs1, vs2};
-            if (s > 0) svelocity(flow, y0, y1, y2, v1);
+            if (s > 0) svelocity(flow, y0, y1, y2, v1, lk);
             float hh = 0.5f * h;
-            svelocity(flow, y0 + hh * v1[0], y1 + hh * v1[1], y2 + hh * v1[2], vf);
+            svelocity(flow, y0 + hh * v1[0], y1 + hh * v1[1], y2 + hh * v1[2], vf, lk);
             b0 = vf[0]; b1 = vf[1]; b2 = vf[2];
             y0 = y0 + h * b0; y1 = y1 + h * b1; y2 = y2 + h * b2;
         } else {
@@ -85,44 +102,53 @@ __global__ __launch_bounds__(256) void k_sadvect_particles(PView ps, int64_t n, 
 }
 
 // sflip.jl:715-731 backtrace: RK2 sub-steps of at most `cells` cells `h` wide.
-__device__ inline void backtrace(const SFlow& f, float& px, float& py, float& pz, float dt, float h) {
+template <class L = SlotsGlobal>
+__device__ inline void backtrace(const SFlow& f, float& px, float& py, float& pz, float dt, float h, const L& lk = L()) {
     float v[3];
-    svelocity(f, px, py, pz, v);
+    svelocity(f, px, py, pz, v, lk);
     int m = substeps(__fdiv_rn(sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * dt, 4.0f * h));
     float k = __fdiv_rn(dt, (float)m);
     for (int s = 0; s < m; ++s) {
         float v1[3] = {v[0], v[1], v[2]}, v2[3];   // the first sub-step starts where v was taken
-        if (s > 0) svelocity(f, px, py, pz, v1);
+        if (s > 0) svelocity(f, px, py, pz, v1, lk);
         float hk = 0.5f * k;
-        svelocity(f, px - hk * v1[0], py - hk * v1[1], pz - hk * v1[2], v2);
+        svelocity(f, px - hk * v1[0], py - hk * v1[1], pz - hk * v1[2], v2, lk);
         px = px - k * v2[0]; py = py - k * v2[1]; pz = pz - k * v2[2];
     }
 }
 
 // sflip.jl:733-763: cells (phi) and faces (the carried velocity) along the flow, into
 // phi0 and u0; fine parts over the pool, coarse cells outside blocks, all coarse faces.
-__global__ void k_sadvect_finecells(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ srcc,
-                                    SFlow f, float dt, int32_t nf) {
+// A workgroup's 256 cells (or faces) lie in one slot: the blocks of its samples are looked
+// up in a table of the 27 blocks around it.
+__global__ __launch_bounds__(256) void k_sadvect_finecells(float* __restrict__ out, const float* __restrict__ src,
+                                                           const float* __restrict__ srcc, SFlow f, float dt, int32_t nf) {
+    __shared__ int32_t s27[27];
     int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nf) return;
     SCell c = finecell(f.t, n);
-    if (c.w.x < 0) return;
+    if (c.w.x < 0) return;   // the whole workgroup
+    stage27(f.t, c.w, s27);
+    __syncthreads();
+    const Slots27 lk{s27, c.w};
     float p[3];
     finecentre(f.t.g, c.w.x * 16 + c.i, c.w.y * 16 + c.j, c.w.z * 16 + c.k, p);
-    backtrace(f, p[0], p[1], p[2], dt, f.t.g.dx);
-    out[n] = cellsample(f.t, src, srcc, p[0], p[1], p[2]);
+    backtrace(f, p[0], p[1], p[2], dt, f.t.g.dx, lk);
+    out[n] = cellsample(f.t, src, srcc, p[0], p[1], p[2], lk);
 }
 template <int A>
-__global__ void k_sadvect_finefaces(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ srcc,
-                                    SFlow f, float dt, int32_t nfa) {
+__global__ __launch_bounds__(256) void k_sadvect_finefaces(float* __restrict__ out, const float* __restrict__ src,
+                                                           const float* __restrict__ srcc, SFlow f, float dt, int32_t nfa) {
+    __shared__ int32_t s27[27];
     int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nfa) return;
     SCell c = fineface(f.t, A, n);
-    if (c.w.x < 0) return;
+    if (c.w.x < 0) return;   // the whole workgroup (4352 = 17 x 256 faces a slot)
+    stage27(f.t, c.w, s27);
+    __syncthreads();
+    const Slots27 lk{s27, c.w};
     float p[3];
     facepos(f.t.g, A, c.w.x * 16 + c.i, c.w.y * 16 + c.j, c.w.z * 16 + c.k, f.t.g.dx, p);
-    backtrace(f, p[0], p[1], p[2], dt, f.t.g.dx);
-    out[n] = facesample(f.t, src, srcc, A, p[0], p[1], p[2]);
+    backtrace(f, p[0], p[1], p[2], dt, f.t.g.dx, lk);
+    out[n] = facesample_a<A>(f.t, src, srcc, p[0], p[1], p[2], lk);
 }
 __global__ void k_sadvect_coarsecells(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ srcc,
                                       SFlow f, float dt) {

@@ -66,9 +66,10 @@ def sparse_bytes(kernel, live_cells, n_particles, precision=0, coarse_cells=0, c
         # (4 B each) and the type (1 B) per fine cell; per coarse cell outside the blocks the
         # same and the activity and cover mask; under a block activity, mask and q
         "search": 21 * live_cells + 23 * (coarse_cells - covered) + 6 * covered,
-        # the flexible CG's update: P, r read and written, r_old written, d, q read (4 B
-        # each) and the type (1 B) per unknown; the cover mask too on the coarse grid
-        "advance": 29 * live_cells + 30 * coarse_cells,
+        # the flexible CG's update: P read and written, r read, the new r written (over
+        # the older residual buffer), d, q read (4 B each) and the type (1 B) per unknown;
+        # the cover mask too on the coarse grid
+        "advance": 25 * live_cells + 26 * coarse_cells,
         # x (12 B) read, v and the APIC rows (48 B) written per particle; u, v, w and their
         # valid flags (15 B) read per fine cell's face triple
         "g2p": 60 * n_particles + 15 * live_cells,

@@ -1364,10 +1364,13 @@ __device__ inline bool unknown(size_t l, int32_t nf, const uint8_t* __restrict__
 
 // sflip.jl:1889-1892 in one pass: P += alpha d, rold = r, r -= alpha q, and r.r per
 // workgroup (alpha from the device scalars).
+// P += alpha d and the new residual r - alpha q into the other residual buffer (the old
+// one stays where it is, as r_old for the flexible CG's beta), with r.r per workgroup.
 __global__ __launch_bounds__(256) void k_fcg_update(double* __restrict__ partials, float* __restrict__ Pf,
-                                                    float* __restrict__ Pc, float* __restrict__ rf, float* __restrict__ rc,
-                                                    float* __restrict__ roldf, float* __restrict__ roldc,
-                                                    const float* __restrict__ df, const float* __restrict__ dc,
+                                                    float* __restrict__ Pc, const float* __restrict__ rf,
+                                                    const float* __restrict__ rc, float* __restrict__ routf,
+                                                    float* __restrict__ routc, const float* __restrict__ df,
+                                                    const float* __restrict__ dc,
                                                     const float* __restrict__ qf, const float* __restrict__ qc,
                                                     const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc,
                                                     const uint8_t* __restrict__ cm, STopo t, int32_t nf,
@@ -1379,13 +1382,11 @@ __global__ __launch_bounds__(256) void k_fcg_update(double* __restrict__ partial
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += (size_t)gridDim.x * blockDim.x) {
         float4 d = grp(df, dc, g, nf4), q = grp(qf, qc, g, nf4);
         float4& P = grp(Pf, Pc, g, nf4);
-        float4& r = grp(rf, rc, g, nf4);
-        float4 p = P, rv = r;
+        float4 p = P, rv = grp(rf, rc, g, nf4);
         p.x = p.x + al * d.x; p.y = p.y + al * d.y; p.z = p.z + al * d.z; p.w = p.w + al * d.w;
         P = p;
-        grp(roldf, roldc, g, nf4) = rv;
         rv.x = rv.x + -al * q.x; rv.y = rv.y + -al * q.y; rv.z = rv.z + -al * q.z; rv.w = rv.w + -al * q.w;
-        r = rv;
+        grp(routf, routc, g, nf4) = rv;
         unsigned m = unknown4(g, nf4, tf, tc, cm, t);
         if (m) v += dot4(m, rv, rv);
     }
@@ -2233,6 +2234,10 @@ struct SparseSolver {
         if (!s.d2.fine.p) s.d2.alloc((size_t)s.blocks * SBCELLS, Cc);
         SVec dbuf[2] = {d, vec(s.d2)};
         int cur = 0;
+        // r and r_old alternate between two buffers: the update writes the new residual
+        // over the older one instead of copying r to r_old
+        SVec rbuf[2] = {r, rold};
+        int rc = 0;
         for (int it = 1; it <= maxiter; ++it) {
             if (it == 1) {
                 apply(q, d);
@@ -2243,10 +2248,13 @@ struct SparseSolver {
                 d = dbuf[cur];
             }
             s.kt.begin(K_ADVANCE, st);
-            hipLaunchKernelGGL(k_fcg_update, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, P.f, P.c, r.f, r.c, rold.f,
-                               rold.c, d.f, d.c, q.f, q.c, s.types.fine.p, s.types.coarse.p, s.cmask.p, t, nf, s.scal.p,
-                               (int)SS_ALPHA);
+            hipLaunchKernelGGL(k_fcg_update, dim3(DOT_BLOCKS), dim3(256), 0, st, s.partials.p, P.f, P.c, r.f, r.c,
+                               rbuf[rc ^ 1].f, rbuf[rc ^ 1].c, d.f, d.c, q.f, q.c, s.types.fine.p, s.types.coarse.p, s.cmask.p,
+                               t, nf, s.scal.p, (int)SS_ALPHA);
             s.kt.end(K_ADVANCE, st);
+            rc ^= 1;
+            r = rbuf[rc];
+            rold = rbuf[rc ^ 1];
             hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(FINISH_THREADS), 0, st, s.partials.p, DOT_BLOCKS, s.scal.p, (int)SF_NORM,
                                (int)SS_RNORM);
             double flags;

@@ -413,6 +413,8 @@ struct SparseBasin {
     DBuf<int32_t> slot;
     DBuf<int4> where;
     DBuf<int32_t> list;       // added / removed slots of a refinement
+    DBuf<int32_t> sorder;     // the live slots in block order (x fastest), for per-slot launches
+    int32_t nlive = 0;        // their count
     DBuf<uint8_t> marks, marks2;
 
     SField<float> phi, phi0, P;

@@ -367,17 +367,23 @@ __global__ void k_covermask(uint8_t* __restrict__ m, STopo t) {
 // loaded first (an index off the grid reads the cell itself); a covered neighbour is
 // always at the edge of its block, so its sub-faces and cells index from one slot
 // lookup (the upper block's copy of a face, or the lower block's high plane). Sums in
-// the reference's order.
-template <int N, class C, class A, class FX, class CX>
-__device__ inline void coarserow_acc(const STopo& t, unsigned msk, FX fx, const FineCoefT<C>& cf, CX cx,
-                                     const CoarseCoefT<C>& ccf, const C* __restrict__ dDc, int I, int J, int K, A& diag,
-                                     A& off, const LevelTypes* ty = nullptr) {
-    constexpr int R = N / 4, V = N * N * N, m = N - 1;
-    constexpr uint32_t per = (N + 1) * N * N;
+// the reference's order. coarserow_pre issues the loads of the uncovered part (the six
+// neighbours' couplings and values, the Dirichlet diagonal), coarserow_post sums them
+// with the covered sides', so a caller can issue the first batch before it knows whether
+// the row is needed.
+template <class A> struct CoarsePre {
+    A cc[6], xv[6], dg;
+    bool in[6];
+};
+template <class C, class A, class CX>
+__device__ inline CoarsePre<A> coarserow_pre(const STopo& t, CX cx, const CoarseCoefT<C>& ccf, const C* __restrict__ dDc, int I,
+                                             int J, int K, const LevelTypes* ty) {
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
     size_t own = cl(I, J, K, n0, n1);
-    A cc[6], xv[6];
-    bool in[6];
+    CoarsePre<A> P;
+    A* cc = P.cc;
+    A* xv = P.xv;
+    bool* in = P.in;
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
         int a = d >> 1, sg = (d & 1) ? 1 : -1;
@@ -393,7 +399,30 @@ __device__ inline void coarserow_acc(const STopo& t, unsigned msk, FX fx, const 
         }
         xv[d] = cx(nidx);
     }
-    diag = (A)lf(dDc[own]);
+    P.dg = (A)lf(dDc[own]);
+    return P;
+}
+// The row of a cell none of whose neighbours is covered, branch-free: a zero coupling
+// adds exactly nothing (the same sums as coarserow_post's in that case).
+template <class A> __device__ inline void coarserow_flat(const CoarsePre<A>& P, A& diag, A& off) {
+    diag = P.dg;
+    off = (A)0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const A c = (P.in[d] && P.cc[d] > (A)0) ? P.cc[d] : (A)0;
+        diag += c;
+        off += c * P.xv[d];
+    }
+}
+template <int N, class C, class A, class FX>
+__device__ inline void coarserow_post(const STopo& t, unsigned msk, FX fx, const FineCoefT<C>& cf, const CoarsePre<A>& P, int I,
+                                      int J, int K, A& diag, A& off, const LevelTypes* ty) {
+    constexpr int R = N / 4, V = N * N * N, m = N - 1;
+    constexpr uint32_t per = (N + 1) * N * N;
+    const A* cc = P.cc;
+    const A* xv = P.xv;
+    const bool* in = P.in;
+    diag = P.dg;
     off = (A)0;
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
@@ -432,6 +461,14 @@ __device__ inline void coarserow_acc(const STopo& t, unsigned msk, FX fx, const 
             off += cc[d] * xv[d];
         }
     }
+}
+
+template <int N, class C, class A, class FX, class CX>
+__device__ inline void coarserow_acc(const STopo& t, unsigned msk, FX fx, const FineCoefT<C>& cf, CX cx,
+                                     const CoarseCoefT<C>& ccf, const C* __restrict__ dDc, int I, int J, int K, A& diag,
+                                     A& off, const LevelTypes* ty = nullptr) {
+    const CoarsePre<A> P = coarserow_pre<C, A>(t, cx, ccf, dDc, I, J, K, ty);
+    coarserow_post<N, C, A>(t, msk, fx, cf, P, I, J, K, diag, off, ty);
 }
 
 // The same reading x from arrays (Z: x = 0, nothing read).
@@ -916,27 +953,41 @@ __device__ __forceinline__ void rowcoarse_wg(const uint32_t cb, const uint32_t p
     if (live) {
     int I, J, K;
     unlin(l, n0, n1, I, J, K);
-    float res = 0.0f;
-    unsigned msk = L.cmask[l];
-    if constexpr (SEARCH) S.dnc[l] = lf(x[l]) + beta * S.doc[l];
-    if (L.actc[l] != 0 && !(msk & 1u)) {
-        float diag, off;
-        const LevelTypes ty{L.actf, L.tc};   // level 0 (N = 16): the couplings from the types
-        const LevelTypes* typ = N == 16 ? &ty : nullptr;
-        if constexpr (SEARCH) {
-            auto fx = [&](uint32_t i) -> float { return lf(xf[i]) + beta * S.dof[i]; };
-            auto cx = [&](size_t i) -> float { return lf(x[i]) + beta * S.doc[i]; };
-            coarserow_acc<N, C, float>(t, msk, fx, L.cf, cx, L.cc, L.dDc, I, J, K, diag, off, typ);
-        } else {
-            coarserow<N, T, C, float, MODE == 3>(t, msk, xf, L.cf, x, L.cc, L.dDc, I, J, K, diag, off, typ);
-        }
-        float xl = MODE == 3 ? 0.0f : (SEARCH ? lf(x[l]) + beta * S.doc[l] : lf(x[l]));
-        float ax = diag * xl - off;
-        if (MODE == 0 || MODE == 3) res = diag > 0.0f ? xl + __fdiv_rn(OMEGA * (lf(b[l]) - ax), diag) : 0.0f;
-        else if (MODE == 1) res = lf(b[l]) - ax;
-        else res = ax;
-        if constexpr (SEARCH) pq = (double)xl * (double)res;
+    // every load of the common row (no covered neighbour) issued in one batch, ahead of
+    // the mask and type tests that decide which sums it takes
+    const unsigned msk = L.cmask[l];
+    const bool row = L.actc[l] != 0 && !(msk & 1u);
+    const LevelTypes ty{L.actf, L.tc};   // level 0 (N = 16): the couplings from the types
+    const LevelTypes* typ = N == 16 ? &ty : nullptr;
+    auto fx = [&](uint32_t i) -> float {
+        if constexpr (SEARCH) return lf(xf[i]) + beta * S.dof[i];
+        else if constexpr (MODE == 3) return 0.0f;
+        else return lf(xf[i]);
+    };
+    auto cx = [&](size_t i) -> float {
+        if constexpr (SEARCH) return lf(x[i]) + beta * S.doc[i];
+        else if constexpr (MODE == 3) return 0.0f;
+        else return lf(x[i]);
+    };
+    const CoarsePre<float> P = coarserow_pre<C, float>(t, cx, L.cc, L.dDc, I, J, K, typ);
+    const float xl = cx(l);
+    const float bl = (MODE == 0 || MODE == 1 || MODE == 3) ? lf(b[l]) : 0.0f;
+    if constexpr (SEARCH) S.dnc[l] = xl;
+    float diag, off;
+    if (msk & 0x7eu) {
+        diag = 0.0f;
+        off = 0.0f;
+        if (row) coarserow_post<N, C, float>(t, msk, fx, L.cf, P, I, J, K, diag, off, typ);
+    } else {
+        coarserow_flat(P, diag, off);
     }
+    const float ax = diag * xl - off;
+    float res;
+    if (MODE == 0 || MODE == 3) res = diag > 0.0f ? xl + __fdiv_rn(OMEGA * (bl - ax), diag) : 0.0f;
+    else if (MODE == 1) res = bl - ax;
+    else res = ax;
+    res = row ? res : 0.0f;
+    if constexpr (SEARCH) pq = row ? (double)xl * (double)res : 0.0;
     out[l] = sf<T>(res);
     }
     if constexpr (SEARCH) {
@@ -1022,15 +1073,14 @@ __global__ __launch_bounds__(256) void k_rowdense(T* __restrict__ out, const T* 
     if (l >= (size_t)L.n.x * L.n.y * L.n.z) return;
     int i, j, k;
     unlin(l, L.n.x, L.n.y, i, j, k);
-    float res = 0.0f;
-    if (L.act[l] != 0) {
-        float diag, off;
-        denserow<T, C, MODE == 3>(L, x, i, j, k, diag, off);
-        float xl = MODE == 3 ? 0.0f : lf(x[l]);
-        float ax = diag * xl - off;
-        res = MODE != 1 ? (diag > 0.0f ? xl + __fdiv_rn(OMEGA * (lf(b[l]) - ax), diag) : 0.0f) : lf(b[l]) - ax;
-    }
-    out[l] = sf<T>(res);
+    // the row's loads issued with the activity flag's, the result selected after
+    const bool act = L.act[l] != 0;
+    float diag, off;
+    denserow<T, C, MODE == 3>(L, x, i, j, k, diag, off);
+    const float xl = MODE == 3 ? 0.0f : lf(x[l]), bl = lf(b[l]);
+    const float ax = diag * xl - off;
+    const float res = MODE != 1 ? (diag > 0.0f ? xl + __fdiv_rn(OMEGA * (bl - ax), diag) : 0.0f) : bl - ax;
+    out[l] = sf<T>(act ? res : 0.0f);
 }
 // The coarser level's right-hand side, the residual and its restriction in one pass:
 // each coarser cell forms its eight finer cells' residuals (k_rowdense mode 1's

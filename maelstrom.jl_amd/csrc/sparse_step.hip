@@ -465,7 +465,7 @@ void SparseBasin::set_stir(const StirParams& p, const float* waves, int nwaves, 
 
 // One layer: an unflagged face takes the mean of its flagged neighbours (fine faces;
 // neighbours through finefaceat's rule, the blocks read from a table of the 27 around
-// the workgroup's block). A workgroup's 256 faces lie in one slot (4352 = 17 x 256). The
+// the slot). A workgroup per live slot (block order), a thread per 17 of its faces. The
 // output arrays hold the layer before the input's (or, with `fresh`, nothing yet). A
 // flagged face keeps its value from layer to layer, and so does an unflagged one until it
 // is flagged: a face whose flag is the same in both and that takes no neighbour's mean is
@@ -475,30 +475,13 @@ template <int A>
 __global__ __launch_bounds__(256) void k_sextrapfine(float* __restrict__ out, uint8_t* __restrict__ fout,
                                                      const float* __restrict__ src, const uint8_t* __restrict__ fin,
                                                      float* __restrict__ out2, const float* __restrict__ src2, STopo t,
-                                                     int fresh) {
+                                                     int fresh, const int32_t* __restrict__ order) {
     __shared__ int32_t s27[27];
-    const int32_t n = (int32_t)(blockIdx.x * 256 + threadIdx.x);
-    const int s = (int)(blockIdx.x / 17) + 1;
+    const int s = order[blockIdx.x];   // 1-based, live
     const int4 w = t.where[s - 1];
-    if (w.x < 0) return;   // the whole workgroup
     stage27(t, w, s27);
     const bool clear = blockclear(t.g, w);
     __syncthreads();
-    const uint8_t f0 = fin[n];
-    const uint8_t fo = fresh ? 2 : fout[n];   // 2: not known
-    if (f0) {
-        if (fo == 1) return;
-        out[n] = src[n];
-        if (out2) out2[n] = src2[n];
-        fout[n] = 1;
-        return;
-    }
-    const int f = n - (s - 1) * SBFACES;
-    int i, j, k;
-    if (A == 0) { i = f % 17; int q = f / 17; j = q & 15; k = q >> 4; }
-    else if (A == 1) { i = f & 15; int q = f >> 4; j = q % 17; k = q / 17; }
-    else { i = f & 15; j = (f >> 4) & 15; k = f >> 8; }
-    const int I = w.x * 16 + i, J = w.y * 16 + j, K = w.z * 16 + k;
     // finefaceat for a neighbour at global (GI, GJ, GK): its own block's copy, else the
     // top face of the block below along A when it lies on that plane
     auto face = [&](int GI, int GJ, int GK) -> int32_t {
@@ -511,40 +494,58 @@ __global__ __launch_bounds__(256) void k_sextrapfine(float* __restrict__ out, ui
         if (s2 == 0) return -1;
         return (s2 - 1) * SBFACES + ffn(16, A, A == 0 ? 16 : (GI & 15), A == 1 ? 16 : (GJ & 15), A == 2 ? 16 : (GK & 15));
     };
-    bool walled = false;
-    if (!clear) {
-        float p[3];
-        facepos(t.g, A, I, J, K, t.g.dx, p);
-        walled = walledface_dev(t.g, p, A, t.g.dx);
-    }
-    int32_t nidx[6];
-    uint8_t nf[6];
-    bool any = false;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        nidx[q] = walled ? -1 : face(I + NBX[q][0], J + NBX[q][1], K + NBX[q][2]);
-        nf[q] = nidx[q] >= 0 ? fin[nidx[q]] : 0;
-        any = any || nf[q] != 0;
-    }
-    if (!any) {   // stays as it is
-        if (fo == 0) return;
-        out[n] = src[n];
-        if (out2) out2[n] = src2[n];
-        fout[n] = 0;
-        return;
-    }
-    float acc = 0.0f, acc2 = 0.0f;
-    int cnt = 0;
-#pragma unroll
-    for (int q = 0; q < 6; ++q)
-        if (nf[q]) {
-            acc += src[nidx[q]];
-            if (out2) acc2 += src2[nidx[q]];
-            ++cnt;
+    for (int r = 0; r < SBFACES / 256; ++r) {
+        const int f = r * 256 + (int)threadIdx.x;
+        const int32_t n = (s - 1) * SBFACES + f;
+        const uint8_t f0 = fin[n];
+        const uint8_t fo = fresh ? 2 : fout[n];   // 2: not known
+        if (f0) {
+            if (fo == 1) continue;
+            out[n] = src[n];
+            if (out2) out2[n] = src2[n];
+            fout[n] = 1;
+            continue;
         }
-    out[n] = __fdiv_rn(acc, (float)cnt);
-    if (out2) out2[n] = __fdiv_rn(acc2, (float)cnt);
-    fout[n] = 1;
+        int i, j, k;
+        if (A == 0) { i = f % 17; int q = f / 17; j = q & 15; k = q >> 4; }
+        else if (A == 1) { i = f & 15; int q = f >> 4; j = q % 17; k = q / 17; }
+        else { i = f & 15; j = (f >> 4) & 15; k = f >> 8; }
+        const int I = w.x * 16 + i, J = w.y * 16 + j, K = w.z * 16 + k;
+        bool walled = false;
+        if (!clear) {
+            float p[3];
+            facepos(t.g, A, I, J, K, t.g.dx, p);
+            walled = walledface_dev(t.g, p, A, t.g.dx);
+        }
+        int32_t nidx[6];
+        uint8_t nf[6];
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            nidx[q] = walled ? -1 : face(I + NBX[q][0], J + NBX[q][1], K + NBX[q][2]);
+            nf[q] = nidx[q] >= 0 ? fin[nidx[q]] : 0;
+            any = any || nf[q] != 0;
+        }
+        if (!any) {   // stays as it is
+            if (fo == 0) continue;
+            out[n] = src[n];
+            if (out2) out2[n] = src2[n];
+            fout[n] = 0;
+            continue;
+        }
+        float acc = 0.0f, acc2 = 0.0f;
+        int cnt = 0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            if (nf[q]) {
+                acc += src[nidx[q]];
+                if (out2) acc2 += src2[nidx[q]];
+                ++cnt;
+            }
+        out[n] = __fdiv_rn(acc, (float)cnt);
+        if (out2) out2[n] = __fdiv_rn(acc2, (float)cnt);
+        fout[n] = 1;
+    }
 }
 __global__ void k_sextrapcoarse(float* __restrict__ out, uint8_t* __restrict__ fout, const float* __restrict__ src,
                                 const uint8_t* __restrict__ fin, float* __restrict__ out2, const float* __restrict__ src2,
@@ -628,8 +629,9 @@ void SparseBasin::extrapolate(int layers, int coarselayers, bool withkick) {
         uint8_t *fin = fl[a].fine.p, *fout = fl2[a].fine.p;
         auto kern = a == 0 ? k_sextrapfine<0> : (a == 1 ? k_sextrapfine<1> : k_sextrapfine<2>);
         for (int l = 0; l < layers; ++l) {   // layer 1 writes over the input of layer 0
-            if (F)
-                hipLaunchKernelGGL(kern, dim3((unsigned)(F / 256)), dim3(256), 0, s, tmp, fout, f, fin, ktmp, k, t, l == 0 ? 1 : 0);
+            if (nlive > 0)   // free slots: nothing reads their faces
+                hipLaunchKernelGGL(kern, dim3((unsigned)nlive), dim3(256), 0, s, tmp, fout, f, fin, ktmp, k, t, l == 0 ? 1 : 0,
+                                   sorder.p);
             std::swap(f, tmp); std::swap(k, ktmp); std::swap(fin, fout);
         }
         if (f != u[a].fine.p) {   // an odd count: copy back

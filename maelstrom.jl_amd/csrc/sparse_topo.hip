@@ -18,7 +18,7 @@ SparseBasin::SparseBasin(const SGrid& grid, int nblk, int64_t cap, int32_t bnd, 
     size_t nf = (size_t)blocks * SBCELLS, nfa = (size_t)blocks * SBFACES;
     hslot.assign(NB, 0);
     hwhere.assign(blocks, make_int4(-1, -1, -1, 0));
-    slot.alloc(NB); where.alloc(blocks); list.alloc(blocks);
+    slot.alloc(NB); where.alloc(blocks); list.alloc(blocks); sorder.alloc(std::max(blocks, 1));
     marks.alloc(NB); marks2.alloc(NB);
     for (auto* f : {&phi, &phi0, &P, &b, &r, &z, &d, &q, &rold, &tmp, &dD}) f->alloc(nf, Cc);
     types.alloc(nf, Cc);
@@ -254,8 +254,18 @@ void SparseBasin::refine_host(const uint8_t* want, std::vector<int32_t>& added, 
 }
 
 void SparseBasin::upload_topology() {
+    // a per-slot launch that walks the live slots in block order runs blocks that are
+    // neighbours in x (and, a row later, in y) together, so the layers they read across
+    // their sides are still in L2
+    std::vector<int32_t> order;
+    order.reserve((size_t)blocks);
+    for (int32_t sl : hslot)
+        if (sl != 0) order.push_back(sl);
+    nlive = (int32_t)order.size();
     MAEL_HIP(hipMemcpyAsync(slot.p, hslot.data(), hslot.size() * sizeof(int32_t), hipMemcpyHostToDevice, st.s));
     MAEL_HIP(hipMemcpyAsync(where.p, hwhere.data(), hwhere.size() * sizeof(int4), hipMemcpyHostToDevice, st.s));
+    if (nlive > 0)
+        MAEL_HIP(hipMemcpyAsync(sorder.p, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice, st.s));
     st.sync();   // the host vectors may change before the copies would otherwise run
 }
 
@@ -620,80 +630,107 @@ void SparseBasin::gravity(float dt, float gacc) {
 
 // ─── redistancing (sflip.jl:833-919) ─────────────────────────────────────────
 
-// A workgroup's 256 cells lie in one slot (4096 = 16 x 256); the neighbours across its
-// sides come from the table of the blocks around it, and a block clear of the walls skips
-// the solid tests.
+// A workgroup per slot: the block's 16^3 values and the layers of the six blocks across
+// its sides (the coarse cell where there is no block, as in the reference's nphi) are
+// staged in LDS once, then each thread takes a 16-cell column along z. A block clear of
+// the walls skips the solid tests.
 __global__ __launch_bounds__(256) void k_sreinit(float* __restrict__ out, const float* __restrict__ phi,
-                                                 const float* __restrict__ phic, STopo t, float phimax) {
+                                                 const float* __restrict__ phic, STopo t, float phimax,
+                                                 const int32_t* __restrict__ order) {
     __shared__ int32_t s27[27];
-    const int32_t n = (int32_t)(blockIdx.x * 256 + threadIdx.x);
-    SCell c = finecell(t, n);
-    if (c.w.x < 0) return;   // the whole workgroup
-    stage27(t, c.w, s27);
-    const bool clear = blockclear(t.g, c.w);
-    __syncthreads();
+    __shared__ float tv[4096];       // the block's cells, x fastest
+    __shared__ float hv[6][256];     // neighbour m's layer at (p, q), the other two axes in order
+    const int32_t slot = order[blockIdx.x] - 1;   // live slots in block order
+    const int4 w = t.where[slot];
+    stage27(t, w, s27);
     const SGrid& g = t.g;
-    int I = c.w.x * 16 + c.i, J = c.w.y * 16 + c.j, K = c.w.z * 16 + c.k;
-    float cv = phi[n], h = g.dx, p[3];
+    const int tid = (int)threadIdx.x, ti = tid & 15, tj = tid >> 4;
+    const float* __restrict__ bp = phi + (size_t)slot * 4096;
+    float own[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) own[k] = bp[tid + 256 * k];
+    __syncthreads();
     const int n0 = g.nc(0), n1 = g.nc(1), n2 = g.nc(2);
-    // nphi of the neighbour m: this block's cell, the face neighbour block's, or the coarse cell
-    float nv[6];
+    float hl[6];
 #pragma unroll
-    for (int m = 0; m < 6; ++m) {
-        const int li = c.i + NBS[m][0], lj = c.j + NBS[m][1], lk = c.k + NBS[m][2];
-        const int di = (li >> 4) + 1, dj = (lj >> 4) + 1, dk = (lk >> 4) + 1;
-        const int32_t sn = s27[di + 3 * dj + 9 * dk];
-        if (sn != 0) {
-            nv[m] = phi[(sn - 1) * 4096 + (li & 15) + 16 * ((lj & 15) + 16 * (lk & 15))];
+    for (int m = 0; m < 6; ++m) {   // entry (p, q) = (ti, tj) of each side's layer
+        const int a = m >> 1, e = NBS[m][a] > 0 ? 16 : -1;
+        const int li = a == 0 ? e : ti, lj = a == 1 ? e : (a == 0 ? ti : tj), lk = a == 2 ? e : tj;
+        const int32_t sn = s27[((li >> 4) + 1) + 3 * ((lj >> 4) + 1) + 9 * ((lk >> 4) + 1)];
+        const int I = w.x * 16 + li, J = w.y * 16 + lj, K = w.z * 16 + lk;
+        const size_t ci = cl(min(max(I >> 2, 0), n0 - 1), min(max(J >> 2, 0), n1 - 1), min(max(K >> 2, 0), n2 - 1), n0, n1);
+        const size_t fi = (size_t)(sn > 0 ? sn - 1 : 0) * 4096 + (li & 15) + 16 * ((lj & 15) + 16 * (lk & 15));
+        hl[m] = sn != 0 ? phi[fi] : phic[ci];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) tv[tid + 256 * k] = own[k];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) hv[m][tid] = hl[m];
+    const bool clear = blockclear(g, w);
+    // a cell whose three upwind distances are all phimax (deep in the water or the air)
+    // takes this one value: the same function of the same arguments
+    const float dsat = jmin(eikonal(phimax, phimax, phimax, g.dx), phimax);
+    __syncthreads();
+    float* __restrict__ op = out + (size_t)slot * 4096;
+    for (int k = 0; k < 16; ++k) {
+        const int l = tid + 256 * k;
+        const int I = w.x * 16 + ti, J = w.y * 16 + tj, K = w.z * 16 + k;
+        const float cv = tv[l], h = g.dx;
+        float nv[6];   // in the order of NBS: +x, -x, +y, -y, +z, -z
+        nv[0] = ti < 15 ? tv[l + 1] : hv[0][tj + 16 * k];
+        nv[1] = ti > 0 ? tv[l - 1] : hv[1][tj + 16 * k];
+        nv[2] = tj < 15 ? tv[l + 16] : hv[2][ti + 16 * k];
+        nv[3] = tj > 0 ? tv[l - 16] : hv[3][ti + 16 * k];
+        nv[4] = k < 15 ? tv[l + 256] : hv[4][tid];
+        nv[5] = k > 0 ? tv[l - 256] : hv[5][tid];
+        bool solid = false, nsolid[6] = {false, false, false, false, false, false};
+        if (!clear) {
+            float p[3];
+            finecentre(g, I, J, K, p);
+            solid = sissolid(g, p[0], p[1], p[2]);
+#pragma unroll
+            for (int m = 0; m < 6; ++m) {
+                finecentre(g, I + NBS[m][0], J + NBS[m][1], K + NBS[m][2], p);
+                nsolid[m] = sissolid(g, p[0], p[1], p[2]);
+            }
+        }
+        float res;
+        if (solid) {
+            float lowest = cv;
+            bool found = false;
+#pragma unroll
+            for (int m = 0; m < 6; ++m) {
+                if (!nsolid[m]) {
+                    float f = nv[m];
+                    lowest = found ? jmin(lowest, f) : f;
+                    found = true;
+                }
+            }
+            res = found ? lowest : jclamp(cv, -phimax, phimax);
         } else {
-            const int a = I + NBS[m][0], b = J + NBS[m][1], cc = K + NBS[m][2];
-            nv[m] = phic[cl(min(max(a >> 2, 0), n0 - 1), min(max(b >> 2, 0), n1 - 1), min(max(cc >> 2, 0), n2 - 1), n0, n1)];
-        }
-    }
-    bool solid = false, nsolid[6] = {false, false, false, false, false, false};
-    if (!clear) {
-        finecentre(g, I, J, K, p);
-        solid = sissolid(g, p[0], p[1], p[2]);
+            bool inside = cv < 0.0f, frozen = false;
+            float ax = phimax, ay = phimax, az = phimax;
 #pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            finecentre(g, I + NBS[m][0], J + NBS[m][1], K + NBS[m][2], p);
-            nsolid[m] = sissolid(g, p[0], p[1], p[2]);
-        }
-    }
-    float res;
-    if (solid) {
-        float lowest = cv;
-        bool found = false;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            if (!nsolid[m]) {
-                float f = nv[m];
-                lowest = found ? jmin(lowest, f) : f;
-                found = true;
+            for (int m = 0; m < 6; ++m) {
+                if (!nsolid[m]) {
+                    float f = nv[m];
+                    if ((f < 0.0f) != inside) frozen = true;
+                    float d = fabsf(f);
+                    if (m < 2) ax = jmin(ax, d);
+                    else if (m < 4) ay = jmin(ay, d);
+                    else az = jmin(az, d);
+                }
+            }
+            if (frozen) res = cv;
+            else {
+                const bool sat = ax == phimax && ay == phimax && az == phimax;
+                float d = dsat;
+                if (!sat) d = jmin(eikonal(ax, ay, az, h), phimax);
+                res = inside ? -d : d;
             }
         }
-        res = found ? lowest : jclamp(cv, -phimax, phimax);
-    } else {
-        bool inside = cv < 0.0f, frozen = false;
-        float ax = phimax, ay = phimax, az = phimax;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            if (!nsolid[m]) {
-                float f = nv[m];
-                if ((f < 0.0f) != inside) frozen = true;
-                float d = fabsf(f);
-                if (m < 2) ax = jmin(ax, d);
-                else if (m < 4) ay = jmin(ay, d);
-                else az = jmin(az, d);
-            }
-        }
-        if (frozen) res = cv;
-        else {
-            float d = jmin(eikonal(ax, ay, az, h), phimax);
-            res = inside ? -d : d;
-        }
+        op[l] = res;
     }
-    out[n] = res;
 }
 __global__ void k_coarsewall(float* __restrict__ out, const float* __restrict__ phic, STopo t) {
     int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
@@ -724,8 +761,9 @@ void SparseBasin::redistance(int iterations) {
     float phimax = (float)(iterations + 1) * g.dx;
     hipStream_t s = st.s;
     for (int it = 0; it < iterations && nslots > 0; ++it) {
-        hipLaunchKernelGGL(k_sreinit, dim3((unsigned)(finecells() / 256)), dim3(256), 0, s, phi0.fine.p, phi.fine.p,
-                           phi.coarse.p, topo(), phimax);
+        if (nlive > 0)   // free slots keep what they hold in both copies: nothing reads them
+            hipLaunchKernelGGL(k_sreinit, dim3((unsigned)nlive), dim3(256), 0, s, phi0.fine.p, phi.fine.p, phi.coarse.p,
+                               topo(), phimax, sorder.p);
         std::swap(phi.fine, phi0.fine);
     }
     hipLaunchKernelGGL(k_coarsewall, dim3(cdiv(ncells(), 256)), dim3(256), 0, s, phi0.coarse.p, phi.coarse.p, topo());

@@ -303,10 +303,12 @@ __global__ __launch_bounds__(1024) void k_coarsest(float* __restrict__ x, const 
         for (int half = 0; half < 2; ++half) {
             int colour = s <= sweeps ? half : 1 - half;
             if (active && ti == LIQUID && par == colour) {
+                T v[6];   // the six neighbours read together (a non-liquid one reads cell 0)
+#pragma unroll
+                for (int m = 0; m < 6; ++m) v[m] = xs[nbr[m]];
                 T acc = bi;
 #pragma unroll
-                for (int m = 0; m < 6; ++m)
-                    if (liq[m]) acc += xs[nbr[m]];
+                for (int m = 0; m < 6; ++m) acc = liq[m] ? acc + v[m] : acc;
                 xs[t] = d > T(0) ? acc / d : T(0);
             }
             __syncthreads();

@@ -1143,6 +1143,12 @@ __global__ __launch_bounds__(1024) void k_scoarsest(T* __restrict__ x, const T* 
         c = lf(L.c[2][cl(i, j, k, n0, n1)]);         if (c > 0.0f && k > 0) { cw[4] = c; nb[4] = t - n0 * n1; }
         c = lf(L.c[2][cl(i, j, k + 1, n0, n1)]);     if (c > 0.0f && k < n2 - 1) { cw[5] = c; nb[5] = t + n0 * n1; }
     }
+    // the diagonal is the same every pass; a pass reads the six neighbours together and
+    // adds the coupled ones (an absent neighbour reads cell 0 and adds nothing)
+    A diag = dg;
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+        if (cw[m] > A(0)) diag += cw[m];
     xs[t] = A(0);
     __syncthreads();
     int par = (i + j + k + 1) & 1;
@@ -1150,10 +1156,12 @@ __global__ __launch_bounds__(1024) void k_scoarsest(T* __restrict__ x, const T* 
         for (int half = 0; half < 2; ++half) {
             int colour = s <= sweeps ? half : 1 - half;
             if (act && par == colour) {
-                A diag = dg, acc = bi;
+                A v[6];
 #pragma unroll
-                for (int m = 0; m < 6; ++m)
-                    if (cw[m] > A(0)) { diag += cw[m]; acc += cw[m] * xs[nb[m]]; }
+                for (int m = 0; m < 6; ++m) v[m] = xs[nb[m]];
+                A acc = bi;
+#pragma unroll
+                for (int m = 0; m < 6; ++m) acc = cw[m] > A(0) ? acc + cw[m] * v[m] : acc;
                 xs[t] = diag > A(0) ? acc / diag : A(0);
             }
             __syncthreads();

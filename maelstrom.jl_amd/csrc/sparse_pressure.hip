@@ -1607,28 +1607,61 @@ __device__ inline float pdiv(float a, float b) { return __fdiv_rn(a, b); }
 __device__ inline float pdiv(double a, float b) { return (float)(a / (double)b); }
 
 template <class PT>
-__global__ void k_projectfine(float* __restrict__ u, uint8_t* __restrict__ valid, const PT* __restrict__ P,
-                              const PT* __restrict__ Pc, const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc,
-                              const float* __restrict__ phi, const float* __restrict__ phic, STopo t, int a, int32_t nfa) {
-    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nfa) return;
-    SCell s = fineface(t, a, n);
-    if (s.w.x < 0) return;
-    int I = s.w.x * 16 + s.i, J = s.w.y * 16 + s.j, K = s.w.z * 16 + s.k;
-    int Im = I - (a == 0), Jm = J - (a == 1), Km = K - (a == 2);
-    bool fp, fm;
-    uint8_t tp = celltype(t, tf, tc, I, J, K, fp), tm = celltype(t, tf, tc, Im, Jm, Km, fm);
-    bool ok = tp != SOLID && tm != SOLID && (tp == LIQUID || tm == LIQUID);
-    if (ok) {
-        float d = (fp && fm ? 1.0f : 2.5f) * t.g.dx;
-        PT Pp = fineorcoarse(t, P, Pc, I, J, K), Pm = fineorcoarse(t, P, Pc, Im, Jm, Km);
-        float Dl;
-        if (tp == LIQUID && tm == LIQUID) Dl = pdiff(Pp, Pm);
-        else if (tm == LIQUID) Dl = pdiv(-Pm, theta(fineorcoarse(t, phi, phic, Im, Jm, Km), fineorcoarse(t, phi, phic, I, J, K)));
-        else Dl = pdiv(Pp, theta(fineorcoarse(t, phi, phic, I, J, K), fineorcoarse(t, phi, phic, Im, Jm, Km)));
-        u[n] -= __fdiv_rn(Dl, d);
+__global__ __launch_bounds__(256) void k_projectfine(float* __restrict__ u, uint8_t* __restrict__ valid,
+                                                     const PT* __restrict__ P, const PT* __restrict__ Pc,
+                                                     const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc,
+                                                     const float* __restrict__ phi, const float* __restrict__ phic, STopo t, int a,
+                                                     const int32_t* __restrict__ order) {
+    // a workgroup per live slot (block order), a thread per 17 of its faces along a: a
+    // face's two cells lie in the slot's block or in the blocks below and above it along
+    // a, looked up once (fineat's index, or -1 where there is no block)
+    const int s = order[blockIdx.x];
+    const int4 w = t.where[s - 1];
+    const SlotsGlobal lk;
+    const int32_t slo = lk(t, w.x - (a == 0), w.y - (a == 1), w.z - (a == 2));
+    const int32_t shi = lk(t, w.x + (a == 0), w.y + (a == 1), w.z + (a == 2));
+    const int n0 = t.g.nc(0), n1 = t.g.nc(1), n2 = t.g.nc(2);
+    // fineat's index of the cell at local (li, lj, lk), the coordinate along a in -1..16
+    auto idx = [&](int li, int lj, int lk) -> int32_t {
+        const int la = a == 0 ? li : (a == 1 ? lj : lk);
+        const int32_t sl = la < 0 ? slo : (la > 15 ? shi : s);
+        return sl == 0 ? -1 : (sl - 1) * 4096 + (li & 15) + 16 * ((lj & 15) + 16 * (lk & 15));
+    };
+    auto ctype = [&](int32_t id, int I, int J, int K, bool& isfine) -> uint8_t {   // celltype
+        if (id >= 0) { isfine = true; return tf[id]; }
+        isfine = false;
+        const int ci = I >> 2, cj = J >> 2, ck = K >> 2;
+        if (ci < 0 || cj < 0 || ck < 0 || ci >= n0 || cj >= n1 || ck >= n2) return SOLID;
+        return tc[cl(ci, cj, ck, n0, n1)];
+    };
+    auto coarsei = [&](int I, int J, int K) -> size_t {   // fineorcoarse's clamped coarse cell
+        return cl(min(max(I >> 2, 0), n0 - 1), min(max(J >> 2, 0), n1 - 1), min(max(K >> 2, 0), n2 - 1), n0, n1);
+    };
+    for (int r = 0; r < SBFACES / 256; ++r) {
+        const int f = r * 256 + (int)threadIdx.x;
+        int i, j, k;
+        if (a == 0) { i = f % 17; int q = f / 17; j = q & 15; k = q >> 4; }
+        else if (a == 1) { i = f & 15; int q = f >> 4; j = q % 17; k = q / 17; }
+        else { i = f & 15; j = (f >> 4) & 15; k = f >> 8; }
+        const int32_t n = (s - 1) * SBFACES + f;
+        const int I = w.x * 16 + i, J = w.y * 16 + j, K = w.z * 16 + k;
+        const int Im = I - (a == 0), Jm = J - (a == 1), Km = K - (a == 2);
+        const int32_t ip = idx(i, j, k), im = idx(i - (a == 0), j - (a == 1), k - (a == 2));
+        bool fp, fm;
+        const uint8_t tp = ctype(ip, I, J, K, fp), tm = ctype(im, Im, Jm, Km, fm);
+        const bool ok = tp != SOLID && tm != SOLID && (tp == LIQUID || tm == LIQUID);
+        if (ok) {
+            const float d = (fp && fm ? 1.0f : 2.5f) * t.g.dx;
+            const PT Pp = ip >= 0 ? P[ip] : Pc[coarsei(I, J, K)], Pm = im >= 0 ? P[im] : Pc[coarsei(Im, Jm, Km)];
+            const float php = ip >= 0 ? phi[ip] : phic[coarsei(I, J, K)], phm = im >= 0 ? phi[im] : phic[coarsei(Im, Jm, Km)];
+            float Dl;
+            if (tp == LIQUID && tm == LIQUID) Dl = pdiff(Pp, Pm);
+            else if (tm == LIQUID) Dl = pdiv(-Pm, theta(phm, php));
+            else Dl = pdiv(Pp, theta(php, phm));
+            u[n] -= __fdiv_rn(Dl, d);
+        }
+        valid[n] = ok ? 1 : 0;
     }
-    valid[n] = ok ? 1 : 0;
 }
 template <class PT>
 __global__ void k_projectcoarse(float* __restrict__ u, uint8_t* __restrict__ valid, const PT* __restrict__ Pc,
@@ -2537,15 +2570,15 @@ void SparseBasin::project(float gamma, const SparseOpts& o, SparseStats& stats) 
     solve_rhs(o, stats);
     // sflip.jl:2008-2013 the faces, then the coarse faces under fine blocks from theirs
     for (int a = 0; a < 3; ++a) {
-        if (S.nf) {
+        if (S.nf && nlive > 0) {
             if (o.refine > 0)
-                hipLaunchKernelGGL(k_projectfine<double>, dim3(cdiv(finefaces(), 256)), dim3(256), 0, s, u[a].fine.p,
-                                   fl[a].fine.p, Pd.fine.p, Pd.coarse.p, types.fine.p, types.coarse.p, phi.fine.p, phi.coarse.p,
-                                   t, a, (int32_t)finefaces());
+                hipLaunchKernelGGL(k_projectfine<double>, dim3((unsigned)nlive), dim3(256), 0, s, u[a].fine.p, fl[a].fine.p,
+                                   Pd.fine.p, Pd.coarse.p, types.fine.p, types.coarse.p, phi.fine.p, phi.coarse.p, t, a,
+                                   sorder.p);
             else
-                hipLaunchKernelGGL(k_projectfine<float>, dim3(cdiv(finefaces(), 256)), dim3(256), 0, s, u[a].fine.p,
-                                   fl[a].fine.p, P.fine.p, P.coarse.p, types.fine.p, types.coarse.p, phi.fine.p, phi.coarse.p,
-                                   t, a, (int32_t)finefaces());
+                hipLaunchKernelGGL(k_projectfine<float>, dim3((unsigned)nlive), dim3(256), 0, s, u[a].fine.p, fl[a].fine.p,
+                                   P.fine.p, P.coarse.p, types.fine.p, types.coarse.p, phi.fine.p, phi.coarse.p, t, a,
+                                   sorder.p);
         }
         if (o.refine > 0)
             hipLaunchKernelGGL(k_projectcoarse<double>, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, s, u[a].coarse.p,

@@ -37,16 +37,63 @@ __device__ inline V fineorcoarse(const STopo& t, const V* __restrict__ fine, con
     return coarse[cl(min(max(I >> 2, 0), n0 - 1), min(max(J >> 2, 0), n1 - 1), min(max(K >> 2, 0), n2 - 1), n0, n1)];
 }
 
-__global__ void k_finecoef(float* __restrict__ c, const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc, STopo t, int a,
-                           int32_t nf) {
-    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nf) return;
-    SCell s = fineface(t, a, n);
-    if (s.w.x < 0) return;
-    int I = s.w.x * 16 + s.i, J = s.w.y * 16 + s.j, K = s.w.z * 16 + s.k;
-    bool fp, fm;
-    uint8_t tp = celltype(t, tf, tc, I, J, K, fp), tm = celltype(t, tf, tc, I - (a == 0), J - (a == 1), K - (a == 2), fm);
-    c[n] = (tp == LIQUID && tm == LIQUID) ? (fp && fm ? FINE_FINE : FINE_COARSE) : 0.0f;
+// The faces along a of a live slot s (1-based), for kernels that take a workgroup per
+// slot and a thread per 17 faces: a face's two cells lie in the slot's block or in the
+// blocks below and above it along a, whose slots are looked up once.
+struct SlotFaces {
+    int s, a, n0, n1, n2;
+    int4 w;
+    int32_t slo, shi;
+    __device__ SlotFaces(const STopo& t, int slot, int axis) : s(slot), a(axis) {
+        w = t.where[s - 1];
+        const SlotsGlobal lk;
+        slo = lk(t, w.x - (a == 0), w.y - (a == 1), w.z - (a == 2));
+        shi = lk(t, w.x + (a == 0), w.y + (a == 1), w.z + (a == 2));
+        n0 = t.g.nc(0); n1 = t.g.nc(1); n2 = t.g.nc(2);
+    }
+    // face f of the slot: its local (i, j, k) (the layout of fineface)
+    __device__ void face(int f, int& i, int& j, int& k) const {
+        if (a == 0) { i = f % 17; int q = f / 17; j = q & 15; k = q >> 4; }
+        else if (a == 1) { i = f & 15; int q = f >> 4; j = q % 17; k = q / 17; }
+        else { i = f & 15; j = (f >> 4) & 15; k = f >> 8; }
+    }
+    // fineat's index of the cell at local (li, lj, lk), its coordinate along a in -1..16
+    __device__ int32_t cell(int li, int lj, int lk) const {
+        const int la = a == 0 ? li : (a == 1 ? lj : lk);
+        const int32_t sl = la < 0 ? slo : (la > 15 ? shi : s);
+        return sl == 0 ? -1 : (sl - 1) * 4096 + (li & 15) + 16 * ((lj & 15) + 16 * (lk & 15));
+    }
+    // celltype from that index (I, J, K global)
+    __device__ uint8_t type(const uint8_t* __restrict__ tf, const uint8_t* __restrict__ tc, int32_t id, int I, int J, int K,
+                            bool& isfine) const {
+        if (id >= 0) { isfine = true; return tf[id]; }
+        isfine = false;
+        const int ci = I >> 2, cj = J >> 2, ck = K >> 2;
+        if (ci < 0 || cj < 0 || ck < 0 || ci >= n0 || cj >= n1 || ck >= n2) return SOLID;
+        return tc[cl(ci, cj, ck, n0, n1)];
+    }
+    // fineorcoarse's coarse cell (clamped)
+    __device__ size_t coarse(int I, int J, int K) const {
+        return cl(min(max(I >> 2, 0), n0 - 1), min(max(J >> 2, 0), n1 - 1), min(max(K >> 2, 0), n2 - 1), n0, n1);
+    }
+};
+
+// A workgroup per live slot (block order), a thread per 17 of its faces along a.
+__global__ __launch_bounds__(256) void k_finecoef(float* __restrict__ c, const uint8_t* __restrict__ tf,
+                                                  const uint8_t* __restrict__ tc, STopo t, int a,
+                                                  const int32_t* __restrict__ order) {
+    const SlotFaces F(t, order[blockIdx.x], a);
+    for (int r = 0; r < SBFACES / 256; ++r) {
+        const int f = r * 256 + (int)threadIdx.x;
+        int i, j, k;
+        F.face(f, i, j, k);
+        const int I = F.w.x * 16 + i, J = F.w.y * 16 + j, K = F.w.z * 16 + k;
+        bool fp, fm;
+        const uint8_t tp = F.type(tf, tc, F.cell(i, j, k), I, J, K, fp);
+        const uint8_t tm = F.type(tf, tc, F.cell(i - (a == 0), j - (a == 1), k - (a == 2)), I - (a == 0), J - (a == 1),
+                                  K - (a == 2), fm);
+        c[(F.s - 1) * SBFACES + f] = (tp == LIQUID && tm == LIQUID) ? (fp && fm ? FINE_FINE : FINE_COARSE) : 0.0f;
+    }
 }
 __global__ void k_coarsecoef(float* __restrict__ c, const uint8_t* __restrict__ tc, STopo t, int a) {
     int nc0 = t.g.nc(0), nc1 = t.g.nc(1), nc2 = t.g.nc(2);
@@ -1873,9 +1920,9 @@ struct SparseSolver {
 
     void coefficients() {
         for (int a = 0; a < 3; ++a) {
-            if (s.nslots)
-                hipLaunchKernelGGL(k_finecoef, dim3(cdiv(s.finefaces(), 256)), dim3(256), 0, st, s.cf[a].p, s.types.fine.p,
-                                   s.types.coarse.p, t, a, (int32_t)s.finefaces());
+            if (s.nlive > 0)
+                hipLaunchKernelGGL(k_finecoef, dim3((unsigned)s.nlive), dim3(256), 0, st, s.cf[a].p, s.types.fine.p,
+                                   s.types.coarse.p, t, a, s.sorder.p);
             hipLaunchKernelGGL(k_coarsecoef, dim3(cdiv(s.nfaces(a), 256)), dim3(256), 0, st, s.cc[a].p, s.types.coarse.p, t, a);
         }
         if (nf)

@@ -564,14 +564,22 @@ void SparseBasin::classify() {
 
 // ─── walls (sflip.jl:1019-1041) ──────────────────────────────────────────────
 
-__global__ void k_solidfine(float* __restrict__ u, STopo t, int a, int32_t nf) {
-    int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nf) return;
-    SCell c = fineface(t, a, n);
-    if (c.w.x < 0) return;
-    float p[3];
-    facepos(t.g, a, c.w.x * 16 + c.i, c.w.y * 16 + c.j, c.w.z * 16 + c.k, t.g.dx, p);
-    if (walledface_dev(t.g, p, a, t.g.dx)) u[n] = 0.0f;
+// A workgroup per live slot (block order), a thread per 17 of its faces; a block clear of
+// the walls has no walled face and leaves at once.
+__global__ __launch_bounds__(256) void k_solidfine(float* __restrict__ u, STopo t, int a, const int32_t* __restrict__ order) {
+    const int s = order[blockIdx.x];
+    const int4 w = t.where[s - 1];
+    if (blockclear(t.g, w)) return;
+    for (int r = 0; r < SBFACES / 256; ++r) {
+        const int f = r * 256 + (int)threadIdx.x;
+        int i, j, k;
+        if (a == 0) { i = f % 17; int q = f / 17; j = q & 15; k = q >> 4; }
+        else if (a == 1) { i = f & 15; int q = f >> 4; j = q % 17; k = q / 17; }
+        else { i = f & 15; j = (f >> 4) & 15; k = f >> 8; }
+        float p[3];
+        facepos(t.g, a, w.x * 16 + i, w.y * 16 + j, w.z * 16 + k, t.g.dx, p);
+        if (walledface_dev(t.g, p, a, t.g.dx)) u[(s - 1) * SBFACES + f] = 0.0f;
+    }
 }
 __global__ void k_solidcoarse(float* __restrict__ u, SGrid g, int a) {
     int nc0 = g.nc(0), nc1 = g.nc(1), nc2 = g.nc(2);
@@ -588,9 +596,8 @@ __global__ void k_solidcoarse(float* __restrict__ u, SGrid g, int a) {
 }
 void SparseBasin::solidify() {
     for (int a = 0; a < 3; ++a) {
-        if (nslots > 0)
-            hipLaunchKernelGGL(k_solidfine, dim3(cdiv(finefaces(), 256)), dim3(256), 0, st.s, u[a].fine.p, topo(), a,
-                               (int32_t)finefaces());
+        if (nlive > 0)
+            hipLaunchKernelGGL(k_solidfine, dim3((unsigned)nlive), dim3(256), 0, st.s, u[a].fine.p, topo(), a, sorder.p);
         hipLaunchKernelGGL(k_solidcoarse, dim3(cdiv(nfaces(a), 256)), dim3(256), 0, st.s, u[a].coarse.p, g, a);
     }
     MAEL_LAUNCH_CHECK();
